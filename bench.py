@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Benchmark of the SPH step on one MI355X.
+
+Workload: the plain dam break with 1,048,576 FP32 particles (128x128x64
+jittered lattice, h = 0.04, rest density 1000, gas constant 3, viscosity
+0.01, dt = 0.003, walls on), synthetic seeded inputs resident in HBM.  A step
+is one full pass of the pipeline (hash, sort, cell start/end, permute,
+density, force + walls + leapfrog), replayed from one hipGraph.
+
+Prints one JSON line: particle-updates/s (N * steps / s), the per-stage
+times, the roofline of the dominant kernel (force, VALU-bound) and a CPU
+baseline (the FP32 CPU restatement of the reference step, OpenMP, timed on a
+bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/sec at 1M SPH (plain vs batched vs fwd+adjoint); HBM % peak"
+PEAK_HBM_GBS = 8000.0          # MI355X HBM3E
+PEAK_VALU_FP32_TFLOPS = 78.6   # non-packed FP32 VALU (packed FP32 is disabled in this build)
+
+
+def cpu_baseline(scene, steps: int = 2):
+    """Oracle FP32 step (bitonic sort, OpenMP) on a bounded sample: `steps`
+    steps of the same 1M-particle workload from its initial state."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    sp, gp, gs = O.derive_params(scene.settings)
+    st = O.State(scene.pos, scene.vel, sp, gp, gs)
+    st.step(scene.dt, O.FLAG_COLLISION | O.FLAG_BITONIC)   # untimed first touch
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.step(scene.dt, O.FLAG_COLLISION | O.FLAG_BITONIC)
+    dt = time.perf_counter() - t0
+    return {
+        "value": scene.n * steps / dt,
+        "unit": "particle-updates/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": f"{steps} steps (after 1 untimed) of the {scene.n}-particle dam break, FP32 CPU restatement "
+                  f"of the reference kernels with a bitonic sort, {O.num_threads()} OpenMP threads",
+        "ms_per_step": 1e3 * dt / steps,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--timer-steps", type=int, default=20, help="eager steps with per-stage events")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--profile", action="store_true", help="timed loop only (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU design is described in DESIGN.md and is not built", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    from enjaparticles_amd import scenes
+    from enjaparticles_amd.rtps import RTPS
+    import enjaparticles_amd._native as N
+
+    sc = scenes.dam1m()
+    s = RTPS.generateSystemInstance(sc.settings)
+    s.pushParticles(sc.pos, sc.vel)
+    s.setOption(N.OPT_GRAPH, 1)
+    s.step(args.warmup)
+    s.synchronize()
+    print(f"# warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n = s.getNum()
+    print(f"# timed {args.steps} steps: {elapsed:.3f} s", file=sys.stderr, flush=True)
+    if args.profile:
+        return
+
+    # per-stage times with HIP events on the handle's stream (eager launches)
+    K, M = s.countPairs()
+    s.setOption(N.OPT_TIMERS, 1)
+    s.resetTimers()
+    s.step(args.timer_steps)
+    s.synchronize()
+    stages = s.stageTimes()
+    K2, M2 = s.countPairs()
+    s.setOption(N.OPT_TIMERS, 0)
+    Kp, Mp = 0.5 * (K + K2) / n, 0.5 * (M + M2) / n
+    # force FLOP model: 8 per candidate pair + 35 per accepted pair (self excluded)
+    force_flop = n * (8.0 * Kp + 35.0 * (Mp - 1.0))
+    force_ms = stages["force"][0]
+    achieved = force_flop / (force_ms * 1e-3) / 1e12
+    dens_flop = n * (8.0 * Kp + 4.0 * Mp)
+    dens_ms = stages["density"][0]
+
+    ms_per_step = 1e3 * elapsed / args.steps
+    out = {
+        "metric": METRIC,
+        "value": n * args.steps / elapsed,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered lattice, SplitMix64)",
+        "config": {"workload": "plain SPH dam break, 1,048,576 particles (128x128x64), h=0.04, dt=0.003, "
+                               "walls on, hipGraph per step", "particles": n, "steps_per_s": 1e3 / ms_per_step},
+        "stages_ms": {k: round(v[0], 5) for k, v in stages.items()},
+        "pairs_per_particle": {"candidates": Kp, "accepted_incl_self": Mp},
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_force (fused force + walls + leapfrog)",
+            "achieved": achieved,
+            "peak": PEAK_VALU_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_VALU_FP32_TFLOPS,
+            "traffic": None,
+            "flop_per_launch": force_flop,
+            "avg_launch_ms": force_ms,
+        },
+        "density_roofline": {
+            "achieved_tflops": dens_flop / (dens_ms * 1e-3) / 1e12,
+            "frac": dens_flop / (dens_ms * 1e-3) / 1e12 / PEAK_VALU_FP32_TFLOPS,
+        },
+    }
+    s.close()
+    if not args.no_cpu_baseline:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,280 @@
+// Stable LSD radix sort of (u32 key, u32 value) pairs for gfx950, replacing
+// the reference's bitonic network (System::bitonic_sort, System.cpp:407-480;
+// Bitonic<T>::Sort, BitonicSort.cpp:62-183).  Only the low key_bits bits are
+// ordered (the cell keys are < 2^key_bits); digits are at most 8 bits wide and
+// the bits are split evenly over ceil(key_bits / 8) passes.
+//
+// Per pass (R = 2^bits buckets, tiles of 4096 keys = 256 threads x 16):
+//   k_radix_hist    per-tile digit histogram (LDS atomics), digit-major in
+//                   hist[d * ntiles + t]; digit totals accumulated in gtot[d]
+//   k_radix_scan    one block per digit: exclusive scan of its row plus the
+//                   total of all smaller digits -> global offsets
+//   k_radix_scatter stable ranking inside the tile with wave64 ballots
+//                   (lanes with equal digit found by `bits` ballots, rank =
+//                   popcount of the lower peers) and a per-digit scan over
+//                   the tile's 64 (round, wave) slots in LDS, then the write
+//                   to offset + rank.  Block 0 clears gtot for the next use.
+// Keys and values ping-pong between the caller's arrays and the alternate
+// arrays in the temp buffer.
+
+#include <hip/hip_runtime.h>
+
+#include "rtps_internal.h"
+
+namespace rtps {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;   // 4096
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kMaxRadix = 256;
+constexpr int kMaxPasses = 4;
+
+int sort_passes(int key_bits) { return key_bits <= 0 ? 0 : (key_bits + 7) / 8; }
+
+static inline int ntiles_for(int n) { return (n + kSortTile - 1) / kSortTile; }
+
+size_t sort_temp_bytes(int n, int key_bits) {
+    (void)key_bits;
+    const size_t nt = (size_t)ntiles_for(n > 0 ? n : 1);
+    size_t b = (size_t)kMaxPasses * kMaxRadix * sizeof(uint32_t);   // digit totals per pass (fixed offset 0)
+    b += 2 * (size_t)n * sizeof(uint32_t);                           // alternate keys / values
+    b = (b + 255) & ~(size_t)255;
+    b += (size_t)kMaxRadix * nt * sizeof(uint32_t);                  // tile histograms
+    return b;
+}
+
+struct SortTemp {
+    uint32_t* kalt;
+    uint32_t* valt;
+    uint32_t* hist;
+    uint32_t* gtot;
+};
+
+// gtot sits at offset 0 so its (kept zero) contents do not depend on n.
+static SortTemp carve(void* temp, int n) {
+    char* p = (char*)temp;
+    SortTemp t;
+    t.gtot = (uint32_t*)p;
+    size_t off = (size_t)kMaxPasses * kMaxRadix * sizeof(uint32_t);
+    t.kalt = (uint32_t*)(p + off);
+    t.valt = t.kalt + n;
+    off = (off + 2 * (size_t)n * sizeof(uint32_t) + 255) & ~(size_t)255;
+    t.hist = (uint32_t*)(p + off);
+    return t;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ kin, int n, int shift,
+                                                             int bits, uint32_t* __restrict__ hist,
+                                                             uint32_t* __restrict__ gtot, int ntiles) {
+    __shared__ uint32_t h[kMaxRadix];
+    const int R = 1 << bits;
+    const uint32_t mask = (uint32_t)R - 1u;
+    for (int d = threadIdx.x; d < R; d += kSortThreads) h[d] = 0;
+    __syncthreads();
+    const long base = (long)blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int k = 0; k < kSortItems; ++k) {
+        const long e = base + k * kSortThreads + threadIdx.x;
+        if (e < n) atomicAdd(&h[(kin[e] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < R; d += kSortThreads) {
+        const uint32_t c = h[d];
+        hist[(long)d * ntiles + blockIdx.x] = c;
+        if (c) atomicAdd(&gtot[d], c);
+    }
+}
+
+// Inclusive scan of v across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive block scan (256 threads); returns the exclusive prefix, total in *tot.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* tot) {
+    __shared__ uint32_t ws[kSortWaves];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int q = 0; q < kSortWaves; ++q) {
+        uint32_t s = ws[q];
+        before += q < w ? s : 0u;
+        all += s;
+    }
+    __syncthreads();
+    *tot = all;
+    return before + inc - v;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scan(uint32_t* __restrict__ hist,
+                                                             const uint32_t* __restrict__ gtot, int ntiles) {
+    const int d = blockIdx.x;
+    // base = total count of all smaller digits
+    uint32_t part = 0;
+    for (int e = threadIdx.x; e < d; e += kSortThreads) part += gtot[e];
+    uint32_t base;
+    block_excl_scan(part, &base);
+    // exclusive scan of row d, ntiles entries, in contiguous chunks per thread
+    uint32_t* row = hist + (long)d * ntiles;
+    const int c = (ntiles + kSortThreads - 1) / kSortThreads;
+    const int lo = threadIdx.x * c;
+    const int hi = min(lo + c, ntiles);
+    uint32_t s = 0;
+    for (int e = lo; e < hi; ++e) s += row[e];
+    uint32_t tot;
+    uint32_t run = base + block_excl_scan(s, &tot);
+    for (int e = lo; e < hi; ++e) {
+        uint32_t v = row[e];
+        row[e] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                                const uint32_t* __restrict__ vin,
+                                                                uint32_t* __restrict__ kout,
+                                                                uint32_t* __restrict__ vout, int n, int shift,
+                                                                int bits, const uint32_t* __restrict__ hist,
+                                                                int ntiles, uint32_t* __restrict__ gtot_clear) {
+    // slot s = round * 4 + wave; wcnt[s * R + d] = count, then exclusive prefix
+    __shared__ uint16_t wcnt[kSortItems * kSortWaves * kMaxRadix];
+    __shared__ uint32_t goff[kMaxRadix];
+    const int R = 1 << bits;
+    const uint32_t mask = (uint32_t)R - 1u;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const long base = (long)blockIdx.x * kSortTile;
+    for (int e = tid; e < kSortItems * kSortWaves * R; e += kSortThreads) wcnt[e] = 0;
+    for (int d = tid; d < R; d += kSortThreads) goff[d] = hist[(long)d * ntiles + blockIdx.x];
+    if (blockIdx.x == 0)
+        for (int d = tid; d < R; d += kSortThreads) gtot_clear[d] = 0;
+    __syncthreads();
+
+    uint32_t key[kSortItems], val[kSortItems];
+    uint32_t rnk[kSortItems];
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const long e = base + k * kSortThreads + tid;
+        const bool valid = e < n;
+        key[k] = valid ? kin[e] : 0xFFFFFFFFu;
+        val[k] = valid ? vin[e] : 0u;
+        const uint32_t d = (key[k] >> shift) & mask;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < bits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const uint32_t r = (uint32_t)__popcll(peers & lt);
+        if (valid && r == 0) wcnt[(k * kSortWaves + w) * R + d] = (uint16_t)__popcll(peers);
+        rnk[k] = r;
+    }
+    __syncthreads();
+    for (int d = tid; d < R; d += kSortThreads) {
+        uint32_t run = 0;
+        for (int s = 0; s < kSortItems * kSortWaves; ++s) {
+            const uint32_t c = wcnt[s * R + d];
+            wcnt[s * R + d] = (uint16_t)run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const long e = base + k * kSortThreads + tid;
+        if (e < n) {
+            const uint32_t d = (key[k] >> shift) & mask;
+            const uint32_t pos = goff[d] + wcnt[(k * kSortWaves + w) * R + d] + rnk[k];
+            kout[pos] = key[k];
+            vout[pos] = val[k];
+        }
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_copy2_u32(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint32_t* __restrict__ ao,
+                            uint32_t* __restrict__ bo, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        ao[i] = a[i];
+        bo[i] = b[i];
+    }
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (n <= 0) return hipSuccess;
+    k_fill_u32<<<(int)((n + 255) / 256), 256, 0, st>>>(p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t sort_clear_temp(void* temp, int n, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    SortTemp t = carve(temp, n);
+    return launch_fill_u32(t.gtot, 0u, (long)kMaxPasses * kMaxRadix, st);
+}
+
+// Sorts (keys, vals) by the low key_bits bits.  The result is in (keys, vals)
+// when *in_alt comes back false, else in the temp buffer's alternate arrays
+// (returned through out_keys / out_vals either way).  gtot must be zero on
+// entry (sort_clear_temp once; every pass leaves it zero again).
+hipError_t sort_pairs_pingpong(int n, int key_bits, uint32_t* keys, uint32_t* vals, void* temp, hipStream_t st,
+                               uint32_t** out_keys, uint32_t** out_vals) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    *out_keys = keys;
+    *out_vals = vals;
+    const int passes = sort_passes(key_bits);
+    if (n <= 1 || passes == 0) return hipSuccess;
+    SortTemp t = carve(temp, n);
+    const int ntiles = ntiles_for(n);
+    const int width = (key_bits + passes - 1) / passes;
+    uint32_t *ki = keys, *vi = vals, *ko = t.kalt, *vo = t.valt;
+    int shift = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int bits = min(width, key_bits - shift);
+        uint32_t* gtot = t.gtot + p * kMaxRadix;
+        k_radix_hist<<<ntiles, kSortThreads, 0, st>>>(ki, n, shift, bits, t.hist, gtot, ntiles);
+        k_radix_scan<<<1 << bits, kSortThreads, 0, st>>>(t.hist, gtot, ntiles);
+        k_radix_scatter<<<ntiles, kSortThreads, 0, st>>>(ki, vi, ko, vo, n, shift, bits, t.hist, ntiles, gtot);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        uint32_t* tk = ki; ki = ko; ko = tk;
+        uint32_t* tv = vi; vi = vo; vo = tv;
+        shift += bits;
+    }
+    *out_keys = ki;
+    *out_vals = vi;
+    return hipSuccess;
+}
+
+// In-place variant for the C ABI: copies back when the pass count is odd.
+hipError_t sort_pairs_inplace(int n, int key_bits, uint32_t* keys, uint32_t* vals, void* temp, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (n <= 1 || sort_passes(key_bits) == 0) return hipSuccess;
+    hipError_t e = sort_clear_temp(temp, n, st);
+    if (e != hipSuccess) return e;
+    uint32_t *ok, *ov;
+    e = sort_pairs_pingpong(n, key_bits, keys, vals, temp, st, &ok, &ov);
+    if (e != hipSuccess) return e;
+    if (ok != keys) {
+        k_copy2_u32<<<(n + 255) / 256, 256, 0, st>>>(ok, ov, keys, vals, n);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+}  // namespace rtps

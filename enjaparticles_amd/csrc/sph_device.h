@@ -1,0 +1,226 @@
+// Device-side math of the SPH step for gfx950 (wave64).  Each helper states
+// the reference code it restates; the kernels in sph_kernels.hip and the
+// fused step kernel share these so every path computes identical values.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rtps_c.h"
+
+namespace rtps {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // empty-cell start marker (CellIndices.cpp:59-62)
+constexpr int kMaxGravitySources = RTPS_MAX_GRAVITY_SOURCES;
+
+// Point attractors, by value as a kernel argument (System.cpp:261-269 buffers).
+struct GravitySources {
+    int n;
+    int pad_[3];
+    float4 pos[kMaxGravitySources];
+    float mass[kMaxGravitySources];
+    float alpha[kMaxGravitySources];
+};
+
+// The subset of GridParams the device needs, passed by value.
+struct Grid {
+    float3 min;
+    float3 delta;
+    float rx, ry;        // grid_res.x, grid_res.y as floats (the hash is evaluated in float)
+    int nb_cells;
+};
+
+inline Grid make_grid(const rtps_grid_params& g) {
+    Grid d;
+    d.min = make_float3(g.grid_min.x, g.grid_min.y, g.grid_min.z);
+    d.delta = make_float3(g.grid_delta.x, g.grid_delta.y, g.grid_delta.z);
+    d.rx = g.grid_res.x;
+    d.ry = g.grid_res.y;
+    d.nb_cells = g.nb_cells;
+    return d;
+}
+
+// Saturating float -> int (NaN -> 0), the hardware conversion the reference's
+// (int) casts rely on (cl_hash.h:39-42, :113).  Written with explicit guards
+// because an out-of-range C++ conversion is undefined.
+__device__ __forceinline__ int sat_f2i(float v) {
+    if (!(v == v)) return 0;
+    if (v >= 2147483648.0f) return 0x7FFFFFFF;
+    return (int)fmaxf(v, -2147483648.0f);
+}
+
+// calcGridCell (cl_hash.h:33-54): truncation toward zero of (p - min) * delta.
+__device__ __forceinline__ int3 grid_cell(float x, float y, float z, const Grid& g) {
+    return make_int3(sat_f2i((x - g.min.x) * g.delta.x), sat_f2i((y - g.min.y) * g.delta.y),
+                     sat_f2i((z - g.min.z) * g.delta.z));
+}
+
+// calcGridHash (cl_hash.h:57-114, no wrap): (gz*res.y + gy)*res.x + gx in float.
+__device__ __forceinline__ int grid_hash(int gx, int gy, int gz, const Grid& g) {
+#pragma clang fp contract(off)
+    float v = ((float)gz * g.ry + (float)gy) * g.rx + (float)gx;
+    return sat_f2i(v);
+}
+
+// Particle count for the kernels after cellindices: the out-of-grid deletion
+// path (SPH.cpp:220-250) shrinks num to nc = start[nb_cells] when
+// 0 <= nc <= num.  Returns true when the deletion path is active.
+__device__ __forceinline__ bool effective_num(int num, const uint32_t* start, int nb_cells, int& out) {
+    int nc = (int)start[nb_cells];
+    bool del = (nc >= 0 && nc <= num);
+    out = del ? nc : num;
+    return del;
+}
+
+// The three consecutive cells (x-1, x, x+1) of one row of the 27-cell
+// neighbourhood (cl_sph_neighbors.h:123-139) occupy one contiguous run of the
+// sorted arrays; returns [a, b) of that run, skipping cells with
+// hash >= nb_cells (unsigned, :68) and empty cells (:79).  Iterating j from a
+// to b visits exactly the particles, and in the order, of the reference's
+// cell-by-cell loop.
+__device__ __forceinline__ void row_range(int cx, int y, int z, const Grid& g, const uint32_t* __restrict__ start,
+                                          const uint32_t* __restrict__ end, uint32_t& a, uint32_t& b) {
+    a = kEmpty;
+    b = 0;
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx) {
+        uint32_t h = (uint32_t)grid_hash(cx + dx, y, z, g);
+        if (h < (uint32_t)g.nb_cells) {
+            uint32_t s = start[h];
+            if (s != kEmpty) {
+                a = min(a, s);
+                b = max(b, end[h]);
+            }
+        }
+    }
+    if (a == kEmpty) { a = 0; b = 0; }
+}
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Per-particle pair-sum accumulator of force.cl:43-140 (pt.force, pt.viscosity, pt.xsph).
+struct ForceAcc {
+    float fx, fy, fz;
+    float vx, vy, vz;
+    float xx, xy, xz;
+};
+
+// One accepted pair (rlen <= h, j != i) of force.cl:62-139.  Self pairs add
+// exactly zero in the reference (iej = 0 multiplies a finite value), so they
+// are skipped.  r = p_i - p_j (scaled), r2 = |r|^2.
+__device__ __forceinline__ void force_pair(ForceAcc& acc, float rx, float ry, float rz, float r2, float h, float h2,
+                                           float eps, float K, float rho0, float Pi, float di, float4 vi, float4 vj,
+                                           float dj) {
+    float rlen = fmaxf(__builtin_sqrtf(r2), eps);   // force.cl:73
+    float hr = h - rlen;
+    float dW = hr * hr * fast_rcp(rlen);            // Wspiky_dr, cl_sph_kernels.h:81-97
+    float idj = fast_rcp(dj);
+    float Pj = K * (dj - rho0);                     // force.cl:84-85
+    float kern = dW * (Pi + Pj) * idj;              // :97
+    float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+    float lv = hr * idj;                            // Wvisc_lapl * idj, :109-112
+    acc.vx += dvx * lv;
+    acc.vy += dvy * lv;
+    acc.vz += dvz * lv;
+    float t = h2 - r2;                              // Wpoly6, cl_sph_kernels.h:29-45
+    float w6 = t * t * t * fast_rcp(di + dj);       // :129-133
+    acc.xx += w6 * dvx;
+    acc.xy += w6 * dvy;
+    acc.xz += w6 * dvz;
+    acc.fx += rx * kern;                            // :136
+    acc.fy += ry * kern;
+    acc.fz += rz * kern;
+}
+
+// force.cl:181-185 epilogue: force = m/rho_i * (-1/2 wspiky_d F + mu wvisc_dd V),
+// xsph = 2 m wpoly6 X.
+__device__ __forceinline__ void force_epilogue(const ForceAcc& acc, float di, const rtps_sph_params& p, float4& F,
+                                               float4& X) {
+    float a = p.mass * (1.0f / di);
+    float cp = -0.5f * p.wspiky_d_coef;
+    float cv = p.viscosity * p.wvisc_dd_coef;
+    F = make_float4(a * (cp * acc.fx + cv * acc.vx), a * (cp * acc.fy + cv * acc.vy), a * (cp * acc.fz + cv * acc.vz),
+                    0.0f);
+    float cx = 2.0f * p.mass * p.wpoly6_coef;
+    X = make_float4(cx * acc.xx, cx * acc.xy, cx * acc.xz, 0.0f);
+}
+
+// gravity.cl:9-36 for one particle: accel += mag * fast_normalize(vect).
+__device__ __forceinline__ void gravity_sources(float4& f, float4 pos, const GravitySources& gs, float scale) {
+    for (int s = 0; s < gs.n; ++s) {
+        float vx = scale * (gs.pos[s].x - pos.x), vy = scale * (gs.pos[s].y - pos.y), vz = scale * (gs.pos[s].z - pos.z);
+        float d2 = vx * vx + vy * vy + vz * vz;
+        float inv = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+        float mag = gs.alpha[s] * gs.mass[s];
+        float cut = 0.5f * scale;
+        cut *= cut;
+        mag = d2 < cut ? mag / -cut : mag / d2;
+        f.x += mag * (vx * inv);
+        f.y += mag * (vy * inv);
+        f.z += mag * (vz * inv);
+    }
+}
+
+// One wall of collision_wall.cl:44-107 with calculateRepulsionForce /
+// calculateFrictionForce (cl_sph_collision.h:31-73); friction coefficients
+// are the hard-coded zeros of collision_wall.cl:61-62.
+__device__ __forceinline__ void wall(float diff, float nx, float ny, float nz, float4 v, const rtps_sph_params& p,
+                                     float3& rf, float3& ff) {
+    if (!(diff > p.EPSILON)) return;
+    float ndv = nx * v.x + ny * v.y + nz * v.z;
+    float m = p.boundary_stiffness * diff - p.boundary_dampening * ndv;
+    rf.x += m * nx;
+    rf.y += m * ny;
+    rf.z += m * nz;
+    float vtx = v.x - v.x * ndv, vty = v.y - v.y * ndv, vtz = v.z - v.z * ndv;   // v_n = v * dot(n, v)
+    if ((vtx + vty + vtz) / 3.0f > 0.0f) {
+        ff.x += -vtx;
+        ff.y += -vty;
+        ff.z += -vtz;
+    } else {
+        ff.x += 0.0f * -vtx;
+        ff.y += 0.0f * -vty;
+        ff.z += 0.0f * -vtz;
+    }
+}
+
+// collision_wall.cl:31-119 for one particle (p = unscaled sorted position).
+__device__ __forceinline__ void collision_wall(float4& f, float4 p, float4 v, const rtps_sph_params& sp,
+                                               const rtps_float4& bmin, const rtps_float4& bmax) {
+    const float s = sp.simulation_scale;
+    const float px = p.x * s, py = p.y * s, pz = p.z * s;
+    const float bd = sp.rest_distance * 0.5f;
+    float3 rf = make_float3(0.f, 0.f, 0.f), ff = make_float3(0.f, 0.f, 0.f);
+    wall(bd - (pz - bmin.z), 0.f, 0.f, 1.f, v, sp, rf, ff);
+    wall(bd - (bmax.z - pz), 0.f, 0.f, -1.f, v, sp, rf, ff);
+    wall(bd - (py - bmin.y), 0.f, 1.f, 0.f, v, sp, rf, ff);
+    wall(bd - (bmax.y - py), 0.f, -1.f, 0.f, v, sp, rf, ff);
+    wall(bd - (px - bmin.x), 1.f, 0.f, 0.f, v, sp, rf, ff);
+    wall(bd - (bmax.x - px), -1.f, 0.f, 0.f, v, sp, rf, ff);
+    f.x += rf.x + ff.x;
+    f.y += rf.y + ff.y;
+    f.z += rf.z + ff.z;
+}
+
+// leapfrog.cl:30-121 for one particle; returns the new unsorted records.
+__device__ __forceinline__ void leapfrog(float4 p, float4 v, float4 f, float4 xs, const rtps_sph_params& sp, float dt,
+                                         float4& pos_out, float4& vel_out, float4& veleval_out) {
+#pragma clang fp contract(off)
+    const float s = sp.simulation_scale;
+    p.x *= s; p.y *= s; p.z *= s;
+    f.x += sp.gravity.x; f.y += sp.gravity.y; f.z += sp.gravity.z;
+    float speed = __builtin_sqrtf(f.x * f.x + f.y * f.y + f.z * f.z);
+    if (speed > sp.velocity_limit) {
+        float k = sp.velocity_limit / speed;
+        f.x *= k; f.y *= k; f.z *= k;
+    }
+    float4 vn = make_float4(v.x + dt * f.x, v.y + dt * f.y, v.z + dt * f.z, v.w);
+    const float xf = sp.xsph_factor;
+    vn.x += xf * xs.x; vn.y += xf * xs.y; vn.z += xf * xs.z; vn.w += xf * xs.w;
+    veleval_out = make_float4(0.5f * (v.x + vn.x), 0.5f * (v.y + vn.y), 0.5f * (v.z + vn.z), 0.5f * (v.w + vn.w));
+    p.x += dt * vn.x; p.y += dt * vn.y; p.z += dt * vn.z;
+    pos_out = make_float4(p.x / s, p.y / s, p.z / s, 1.0f);
+    vel_out = vn;
+}
+
+}  // namespace rtps

@@ -1,0 +1,411 @@
+// Per-stage HIP kernels of the SPH step for gfx950 and their C-ABI launchers
+// (hash, cellindices, permute, density, force, gravity, collision, leapfrog),
+// plus the fused force+gravity+collision+leapfrog kernel used by the system
+// step.  The radix sort lives in radix_sort.hip.
+//
+// Layout in HBM: one float4 (16 B) per particle for position, velocity and
+// veleval; sorted copies are separate arrays; keys/idx are u32; start/end
+// are u32[nb_cells + 1].  Kernels run one thread per particle (wave64,
+// 256-thread blocks); the neighbour loops walk nine contiguous runs of the
+// sorted arrays (one per (y, z) row of the 27-cell block).
+
+#include <hip/hip_runtime.h>
+
+#include "rtps_internal.h"
+#include "sph_device.h"
+
+namespace rtps {
+
+constexpr int kBlock = 256;
+
+static inline int blocks_for(long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------
+// a2: hash.cl:50-157.  Optionally also clears start[0..n_clear) to kEmpty
+// (the per-step host clear of CellIndices.cpp:59-62, moved on device).
+__global__ __launch_bounds__(kBlock) void k_hash(int num_host, const int* __restrict__ d_num, int n_pad,
+                                                 uint32_t pad_key, const float4* __restrict__ pos_u,
+                                                 uint32_t* __restrict__ keys, uint32_t* __restrict__ idx, Grid g,
+                                                 uint32_t* __restrict__ start_clear, int n_clear) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_clear) start_clear[i] = kEmpty;
+    const int num = d_num ? *d_num : num_host;
+    if (i < num) {
+        float4 p = pos_u[i];
+        int3 c = grid_cell(p.x, p.y, p.z, g);
+        int h = grid_hash(c.x, c.y, c.z, g);
+        h = h > g.nb_cells ? g.nb_cells : h;   // hash.cl:91
+        h = h < 0 ? g.nb_cells : h;            // hash.cl:92
+        keys[i] = (uint32_t)h;
+        idx[i] = (uint32_t)i;
+    } else if (i < n_pad) {
+        keys[i] = pad_key;
+        idx[i] = (uint32_t)i;
+    }
+}
+
+// a4: cell start/end by boundary detection on the sorted keys (semantics of
+// cellindices.cl:12-107: start[k] = first i with key k, end[k] = one past the
+// last; keys > nb_cells are ignored).
+__global__ __launch_bounds__(kBlock) void k_cellindices(int num_host, const int* __restrict__ d_num,
+                                                        const uint32_t* __restrict__ keys, uint32_t nb_cells,
+                                                        uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int num = d_num ? *d_num : num_host;
+    if (i >= num) return;
+    uint32_t h = keys[i];
+    if (h > nb_cells) return;
+    if (i == 0) {
+        start[h] = 0;
+    } else {
+        uint32_t prev = min(keys[i - 1], nb_cells);
+        if (prev != h) {
+            start[h] = (uint32_t)i;
+            end[prev] = (uint32_t)i;
+        }
+    }
+    if (i == num - 1) end[h] = (uint32_t)num;
+}
+
+__global__ void k_read_nc(const uint32_t* __restrict__ start, int nb_cells, int* __restrict__ nc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *nc = (int)start[nb_cells];
+}
+
+// a5: permute.cl:10-46 (pos, vel, veleval).
+__global__ __launch_bounds__(kBlock) void k_permute(int num_host, const int* __restrict__ d_num,
+                                                    const uint32_t* __restrict__ idx, const float4* __restrict__ pu,
+                                                    const float4* __restrict__ vu, const float4* __restrict__ eu,
+                                                    float4* __restrict__ ps, float4* __restrict__ vs,
+                                                    float4* __restrict__ es) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int num = d_num ? *d_num : num_host;
+    if (i >= num) return;
+    const uint32_t s = idx[i];
+    ps[i] = pu[s];
+    vs[i] = vu[s];
+    es[i] = eu[s];
+}
+
+// ---------------------------------------------------------------------------
+// a7: density.cl:86-126.  PACKED: the density is written to veleval_s[i].w
+// (the system layout, so the force loop loads one float4 per neighbour).
+template <bool SCALED, bool PACKED>
+__global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __restrict__ d_num,
+                                                    const float4* __restrict__ pos_s,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
+                                                    float* __restrict__ dens, float4* __restrict__ packed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int num = d_num ? *d_num : num_host;
+    if (d_num) effective_num(num, start, g.nb_cells, num);
+    if (i >= num) return;
+    const float scale = sp.simulation_scale;
+    const float h = sp.smoothing_distance, h2 = h * h;
+    float4 pi = pos_s[i];
+    if (SCALED) { pi.x *= scale; pi.y *= scale; pi.z *= scale; }
+    const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
+    float sum = 0.0f;
+    for (int z = c.z - 1; z <= c.z + 1; ++z) {
+        for (int y = c.y - 1; y <= c.y + 1; ++y) {
+            uint32_t a, b;
+            row_range(c.x, y, z, g, start, end, a, b);
+            for (uint32_t j = a; j < b; ++j) {
+                float4 pj = pos_s[j];
+                if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                float r2 = rx * rx + ry * ry + rz * rz;
+                float t = h2 - r2;
+                sum += (r2 <= h2) ? t * t * t : 0.0f;
+            }
+        }
+    }
+    const float rho = sp.wpoly6_coef * sp.mass * sum;
+    if (PACKED) packed[i].w = rho; else dens[i] = rho;
+}
+
+// ---------------------------------------------------------------------------
+// a8 (+a9, a10, a11 when FUSED): force.cl:148-186.  PACKED: veleval and
+// density come from one float4 (xyz = veleval, w = density).  FUSED: the
+// epilogue goes on to the point sources (gravity.cl), the walls
+// (collision_wall.cl) and the leapfrog step (leapfrog.cl), scattering to the
+// unsorted arrays; force/xsph are then stored only if force_out != nullptr.
+template <bool SCALED, bool PACKED, bool FUSED>
+__global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __restrict__ d_num,
+                                                  const float4* __restrict__ pos_s,
+                                                  const float* __restrict__ dens_in,
+                                                  const float4* __restrict__ ve_in,
+                                                  const uint32_t* __restrict__ start,
+                                                  const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
+                                                  float4* __restrict__ force_out, float4* __restrict__ xsph_out,
+                                                  // FUSED only:
+                                                  const float4* __restrict__ vel_s, const uint32_t* __restrict__ idx,
+                                                  GravitySources gsrc, int collision, rtps_float4 bmin,
+                                                  rtps_float4 bmax, float dt, float4* __restrict__ pos_u,
+                                                  float4* __restrict__ vel_u, float4* __restrict__ veleval_u) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int num = d_num ? *d_num : num_host;
+    bool deleted = false;
+    if (d_num) deleted = effective_num(num, start, g.nb_cells, num);
+    if (i >= num) return;
+    const float scale = sp.simulation_scale;
+    const float h = sp.smoothing_distance, h2 = h * h, eps = sp.EPSILON;
+    const float K = sp.K, rho0 = sp.rest_density;
+    float4 pi = pos_s[i];
+    const float4 p_unscaled = pi;
+    if (SCALED) { pi.x *= scale; pi.y *= scale; pi.z *= scale; }
+    float4 vi;
+    float di;
+    if (PACKED) { vi = ve_in[i]; di = vi.w; } else { vi = ve_in[i]; di = dens_in[i]; }
+    const float Pi = K * (di - rho0);
+    const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
+    ForceAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int z = c.z - 1; z <= c.z + 1; ++z) {
+        for (int y = c.y - 1; y <= c.y + 1; ++y) {
+            uint32_t a, b;
+            row_range(c.x, y, z, g, start, end, a, b);
+            for (uint32_t j = a; j < b; ++j) {
+                float4 pj = pos_s[j];
+                if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 <= h2 && j != (uint32_t)i) {
+                    float4 vj = ve_in[j];
+                    float dj = PACKED ? vj.w : dens_in[j];
+                    force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, dj);
+                }
+            }
+        }
+    }
+    float4 F, X;
+    force_epilogue(acc, di, sp, F, X);
+    if (force_out) { force_out[i] = F; xsph_out[i] = X; }
+    if (FUSED) {
+        gravity_sources(F, p_unscaled, gsrc, scale);
+        const float4 v = vel_s[i];
+        if (collision) collision_wall(F, p_unscaled, v, sp, bmin, bmax);
+        float4 po, vo, eo;
+        leapfrog(p_unscaled, v, F, X, sp, dt, po, vo, eo);
+        const uint32_t o = deleted ? (uint32_t)i : idx[i];
+        pos_u[o] = po;
+        vel_u[o] = vo;
+        veleval_u[o] = eo;
+    }
+}
+
+// a9 + a10 on stored forces (the unfused update() path): point sources then
+// walls, in the reference's order (SPH.cpp:289-297).
+__global__ __launch_bounds__(kBlock) void k_post_force(int num_host, const int* __restrict__ d_num,
+                                                       const uint32_t* __restrict__ start, int nb_cells,
+                                                       const float4* __restrict__ pos_s,
+                                                       const float4* __restrict__ vel_s, float4* __restrict__ force,
+                                                       rtps_sph_params sp, GravitySources gsrc, int collision,
+                                                       rtps_float4 bmin, rtps_float4 bmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int num = d_num ? *d_num : num_host;
+    if (d_num) effective_num(num, start, nb_cells, num);
+    if (i >= num) return;
+    float4 f = force[i];
+    const float4 p = pos_s[i];
+    gravity_sources(f, p, gsrc, sp.simulation_scale);
+    if (collision) collision_wall(f, p, vel_s[i], sp, bmin, bmax);
+    force[i] = f;
+}
+
+// a11: leapfrog.cl:30-121.
+__global__ __launch_bounds__(kBlock) void k_leapfrog(int num_host, const int* __restrict__ d_num,
+                                                     const uint32_t* __restrict__ start, int nb_cells,
+                                                     const float4* __restrict__ pos_s,
+                                                     const float4* __restrict__ vel_s,
+                                                     const float4* __restrict__ force,
+                                                     const float4* __restrict__ xsph,
+                                                     const uint32_t* __restrict__ idx, rtps_sph_params sp, float dt,
+                                                     float4* __restrict__ pos_u, float4* __restrict__ vel_u,
+                                                     float4* __restrict__ veleval_u) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int num = d_num ? *d_num : num_host;
+    bool deleted = false;
+    if (d_num) deleted = effective_num(num, start, nb_cells, num);
+    if (i >= num) return;
+    float4 po, vo, eo;
+    leapfrog(pos_s[i], vel_s[i], force[i], xsph[i], sp, dt, po, vo, eo);
+    const uint32_t o = deleted ? (uint32_t)i : idx[i];
+    pos_u[o] = po;
+    vel_u[o] = vo;
+    veleval_u[o] = eo;
+}
+
+// Candidate (K) and accepted (M, self included) pair totals of the
+// neighbour loop on the current sorted state, for the FLOP model.
+__global__ __launch_bounds__(kBlock) void k_count_pairs(int num_host, const int* __restrict__ d_num,
+                                                        const float4* __restrict__ pos_s,
+                                                        const uint32_t* __restrict__ start,
+                                                        const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
+                                                        unsigned long long* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int num = d_num ? *d_num : num_host;
+    if (d_num) effective_num(num, start, g.nb_cells, num);
+    unsigned long long k = 0, m = 0;
+    if (i < num) {
+        const float scale = sp.simulation_scale, h = sp.smoothing_distance, h2 = h * h;
+        float4 pi = pos_s[i];
+        pi.x *= scale; pi.y *= scale; pi.z *= scale;
+        const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
+        for (int z = c.z - 1; z <= c.z + 1; ++z)
+            for (int y = c.y - 1; y <= c.y + 1; ++y) {
+                uint32_t a, b;
+                row_range(c.x, y, z, g, start, end, a, b);
+                k += b - a;
+                for (uint32_t j = a; j < b; ++j) {
+                    float4 pj = pos_s[j];
+                    float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
+                    m += (rx * rx + ry * ry + rz * rz <= h2) ? 1u : 0u;
+                }
+            }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        k += __shfl_down(k, o, 64);
+        m += __shfl_down(m, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && (k | m)) {
+        atomicAdd(&out[0], k);
+        atomicAdd(&out[1], m);
+    }
+}
+
+hipError_t launch_count_pairs(int num, const int* d_num, const float4* pos_s, const uint32_t* start,
+                              const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
+                              unsigned long long* out, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (num <= 0) return hipSuccess;
+    k_count_pairs<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, out);
+    return hipGetLastError();
+}
+
+// Commits the deletion path's particle count for the next step.
+__global__ void k_commit_num(int* __restrict__ d_num, const uint32_t* __restrict__ start, int nb_cells) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int num = *d_num;
+        int nc = (int)start[nb_cells];
+        if (nc >= 0 && nc <= num) *d_num = nc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers shared by the C ABI and the system.
+
+hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
+                       uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    const int n = n_pad > n_clear ? n_pad : n_clear;
+    if (n <= 0) return hipSuccess;
+    k_hash<<<blocks_for(n), kBlock, 0, st>>>(num, d_num, n_pad, pad_key, pos_u, keys, idx, g, start_clear, n_clear);
+    return hipGetLastError();
+}
+
+hipError_t launch_cellindices(int num, const int* d_num, const uint32_t* keys, int nb_cells, uint32_t* start,
+                              uint32_t* end, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (num <= 0) return hipSuccess;
+    k_cellindices<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, keys, (uint32_t)nb_cells, start, end);
+    return hipGetLastError();
+}
+
+hipError_t launch_read_nc(const uint32_t* start, int nb_cells, int* nc, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    k_read_nc<<<1, 64, 0, st>>>(start, nb_cells, nc);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st);
+
+hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const float4* pu, const float4* vu,
+                          const float4* eu, float4* ps, float4* vs, float4* es, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (num <= 0) return hipSuccess;
+    k_permute<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, idx, pu, vu, eu, ps, vs, es);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
+                          const rtps_sph_params& sp, const Grid& g, float* dens, float4* packed, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (num <= 0) return hipSuccess;
+    const bool scaled = sp.simulation_scale != 1.0f;
+    const int nb = blocks_for(num);
+    if (packed) {
+        if (scaled) k_density<true, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, dens, packed);
+        else k_density<false, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, dens, packed);
+    } else {
+        if (scaled) k_density<true, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, dens, packed);
+        else k_density<false, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, dens, packed);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const float* dens, const float4* ve,
+                        bool packed, const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp,
+                        const Grid& g, float4* force, float4* xsph, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (num <= 0) return hipSuccess;
+    const bool scaled = sp.simulation_scale != 1.0f;
+    const int nb = blocks_for(num);
+    GravitySources none{};
+    rtps_float4 z{0, 0, 0, 0};
+#define RTPS_FORCE(S, P)                                                                                            \
+    k_force<S, P, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, force, xsph,       \
+                                                nullptr, nullptr, none, 0, z, z, 0.f, nullptr, nullptr, nullptr)
+    if (packed) { if (scaled) RTPS_FORCE(true, true); else RTPS_FORCE(false, true); }
+    else { if (scaled) RTPS_FORCE(true, false); else RTPS_FORCE(false, false); }
+#undef RTPS_FORCE
+    return hipGetLastError();
+}
+
+hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, const float4* ve_packed,
+                              const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
+                              float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
+                              const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,
+                              float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (num <= 0) return hipSuccess;
+    const bool scaled = sp.simulation_scale != 1.0f;
+    const int nb = blocks_for(num);
+    if (scaled)
+        k_force<true, true, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g,
+                                                         force, xsph, vel_s, idx, gsrc, collision, gps.bnd_min,
+                                                         gps.bnd_max, dt, pos_u, vel_u, veleval_u);
+    else
+        k_force<false, true, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g,
+                                                          force, xsph, vel_s, idx, gsrc, collision, gps.bnd_min,
+                                                          gps.bnd_max, dt, pos_u, vel_u, veleval_u);
+    return hipGetLastError();
+}
+
+hipError_t launch_post_force(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
+                             const float4* vel_s, float4* force, const rtps_sph_params& sp,
+                             const GravitySources& gsrc, int collision, const rtps_grid_params& gps, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (num <= 0) return hipSuccess;
+    if (gsrc.n == 0 && !collision) return hipSuccess;
+    k_post_force<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, start, nb_cells, pos_s, vel_s, force, sp, gsrc,
+                                                      collision, gps.bnd_min, gps.bnd_max);
+    return hipGetLastError();
+}
+
+hipError_t launch_leapfrog(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
+                           const float4* vel_s, const float4* force, const float4* xsph, const uint32_t* idx,
+                           const rtps_sph_params& sp, float dt, float4* pos_u, float4* vel_u, float4* veleval_u,
+                           hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (num <= 0) return hipSuccess;
+    k_leapfrog<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, start, nb_cells, pos_s, vel_s, force, xsph, idx, sp, dt,
+                                                    pos_u, vel_u, veleval_u);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit_num(int* d_num, const uint32_t* start, int nb_cells, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    k_commit_num<<<1, 64, 0, st>>>(d_num, start, nb_cells);
+    return hipGetLastError();
+}
+
+}  // namespace rtps

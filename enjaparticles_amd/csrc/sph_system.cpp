@@ -1,0 +1,495 @@
+// The SPH system handle: RTPS::generateSystemInstance + SPH (rtpslib/RTPS.cpp:37-67,
+// rtpslib/system/SPH.cpp) on one HIP stream.
+//
+// Device state (all sized for max_num_particles, one HIP allocation each):
+//   unsorted  pos_u, vel_u, veleval_u        float4   (original index order)
+//   sorted    pos_s, vel_s, ve_s             float4   (ve_s = veleval.xyz | density)
+//             force_s, xsph_s                float4   (unfused path / force output)
+//   keys/idx (+ alternates in the sort temp) u32
+//   start/end                                u32[nb_cells + 1]
+//   d_num                                    the particle count, on the device
+// The step never synchronises with the host: the per-step clear of start[]
+// is fused into the hash kernel, nc stays on the device and the deletion
+// path's new count is committed by a one-thread kernel, so rtps_sph_step can
+// replay one captured hipGraph per step.
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rtps_internal.h"
+#include "sph_device.h"
+
+using namespace rtps;
+
+enum Stage { ST_HASH = 0, ST_SORT, ST_CELL, ST_PERMUTE, ST_DENSITY, ST_FORCE, ST_N };
+
+struct rtps_sph {
+    RTPSSettings settings;
+    rtps_sph_params sp{};
+    rtps_grid_params gp{}, gps{};
+    Grid g{}, gs{};
+    float dt = 0.f;
+    int max_num = 0, num_host = 0, nb_cells = 0, key_bits = 0;
+    uint32_t pad_key = 0;
+    hipStream_t stream = nullptr;
+    float4 *pos_u = nullptr, *vel_u = nullptr, *veleval_u = nullptr;
+    float4 *pos_s = nullptr, *vel_s = nullptr, *ve_s = nullptr, *force_s = nullptr, *xsph_s = nullptr;
+    uint32_t *keys = nullptr, *idx = nullptr, *start = nullptr, *end = nullptr;
+    uint32_t *keys_sorted = nullptr, *idx_sorted = nullptr;
+    int* d_num = nullptr;
+    void* sort_tmp = nullptr;
+    GravitySources gsrc{};
+    int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_valid = false;
+    hipEvent_t ev[ST_N + 1] = {};
+    double stage_ms[ST_N] = {};
+    long stage_cnt[ST_N] = {};
+    bool update_pending = false;
+
+    ~rtps_sph() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        void* bufs[] = {pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s, force_s, xsph_s, keys, idx, start, end, d_num,
+                        sort_tmp};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    return hipMalloc((void**)p, count * sizeof(T) + 16);
+}
+
+#define RTPS_TRY_HIP(expr)                                     \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) return hip_error(e_, #expr);     \
+    } while (0)
+
+void load_gravity_sources(rtps_sph* s) {
+    std::vector<rtps_float4> src = s->settings.GetSettingAs<std::vector<rtps_float4>>("gravity_sources");
+    std::vector<float> mass = s->settings.GetSettingAs<std::vector<float>>("gravity_mass");
+    std::vector<float> alpha = s->settings.GetSettingAs<std::vector<float>>("gravity_alphas");
+    std::memset(&s->gsrc, 0, sizeof(s->gsrc));
+    int n = (int)std::min(src.size(), (size_t)kMaxGravitySources);
+    bool any = false;
+    for (int i = 0; i < n; ++i) {
+        s->gsrc.pos[i] = make_float4(src[i].x, src[i].y, src[i].z, src[i].w);
+        s->gsrc.mass[i] = i < (int)mass.size() ? mass[i] : 0.f;
+        s->gsrc.alpha[i] = i < (int)alpha.size() ? alpha[i] : 0.f;
+        any = any || s->gsrc.alpha[i] * s->gsrc.mass[i] != 0.f;
+    }
+    // With every alpha * mass zero the sources add exactly zero (gravity.cl:25-34).
+    s->gsrc.n = any ? n : 0;
+}
+
+void apply_settings(rtps_sph* s) {
+    read_sph_params(s->settings, s->sp);
+    s->sp.num = s->num_host;
+    s->dt = s->settings.GetSettingAs<float>("time_step");
+    load_gravity_sources(s);
+    s->settings.updated();
+    s->graph_valid = false;
+}
+
+int maybe_update_params(rtps_sph* s) {
+    if (s->settings.hasChanged()) apply_settings(s);   // SPH.cpp:154
+    return RTPS_OK;
+}
+
+struct StageTimer {
+    rtps_sph* s;
+    bool on;
+    explicit StageTimer(rtps_sph* s_) : s(s_), on(s_->opt_timers != 0) {}
+    void mark(int i) {
+        if (on) (void)hipEventRecord(s->ev[i], s->stream);
+    }
+};
+
+// hash -> sort -> cellindices -> permute (System::hash_and_sort, SPH.cpp:170-213)
+int launch_front(rtps_sph* s, StageTimer& t) {
+    const int n = s->num_host;
+    t.mark(ST_HASH);
+    RTPS_TRY_HIP(launch_hash(n, s->d_num, n, s->pad_key, s->pos_u, s->keys, s->idx, s->g, s->start,
+                             s->nb_cells + 1, s->stream));
+    t.mark(ST_SORT);
+    RTPS_TRY_HIP(sort_pairs_pingpong(n, s->key_bits, s->keys, s->idx, s->sort_tmp, s->stream, &s->keys_sorted,
+                                     &s->idx_sorted));
+    t.mark(ST_CELL);
+    RTPS_TRY_HIP(launch_cellindices(n, s->d_num, s->keys_sorted, s->nb_cells, s->start, s->end, s->stream));
+    t.mark(ST_PERMUTE);
+    RTPS_TRY_HIP(launch_permute(n, s->d_num, s->idx_sorted, s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s,
+                                s->ve_s, s->stream));
+    t.mark(ST_DENSITY);
+    RTPS_TRY_HIP(launch_density(n, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, nullptr, s->ve_s, s->stream));
+    return RTPS_OK;
+}
+
+// One fused step: front + force/gravity/collision/leapfrog + count commit.
+int launch_step(rtps_sph* s, StageTimer& t) {
+    int rc = launch_front(s, t);
+    if (rc) return rc;
+    t.mark(ST_FORCE);
+    const bool fo = s->opt_force_out != 0;
+    RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs,
+                                    fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s, s->idx_sorted,
+                                    s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u, s->veleval_u,
+                                    s->stream));
+    RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
+    t.mark(ST_N);
+    return RTPS_OK;
+}
+
+int collect_timers(rtps_sph* s) {
+    RTPS_TRY_HIP(hipEventSynchronize(s->ev[ST_N]));
+    for (int i = 0; i < ST_N; ++i) {
+        float ms = 0.f;
+        RTPS_TRY_HIP(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+        s->stage_ms[i] += ms;
+        s->stage_cnt[i] += 1;
+    }
+    return RTPS_OK;
+}
+
+int capture_graph(rtps_sph* s) {
+    if (s->gexec) {
+        (void)hipGraphExecDestroy(s->gexec);
+        s->gexec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    RTPS_TRY_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    StageTimer off(s);
+    off.on = false;
+    int rc = launch_step(s, off);
+    hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
+    s->graph_valid = true;
+    return RTPS_OK;
+}
+
+float4* field_f4(rtps_sph* s, int field) {
+    switch (field) {
+        case RTPS_FIELD_POS: return s->pos_u;
+        case RTPS_FIELD_VEL: return s->vel_u;
+        case RTPS_FIELD_VELEVAL: return s->veleval_u;
+        case RTPS_FIELD_POS_S: return s->pos_s;
+        case RTPS_FIELD_VEL_S: return s->vel_s;
+        case RTPS_FIELD_VELEVAL_S: return s->ve_s;
+        case RTPS_FIELD_FORCE: return s->force_s;
+        case RTPS_FIELD_XSPH: return s->xsph_s;
+        default: return nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtps_sph** out) {
+    try {
+        if (!out || n < 0 || (n > 0 && (!keys || !vals))) return set_error(RTPS_ERR_ARG, "rtps_sph_create: bad arguments");
+        *out = nullptr;
+        std::unique_ptr<rtps_sph> s(new rtps_sph());
+        for (int i = 0; i < n; ++i) {
+            if (!keys[i] || !vals[i]) return set_error(RTPS_ERR_ARG, "rtps_sph_create: null key or value");
+            s->settings.SetSetting<std::string>(keys[i], vals[i]);
+        }
+        if (s->settings.GetSettingAs<std::string>("system") != "sph")
+            return set_error(RTPS_ERR_UNKNOWN_SYSTEM,
+                             "No system of type: " + s->settings.GetSettingAs<std::string>("system"));
+        s->max_num = s->settings.GetSettingAs<int>("max_num_particles");
+        if (s->max_num <= 0) return set_error(RTPS_ERR_ARG, "max_num_particles must be positive");
+        derive_params(s->settings, s->sp, s->gp, s->gps);
+        if (s->gp.nb_cells <= 0 || s->gp.nb_cells >= (1 << 30))
+            return set_error(RTPS_ERR_ARG, "grid has no cells or too many cells");
+        s->g = make_grid(s->gp);
+        s->gs = make_grid(s->gps);
+        s->nb_cells = s->gp.nb_cells;
+        s->key_bits = sort_key_bits(s->nb_cells);
+        s->pad_key = (s->key_bits >= 32) ? 0xFFFFFFFFu : ((1u << s->key_bits) - 1u);
+        apply_settings(s.get());
+
+        RTPS_TRY_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        const size_t N = (size_t)s->max_num, C = (size_t)s->nb_cells + 1;
+        RTPS_TRY_HIP(dalloc(&s->pos_u, N));
+        RTPS_TRY_HIP(dalloc(&s->vel_u, N));
+        RTPS_TRY_HIP(dalloc(&s->veleval_u, N));
+        RTPS_TRY_HIP(dalloc(&s->pos_s, N));
+        RTPS_TRY_HIP(dalloc(&s->vel_s, N));
+        RTPS_TRY_HIP(dalloc(&s->ve_s, N));
+        RTPS_TRY_HIP(dalloc(&s->force_s, N));
+        RTPS_TRY_HIP(dalloc(&s->xsph_s, N));
+        RTPS_TRY_HIP(dalloc(&s->keys, N));
+        RTPS_TRY_HIP(dalloc(&s->idx, N));
+        RTPS_TRY_HIP(dalloc(&s->start, C));
+        RTPS_TRY_HIP(dalloc(&s->end, C));
+        RTPS_TRY_HIP(dalloc(&s->d_num, 1));
+        RTPS_TRY_HIP(hipMalloc(&s->sort_tmp, sort_temp_bytes((int)N, s->key_bits)));
+        float4* zf[] = {s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s, s->ve_s, s->force_s, s->xsph_s};
+        for (float4* p : zf) RTPS_TRY_HIP(hipMemsetAsync(p, 0, N * sizeof(float4), s->stream));
+        RTPS_TRY_HIP(hipMemsetAsync(s->keys, 0, N * 4, s->stream));
+        RTPS_TRY_HIP(hipMemsetAsync(s->idx, 0, N * 4, s->stream));
+        RTPS_TRY_HIP(hipMemsetAsync(s->d_num, 0, sizeof(int), s->stream));
+        // cell arrays start at 666 (SPH.cpp:512-517)
+        RTPS_TRY_HIP(launch_fill_u32(s->start, 666u, (long)C, s->stream));
+        RTPS_TRY_HIP(launch_fill_u32(s->end, 666u, (long)C, s->stream));
+        RTPS_TRY_HIP(sort_clear_temp(s->sort_tmp, (int)N, s->stream));
+        s->keys_sorted = s->keys;
+        s->idx_sorted = s->idx;
+        for (auto& e : s->ev) RTPS_TRY_HIP(hipEventCreate(&e));
+        RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+        *out = s.release();
+        return RTPS_OK;
+    } catch (const std::exception& e) {
+        return set_error(RTPS_ERR_ARG, std::string("rtps_sph_create: ") + e.what());
+    }
+}
+
+int rtps_sph_destroy(rtps_sph* s) {
+    delete s;
+    return RTPS_OK;
+}
+
+int rtps_sph_get_num(rtps_sph* s, int* num) {
+    if (!s || !num) return set_error(RTPS_ERR_ARG, "rtps_sph_get_num: null argument");
+    int v = 0;
+    RTPS_TRY_HIP(hipMemcpyAsync(&v, s->d_num, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    *num = v;
+    return RTPS_OK;
+}
+
+int rtps_sph_push_particles(rtps_sph* s, const rtps_float4* pos, const rtps_float4* vel, int n) {
+    if (!s || (!pos && n > 0) || n < 0) return set_error(RTPS_ERR_ARG, "rtps_sph_push_particles: bad arguments");
+    if (n == 0) return RTPS_OK;
+    int num = 0;
+    int rc = rtps_sph_get_num(s, &num);
+    if (rc) return rc;
+    if ((long)num + n > s->max_num)
+        return set_error(RTPS_ERR_CAPACITY, "pushParticles: exceeded max nb (" + std::to_string(s->max_num) +
+                                                ") of particles allowed");
+    RTPS_TRY_HIP(hipMemcpyAsync(s->pos_u + num, pos, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+    if (vel) RTPS_TRY_HIP(hipMemcpyAsync(s->vel_u + num, vel, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+    else RTPS_TRY_HIP(hipMemsetAsync(s->vel_u + num, 0, (size_t)n * 16, s->stream));
+    num += n;
+    RTPS_TRY_HIP(hipMemcpyAsync(s->d_num, &num, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    s->num_host = num;
+    s->settings.SetSetting("num_particles", num);   // SPH.cpp:588-589
+    apply_settings(s);
+    return RTPS_OK;
+}
+
+int rtps_sph_update(rtps_sph* s) {
+    if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_update: null handle");
+    maybe_update_params(s);
+    StageTimer t(s);
+    t.on = false;
+    int rc = launch_front(s, t);
+    if (rc) return rc;
+    RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp, s->gs,
+                              s->force_s, s->xsph_s, s->stream));
+    RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
+                                   s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
+    s->update_pending = true;
+    return RTPS_OK;
+}
+
+int rtps_sph_integrate(rtps_sph* s) {
+    if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_integrate: null handle");
+    if (!s->update_pending) return set_error(RTPS_ERR_STATE, "integrate() needs a preceding update()");
+    RTPS_TRY_HIP(launch_leapfrog(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
+                                 s->xsph_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u, s->stream));
+    RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
+    s->update_pending = false;
+    return RTPS_OK;
+}
+
+int rtps_sph_step(rtps_sph* s, int nsteps) {
+    if (!s || nsteps < 0) return set_error(RTPS_ERR_ARG, "rtps_sph_step: bad arguments");
+    if (s->update_pending) return set_error(RTPS_ERR_STATE, "rtps_sph_step after update() without integrate()");
+    maybe_update_params(s);
+    if (s->num_host == 0) return RTPS_OK;
+    if (s->opt_timers) {
+        for (int k = 0; k < nsteps; ++k) {
+            StageTimer t(s);
+            int rc = launch_step(s, t);
+            if (rc) return rc;
+            rc = collect_timers(s);
+            if (rc) return rc;
+        }
+        return RTPS_OK;
+    }
+    if (s->opt_graph) {
+        if (!s->graph_valid) {
+            int rc = capture_graph(s);
+            if (rc) return rc;
+        }
+        for (int k = 0; k < nsteps; ++k) RTPS_TRY_HIP(hipGraphLaunch(s->gexec, s->stream));
+        return RTPS_OK;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        StageTimer t(s);
+        t.on = false;
+        int rc = launch_step(s, t);
+        if (rc) return rc;
+    }
+    return RTPS_OK;
+}
+
+int rtps_sph_synchronize(rtps_sph* s) {
+    if (!s) return set_error(RTPS_ERR_ARG, "null handle");
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    return RTPS_OK;
+}
+
+int rtps_sph_read(rtps_sph* s, int field, void* host, int count) {
+    if (!s || !host || count < 0) return set_error(RTPS_ERR_ARG, "rtps_sph_read: bad arguments");
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    if (field == RTPS_FIELD_CELL_START || field == RTPS_FIELD_CELL_END) {
+        if (count > s->nb_cells + 1) return set_error(RTPS_ERR_ARG, "rtps_sph_read: count > nb_cells + 1");
+        RTPS_TRY_HIP(hipMemcpy(host, field == RTPS_FIELD_CELL_START ? s->start : s->end, (size_t)count * 4,
+                               hipMemcpyDeviceToHost));
+        return RTPS_OK;
+    }
+    if (count > s->max_num) return set_error(RTPS_ERR_ARG, "rtps_sph_read: count > max_num");
+    if (field == RTPS_FIELD_KEYS || field == RTPS_FIELD_IDX) {
+        RTPS_TRY_HIP(hipMemcpy(host, field == RTPS_FIELD_KEYS ? s->keys_sorted : s->idx_sorted, (size_t)count * 4,
+                               hipMemcpyDeviceToHost));
+        return RTPS_OK;
+    }
+    if (field == RTPS_FIELD_DENSITY) {
+        std::vector<float> tmp((size_t)count * 4);
+        RTPS_TRY_HIP(hipMemcpy(tmp.data(), s->ve_s, (size_t)count * 16, hipMemcpyDeviceToHost));
+        float* d = (float*)host;
+        for (int i = 0; i < count; ++i) d[i] = tmp[4 * (size_t)i + 3];
+        return RTPS_OK;
+    }
+    float4* src = field_f4(s, field);
+    if (!src) return set_error(RTPS_ERR_ARG, "rtps_sph_read: unknown field");
+    RTPS_TRY_HIP(hipMemcpy(host, src, (size_t)count * 16, hipMemcpyDeviceToHost));
+    return RTPS_OK;
+}
+
+int rtps_sph_write(rtps_sph* s, int field, const void* host, int count) {
+    if (!s || !host || count < 0 || count > s->max_num) return set_error(RTPS_ERR_ARG, "rtps_sph_write: bad arguments");
+    if (field != RTPS_FIELD_POS && field != RTPS_FIELD_VEL && field != RTPS_FIELD_VELEVAL)
+        return set_error(RTPS_ERR_ARG, "rtps_sph_write: only POS, VEL and VELEVAL are writable");
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    RTPS_TRY_HIP(hipMemcpy(field_f4(s, field), host, (size_t)count * 16, hipMemcpyHostToDevice));
+    return RTPS_OK;
+}
+
+int rtps_sph_device_ptr(rtps_sph* s, int field, void** dev) {
+    if (!s || !dev) return set_error(RTPS_ERR_ARG, "rtps_sph_device_ptr: null argument");
+    switch (field) {
+        case RTPS_FIELD_KEYS: *dev = s->keys_sorted; return RTPS_OK;
+        case RTPS_FIELD_IDX: *dev = s->idx_sorted; return RTPS_OK;
+        case RTPS_FIELD_CELL_START: *dev = s->start; return RTPS_OK;
+        case RTPS_FIELD_CELL_END: *dev = s->end; return RTPS_OK;
+        case RTPS_FIELD_DENSITY: return set_error(RTPS_ERR_ARG, "density lives in VELEVAL_S .w (packed)");
+        default: break;
+    }
+    float4* p = field_f4(s, field);
+    if (!p) return set_error(RTPS_ERR_ARG, "rtps_sph_device_ptr: unknown field");
+    *dev = p;
+    return RTPS_OK;
+}
+
+int rtps_sph_set_setting(rtps_sph* s, const char* key, const char* val) {
+    if (!s || !key || !val) return set_error(RTPS_ERR_ARG, "rtps_sph_set_setting: null argument");
+    s->settings.SetSetting<std::string>(key, val);
+    return RTPS_OK;
+}
+
+int rtps_sph_get_setting(rtps_sph* s, const char* key, char* buf, int buflen) {
+    if (!s || !key || !buf || buflen <= 0) return set_error(RTPS_ERR_ARG, "rtps_sph_get_setting: bad arguments");
+    if (!s->settings.Exists(key)) return set_error(RTPS_ERR_ARG, std::string("no setting ") + key);
+    std::string v = s->settings.GetSettingAs<std::string>(key);
+    std::strncpy(buf, v.c_str(), (size_t)buflen - 1);
+    buf[buflen - 1] = 0;
+    return RTPS_OK;
+}
+
+int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sp, rtps_grid_params* gp, rtps_grid_params* gps) {
+    if (!s) return set_error(RTPS_ERR_ARG, "null handle");
+    maybe_update_params(s);
+    if (sp) *sp = s->sp;
+    if (gp) *gp = s->gp;
+    if (gps) *gps = s->gps;
+    return RTPS_OK;
+}
+
+int rtps_sph_set_option(rtps_sph* s, int option, int value) {
+    if (!s) return set_error(RTPS_ERR_ARG, "null handle");
+    switch (option) {
+        case RTPS_OPT_COLLISION: s->opt_collision = value ? 1 : 0; break;
+        case RTPS_OPT_GRAPH: s->opt_graph = value ? 1 : 0; break;
+        case RTPS_OPT_FORCE_OUT: s->opt_force_out = value ? 1 : 0; break;
+        case RTPS_OPT_TIMERS: s->opt_timers = value ? 1 : 0; break;
+        default: return set_error(RTPS_ERR_ARG, "unknown option");
+    }
+    s->graph_valid = false;
+    return RTPS_OK;
+}
+
+int rtps_sph_stage_times(rtps_sph* s, float* avg_ms, int* counts, int n) {
+    if (!s || n < 0) return set_error(RTPS_ERR_ARG, "bad arguments");
+    for (int i = 0; i < n && i < ST_N; ++i) {
+        if (avg_ms) avg_ms[i] = s->stage_cnt[i] ? (float)(s->stage_ms[i] / s->stage_cnt[i]) : 0.f;
+        if (counts) counts[i] = (int)s->stage_cnt[i];
+    }
+    return RTPS_OK;
+}
+
+int rtps_sph_reset_timers(rtps_sph* s) {
+    if (!s) return set_error(RTPS_ERR_ARG, "null handle");
+    for (int i = 0; i < ST_N; ++i) {
+        s->stage_ms[i] = 0;
+        s->stage_cnt[i] = 0;
+    }
+    return RTPS_OK;
+}
+
+int rtps_sph_count_pairs(rtps_sph* s, long long* K, long long* M) {
+    if (!s || !K || !M) return set_error(RTPS_ERR_ARG, "rtps_sph_count_pairs: null argument");
+    unsigned long long* d = nullptr;
+    RTPS_TRY_HIP(hipMalloc((void**)&d, 2 * sizeof(unsigned long long)));
+    hipError_t e = hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), s->stream);
+    if (e == hipSuccess)
+        e = launch_count_pairs(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, d, s->stream);
+    unsigned long long h[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_error(e, "rtps_sph_count_pairs");
+    *K = (long long)h[0];
+    *M = (long long)h[1];
+    return RTPS_OK;
+}
+
+int rtps_sph_stream(rtps_sph* s, void** stream) {
+    if (!s || !stream) return set_error(RTPS_ERR_ARG, "null argument");
+    *stream = (void*)s->stream;
+    return RTPS_OK;
+}
+
+}  // extern "C"

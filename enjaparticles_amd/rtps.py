@@ -1,0 +1,235 @@
+"""Python mirror of rtpslib's System/SPH API over the C ABI.
+
+Names follow the reference: ``RTPSSettings`` (rtpslib/RTPSSettings.h:43-107),
+``RTPS.generateSystemInstance`` (rtpslib/RTPS.h:45, RTPS.cpp:37-67), and the
+``SPH`` system (rtpslib/system/SPH.h:56-155, System.h:61-271): ``update``,
+``integrate``, ``pushParticles``, ``addBox``, ``getNum`` and the sorted /
+unsorted buffer getters.  Every call goes to librtps_hip.so; buffers come back
+as numpy copies (``device_ptr`` gives zero-copy device access).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+class RTPSSettings:
+    """String map with typed access (values are kept as strings)."""
+
+    def __init__(self, init: dict | None = None):
+        self._s: dict[str, str] = {}
+        self._changed = False
+        for k, v in (init or {}).items():
+            self.SetSetting(k, v)
+
+    def SetSetting(self, key: str, value) -> None:
+        if isinstance(value, (list, tuple, np.ndarray)):
+            value = " ".join(str(x) for x in np.asarray(value).ravel())
+        self._s[key] = str(value)
+        self._changed = True
+
+    def GetSettingAs(self, kind, key: str, default: str = "0"):
+        raw = self._s.get(key, default)
+        if kind is str:
+            return raw
+        vals = raw.split()
+        if kind in (list, np.ndarray):
+            return [float(x) for x in vals]
+        return kind(vals[0]) if vals else kind(0)
+
+    def Exists(self, key: str) -> bool:
+        return key in self._s
+
+    def hasChanged(self) -> bool:
+        return self._changed
+
+    def updated(self) -> None:
+        self._changed = False
+
+    def items(self):
+        return self._s.items()
+
+    def as_dict(self) -> dict:
+        return dict(self._s)
+
+
+class SPH:
+    """Handle of one SPH system on the GPU (rtps_sph in include/rtps_c.h)."""
+
+    def __init__(self, settings: RTPSSettings | dict):
+        if isinstance(settings, dict):
+            settings = RTPSSettings(settings)
+        self.settings = settings
+        ka, va, n = N.settings_arrays(settings.as_dict())
+        h = C.c_void_p()
+        N.check(N.lib().rtps_sph_create(C.cast(ka, C.c_void_p), C.cast(va, C.c_void_p), n, C.byref(h)))
+        self._h = h
+        self.max_num = int(settings.GetSettingAs(int, "max_num_particles"))
+        sp, gp, gs = self.params()
+        self.nb_cells = gp.nb_cells
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            N.lib().rtps_sph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- System API -------------------------------------------------------
+    def pushParticles(self, pos, vel=None) -> None:
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+        vptr = None
+        if vel is not None:
+            vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 4)
+            assert vel.shape == pos.shape
+            vptr = vel.ctypes.data
+        N.check(N.lib().rtps_sph_push_particles(self._h, pos.ctypes.data, vptr, pos.shape[0]))
+
+    def addBox(self, nn: int, bmin, bmax, spacing: float | None = None) -> int:
+        """System::addBox (System.cpp:331-338) with addRect (IV.cpp:165-202):
+        float-accumulated lattice, z outer / x inner, at most nn particles."""
+        if spacing is None:
+            spacing = float(self.getSetting("spacing"))
+        sp = np.float32(spacing)
+        out = []
+        z = np.float32(bmin[2])
+        while z <= np.float32(bmax[2]) and len(out) < nn:
+            y = np.float32(bmin[1])
+            while y <= np.float32(bmax[1]) and len(out) < nn:
+                x = np.float32(bmin[0])
+                while x <= np.float32(bmax[0]) and len(out) < nn:
+                    out.append((x, y, z, 1.0))
+                    x = np.float32(x + sp)
+                y = np.float32(y + sp)
+            z = np.float32(z + sp)
+        pos = np.array(out, np.float32).reshape(-1, 4)
+        if len(pos):
+            self.pushParticles(pos)
+        return len(pos)
+
+    def update(self) -> None:
+        N.check(N.lib().rtps_sph_update(self._h))
+
+    def interact(self) -> None:
+        """No interaction systems are supported (rigid-body coupling is out of scope)."""
+
+    def integrate(self) -> None:
+        N.check(N.lib().rtps_sph_integrate(self._h))
+
+    def postProcess(self) -> None:
+        """Colour field / surface extraction is out of scope; nothing to do."""
+
+    def step(self, nsteps: int = 1) -> None:
+        """nsteps x (update + integrate), fused kernels, one hipGraph replay per step."""
+        N.check(N.lib().rtps_sph_step(self._h, int(nsteps)))
+
+    def synchronize(self) -> None:
+        N.check(N.lib().rtps_sph_synchronize(self._h))
+
+    def getNum(self) -> int:
+        v = C.c_int()
+        N.check(N.lib().rtps_sph_get_num(self._h, C.byref(v)))
+        return v.value
+
+    def setOption(self, option: int, value: int) -> None:
+        N.check(N.lib().rtps_sph_set_option(self._h, option, int(value)))
+
+    def setSetting(self, key: str, value) -> None:
+        self.settings.SetSetting(key, value)
+        N.check(N.lib().rtps_sph_set_setting(self._h, key.encode(), str(value).encode()))
+
+    def getSetting(self, key: str) -> str:
+        buf = C.create_string_buffer(256)
+        N.check(N.lib().rtps_sph_get_setting(self._h, key.encode(), buf, 256))
+        return buf.value.decode()
+
+    def params(self):
+        sp, gp, gs = N.SPHParams(), N.GridParams(), N.GridParams()
+        N.check(N.lib().rtps_sph_get_params(self._h, C.byref(sp), C.byref(gp), C.byref(gs)))
+        return sp, gp, gs
+
+    # -- buffers ----------------------------------------------------------
+    def read(self, field: str, count: int | None = None) -> np.ndarray:
+        f = N.FIELD[field]
+        if count is None:
+            count = self.nb_cells + 1 if field in ("cell_start", "cell_end") else self.getNum()
+        if field in ("keys", "idx", "cell_start", "cell_end"):
+            out = np.empty(count, np.uint32)
+        elif field == "density":
+            out = np.empty(count, np.float32)
+        else:
+            out = np.empty((count, 4), np.float32)
+        if count:
+            N.check(N.lib().rtps_sph_read(self._h, f, out.ctypes.data, count))
+        return out
+
+    def write(self, field: str, arr) -> None:
+        a = np.ascontiguousarray(arr, np.float32).reshape(-1, 4)
+        N.check(N.lib().rtps_sph_write(self._h, N.FIELD[field], a.ctypes.data, a.shape[0]))
+
+    def device_ptr(self, field: str) -> int:
+        p = C.c_void_p()
+        N.check(N.lib().rtps_sph_device_ptr(self._h, N.FIELD[field], C.byref(p)))
+        return p.value
+
+    def countPairs(self):
+        """(K, M): candidate and accepted (self included) neighbour pairs of the last step."""
+        K, M = C.c_longlong(), C.c_longlong()
+        N.check(N.lib().rtps_sph_count_pairs(self._h, C.byref(K), C.byref(M)))
+        return K.value, M.value
+
+    def stream(self) -> int:
+        p = C.c_void_p()
+        N.check(N.lib().rtps_sph_stream(self._h, C.byref(p)))
+        return p.value or 0
+
+    # reference getter names (System.h:168-177, SPH.h:71)
+    def getPositionBufferUnsorted(self): return self.read("pos")
+    def getVelocityBufferUnsorted(self): return self.read("vel")
+    def getPositionBuffer(self): return self.read("pos_s")
+    def getVelocityBuffer(self): return self.read("vel_s")
+    def getDensityBuffer(self): return self.read("density")
+    def getCellStartBuffer(self): return self.read("cell_start")
+    def getCellEndBuffer(self): return self.read("cell_end")
+
+    # -- timers (SPH::printTimers, SPH.cpp:664-672) -------------------------
+    def stageTimes(self) -> dict:
+        ms = (C.c_float * N.NUM_STAGES)()
+        cnt = (C.c_int * N.NUM_STAGES)()
+        N.check(N.lib().rtps_sph_stage_times(self._h, ms, cnt, N.NUM_STAGES))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(N.STAGE_NAMES)}
+
+    def resetTimers(self) -> None:
+        N.check(N.lib().rtps_sph_reset_timers(self._h))
+
+    def printTimers(self) -> str:
+        lines = ["Timers [All times in ms (1/1000 s)]:"]
+        for name, (avg, cnt) in self.stageTimes().items():
+            lines.append(f"{name:<40s}|  avg: {avg:10.4f}  |  count= {cnt:6d}")
+        text = "\n".join(lines)
+        print(text)
+        return text
+
+
+class RTPS:
+    @staticmethod
+    def generateSystemInstance(settings: RTPSSettings | dict):
+        """Returns an SPH system, or None for an unknown "system" (RTPS.cpp:62-66)."""
+        s = settings if isinstance(settings, RTPSSettings) else RTPSSettings(settings)
+        if s.GetSettingAs(str, "system") != "sph":
+            return None
+        return SPH(s)

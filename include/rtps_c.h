@@ -1,0 +1,228 @@
+/*
+ * rtps_c.h -- C ABI of the MI355X-native SPH step (EnjaParticles / rtpslib drop-in).
+ *
+ * Plain C types only (no HIP, no torch): every entry point takes plain
+ * pointers, sizes and an opaque stream (a hipStream_t passed as void*, NULL =
+ * the default stream).  Device pointers are marked "dev", host pointers "host".
+ * Every function returns an int status (RTPS_OK = 0 or a negative RTPS_ERR_*);
+ * rtps_last_error() gives the message of the last failure on the calling
+ * thread.  No C++ exception crosses this boundary.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference root).
+ */
+#ifndef RTPS_C_H
+#define RTPS_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTPS_OK 0
+#define RTPS_ERR_ARG (-1)           /* bad argument (null, negative size, size limit) */
+#define RTPS_ERR_HIP (-2)           /* a HIP runtime call failed */
+#define RTPS_ERR_CAPACITY (-3)      /* more particles than max_num_particles */
+#define RTPS_ERR_UNKNOWN_SYSTEM (-4) /* settings "system" is not "sph" */
+#define RTPS_ERR_STATE (-5)         /* call not valid in the handle's state */
+
+/* float4 of rtpslib/structs.h:403-581 (16-byte aligned). */
+typedef struct rtps_float4 {
+    float x, y, z, w;
+} __attribute__((aligned(16))) rtps_float4;
+
+/* SPHParams, rtpslib/system/SPHSettings.h:38-136 (device twin
+ * sph/cl_src/cl_sph_structs.h:30-75): 144 bytes, gravity at offset 128. */
+typedef struct rtps_sph_params {
+    float rest_density, mass, rest_distance, smoothing_distance, simulation_scale;
+    float boundary_stiffness, boundary_dampening, boundary_distance, K;
+    float viscosity, velocity_limit, xsph_factor;
+    float friction_coef, restitution_coef, shear, attraction, spring;
+    float EPSILON;
+    float wpoly6_coef, wpoly6_d_coef, wpoly6_dd_coef, wspiky_coef, wspiky_d_coef;
+    float wspiky_dd_coef, wvisc_coef, wvisc_d_coef, wvisc_dd_coef;
+    int num, max_num;
+    int pad_[3];
+    rtps_float4 gravity;
+} __attribute__((aligned(16))) rtps_sph_params;
+
+/* GridParams, rtpslib/domain/Domain.h:144-172 (twin common/cl_src/cl_structs.h:29-43): 128 bytes. */
+typedef struct rtps_grid_params {
+    rtps_float4 grid_size, grid_min, grid_max, bnd_min, bnd_max, grid_res, grid_delta;
+    int nb_cells;
+    int pad_[3];
+} __attribute__((aligned(16))) rtps_grid_params;
+
+/* ------------------------------------------------------------------------ */
+/* Errors and version                                                        */
+const char* rtps_last_error(void);
+int rtps_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* a1. Parameter derivation: SPH::calculate + SPH::updateParams
+ * (rtpslib/system/SPHSettings.cpp:30-201) with the 6-digit settings round
+ * trip (rtpslib/RTPSSettings.h:65-85), and System::setupDomain /
+ * Domain::calculateCells (rtpslib/system/System.cpp:294-329,
+ * rtpslib/domain/Domain.cpp:38-81).  keys/vals: n host C strings. */
+int rtps_derive_params(const char* const* keys, const char* const* vals, int n, rtps_sph_params* sphp,
+                       rtps_grid_params* gp, rtps_grid_params* gp_scaled);
+
+/* ------------------------------------------------------------------------ */
+/* Stage operators (device pointers).  One per reference stage.             */
+
+/* a2. Hash::execute (rtpslib/system/common/Hash.h:49-61, hash.cl:50-157):
+ * keys[i] = linear cell of pos_u[i] (clamped to nb_cells), idx[i] = i for
+ * i < num; entries num <= i < n_pad get the pad key (all ones in
+ * rtps_sort_key_bits(nb_cells) bits) so they sort last. */
+int rtps_hash(int num, int n_pad, const rtps_float4* pos_u /*dev*/, uint32_t* keys /*dev*/, uint32_t* idx /*dev*/,
+              const rtps_grid_params* gp /*host*/, void* stream);
+
+/* Number of low key bits the sort must order for a grid of nb_cells cells
+ * (the sentinel nb_cells and the pad key included). */
+int rtps_sort_key_bits(int nb_cells);
+
+/* a3. Bitonic<T>::Sort (rtpslib/system/common/BitonicSort.h:26-28) as used by
+ * System::bitonic_sort (System.cpp:407-480): ascending key/value sort of n
+ * pairs, in place.  Implemented as a stable LSD radix sort over the low
+ * key_bits bits.  temp: dev scratch of rtps_sort_temp_bytes(n, key_bits). */
+size_t rtps_sort_temp_bytes(int n, int key_bits);
+int rtps_sort_pairs(int n, int key_bits, uint32_t* keys /*dev*/, uint32_t* vals /*dev*/, void* temp /*dev*/,
+                    size_t temp_bytes, void* stream);
+
+/* a4. CellIndices::execute (rtpslib/system/common/CellIndices.h:47-58,
+ * cellindices.cl:12-107): start[0..nb_cells] := 0xFFFFFFFF, then start/end of
+ * every occupied cell from the sorted keys.  end[] of empty cells is left
+ * untouched (as in the reference).  nc: optional dev int receiving
+ * (int)start[nb_cells] (the reference reads it back to the host). */
+int rtps_cellindices(int num, const uint32_t* keys /*dev*/, int nb_cells, uint32_t* start /*dev*/,
+                     uint32_t* end /*dev*/, int* nc /*dev, may be NULL*/, void* stream);
+
+/* a5. Permute::execute (rtpslib/system/common/Permute.h:46-79, permute.cl:10-46):
+ * X_s[i] = X_u[idx[i]] for position, velocity and veleval. */
+int rtps_permute(int num, const uint32_t* idx, const rtps_float4* pos_u, const rtps_float4* vel_u,
+                 const rtps_float4* veleval_u, rtps_float4* pos_s, rtps_float4* vel_s, rtps_float4* veleval_s,
+                 void* stream);
+
+/* a7. Density::execute (rtpslib/system/sph/Density.h:40-53, density.cl:86-126):
+ * density[i] = wpoly6 * m * sum_{|r|<=h} (h^2-r^2)^3 over the 27 cells, self
+ * included.  gps = the scaled grid (SPH.cpp:255-267). */
+int rtps_density(int num, const rtps_float4* pos_s, const uint32_t* start, const uint32_t* end,
+                 const rtps_sph_params* sphp /*host*/, const rtps_grid_params* gps /*host*/, float* density,
+                 void* stream);
+
+/* a8. Force::execute (rtpslib/system/sph/Force.h:34, force.cl:148-186):
+ * pressure (spiky), viscosity (Laplacian) and XSPH sums on veleval;
+ * force[i] is an acceleration, xsph[i] the XSPH velocity correction. */
+int rtps_force(int num, const rtps_float4* pos_s, const float* density, const rtps_float4* veleval_s,
+               const uint32_t* start, const uint32_t* end, const rtps_sph_params* sphp, const rtps_grid_params* gps,
+               rtps_float4* force, rtps_float4* xsph, void* stream);
+
+/* a9. Gravity::execute (rtpslib/system/common/Gravity.cpp:42-79, gravity.cl:9-36):
+ * force[i] += sum_s alpha_s m_s dir/dist^2 for nsrc point sources (host arrays,
+ * at most RTPS_MAX_GRAVITY_SOURCES). */
+#define RTPS_MAX_GRAVITY_SOURCES 16
+int rtps_gravity(int num, int nsrc, const rtps_float4* sources /*host*/, const float* masses /*host*/,
+                 const float* alphas /*host*/, const rtps_float4* pos_s, rtps_float4* force, float scale,
+                 void* stream);
+
+/* a10. CollisionWall::execute (rtpslib/system/sph/Collision_wall.h:35,
+ * collision_wall.cl:31-119): six-wall penalty + friction added to force. */
+int rtps_collision_wall(int num, const rtps_float4* pos_s, const rtps_float4* vel_s, rtps_float4* force,
+                        const rtps_sph_params* sphp, const rtps_grid_params* gps, void* stream);
+
+/* a11. LeapFrog::execute (rtpslib/system/sph/LeapFrog.h:44-63, leapfrog.cl:30-121):
+ * integrates sorted particle i and scatters it to pos_u/vel_u/veleval_u[idx[i]]. */
+int rtps_leapfrog(int num, const rtps_float4* pos_s, const rtps_float4* vel_s, const rtps_float4* force,
+                  const rtps_float4* xsph, const uint32_t* idx, const rtps_sph_params* sphp, float dt,
+                  rtps_float4* pos_u, rtps_float4* vel_u, rtps_float4* veleval_u, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* a12/a13. The SPH system: RTPS::generateSystemInstance (rtpslib/RTPS.h:45,
+ * RTPS.cpp:37-67) + SPH (rtpslib/system/SPH.h:56-155).  A handle owns its
+ * device buffers and one HIP stream; calls on one handle are not thread-safe,
+ * calls on different handles are. */
+typedef struct rtps_sph rtps_sph;
+
+/* Settings as key/value strings (RTPSSettings).  Fails with
+ * RTPS_ERR_UNKNOWN_SYSTEM unless "system" is "sph" (RTPS.cpp:62-66 returns NULL). */
+int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtps_sph** out);
+int rtps_sph_destroy(rtps_sph* s);
+
+/* SPH::pushParticles (SPH.cpp:560-602): append n particles (host arrays of
+ * float4; vel may be NULL for zero).  RTPS_ERR_CAPACITY if num + n > max_num
+ * (the reference prints and returns). */
+int rtps_sph_push_particles(rtps_sph* s, const rtps_float4* pos, const rtps_float4* vel, int n);
+
+/* SPH::update (SPH.cpp:117-125,149-300) and SPH::integrate (SPH.cpp:389-441). */
+int rtps_sph_update(rtps_sph* s);
+int rtps_sph_integrate(rtps_sph* s);
+/* nsteps x (update + integrate), captured in a hipGraph when enabled. */
+int rtps_sph_step(rtps_sph* s, int nsteps);
+/* Wait for all work on the handle's stream. */
+int rtps_sph_synchronize(rtps_sph* s);
+
+/* System::getNum (System.h:82-89); reads the device count (it can shrink
+ * through the out-of-grid deletion path, SPH.cpp:220-250). */
+int rtps_sph_get_num(rtps_sph* s, int* num);
+
+/* Fields for rtps_sph_read / rtps_sph_device_ptr. */
+enum rtps_field {
+    RTPS_FIELD_POS = 0,        /* float4, unsorted (getPositionBufferUnsorted) */
+    RTPS_FIELD_VEL = 1,        /* float4, unsorted */
+    RTPS_FIELD_VELEVAL = 2,    /* float4, unsorted */
+    RTPS_FIELD_POS_S = 3,      /* float4, sorted (getPositionBuffer) */
+    RTPS_FIELD_VEL_S = 4,      /* float4, sorted */
+    RTPS_FIELD_VELEVAL_S = 5,  /* float4, sorted; .w holds the density after a step */
+    RTPS_FIELD_DENSITY = 6,    /* float, sorted (getDensityBuffer) */
+    RTPS_FIELD_FORCE = 7,      /* float4, sorted (when force output is enabled) */
+    RTPS_FIELD_XSPH = 8,       /* float4, sorted (when force output is enabled) */
+    RTPS_FIELD_KEYS = 9,       /* uint32, sorted cell keys */
+    RTPS_FIELD_IDX = 10,       /* uint32, sorted -> original index */
+    RTPS_FIELD_CELL_START = 11, /* uint32[nb_cells+1] (getCellStartBuffer) */
+    RTPS_FIELD_CELL_END = 12   /* uint32[nb_cells+1] (getCellEndBuffer) */
+};
+/* Blocking copy of the first `count` elements of a field into host memory. */
+int rtps_sph_read(rtps_sph* s, int field, void* host, int count);
+/* Device pointer of a field (zero-copy access on the handle's stream). */
+int rtps_sph_device_ptr(rtps_sph* s, int field, void** dev);
+/* Overwrite the first `count` elements of an unsorted float4 field (POS, VEL, VELEVAL). */
+int rtps_sph_write(rtps_sph* s, int field, const void* host, int count);
+
+/* RTPSSettings::SetSetting / GetSettingAs<string> on the handle's settings;
+ * a change is applied at the next update (SPH.cpp:154). */
+int rtps_sph_set_setting(rtps_sph* s, const char* key, const char* val);
+int rtps_sph_get_setting(rtps_sph* s, const char* key, char* buf, int buflen);
+int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp, rtps_grid_params* gps);
+
+/* Options (not in the reference):
+ *   RTPS_OPT_COLLISION   1 = wall collision on (default; the shipped SPH.cpp:297 has it off)
+ *   RTPS_OPT_GRAPH       1 = capture rtps_sph_step's inner loop in a hipGraph (default 1)
+ *   RTPS_OPT_FORCE_OUT   1 = also store force/xsph per step (default 0)
+ *   RTPS_OPT_TIMERS      1 = per-stage hipEvent timers (forces eager launches) */
+#define RTPS_OPT_COLLISION 0
+#define RTPS_OPT_GRAPH 1
+#define RTPS_OPT_FORCE_OUT 2
+#define RTPS_OPT_TIMERS 3
+int rtps_sph_set_option(rtps_sph* s, int option, int value);
+
+/* Per-stage GPU times (SPH::printTimers, SPH.cpp:664-672): average ms per
+ * call since the last reset, in the order hash, sort, cellindices, permute,
+ * density, force (force + gravity + collision + leapfrog when fused).  n <= 8. */
+#define RTPS_NUM_STAGES 6
+int rtps_sph_stage_times(rtps_sph* s, float* avg_ms, int* counts, int n);
+int rtps_sph_reset_timers(rtps_sph* s);
+
+/* Candidate (K) and accepted (M, self included) pair totals of the 27-cell
+ * neighbour loop on the last step's sorted state (the FLOP model's inputs). */
+int rtps_sph_count_pairs(rtps_sph* s, long long* K, long long* M);
+
+/* The handle's stream (hipStream_t as void*). */
+int rtps_sph_stream(rtps_sph* s, void** stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTPS_C_H */

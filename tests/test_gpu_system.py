@@ -1,0 +1,151 @@
+"""Whole-step parity of the SPH system (rtps_sph_* C ABI) with the CPU oracle.
+
+Bars (written per test): over the first 10 steps of the 16k dam-break the FP32
+positions, velocities and densities stay within 1e-4 relative of the FP32
+oracle; the FP32 HIP run stays within a stated tolerance of the FP64 oracle;
+the fused/graph path equals the update()+integrate() path; the out-of-grid
+deletion path shrinks the count exactly as the oracle does.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from enjaparticles_amd import scenes
+from enjaparticles_amd.rtps import RTPS, SPH
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(sc, collision=True, graph=True):
+    s = RTPS.generateSystemInstance(sc.settings)
+    s.setOption(0, int(collision))
+    s.setOption(1, int(graph))
+    s.pushParticles(sc.pos, sc.vel)
+    return s
+
+
+def _oracle(sc, dtype=np.float32):
+    sp, gp, gs = O.derive_params(sc.settings)
+    return O.State(sc.pos, sc.vel, sp, gp, gs, dtype=dtype)
+
+
+def _rel_max(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+def test_unknown_system_returns_none(cuda):
+    assert RTPS.generateSystemInstance({"system": "flock"}) is None
+
+
+def test_ten_steps_match_fp32_oracle(cuda):
+    sc = scenes.dam16k()
+    s = _system(sc)
+    st = _oracle(sc)
+    for k in range(10):
+        s.step(1)
+        st.step(sc.dt, O.FLAG_COLLISION)
+        pos, vel = s.read("pos"), s.read("vel")
+        dens = s.read("density")
+        # per-element 1e-4 relative on positions; velocities and densities
+        # max-norm 1e-4 relative of their fields
+        np.testing.assert_allclose(pos[:, :3], st.pos_u[:, :3], rtol=1e-4, atol=0)
+        assert _rel_max(vel[:, :3], st.vel_u[:, :3]) <= 1e-4, k
+        np.testing.assert_allclose(dens, st.density[: st.num], rtol=1e-4)
+    assert s.getNum() == st.num == sc.n
+    s.close()
+
+
+def test_hundred_steps_vs_fp64_oracle(cuda):
+    """FP32 HIP vs FP64 oracle after 100 steps: positions within 1e-3 of the
+    box size per element (chaotic pair sums amplify the FP32 rounding)."""
+    sc = scenes.dam16k()
+    s = _system(sc)
+    st = _oracle(sc, np.float64)
+    s.step(100)
+    for _ in range(100):
+        st.step(sc.dt, O.FLAG_COLLISION)
+    pos = s.read("pos")
+    err = np.abs(pos[:, :3] - st.pos_u[:, :3]).max()
+    assert err <= 1e-3, err
+    s.close()
+
+
+def test_fused_graph_equals_update_integrate(cuda):
+    sc = scenes.dam_custom(24, 24, 12, seed=77)
+    a = _system(sc, graph=True)
+    b = _system(sc, graph=False)
+    c = _system(sc)
+    a.step(5)
+    b.step(5)
+    for _ in range(5):
+        c.update()
+        c.interact()
+        c.integrate()
+    pa, pb, pc = a.read("pos"), b.read("pos"), c.read("pos")
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_allclose(pc, pa, rtol=1e-6, atol=1e-7)
+    for s in (a, b, c):
+        s.close()
+
+
+def test_deletion_path_matches_oracle(cuda):
+    """Particles pushed out of the grid are deleted (SPH.cpp:220-250) on the
+    device, with the same survivors as the oracle."""
+    sc = scenes.dam16k()
+    pos = sc.pos.copy()
+    pos[5::211, 0] = 7.0
+    sc2 = scenes.Scene("del", sc.settings, pos, sc.vel, 3, sc.dt)
+    s = _system(sc2)
+    st = _oracle(sc2)
+    for _ in range(3):
+        s.step(1)
+        st.step(sc.dt, O.FLAG_COLLISION)
+    n = s.getNum()
+    assert n == st.num == sc.n - len(pos[5::211])
+    np.testing.assert_allclose(s.read("pos")[:, :3], st.pos_u[:n, :3], rtol=1e-4)
+    s.close()
+
+
+def test_shipped_order_without_walls(cuda):
+    """Collision off (the shipped SPH.cpp:297): still matches the oracle."""
+    sc = scenes.dam16k()
+    s = _system(sc, collision=False)
+    st = _oracle(sc)
+    for _ in range(5):
+        s.step(1)
+        st.step(sc.dt, 0)
+    np.testing.assert_allclose(s.read("pos")[:, :3], st.pos_u[:, :3], rtol=1e-4)
+    s.close()
+
+
+def test_push_capacity_and_settings_change(cuda):
+    sc = scenes.dam_custom(16, 16, 8, seed=3)
+    s = RTPS.generateSystemInstance(sc.settings)
+    s.pushParticles(sc.pos[:1000])
+    s.pushParticles(sc.pos[1000:], sc.vel[1000:])
+    assert s.getNum() == sc.n
+    from enjaparticles_amd._native import RTPSError
+    with pytest.raises(RTPSError):
+        s.pushParticles(sc.pos[:1])
+    s.step(2)
+    s.setSetting("viscosity", "0.05")
+    st = _oracle(sc)
+    st.step(sc.dt, 1)
+    st.step(sc.dt, 1)
+    st.sphp, _, _ = O.derive_params({**sc.settings, "viscosity": "0.05"})
+    s.step(2)
+    st.step(sc.dt, 1)
+    st.step(sc.dt, 1)
+    np.testing.assert_allclose(s.read("pos")[:, :3], st.pos_u[:, :3], rtol=1e-4)
+    s.close()
+
+
+def test_addbox_matches_addrect(cuda):
+    sc = scenes.dam16k()
+    s = SPH(sc.settings)
+    n = s.addBox(1000, (0.1, 0.1, 0.1), (0.3, 0.3, 0.3), spacing=0.02)
+    p = s.read("pos")
+    assert n == len(p) == 1000
+    assert np.all(p[:, 3] == 1.0)
+    assert p[1, 0] == np.float32(np.float32(0.1) + np.float32(0.02))
+    s.close()
