@@ -60,6 +60,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--profile", action="store_true", help="timed loop only (for rocprofv3 runs)")
+    ap.add_argument("--force-variant", type=int, default=1, help="1 = compacted neighbour lists, 0 = direct")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -78,6 +79,7 @@ def main():
     s = RTPS.generateSystemInstance(sc.settings)
     s.pushParticles(sc.pos, sc.vel)
     s.setOption(N.OPT_GRAPH, 1)
+    s.setOption(N.OPT_FORCE_VARIANT, args.force_variant)
     s.step(args.warmup)
     s.synchronize()
     print(f"# warmup {args.warmup} steps done", file=sys.stderr, flush=True)
@@ -100,6 +102,7 @@ def main():
     s.step(args.timer_steps)
     s.synchronize()
     stages = s.stageTimes()
+    print("# stages (ms): " + ", ".join(f"{k} {v[0]:.4f}" for k, v in stages.items()), file=sys.stderr, flush=True)
     K2, M2 = s.countPairs()
     s.setOption(N.OPT_TIMERS, 0)
     Kp, Mp = 0.5 * (K + K2) / n, 0.5 * (M + M2) / n

@@ -23,7 +23,7 @@ FIELD = {
     "pos": 0, "vel": 1, "veleval": 2, "pos_s": 3, "vel_s": 4, "veleval_s": 5, "density": 6,
     "force": 7, "xsph": 8, "keys": 9, "idx": 10, "cell_start": 11, "cell_end": 12,
 }
-OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS = 0, 1, 2, 3
+OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS, OPT_FORCE_VARIANT = 0, 1, 2, 3, 4
 NUM_STAGES = 6
 STAGE_NAMES = ("hash", "sort", "cellindices", "permute", "density", "force")
 MAX_GRAVITY_SOURCES = 16
@@ -113,6 +113,14 @@ def lib():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
                 "or `make -C enjaparticles_amd/csrc` (there is no CPU fallback)")
+        # torch ships its own HIP runtime.  If this library were loaded first it
+        # would bind the system runtime and a later `import torch` would bring a
+        # second runtime into the process, which then cannot open the device.
+        # Loading torch first makes both share torch's runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -27,6 +27,7 @@ void read_sph_params(const RTPSSettings& s, rtps_sph_params& sp);
 int sort_key_bits(int nb_cells);
 
 // ---- kernels (sph_kernels.hip) ----
+extern int g_force_variant;   // 0 = direct branch per candidate, 1 = compacted neighbour lists
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
                        uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, hipStream_t st);
 hipError_t launch_cellindices(int num, const int* d_num, const uint32_t* keys, int nb_cells, uint32_t* start,

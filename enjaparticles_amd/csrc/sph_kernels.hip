@@ -129,7 +129,17 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
 // epilogue goes on to the point sources (gravity.cl), the walls
 // (collision_wall.cl) and the leapfrog step (leapfrog.cl), scattering to the
 // unsorted arrays; force/xsph are then stored only if force_out != nullptr.
-template <bool SCALED, bool PACKED, bool FUSED>
+//
+// COMPACT: only ~15% of the candidates pass the r <= h test, so a branch on
+// it keeps the whole wave in the pair math for almost every candidate.  The
+// compact variant first tests candidates in chunks of kChunk and appends the
+// accepted j to a per-thread list in LDS (kNbrCap entries, entry k of thread t
+// at word k * kBlock + t: conflict-free), then drains the lists with one pair
+// per lane per iteration.  Pairs are visited in the same ascending-j order.
+constexpr int kNbrCap = 32;
+constexpr int kChunk = 16;
+
+template <bool SCALED, bool PACKED, bool FUSED, bool COMPACT>
 __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __restrict__ d_num,
                                                   const float4* __restrict__ pos_s,
                                                   const float* __restrict__ dens_in,
@@ -159,23 +169,67 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
     const float Pi = K * (di - rho0);
     const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
     ForceAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int z = c.z - 1; z <= c.z + 1; ++z) {
-        for (int y = c.y - 1; y <= c.y + 1; ++y) {
-            uint32_t a, b;
-            row_range(c.x, y, z, g, start, end, a, b);
-            for (uint32_t j = a; j < b; ++j) {
-                float4 pj = pos_s[j];
-                if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
-                float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                float r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 <= h2 && j != (uint32_t)i) {
-                    float4 vj = ve_in[j];
-                    float dj = PACKED ? vj.w : dens_in[j];
-                    force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, dj);
+
+    auto pair = [&](uint32_t j, float rx, float ry, float rz, float r2) {
+        float4 vj = ve_in[j];
+        float dj = PACKED ? vj.w : dens_in[j];
+        force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, dj);
+    };
+    auto sep = [&](uint32_t j, float& rx, float& ry, float& rz) {
+        float4 pj = pos_s[j];
+        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+        rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
+        return rx * rx + ry * ry + rz * rz;
+    };
+
+    if constexpr (!COMPACT) {
+        for (int z = c.z - 1; z <= c.z + 1; ++z) {
+            for (int y = c.y - 1; y <= c.y + 1; ++y) {
+                uint32_t a, b;
+                row_range(c.x, y, z, g, start, end, a, b);
+                for (uint32_t j = a; j < b; ++j) {
+                    float rx, ry, rz;
+                    float r2 = sep(j, rx, ry, rz);
+                    if (r2 <= h2 && j != (uint32_t)i) pair(j, rx, ry, rz, r2);
                 }
             }
         }
+    } else {
+        __shared__ uint32_t nbr[kNbrCap * kBlock];
+        uint32_t* mine = nbr + threadIdx.x;
+        int cnt = 0;
+        auto drain = [&]() {
+            for (int k = 0; __ballot(k < cnt) != 0ull; ++k) {
+                if (k < cnt) {
+                    const uint32_t j = mine[k * kBlock];
+                    float rx, ry, rz;
+                    float r2 = sep(j, rx, ry, rz);
+                    pair(j, rx, ry, rz, r2);
+                }
+            }
+            cnt = 0;
+        };
+        for (int z = c.z - 1; z <= c.z + 1; ++z) {
+            for (int y = c.y - 1; y <= c.y + 1; ++y) {
+                uint32_t a, b;
+                row_range(c.x, y, z, g, start, end, a, b);
+                for (uint32_t c0 = a; __ballot(c0 < b) != 0ull; c0 += kChunk) {
+                    if (__ballot(cnt > kNbrCap - kChunk) != 0ull) drain();
+                    const uint32_t c1 = min(c0 + (uint32_t)kChunk, b);
+                    for (uint32_t j = c0; j < c1; ++j) {
+                        float rx, ry, rz;
+                        float r2 = sep(j, rx, ry, rz);
+                        if (r2 <= h2 && j != (uint32_t)i) {
+                            mine[cnt * kBlock] = j;
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+        }
+        drain();
     }
+
     float4 F, X;
     force_epilogue(acc, di, sp, F, X);
     if (force_out) { force_out[i] = F; xsph_out[i] = X; }
@@ -293,6 +347,9 @@ __global__ void k_commit_num(int* __restrict__ d_num, const uint32_t* __restrict
 // ---------------------------------------------------------------------------
 // Launchers shared by the C ABI and the system.
 
+// Force-loop variant: 0 = one branch per candidate, 1 = compacted LDS lists.
+int g_force_variant = 1;
+
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
                        uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
@@ -352,8 +409,14 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
     GravitySources none{};
     rtps_float4 z{0, 0, 0, 0};
 #define RTPS_FORCE(S, P)                                                                                            \
-    k_force<S, P, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, force, xsph,       \
-                                                nullptr, nullptr, none, 0, z, z, 0.f, nullptr, nullptr, nullptr)
+    if (g_force_variant)                                                                                            \
+        k_force<S, P, false, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, force,   \
+                                                          xsph, nullptr, nullptr, none, 0, z, z, 0.f, nullptr,     \
+                                                          nullptr, nullptr);                                       \
+    else                                                                                                            \
+        k_force<S, P, false, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, force,  \
+                                                           xsph, nullptr, nullptr, none, 0, z, z, 0.f, nullptr,    \
+                                                           nullptr, nullptr)
     if (packed) { if (scaled) RTPS_FORCE(true, true); else RTPS_FORCE(false, true); }
     else { if (scaled) RTPS_FORCE(true, false); else RTPS_FORCE(false, false); }
 #undef RTPS_FORCE
@@ -369,14 +432,13 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
     const int nb = blocks_for(num);
-    if (scaled)
-        k_force<true, true, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g,
-                                                         force, xsph, vel_s, idx, gsrc, collision, gps.bnd_min,
-                                                         gps.bnd_max, dt, pos_u, vel_u, veleval_u);
-    else
-        k_force<false, true, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g,
-                                                          force, xsph, vel_s, idx, gsrc, collision, gps.bnd_min,
-                                                          gps.bnd_max, dt, pos_u, vel_u, veleval_u);
+#define RTPS_FFUSED(S, C)                                                                                           \
+    k_force<S, true, true, C><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, force, \
+                                                     xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt, \
+                                                     pos_u, vel_u, veleval_u)
+    if (scaled) { if (g_force_variant) RTPS_FFUSED(true, true); else RTPS_FFUSED(true, false); }
+    else { if (g_force_variant) RTPS_FFUSED(false, true); else RTPS_FFUSED(false, false); }
+#undef RTPS_FFUSED
     return hipGetLastError();
 }
 

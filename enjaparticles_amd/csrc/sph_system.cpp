@@ -445,6 +445,7 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_GRAPH: s->opt_graph = value ? 1 : 0; break;
         case RTPS_OPT_FORCE_OUT: s->opt_force_out = value ? 1 : 0; break;
         case RTPS_OPT_TIMERS: s->opt_timers = value ? 1 : 0; break;
+        case RTPS_OPT_FORCE_VARIANT: g_force_variant = value ? 1 : 0; break;
         default: return set_error(RTPS_ERR_ARG, "unknown option");
     }
     s->graph_valid = false;

@@ -200,11 +200,14 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *   RTPS_OPT_COLLISION   1 = wall collision on (default; the shipped SPH.cpp:297 has it off)
  *   RTPS_OPT_GRAPH       1 = capture rtps_sph_step's inner loop in a hipGraph (default 1)
  *   RTPS_OPT_FORCE_OUT   1 = also store force/xsph per step (default 0)
- *   RTPS_OPT_TIMERS      1 = per-stage hipEvent timers (forces eager launches) */
+ *   RTPS_OPT_TIMERS      1 = per-stage hipEvent timers (forces eager launches)
+ *   RTPS_OPT_FORCE_VARIANT  force-loop kernel, process-wide: 1 = compacted
+ *                        neighbour lists in LDS (default), 0 = one branch per candidate */
 #define RTPS_OPT_COLLISION 0
 #define RTPS_OPT_GRAPH 1
 #define RTPS_OPT_FORCE_OUT 2
 #define RTPS_OPT_TIMERS 3
+#define RTPS_OPT_FORCE_VARIANT 4
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Per-stage GPU times (SPH::printTimers, SPH.cpp:664-672): average ms per

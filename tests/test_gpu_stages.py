@@ -128,7 +128,7 @@ def test_cellindices_bitexact(cuda, case):
     sc, sp, gp, gs = _dam()
     pos = sc.pos.copy()
     if case == "with_out_of_grid":
-        pos[::97, 0] = 5.0          # far outside: key = nb_cells (deletion sentinel)
+        pos[::97, 2] = 50.0         # far above: linear index > nb_cells -> key nb_cells (deletion sentinel)
     elif case == "single_cell":
         pos[:, :3] = 0.5
     ok, oi = O.hash(pos, gp)

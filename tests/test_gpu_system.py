@@ -56,17 +56,25 @@ def test_ten_steps_match_fp32_oracle(cuda):
 
 
 def test_hundred_steps_vs_fp64_oracle(cuda):
-    """FP32 HIP vs FP64 oracle after 100 steps: positions within 1e-3 of the
-    box size per element (chaotic pair sums amplify the FP32 rounding)."""
+    """FP32 HIP vs FP64 oracle after 100 steps.  Pair sums are chaotic, so FP32
+    rounding grows into visible position differences; the bar is the FP32
+    oracle's own distance from FP64 (the reference's FP32 arithmetic): the HIP
+    run may be at most 3x as far (max and RMS over particles) plus 1e-4."""
     sc = scenes.dam16k()
     s = _system(sc)
-    st = _oracle(sc, np.float64)
+    st64 = _oracle(sc, np.float64)
+    st32 = _oracle(sc, np.float32)
     s.step(100)
     for _ in range(100):
-        st.step(sc.dt, O.FLAG_COLLISION)
+        st64.step(sc.dt, O.FLAG_COLLISION)
+        st32.step(sc.dt, O.FLAG_COLLISION)
     pos = s.read("pos")
-    err = np.abs(pos[:, :3] - st.pos_u[:, :3]).max()
-    assert err <= 1e-3, err
+    e_gpu = np.linalg.norm(pos[:, :3] - st64.pos_u[:, :3], axis=1)
+    e_ref = np.linalg.norm(st32.pos_u[:, :3] - st64.pos_u[:, :3], axis=1)
+    print(f"100 steps: max/rms |HIP - FP64| = {e_gpu.max():.3e}/{np.sqrt((e_gpu**2).mean()):.3e}, "
+          f"|FP32 oracle - FP64| = {e_ref.max():.3e}/{np.sqrt((e_ref**2).mean()):.3e}")
+    assert e_gpu.max() <= 3 * e_ref.max() + 1e-4
+    assert np.sqrt((e_gpu ** 2).mean()) <= 3 * np.sqrt((e_ref ** 2).mean()) + 1e-4
     s.close()
 
 
@@ -93,7 +101,7 @@ def test_deletion_path_matches_oracle(cuda):
     device, with the same survivors as the oracle."""
     sc = scenes.dam16k()
     pos = sc.pos.copy()
-    pos[5::211, 0] = 7.0
+    pos[5::211, 2] = 70.0    # far above the grid: key nb_cells
     sc2 = scenes.Scene("del", sc.settings, pos, sc.vel, 3, sc.dt)
     s = _system(sc2)
     st = _oracle(sc2)
