@@ -51,6 +51,51 @@ def cpu_baseline(scene, steps: int = 2):
     }
 
 
+def batched_leg(steps: int, num_scenes: int = 256, seed: int = 1003):
+    """Batched scenes (256 scenes, counts log-uniform in [4096, 65536]) stepped
+    with one launch per stage, against the same scenes stepped one after
+    another as plain systems (one graph replay per scene-step)."""
+    import torch
+    from enjaparticles_amd import scenes
+    from enjaparticles_amd.rtps import SPH, BatchedSPH
+    scs = scenes.batched(num_scenes, 4096, 65536, seed=seed)
+    total = sum(sc.n for sc in scs)
+    b = BatchedSPH([sc.settings for sc in scs])
+    for i, sc in enumerate(scs):
+        b.pushParticles(i, sc.pos, sc.vel)
+    b.step(2)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    tb = time.perf_counter() - t0
+    b.setOption(3, 1)
+    b.resetTimers()
+    b.step(3)
+    stages = {k: round(v[0], 4) for k, v in b.stageTimes().items()}
+    b.close()
+    plains = []
+    for sc in scs:
+        p = SPH(sc.settings)
+        p.pushParticles(sc.pos, sc.vel)
+        p.step(2)
+        plains.append(p)
+    torch.cuda.synchronize()
+    for p in plains:
+        p.synchronize()
+    t0 = time.perf_counter()
+    for p in plains:
+        p.step(steps)
+        p.synchronize()
+    ts = time.perf_counter() - t0
+    for p in plains:
+        p.close()
+    vb, vs = total * steps / tb, total * steps / ts
+    return {"scenes": num_scenes, "particles": total, "steps": steps, "value": vb,
+            "unit": "particle-updates/s", "sequential_plain_value": vs, "ratio": vb / vs,
+            "ms_per_step": 1e3 * tb / steps, "stages_ms": stages}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,7 +105,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--profile", action="store_true", help="timed loop only (for rocprofv3 runs)")
-    ap.add_argument("--force-variant", type=int, default=1, help="1 = compacted neighbour lists, 0 = direct")
+    ap.add_argument("--force-variant", type=int, default=0, help="1 = compacted neighbour lists, 0 = direct")
+    ap.add_argument("--batched-steps", type=int, default=30, help="steps of the batched leg (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -93,19 +139,26 @@ def main():
     n = s.getNum()
     print(f"# timed {args.steps} steps: {elapsed:.3f} s", file=sys.stderr, flush=True)
     if args.profile:
+        # a plain 256 MiB device copy of known size, to calibrate the HBM
+        # byte counters (FETCH_SIZE / WRITE_SIZE) of the same profiling run
+        src = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# reference copy: 268435456 bytes read + written", file=sys.stderr, flush=True)
         return
 
     # per-stage times with HIP events on the handle's stream (eager launches)
-    K, M = s.countPairs()
+    K, M, Kt = s.countPairs()
     s.setOption(N.OPT_TIMERS, 1)
     s.resetTimers()
     s.step(args.timer_steps)
     s.synchronize()
     stages = s.stageTimes()
     print("# stages (ms): " + ", ".join(f"{k} {v[0]:.4f}" for k, v in stages.items()), file=sys.stderr, flush=True)
-    K2, M2 = s.countPairs()
+    K2, M2, Kt2 = s.countPairs()
     s.setOption(N.OPT_TIMERS, 0)
-    Kp, Mp = 0.5 * (K + K2) / n, 0.5 * (M + M2) / n
+    Kp, Mp, Ktp = 0.5 * (K + K2) / n, 0.5 * (M + M2) / n, 0.5 * (Kt + Kt2) / n
     # force FLOP model: 8 per candidate pair + 35 per accepted pair (self excluded)
     force_flop = n * (8.0 * Kp + 35.0 * (Mp - 1.0))
     force_ms = stages["force"][0]
@@ -130,7 +183,7 @@ def main():
         "config": {"workload": "plain SPH dam break, 1,048,576 particles (128x128x64), h=0.04, dt=0.003, "
                                "walls on, hipGraph per step", "particles": n, "steps_per_s": 1e3 / ms_per_step},
         "stages_ms": {k: round(v[0], 5) for k, v in stages.items()},
-        "pairs_per_particle": {"candidates": Kp, "accepted_incl_self": Mp},
+        "pairs_per_particle": {"candidates_27cell": Kp, "accepted_incl_self": Mp, "candidates_tested": Ktp},
         "roofline": {
             "bound": "valu",
             "kernel": "k_force (fused force + walls + leapfrog)",
@@ -148,6 +201,11 @@ def main():
         },
     }
     s.close()
+    if args.batched_steps > 0:
+        print("# batched leg ...", file=sys.stderr, flush=True)
+        out["batched"] = batched_leg(args.batched_steps)
+        print(f"# batched: {out['batched']['value']:.3e} p-u/s, x{out['batched']['ratio']:.2f} vs sequential plain",
+              file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)

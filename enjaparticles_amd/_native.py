@@ -92,7 +92,18 @@ SIGNATURES = {
     "rtps_sph_stage_times": (_i, [P, P, P, _i]),
     "rtps_sph_reset_timers": (_i, [P]),
     "rtps_sph_stream": (_i, [P, P]),
-    "rtps_sph_count_pairs": (_i, [P, P, P]),
+    "rtps_sph_count_pairs": (_i, [P, P, P, P]),
+    "rtps_batch_create": (_i, [_i, P, P, P, P]),
+    "rtps_batch_destroy": (_i, [P]),
+    "rtps_batch_num_scenes": (_i, [P, P, P]),
+    "rtps_batch_push_particles": (_i, [P, _i, P, P, _i]),
+    "rtps_batch_step": (_i, [P, _i]),
+    "rtps_batch_synchronize": (_i, [P]),
+    "rtps_batch_get_num": (_i, [P, _i, P]),
+    "rtps_batch_read": (_i, [P, _i, _i, P, _i]),
+    "rtps_batch_set_option": (_i, [P, _i, _i]),
+    "rtps_batch_stage_times": (_i, [P, P, P, _i]),
+    "rtps_batch_reset_timers": (_i, [P]),
 }
 
 

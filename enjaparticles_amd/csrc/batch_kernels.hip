@@ -1,0 +1,223 @@
+// Batched scenes: many independent SPH systems stepped together, one launch
+// per stage for all of them (no reference counterpart; each scene follows the
+// single-system step of sph_kernels.hip exactly).
+//
+// Layout: scene s owns the slots [off_s, off_s + cap_s) of every particle
+// array, its own parameters (SceneDev) and its own cell table at cell_base_s.
+// Keys are segmented, (s << 24) | (cell << 4) | xq, so one sort over all
+// slots keeps every scene inside its own slot range; pads (unused slots) get
+// (s << 24) | 0xFFFFFF and sort to the end of their scene.  Density and force
+// run as persistent kernels over a queue of (scene, 256-slot tile) items: a
+// block takes the next item with one atomic, so uneven scene sizes balance
+// themselves, and the scene's parameters are wave-uniform (scalar loads).
+
+#include <hip/hip_runtime.h>
+
+#include "batch_types.h"
+#include "rtps_internal.h"
+#include "sph_device.h"
+
+namespace rtps {
+
+constexpr int kBBlock = 256;
+constexpr uint32_t kLocalMask = (1u << kSceneShift) - 1u;
+
+static inline int bblocks(long n) { return (int)((n + kBBlock - 1) / kBBlock); }
+
+// Count of scene s after its deletion path (SPH.cpp:220-250): the first
+// out-of-grid particle's sorted index, when it lies inside [0, num].
+__device__ __forceinline__ bool scene_num(const SceneDev& S, int num, const uint32_t* __restrict__ start, int& out) {
+    const uint32_t nc = start[S.cell_base + S.g.nb_cells];
+    const int local = nc == kEmpty ? -1 : (int)nc - S.off;
+    const bool del = local >= 0 && local <= num;
+    out = del ? local : num;
+    return del;
+}
+
+__global__ __launch_bounds__(kBBlock) void k_hash_b(int T, const uint16_t* __restrict__ sid,
+                                                    const SceneDev* __restrict__ scenes, const int* __restrict__ d_num,
+                                                    const float4* __restrict__ pos_u, uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ idx, uint32_t* __restrict__ start_clear,
+                                                    int n_clear, int* __restrict__ queues) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_clear) start_clear[i] = kEmpty;
+    if (i == 0) {
+        queues[0] = 0;
+        queues[1] = 0;
+    }
+    if (i >= T) return;
+    const int s = sid[i];
+    const SceneDev& S = scenes[s];
+    uint32_t key = ((uint32_t)s << kSceneShift) | kLocalMask;
+    if (i - S.off < d_num[s]) {
+        const Grid& g = S.g;
+        const float4 p = pos_u[i];
+        const float ux = (p.x - g.min.x) * g.delta.x;
+        const int cx = sat_f2i(ux);
+        const int cy = sat_f2i((p.y - g.min.y) * g.delta.y);
+        const int cz = sat_f2i((p.z - g.min.z) * g.delta.z);
+        int h = grid_hash(cx, cy, cz, g);
+        const bool out = h > g.nb_cells || h < 0;
+        h = out ? g.nb_cells : h;
+        int xq = sat_f2i((ux - (float)cx) * (float)kSubCells);
+        xq = out || h == g.nb_cells ? 0 : min(max(xq, 0), kSubCells - 1);
+        key = ((uint32_t)s << kSceneShift) | ((uint32_t)h << kSubBits) | (uint32_t)xq;
+    }
+    keys[i] = key;
+    idx[i] = (uint32_t)i;
+}
+
+__device__ __forceinline__ bool cell_of(uint32_t key, const SceneDev* __restrict__ scenes, uint32_t& gcell) {
+    const int s = (int)(key >> kSceneShift);
+    const uint32_t cell = (key & kLocalMask) >> kSubBits;
+    const SceneDev& S = scenes[s];
+    gcell = (uint32_t)S.cell_base + cell;
+    return cell <= (uint32_t)S.g.nb_cells;
+}
+
+// Cell start/end on the segmented keys; start/end hold global sorted indices.
+__global__ __launch_bounds__(kBBlock) void k_cellindices_b(int T, const uint32_t* __restrict__ keys,
+                                                           const SceneDev* __restrict__ scenes,
+                                                           uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    uint32_t g;
+    if (!cell_of(keys[i], scenes, g)) return;
+    uint32_t gp = 0, gn = 0;
+    const bool first = i == 0 || !cell_of(keys[i - 1], scenes, gp) || gp != g;
+    const bool last = i == T - 1 || !cell_of(keys[i + 1], scenes, gn) || gn != g;
+    if (first) start[g] = (uint32_t)i;
+    if (last) end[g] = (uint32_t)(i + 1);
+}
+
+template <bool FORCE>
+__global__ __launch_bounds__(kBBlock) void k_pairs_b(const BatchItem* __restrict__ items, int n_items,
+                                                     int* __restrict__ queue, const SceneDev* __restrict__ scenes,
+                                                     const int* __restrict__ d_num, const float4* __restrict__ pos_s,
+                                                     const uint32_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ end,
+                                                     const uint32_t* __restrict__ keys, float4* __restrict__ ve,
+                                                     const float4* __restrict__ vel_s,
+                                                     const uint32_t* __restrict__ idx, float4* __restrict__ pos_u,
+                                                     float4* __restrict__ vel_u, float4* __restrict__ veleval_u) {
+    __shared__ int sh_item;
+    for (;;) {
+        if (threadIdx.x == 0) sh_item = atomicAdd(queue, 1);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(sh_item);
+        __syncthreads();
+        if (it >= n_items) return;   // every block leaves once the queue is drained
+        const BatchItem I = items[it];
+        const int s = __builtin_amdgcn_readfirstlane(I.scene);
+        const SceneDev& S = scenes[s];
+        int num;
+        const bool deleted = scene_num(S, d_num[s], start, num);
+        const int i = I.first + (int)threadIdx.x;
+        if ((int)threadIdx.x >= I.count || i - S.off >= num) continue;
+        const rtps_sph_params& sp = S.sp;
+        const float scale = sp.simulation_scale;
+        const float h = sp.smoothing_distance, h2 = h * h;
+        float4 pi = pos_s[i];
+        const float4 p_unscaled = pi;
+        pi.x *= scale;
+        pi.y *= scale;
+        pi.z *= scale;
+        const Grid& g = S.gs;
+        const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
+        const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, keys, S.res_x, S.res_y, S.res_z);
+        const uint32_t* st = start + S.cell_base;
+        const uint32_t* en = end + S.cell_base;
+        const uint32_t prefix = (uint32_t)s << kSceneShift;
+        if (!FORCE) {
+            float sum = 0.0f;
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy) {
+                    uint32_t a, b;
+                    row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix);
+                    for (uint32_t j = a; j < b; ++j) {
+                        float4 pj = pos_s[j];
+                        float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
+                        float r2 = rx * rx + ry * ry + rz * rz;
+                        float t = h2 - r2;
+                        sum += (r2 <= h2) ? t * t * t : 0.0f;
+                    }
+                }
+            ve[i].w = sp.wpoly6_coef * sp.mass * sum;
+        } else {
+            const float eps = sp.EPSILON, K = sp.K, rho0 = sp.rest_density;
+            const float4 vi = ve[i];
+            const float di = vi.w;
+            const float Pi = K * (di - rho0);
+            ForceAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy) {
+                    uint32_t a, b;
+                    row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix);
+                    for (uint32_t j = a; j < b; ++j) {
+                        float4 pj = pos_s[j];
+                        float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
+                        float r2 = rx * rx + ry * ry + rz * rz;
+                        if (r2 <= h2 && j != (uint32_t)i) {
+                            const float4 vj = ve[j];
+                            force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, vj.w);
+                        }
+                    }
+                }
+            float4 F, X;
+            force_epilogue(acc, di, sp, F, X);
+            const float4 v = vel_s[i];
+            if (S.collision) collision_wall(F, p_unscaled, v, sp, S.bmin, S.bmax);
+            float4 po, vo, eo;
+            leapfrog(p_unscaled, v, F, X, sp, S.dt, po, vo, eo);
+            const uint32_t o = deleted ? (uint32_t)i : idx[i];
+            pos_u[o] = po;
+            vel_u[o] = vo;
+            veleval_u[o] = eo;
+        }
+    }
+}
+
+__global__ void k_commit_b(int S, const SceneDev* __restrict__ scenes, int* __restrict__ d_num,
+                           const uint32_t* __restrict__ start) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    int num;
+    if (scene_num(scenes[s], d_num[s], start, num)) d_num[s] = num;
+}
+
+hipError_t launch_hash_b(int T, const uint16_t* sid, const SceneDev* scenes, const int* d_num, const float4* pos_u,
+                         uint32_t* keys, uint32_t* idx, uint32_t* start, int n_cells, int* queues, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    const int n = T > n_cells ? T : n_cells;
+    k_hash_b<<<bblocks(n), kBBlock, 0, st>>>(T, sid, scenes, d_num, pos_u, keys, idx, start, n_cells, queues);
+    return hipGetLastError();
+}
+
+hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* scenes, uint32_t* start, uint32_t* end,
+                                hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    k_cellindices_b<<<bblocks(T), kBBlock, 0, st>>>(T, keys, scenes, start, end);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs_b(bool force, int grid, const BatchItem* items, int n_items, int* queue,
+                          const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
+                          const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,
+                          const uint32_t* idx, float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (force)
+        k_pairs_b<true><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start, end, keys, ve,
+                                                  vel_s, idx, pos_u, vel_u, veleval_u);
+    else
+        k_pairs_b<false><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start, end, keys, ve,
+                                                   vel_s, idx, pos_u, vel_u, veleval_u);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit_b(int S, const SceneDev* scenes, int* d_num, const uint32_t* start, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    k_commit_b<<<(S + 63) / 64, 64, 0, st>>>(S, scenes, d_num, start);
+    return hipGetLastError();
+}
+
+}  // namespace rtps

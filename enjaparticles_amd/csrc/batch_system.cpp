@@ -1,0 +1,406 @@
+// Batched scenes handle (rtps_batch): S independent SPH systems, each with
+// its own settings (viscosity, gas constant, box, capacity ...), stepped with
+// one launch per stage and one hipGraph per step.  No reference counterpart:
+// every scene computes what a single rtps_sph system with the same settings
+// and particles computes.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "batch_types.h"
+#include "rtps_internal.h"
+#include "sph_device.h"
+
+using namespace rtps;
+
+enum BStage { B_HASH = 0, B_SORT, B_CELL, B_PERMUTE, B_DENSITY, B_FORCE, B_N };
+
+struct rtps_batch {
+    int S = 0, T = 0, total_cells = 0;
+    std::vector<RTPSSettings> settings;
+    std::vector<SceneDev> scenes_h;
+    std::vector<int> num_h;
+    hipStream_t stream = nullptr;
+    SceneDev* scenes = nullptr;
+    int* d_num = nullptr;
+    uint16_t* sid = nullptr;
+    float4 *pos_u = nullptr, *vel_u = nullptr, *veleval_u = nullptr, *pos_s = nullptr, *vel_s = nullptr,
+           *ve_s = nullptr;
+    uint32_t *keys = nullptr, *idx = nullptr, *start = nullptr, *end = nullptr;
+    uint32_t *keys_sorted = nullptr, *idx_sorted = nullptr;
+    BatchItem* items = nullptr;
+    int n_items = 0, persistent_grid = 0;
+    int* queues = nullptr;
+    void* sort_tmp = nullptr;
+    int shifts[4] = {}, widths[4] = {}, npasses = 0;
+    int opt_graph = 1, opt_timers = 0;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_valid = false;
+    hipEvent_t ev[B_N + 1] = {};
+    double stage_ms[B_N] = {};
+    long stage_cnt[B_N] = {};
+
+    ~rtps_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        void* bufs[] = {scenes, d_num, sid, pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s, keys, idx,
+                        start, end, items, queues, sort_tmp};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+#define RTPS_TRY_HIP(expr)                                 \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return hip_error(e_, #expr); \
+    } while (0)
+
+int bits_for(unsigned v) {   // number of bits to represent values < v
+    int b = 0;
+    while ((1u << b) < v && b < 32) ++b;
+    return b;
+}
+
+void mark(rtps_batch* b, int i) {
+    if (b->opt_timers) (void)hipEventRecord(b->ev[i], b->stream);
+}
+
+int launch_step(rtps_batch* b) {
+    mark(b, B_HASH);
+    RTPS_TRY_HIP(launch_hash_b(b->T, b->sid, b->scenes, b->d_num, b->pos_u, b->keys, b->idx, b->start,
+                               b->total_cells, b->queues, b->stream));
+    mark(b, B_SORT);
+    RTPS_TRY_HIP(sort_pairs_passes(b->T, b->shifts, b->widths, b->npasses, b->keys, b->idx, b->sort_tmp, b->stream,
+                                   &b->keys_sorted, &b->idx_sorted));
+    mark(b, B_CELL);
+    RTPS_TRY_HIP(launch_cellindices_b(b->T, b->keys_sorted, b->scenes, b->start, b->end, b->stream));
+    mark(b, B_PERMUTE);
+    RTPS_TRY_HIP(launch_permute(b->T, nullptr, b->idx_sorted, b->pos_u, b->vel_u, b->veleval_u, b->pos_s, b->vel_s,
+                                b->ve_s, b->stream));
+    mark(b, B_DENSITY);
+    RTPS_TRY_HIP(launch_pairs_b(false, b->persistent_grid, b->items, b->n_items, b->queues, b->scenes, b->d_num,
+                                b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted,
+                                b->pos_u, b->vel_u, b->veleval_u, b->stream));
+    mark(b, B_FORCE);
+    RTPS_TRY_HIP(launch_pairs_b(true, b->persistent_grid, b->items, b->n_items, b->queues + 1, b->scenes, b->d_num,
+                                b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted,
+                                b->pos_u, b->vel_u, b->veleval_u, b->stream));
+    RTPS_TRY_HIP(launch_commit_b(b->S, b->scenes, b->d_num, b->start, b->stream));
+    mark(b, B_N);
+    return RTPS_OK;
+}
+
+int capture(rtps_batch* b) {
+    if (b->gexec) {
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    RTPS_TRY_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int timers = b->opt_timers;
+    b->opt_timers = 0;
+    int rc = launch_step(b);
+    b->opt_timers = timers;
+    hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
+    b->graph_valid = true;
+    return RTPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* keys, const char* const* vals,
+                      rtps_batch** out) {
+    try {
+        if (!out || num_scenes <= 0 || num_scenes > kMaxScenes || !n_settings)
+            return set_error(RTPS_ERR_ARG, "rtps_batch_create: 1..256 scenes and their settings are required");
+        *out = nullptr;
+        std::unique_ptr<rtps_batch> b(new rtps_batch());
+        b->S = num_scenes;
+        b->settings.resize(num_scenes);
+        b->scenes_h.resize(num_scenes);
+        b->num_h.assign(num_scenes, 0);
+        long k = 0, T = 0, cells = 0;
+        int max_cells = 0;
+        for (int s = 0; s < num_scenes; ++s) {
+            if (n_settings[s] < 0) return set_error(RTPS_ERR_ARG, "negative settings count");
+            RTPSSettings& st = b->settings[s];
+            for (int q = 0; q < n_settings[s]; ++q, ++k) {
+                if (!keys[k] || !vals[k]) return set_error(RTPS_ERR_ARG, "null key or value");
+                st.SetSetting<std::string>(keys[k], vals[k]);
+            }
+            if (st.GetSettingAs<std::string>("system") != "sph")
+                return set_error(RTPS_ERR_UNKNOWN_SYSTEM, "scene " + std::to_string(s) + ": system is not sph");
+            const int cap = st.GetSettingAs<int>("max_num_particles");
+            if (cap <= 0) return set_error(RTPS_ERR_ARG, "scene " + std::to_string(s) + ": max_num_particles <= 0");
+            SceneDev& d = b->scenes_h[s];
+            std::memset(&d, 0, sizeof(d));
+            rtps_grid_params gp, gps;
+            derive_params(st, d.sp, gp, gps);
+            if (gp.nb_cells <= 0 || gp.nb_cells >= (1 << 19))
+                return set_error(RTPS_ERR_ARG, "scene " + std::to_string(s) + ": grid too large for segmented keys");
+            d.g = make_grid(gp);
+            d.gs = make_grid(gps);
+            d.bmin = gps.bnd_min;
+            d.bmax = gps.bnd_max;
+            d.res_x = gps.grid_res.x;
+            d.res_y = gps.grid_res.y;
+            d.res_z = gps.grid_res.z;
+            d.dt = st.GetSettingAs<float>("time_step");
+            d.off = (int)T;
+            d.cap = cap;
+            d.cell_base = (int)cells;
+            d.collision = 1;
+            T += cap;
+            cells += gp.nb_cells + 1;
+            max_cells = std::max(max_cells, gp.nb_cells);
+            if (T >= (1L << 31) - 1) return set_error(RTPS_ERR_ARG, "too many particle slots");
+        }
+        b->T = (int)T;
+        b->total_cells = (int)cells;
+        // sort plan: the low local bits (cell << 4 | xq; pads are all ones) then the scene bits
+        const int low = sort_key_bits(max_cells) + kSubBits;
+        int np = sort_plan(low, b->shifts, b->widths);
+        const int sb = bits_for((unsigned)num_scenes);
+        if (sb > 0) {
+            b->shifts[np] = (int)kSceneShift;
+            b->widths[np] = sb;
+            ++np;
+        }
+        b->npasses = np;
+        // work items: 256-slot tiles of every scene
+        std::vector<BatchItem> items;
+        for (int s = 0; s < num_scenes; ++s) {
+            const SceneDev& d = b->scenes_h[s];
+            for (int f = 0; f < d.cap; f += 256) items.push_back(BatchItem{s, d.off + f, std::min(256, d.cap - f), 0});
+        }
+        b->n_items = (int)items.size();
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+                cus = prop.multiProcessorCount;
+        }
+        b->persistent_grid = std::max(1, std::min(b->n_items, cus * 4));
+
+        std::vector<uint16_t> sid((size_t)T);
+        for (int s = 0; s < num_scenes; ++s)
+            std::fill(sid.begin() + b->scenes_h[s].off, sid.begin() + b->scenes_h[s].off + b->scenes_h[s].cap,
+                      (uint16_t)s);
+
+        RTPS_TRY_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        const size_t NT = (size_t)T;
+        RTPS_TRY_HIP(hipMalloc((void**)&b->scenes, sizeof(SceneDev) * num_scenes));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->d_num, sizeof(int) * num_scenes));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->sid, sizeof(uint16_t) * NT));
+        float4** f4[] = {&b->pos_u, &b->vel_u, &b->veleval_u, &b->pos_s, &b->vel_s, &b->ve_s};
+        for (float4** p : f4) {
+            RTPS_TRY_HIP(hipMalloc((void**)p, NT * sizeof(float4)));
+            RTPS_TRY_HIP(hipMemsetAsync(*p, 0, NT * sizeof(float4), b->stream));
+        }
+        RTPS_TRY_HIP(hipMalloc((void**)&b->keys, NT * 4));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->idx, NT * 4));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->start, (size_t)cells * 4));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->end, (size_t)cells * 4));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->items, sizeof(BatchItem) * items.size()));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->queues, 2 * sizeof(int)));
+        RTPS_TRY_HIP(hipMalloc(&b->sort_tmp, sort_temp_bytes((int)T, 32)));
+        RTPS_TRY_HIP(hipMemcpyAsync(b->scenes, b->scenes_h.data(), sizeof(SceneDev) * num_scenes,
+                                    hipMemcpyHostToDevice, b->stream));
+        RTPS_TRY_HIP(hipMemsetAsync(b->d_num, 0, sizeof(int) * num_scenes, b->stream));
+        RTPS_TRY_HIP(hipMemcpyAsync(b->sid, sid.data(), NT * sizeof(uint16_t), hipMemcpyHostToDevice, b->stream));
+        RTPS_TRY_HIP(hipMemcpyAsync(b->items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice,
+                                    b->stream));
+        RTPS_TRY_HIP(hipMemsetAsync(b->queues, 0, 2 * sizeof(int), b->stream));
+        RTPS_TRY_HIP(launch_fill_u32(b->start, 666u, cells, b->stream));
+        RTPS_TRY_HIP(launch_fill_u32(b->end, 666u, cells, b->stream));
+        RTPS_TRY_HIP(sort_clear_temp(b->sort_tmp, (int)T, b->stream));
+        b->keys_sorted = b->keys;
+        b->idx_sorted = b->idx;
+        for (auto& e : b->ev) RTPS_TRY_HIP(hipEventCreate(&e));
+        RTPS_TRY_HIP(hipStreamSynchronize(b->stream));
+        *out = b.release();
+        return RTPS_OK;
+    } catch (const std::exception& e) {
+        return set_error(RTPS_ERR_ARG, std::string("rtps_batch_create: ") + e.what());
+    }
+}
+
+int rtps_batch_destroy(rtps_batch* b) {
+    delete b;
+    return RTPS_OK;
+}
+
+int rtps_batch_num_scenes(rtps_batch* b, int* num_scenes, int* total_slots) {
+    if (!b) return set_error(RTPS_ERR_ARG, "null handle");
+    if (num_scenes) *num_scenes = b->S;
+    if (total_slots) *total_slots = b->T;
+    return RTPS_OK;
+}
+
+int rtps_batch_get_num(rtps_batch* b, int scene, int* num) {
+    if (!b || !num || scene < 0 || scene >= b->S) return set_error(RTPS_ERR_ARG, "rtps_batch_get_num: bad arguments");
+    RTPS_TRY_HIP(hipMemcpyAsync(num, b->d_num + scene, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    RTPS_TRY_HIP(hipStreamSynchronize(b->stream));
+    return RTPS_OK;
+}
+
+int rtps_batch_push_particles(rtps_batch* b, int scene, const rtps_float4* pos, const rtps_float4* vel, int n) {
+    if (!b || scene < 0 || scene >= b->S || n < 0 || (n > 0 && !pos))
+        return set_error(RTPS_ERR_ARG, "rtps_batch_push_particles: bad arguments");
+    if (n == 0) return RTPS_OK;
+    int num = 0;
+    int rc = rtps_batch_get_num(b, scene, &num);
+    if (rc) return rc;
+    const SceneDev& d = b->scenes_h[scene];
+    if ((long)num + n > d.cap)
+        return set_error(RTPS_ERR_CAPACITY, "pushParticles: exceeded max nb (" + std::to_string(d.cap) +
+                                                ") of particles allowed in scene " + std::to_string(scene));
+    RTPS_TRY_HIP(hipMemcpyAsync(b->pos_u + d.off + num, pos, (size_t)n * 16, hipMemcpyHostToDevice, b->stream));
+    if (vel)
+        RTPS_TRY_HIP(hipMemcpyAsync(b->vel_u + d.off + num, vel, (size_t)n * 16, hipMemcpyHostToDevice, b->stream));
+    else
+        RTPS_TRY_HIP(hipMemsetAsync(b->vel_u + d.off + num, 0, (size_t)n * 16, b->stream));
+    num += n;
+    RTPS_TRY_HIP(hipMemcpyAsync(b->d_num + scene, &num, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    RTPS_TRY_HIP(hipStreamSynchronize(b->stream));
+    b->num_h[scene] = num;
+    return RTPS_OK;
+}
+
+int rtps_batch_step(rtps_batch* b, int nsteps) {
+    if (!b || nsteps < 0) return set_error(RTPS_ERR_ARG, "rtps_batch_step: bad arguments");
+    if (b->opt_timers) {
+        for (int k = 0; k < nsteps; ++k) {
+            int rc = launch_step(b);
+            if (rc) return rc;
+            RTPS_TRY_HIP(hipEventSynchronize(b->ev[B_N]));
+            for (int i = 0; i < B_N; ++i) {
+                float ms = 0.f;
+                RTPS_TRY_HIP(hipEventElapsedTime(&ms, b->ev[i], b->ev[i + 1]));
+                b->stage_ms[i] += ms;
+                b->stage_cnt[i] += 1;
+            }
+        }
+        return RTPS_OK;
+    }
+    if (b->opt_graph) {
+        if (!b->graph_valid) {
+            int rc = capture(b);
+            if (rc) return rc;
+        }
+        for (int k = 0; k < nsteps; ++k) RTPS_TRY_HIP(hipGraphLaunch(b->gexec, b->stream));
+        return RTPS_OK;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = launch_step(b);
+        if (rc) return rc;
+    }
+    return RTPS_OK;
+}
+
+int rtps_batch_synchronize(rtps_batch* b) {
+    if (!b) return set_error(RTPS_ERR_ARG, "null handle");
+    RTPS_TRY_HIP(hipStreamSynchronize(b->stream));
+    return RTPS_OK;
+}
+
+int rtps_batch_read(rtps_batch* b, int scene, int field, void* host, int count) {
+    if (!b || !host || scene < 0 || scene >= b->S || count < 0)
+        return set_error(RTPS_ERR_ARG, "rtps_batch_read: bad arguments");
+    const SceneDev& d = b->scenes_h[scene];
+    if (count > d.cap) return set_error(RTPS_ERR_ARG, "rtps_batch_read: count > scene capacity");
+    RTPS_TRY_HIP(hipStreamSynchronize(b->stream));
+    const float4* src = nullptr;
+    switch (field) {
+        case RTPS_FIELD_POS: src = b->pos_u; break;
+        case RTPS_FIELD_VEL: src = b->vel_u; break;
+        case RTPS_FIELD_VELEVAL: src = b->veleval_u; break;
+        case RTPS_FIELD_POS_S: src = b->pos_s; break;
+        case RTPS_FIELD_VEL_S: src = b->vel_s; break;
+        case RTPS_FIELD_VELEVAL_S: src = b->ve_s; break;
+        default: break;
+    }
+    if (src) {
+        RTPS_TRY_HIP(hipMemcpy(host, src + d.off, (size_t)count * 16, hipMemcpyDeviceToHost));
+        return RTPS_OK;
+    }
+    if (field == RTPS_FIELD_DENSITY) {
+        std::vector<float> tmp((size_t)count * 4);
+        RTPS_TRY_HIP(hipMemcpy(tmp.data(), b->ve_s + d.off, (size_t)count * 16, hipMemcpyDeviceToHost));
+        for (int i = 0; i < count; ++i) ((float*)host)[i] = tmp[4 * (size_t)i + 3];
+        return RTPS_OK;
+    }
+    if (field == RTPS_FIELD_KEYS || field == RTPS_FIELD_IDX) {
+        RTPS_TRY_HIP(hipMemcpy(host, (field == RTPS_FIELD_KEYS ? b->keys_sorted : b->idx_sorted) + d.off,
+                               (size_t)count * 4, hipMemcpyDeviceToHost));
+        uint32_t* k = (uint32_t*)host;
+        if (field == RTPS_FIELD_KEYS)   // segmented reference keys: (scene << 24) | cell
+            for (int i = 0; i < count; ++i) {
+                const uint32_t local = k[i] & ((1u << kSceneShift) - 1u);
+                k[i] = (k[i] & ~((1u << kSceneShift) - 1u)) | (local >> kSubBits);
+            }
+        else   // scene-local original indices
+            for (int i = 0; i < count; ++i) k[i] -= (uint32_t)d.off;
+        return RTPS_OK;
+    }
+    return set_error(RTPS_ERR_ARG, "rtps_batch_read: unsupported field");
+}
+
+int rtps_batch_set_option(rtps_batch* b, int option, int value) {
+    if (!b) return set_error(RTPS_ERR_ARG, "null handle");
+    switch (option) {
+        case RTPS_OPT_GRAPH: b->opt_graph = value ? 1 : 0; break;
+        case RTPS_OPT_TIMERS: b->opt_timers = value ? 1 : 0; break;
+        case RTPS_OPT_COLLISION: {
+            for (auto& d : b->scenes_h) d.collision = value ? 1 : 0;
+            RTPS_TRY_HIP(hipMemcpyAsync(b->scenes, b->scenes_h.data(), sizeof(SceneDev) * b->S,
+                                        hipMemcpyHostToDevice, b->stream));
+            RTPS_TRY_HIP(hipStreamSynchronize(b->stream));
+            break;
+        }
+        default: return set_error(RTPS_ERR_ARG, "unknown option");
+    }
+    b->graph_valid = false;
+    return RTPS_OK;
+}
+
+int rtps_batch_stage_times(rtps_batch* b, float* avg_ms, int* counts, int n) {
+    if (!b || n < 0) return set_error(RTPS_ERR_ARG, "bad arguments");
+    for (int i = 0; i < n && i < B_N; ++i) {
+        if (avg_ms) avg_ms[i] = b->stage_cnt[i] ? (float)(b->stage_ms[i] / b->stage_cnt[i]) : 0.f;
+        if (counts) counts[i] = (int)b->stage_cnt[i];
+    }
+    return RTPS_OK;
+}
+
+int rtps_batch_reset_timers(rtps_batch* b) {
+    if (!b) return set_error(RTPS_ERR_ARG, "null handle");
+    for (int i = 0; i < B_N; ++i) {
+        b->stage_ms[i] = 0;
+        b->stage_cnt[i] = 0;
+    }
+    return RTPS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,41 @@
+// Device-side description of one scene of a batch and of one work item of
+// the persistent neighbour kernels.
+#pragma once
+
+#include <stdint.h>
+
+#include "sph_device.h"
+
+namespace rtps {
+
+constexpr uint32_t kSceneShift = 24;   // segmented key: (scene << 24) | local key
+constexpr int kMaxScenes = 256;
+
+struct SceneDev {
+    rtps_sph_params sp;          // SPHParams of the scene
+    Grid g;                      // unscaled grid (hash)
+    Grid gs;                     // scaled grid (neighbour search)
+    rtps_float4 bmin, bmax;      // scaled domain bounds (walls)
+    float res_x, res_y, res_z;   // grid resolution
+    float dt;
+    int off;                     // first slot of the scene in the particle arrays
+    int cap;                     // slots of the scene (its max_num_particles)
+    int cell_base;               // first entry of the scene's cell table
+    int collision;               // wall collision on
+};
+
+struct BatchItem {
+    int scene, first, count, pad_;
+};
+
+hipError_t launch_hash_b(int T, const uint16_t* sid, const SceneDev* scenes, const int* d_num, const float4* pos_u,
+                         uint32_t* keys, uint32_t* idx, uint32_t* start, int n_cells, int* queues, hipStream_t st);
+hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* scenes, uint32_t* start, uint32_t* end,
+                                hipStream_t st);
+hipError_t launch_pairs_b(bool force, int grid, const BatchItem* items, int n_items, int* queue,
+                          const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
+                          const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,
+                          const uint32_t* idx, float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st);
+hipError_t launch_commit_b(int S, const SceneDev* scenes, int* d_num, const uint32_t* start, hipStream_t st);
+
+}  // namespace rtps

@@ -71,7 +71,7 @@ int rtps_hash(int num, int n_pad, const rtps_float4* pos_u, uint32_t* keys, uint
     RTPS_CHECK(num == 0 || (pos_u && keys && idx), "rtps_hash: null buffer");
     const int bits = sort_key_bits(gp->nb_cells);
     const uint32_t pad = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
-    RTPS_HIP(launch_hash(num, nullptr, n_pad, pad, (const float4*)pos_u, keys, idx, make_grid(*gp), nullptr, 0,
+    RTPS_HIP(launch_hash(num, nullptr, n_pad, pad, (const float4*)pos_u, keys, idx, make_grid(*gp), nullptr, 0, 0,
                          S(stream)));
     return RTPS_OK;
 }
@@ -92,7 +92,7 @@ int rtps_cellindices(int num, const uint32_t* keys, int nb_cells, uint32_t* star
                      void* stream) {
     RTPS_CHECK(num >= 0 && nb_cells > 0 && start && end && (num == 0 || keys), "rtps_cellindices: bad arguments");
     RTPS_HIP(launch_fill_u32(start, kEmpty, (long)nb_cells + 1, S(stream)));
-    RTPS_HIP(launch_cellindices(num, nullptr, keys, nb_cells, start, end, S(stream)));
+    RTPS_HIP(launch_cellindices(num, nullptr, keys, nb_cells, start, end, 0, S(stream)));
     if (nc) RTPS_HIP(launch_read_nc(start, nb_cells, nc, S(stream)));
     return RTPS_OK;
 }
@@ -113,8 +113,9 @@ int rtps_density(int num, const rtps_float4* pos_s, const uint32_t* start, const
     RTPS_CHECK(num >= 0 && sphp && gps, "rtps_density: bad arguments");
     if (num == 0) return RTPS_OK;
     RTPS_CHECK(pos_s && start && end && density, "rtps_density: null buffer");
-    RTPS_HIP(launch_density(num, nullptr, (const float4*)pos_s, start, end, *sphp, make_grid(*gps), density, nullptr,
-                            S(stream)));
+    const Win plain{nullptr, 0.f, 0.f, 0.f};
+    RTPS_HIP(launch_density(num, nullptr, (const float4*)pos_s, start, end, *sphp, make_grid(*gps), plain, density,
+                            nullptr, S(stream)));
     return RTPS_OK;
 }
 
@@ -124,8 +125,9 @@ int rtps_force(int num, const rtps_float4* pos_s, const float* density, const rt
     RTPS_CHECK(num >= 0 && sphp && gps, "rtps_force: bad arguments");
     if (num == 0) return RTPS_OK;
     RTPS_CHECK(pos_s && density && veleval_s && start && end && force && xsph, "rtps_force: null buffer");
+    const Win plain{nullptr, 0.f, 0.f, 0.f};
     RTPS_HIP(launch_force(num, nullptr, (const float4*)pos_s, density, (const float4*)veleval_s, false, start, end,
-                          *sphp, make_grid(*gps), (float4*)force, (float4*)xsph, S(stream)));
+                          *sphp, make_grid(*gps), plain, (float4*)force, (float4*)xsph, S(stream)));
     return RTPS_OK;
 }
 

@@ -5,15 +5,19 @@
 // the bits are split evenly over ceil(key_bits / 8) passes.
 //
 // Per pass (R = 2^bits buckets, tiles of 4096 keys = 256 threads x 16):
-//   k_radix_hist    per-tile digit histogram (LDS atomics), digit-major in
-//                   hist[d * ntiles + t]; digit totals accumulated in gtot[d]
+//   k_radix_hist    per-tile digit histogram, digit-major in hist[d * ntiles + t];
+//                   lanes of a wave holding the same digit are found with
+//                   `bits` ballots and add their count with one LDS atomic;
+//                   digit totals accumulated in gtot[d]
 //   k_radix_scan    one block per digit: exclusive scan of its row plus the
 //                   total of all smaller digits -> global offsets
-//   k_radix_scatter stable ranking inside the tile with wave64 ballots
-//                   (lanes with equal digit found by `bits` ballots, rank =
-//                   popcount of the lower peers) and a per-digit scan over
-//                   the tile's 64 (round, wave) slots in LDS, then the write
-//                   to offset + rank.  Block 0 clears gtot for the next use.
+//   k_radix_scatter stable ranking inside the tile: wave w owns the tile's
+//                   w-th quarter and walks it in rounds of 64 keys; the rank
+//                   of a key = popcount of the lower lanes with the same digit
+//                   (ballots) + the wave's running count of that digit (LDS);
+//                   a 4-wave exclusive scan per digit then gives each wave's
+//                   offset and the key goes to digit offset + rank.  Block 0
+//                   clears gtot for the next use.
 // Keys and values ping-pong between the caller's arrays and the alternate
 // arrays in the temp buffer.
 
@@ -64,19 +68,34 @@ static SortTemp carve(void* temp, int n) {
     return t;
 }
 
+// Lanes of the calling wave whose digit equals this lane's (restricted to `valid` lanes).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, bool valid) {
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+    }
+    return peers;
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ kin, int n, int shift,
                                                              int bits, uint32_t* __restrict__ hist,
                                                              uint32_t* __restrict__ gtot, int ntiles) {
     __shared__ uint32_t h[kMaxRadix];
     const int R = 1 << bits;
     const uint32_t mask = (uint32_t)R - 1u;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
     for (int d = threadIdx.x; d < R; d += kSortThreads) h[d] = 0;
     __syncthreads();
     const long base = (long)blockIdx.x * kSortTile;
-#pragma unroll 4
     for (int k = 0; k < kSortItems; ++k) {
         const long e = base + k * kSortThreads + threadIdx.x;
-        if (e < n) atomicAdd(&h[(kin[e] >> shift) & mask], 1u);
+        const bool valid = e < n;
+        const uint32_t d = valid ? (kin[e] >> shift) & mask : 0u;
+        const uint64_t peers = match_digit(d, bits, valid);
+        if (valid && (peers & lt) == 0ull) atomicAdd(&h[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
     for (int d = threadIdx.x; d < R; d += kSortThreads) {
@@ -146,57 +165,52 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
                                                                 uint32_t* __restrict__ vout, int n, int shift,
                                                                 int bits, const uint32_t* __restrict__ hist,
                                                                 int ntiles, uint32_t* __restrict__ gtot_clear) {
-    // slot s = round * 4 + wave; wcnt[s * R + d] = count, then exclusive prefix
-    __shared__ uint16_t wcnt[kSortItems * kSortWaves * kMaxRadix];
+    constexpr int kQuarter = kSortTile / kSortWaves;   // 1024 keys per wave, in rounds of 64
+    __shared__ uint32_t cnt[kSortWaves][kMaxRadix];    // running count per (wave, digit), then wave offsets
     __shared__ uint32_t goff[kMaxRadix];
     const int R = 1 << bits;
     const uint32_t mask = (uint32_t)R - 1u;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const long base = (long)blockIdx.x * kSortTile;
-    for (int e = tid; e < kSortItems * kSortWaves * R; e += kSortThreads) wcnt[e] = 0;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int e = tid; e < kSortWaves * kMaxRadix; e += kSortThreads) (&cnt[0][0])[e] = 0;
     for (int d = tid; d < R; d += kSortThreads) goff[d] = hist[(long)d * ntiles + blockIdx.x];
     if (blockIdx.x == 0)
         for (int d = tid; d < R; d += kSortThreads) gtot_clear[d] = 0;
     __syncthreads();
 
-    uint32_t key[kSortItems], val[kSortItems];
-    uint32_t rnk[kSortItems];
-    const uint64_t lt = (1ull << lane) - 1ull;
+    const long base = (long)blockIdx.x * kSortTile + (long)w * kQuarter;
+    uint32_t key[kSortItems], val[kSortItems], pos[kSortItems];
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        const long e = base + k * kSortThreads + tid;
+        const long e = base + k * 64 + lane;
         const bool valid = e < n;
         key[k] = valid ? kin[e] : 0xFFFFFFFFu;
         val[k] = valid ? vin[e] : 0u;
         const uint32_t d = (key[k] >> shift) & mask;
-        uint64_t peers = __ballot(valid);
-        for (int b = 0; b < bits; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bb = __ballot(bit);
-            peers &= bit ? bb : ~bb;
-        }
-        const uint32_t r = (uint32_t)__popcll(peers & lt);
-        if (valid && r == 0) wcnt[(k * kSortWaves + w) * R + d] = (uint16_t)__popcll(peers);
-        rnk[k] = r;
+        const uint64_t peers = match_digit(d, bits, valid);
+        const uint32_t before = cnt[w][d];                 // this wave's earlier rounds
+        pos[k] = before + (uint32_t)__popcll(peers & lt);
+        if (valid && (peers & lt) == 0ull) cnt[w][d] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    for (int d = tid; d < R; d += kSortThreads) {
+    for (int d = tid; d < R; d += kSortThreads) {          // exclusive scan over the 4 waves
         uint32_t run = 0;
-        for (int s = 0; s < kSortItems * kSortWaves; ++s) {
-            const uint32_t c = wcnt[s * R + d];
-            wcnt[s * R + d] = (uint16_t)run;
+#pragma unroll
+        for (int q = 0; q < kSortWaves; ++q) {
+            const uint32_t c = cnt[q][d];
+            cnt[q][d] = run;
             run += c;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        const long e = base + k * kSortThreads + tid;
+        const long e = base + k * 64 + lane;
         if (e < n) {
             const uint32_t d = (key[k] >> shift) & mask;
-            const uint32_t pos = goff[d] + wcnt[(k * kSortWaves + w) * R + d] + rnk[k];
-            kout[pos] = key[k];
-            vout[pos] = val[k];
+            const uint32_t p = goff[d] + cnt[w][d] + pos[k];
+            kout[p] = key[k];
+            vout[p] = val[k];
         }
     }
 }
@@ -228,24 +242,24 @@ hipError_t sort_clear_temp(void* temp, int n, hipStream_t st) {
     return launch_fill_u32(t.gtot, 0u, (long)kMaxPasses * kMaxRadix, st);
 }
 
-// Sorts (keys, vals) by the low key_bits bits.  The result is in (keys, vals)
-// when *in_alt comes back false, else in the temp buffer's alternate arrays
-// (returned through out_keys / out_vals either way).  gtot must be zero on
-// entry (sort_clear_temp once; every pass leaves it zero again).
-hipError_t sort_pairs_pingpong(int n, int key_bits, uint32_t* keys, uint32_t* vals, void* temp, hipStream_t st,
-                               uint32_t** out_keys, uint32_t** out_vals) {
+// Sorts (keys, vals) by the bits [shift[p], shift[p] + width[p]) of each pass
+// p (least significant range first, every width <= 8).  The result ends in
+// (keys, vals) or in the temp buffer's alternate arrays, returned through
+// out_keys / out_vals.  gtot must be zero on entry (sort_clear_temp once;
+// every pass leaves it zero again).
+hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int npasses, uint32_t* keys,
+                             uint32_t* vals, void* temp, hipStream_t st, uint32_t** out_keys, uint32_t** out_vals) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     *out_keys = keys;
     *out_vals = vals;
-    const int passes = sort_passes(key_bits);
-    if (n <= 1 || passes == 0) return hipSuccess;
+    if (n <= 1 || npasses == 0) return hipSuccess;
+    if (npasses > kMaxPasses) return hipErrorInvalidValue;
     SortTemp t = carve(temp, n);
     const int ntiles = ntiles_for(n);
-    const int width = (key_bits + passes - 1) / passes;
     uint32_t *ki = keys, *vi = vals, *ko = t.kalt, *vo = t.valt;
-    int shift = 0;
-    for (int p = 0; p < passes; ++p) {
-        const int bits = min(width, key_bits - shift);
+    for (int p = 0; p < npasses; ++p) {
+        const int bits = widths[p], shift = shifts[p];
+        if (bits < 1 || bits > 8) return hipErrorInvalidValue;
         uint32_t* gtot = t.gtot + p * kMaxRadix;
         k_radix_hist<<<ntiles, kSortThreads, 0, st>>>(ki, n, shift, bits, t.hist, gtot, ntiles);
         k_radix_scan<<<1 << bits, kSortThreads, 0, st>>>(t.hist, gtot, ntiles);
@@ -254,11 +268,31 @@ hipError_t sort_pairs_pingpong(int n, int key_bits, uint32_t* keys, uint32_t* va
         if (e != hipSuccess) return e;
         uint32_t* tk = ki; ki = ko; ko = tk;
         uint32_t* tv = vi; vi = vo; vo = tv;
-        shift += bits;
     }
     *out_keys = ki;
     *out_vals = vi;
     return hipSuccess;
+}
+
+// Passes over the low key_bits bits, split evenly into digits of <= 8 bits.
+int sort_plan(int key_bits, int* shifts, int* widths) {
+    const int passes = sort_passes(key_bits);
+    if (passes == 0) return 0;
+    const int width = (key_bits + passes - 1) / passes;
+    int shift = 0;
+    for (int p = 0; p < passes; ++p) {
+        widths[p] = min(width, key_bits - shift);
+        shifts[p] = shift;
+        shift += widths[p];
+    }
+    return passes;
+}
+
+hipError_t sort_pairs_pingpong(int n, int key_bits, uint32_t* keys, uint32_t* vals, void* temp, hipStream_t st,
+                               uint32_t** out_keys, uint32_t** out_vals) {
+    int shifts[kMaxPasses], widths[kMaxPasses];
+    const int np = sort_plan(key_bits, shifts, widths);
+    return sort_pairs_passes(n, shifts, widths, np, keys, vals, temp, st, out_keys, out_vals);
 }
 
 // In-place variant for the C ABI: copies back when the pass count is odd.

@@ -13,6 +13,7 @@ namespace rtps {
 
 struct Grid;
 struct GravitySources;
+struct Win;
 
 // ---- error reporting (capi.cpp) ----
 int set_error(int code, const std::string& msg);
@@ -29,20 +30,22 @@ int sort_key_bits(int nb_cells);
 // ---- kernels (sph_kernels.hip) ----
 extern int g_force_variant;   // 0 = direct branch per candidate, 1 = compacted neighbour lists
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
-                       uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, hipStream_t st);
+                       uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, int sub_bits,
+                       hipStream_t st);
 hipError_t launch_cellindices(int num, const int* d_num, const uint32_t* keys, int nb_cells, uint32_t* start,
-                              uint32_t* end, hipStream_t st);
+                              uint32_t* end, int sub_bits, hipStream_t st);
 hipError_t launch_read_nc(const uint32_t* start, int nb_cells, int* nc, hipStream_t st);
 hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const float4* pu, const float4* vu,
                           const float4* eu, float4* ps, float4* vs, float4* es, hipStream_t st);
 hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
-                          const rtps_sph_params& sp, const Grid& g, float* dens, float4* packed, hipStream_t st);
+                          const rtps_sph_params& sp, const Grid& g, const Win& win, float* dens, float4* packed,
+                          hipStream_t st);
 hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const float* dens, const float4* ve,
                         bool packed, const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp,
-                        const Grid& g, float4* force, float4* xsph, hipStream_t st);
+                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st);
 hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, const float4* ve_packed,
                               const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
-                              float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
+                              const Win& win, float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
                               const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,
                               float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st);
 hipError_t launch_post_force(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
@@ -53,7 +56,7 @@ hipError_t launch_leapfrog(int num, const int* d_num, const uint32_t* start, int
                            const rtps_sph_params& sp, float dt, float4* pos_u, float4* vel_u, float4* veleval_u,
                            hipStream_t st);
 hipError_t launch_count_pairs(int num, const int* d_num, const float4* pos_s, const uint32_t* start,
-                              const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
+                              const uint32_t* end, const rtps_sph_params& sp, const Grid& g, const Win& win,
                               unsigned long long* out, hipStream_t st);
 hipError_t launch_commit_num(int* d_num, const uint32_t* start, int nb_cells, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st);
@@ -65,5 +68,8 @@ hipError_t sort_clear_temp(void* temp, int n, hipStream_t st);
 hipError_t sort_pairs_pingpong(int n, int key_bits, uint32_t* keys, uint32_t* vals, void* temp, hipStream_t st,
                                uint32_t** out_keys, uint32_t** out_vals);
 hipError_t sort_pairs_inplace(int n, int key_bits, uint32_t* keys, uint32_t* vals, void* temp, hipStream_t st);
+hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int npasses, uint32_t* keys,
+                             uint32_t* vals, void* temp, hipStream_t st, uint32_t** out_keys, uint32_t** out_vals);
+int sort_plan(int key_bits, int* shifts, int* widths);
 
 }  // namespace rtps

@@ -96,6 +96,93 @@ __device__ __forceinline__ void row_range(int cx, int y, int z, const Grid& g, c
     if (a == kEmpty) { a = 0; b = 0; }
 }
 
+// ---------------------------------------------------------------------------
+// Windowed rows.  The system sorts by an internal key (cell << kSubBits) | xq,
+// xq = the particle's x position inside its cell in 1/16ths, so the cells
+// keep the reference's order (keys >> kSubBits are the reference keys) and
+// every row of three cells is sorted by x.  Only particles with
+// |x_j - x_i| <= sqrt(h^2 - d_yz^2) can be neighbours, where d_yz is the
+// distance from particle i to the row's y/z column; those form one contiguous
+// sub-run, found by two binary searches on the keys.  Rows farther than h in
+// y/z are skipped.  The bounds are widened by one sub-cell (and d^2 by 1e-4
+// relative) so rounding never drops a candidate the r <= h test would accept.
+constexpr int kSubBits = 4;
+constexpr int kSubCells = 1 << kSubBits;
+
+// Windowing inputs of a launch: the sorted internal keys (nullptr = plain
+// 27-cell rows) and the grid resolution.
+struct Win {
+    const uint32_t* keys;
+    float rx, ry, rz;
+};
+
+struct WindowCtx {
+    float ux, uy, uz;        // continuous cell coordinates (p - min) * delta
+    float wy, wz;            // cell widths in y and z (1 / delta)
+    float hx;                // h in x cell units (h * delta.x)
+    bool on;                 // windowing allowed for this particle
+};
+
+__device__ __forceinline__ WindowCtx window_ctx(float px, float py, float pz, const Grid& g, int3 c, float h,
+                                                const uint32_t* keys, float res_x, float res_y, float res_z) {
+    WindowCtx w;
+    w.ux = (px - g.min.x) * g.delta.x;
+    w.uy = (py - g.min.y) * g.delta.y;
+    w.uz = (pz - g.min.z) * g.delta.z;
+    w.wy = 1.0f / g.delta.y;
+    w.wz = 1.0f / g.delta.z;
+    w.hx = h * g.delta.x;
+    // inside the grid and away from its x edges (no aliased cells in the row)
+    w.on = keys != nullptr && w.ux >= 1.0f && w.uy >= 0.0f && w.uz >= 0.0f && c.x + 1 < (int)res_x &&
+           w.uy < res_y && w.uz < res_z;
+    return w;
+}
+
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ k, uint32_t lo, uint32_t hi,
+                                                    uint32_t v) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (k[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Row (y, z) = (c.y + dy, c.z + dz) of particle i: [a, b) of its candidates.
+// key_prefix: high key bits shared by the whole row (the scene id of batched keys).
+__device__ __forceinline__ void row_window(const WindowCtx& w, int3 c, int dy, int dz, float h2, const Grid& g,
+                                           const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                           const uint32_t* __restrict__ keys, uint32_t& a, uint32_t& b,
+                                           uint32_t key_prefix = 0u) {
+    row_range(c.x, c.y + dy, c.z + dz, g, start, end, a, b);
+    if (!w.on || a >= b) return;
+    const float ey = dy < 0 ? (w.uy - (float)c.y) * w.wy : (dy > 0 ? ((float)(c.y + 1) - w.uy) * w.wy : 0.0f);
+    const float ez = dz < 0 ? (w.uz - (float)c.z) * w.wz : (dz > 0 ? ((float)(c.z + 1) - w.uz) * w.wz : 0.0f);
+    const float d2 = ey * ey + ez * ez;
+    if (d2 > h2 * 1.0001f) {
+        b = a;
+        return;
+    }
+    const float half = __builtin_sqrtf(fmaxf(h2 - d2, 0.0f)) * (w.hx * __builtin_amdgcn_rsqf(h2));
+    const float rel = w.ux - (float)(c.x - 1);
+    const int r_lo = max((int)floorf((rel - half) * (float)kSubCells) - 1, 0);
+    const int r_hi = min((int)floorf((rel + half) * (float)kSubCells) + 1, 3 * kSubCells - 1);
+    const uint32_t base = key_prefix | ((uint32_t)grid_hash(c.x - 1, c.y + dy, c.z + dz, g) << kSubBits);
+    const uint32_t na = lower_bound_u32(keys, a, b, base + (uint32_t)r_lo);
+    b = lower_bound_u32(keys, na, b, base + (uint32_t)r_hi + 1u);
+    a = na;
+}
+
+// XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
+// hardware block b runs on XCD b % 8.  Remapping gives each XCD one
+// contiguous range of tiles (neighbouring particles share neighbour rows,
+// which then hit the same L2).
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    constexpr int X = 8;
+    const int q = nb / X, r = nb % X;
+    const int x = b % X, k = b / X;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Per-particle pair-sum accumulator of force.cl:43-140 (pt.force, pt.viscosity, pt.xsph).

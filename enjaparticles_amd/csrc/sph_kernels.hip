@@ -23,20 +23,32 @@ static inline int blocks_for(long n, int bs = kBlock) { return (int)((n + bs - 1
 // ---------------------------------------------------------------------------
 // a2: hash.cl:50-157.  Optionally also clears start[0..n_clear) to kEmpty
 // (the per-step host clear of CellIndices.cpp:59-62, moved on device).
+// sub_bits > 0 (system path): the stored key is (cell << sub_bits) | xq with
+// xq the x position inside the cell in 1/2^sub_bits (see row_window).
 __global__ __launch_bounds__(kBlock) void k_hash(int num_host, const int* __restrict__ d_num, int n_pad,
                                                  uint32_t pad_key, const float4* __restrict__ pos_u,
                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ idx, Grid g,
-                                                 uint32_t* __restrict__ start_clear, int n_clear) {
+                                                 uint32_t* __restrict__ start_clear, int n_clear, int sub_bits) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_clear) start_clear[i] = kEmpty;
     const int num = d_num ? *d_num : num_host;
     if (i < num) {
         float4 p = pos_u[i];
-        int3 c = grid_cell(p.x, p.y, p.z, g);
-        int h = grid_hash(c.x, c.y, c.z, g);
+        const float ux = (p.x - g.min.x) * g.delta.x;
+        const int cx = sat_f2i(ux);
+        const int cy = sat_f2i((p.y - g.min.y) * g.delta.y);
+        const int cz = sat_f2i((p.z - g.min.z) * g.delta.z);
+        int h = grid_hash(cx, cy, cz, g);
+        const bool out = h > g.nb_cells || h < 0;
         h = h > g.nb_cells ? g.nb_cells : h;   // hash.cl:91
         h = h < 0 ? g.nb_cells : h;            // hash.cl:92
-        keys[i] = (uint32_t)h;
+        uint32_t key = (uint32_t)h;
+        if (sub_bits) {
+            int xq = sat_f2i((ux - (float)cx) * (float)(1 << sub_bits));
+            xq = out || h == g.nb_cells ? 0 : min(max(xq, 0), (1 << sub_bits) - 1);
+            key = (key << sub_bits) | (uint32_t)xq;
+        }
+        keys[i] = key;
         idx[i] = (uint32_t)i;
     } else if (i < n_pad) {
         keys[i] = pad_key;
@@ -49,16 +61,17 @@ __global__ __launch_bounds__(kBlock) void k_hash(int num_host, const int* __rest
 // last; keys > nb_cells are ignored).
 __global__ __launch_bounds__(kBlock) void k_cellindices(int num_host, const int* __restrict__ d_num,
                                                         const uint32_t* __restrict__ keys, uint32_t nb_cells,
-                                                        uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+                                                        uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                        int sub_bits) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int num = d_num ? *d_num : num_host;
     if (i >= num) return;
-    uint32_t h = keys[i];
+    uint32_t h = keys[i] >> sub_bits;
     if (h > nb_cells) return;
     if (i == 0) {
         start[h] = 0;
     } else {
-        uint32_t prev = min(keys[i - 1], nb_cells);
+        uint32_t prev = min(keys[i - 1] >> sub_bits, nb_cells);
         if (prev != h) {
             start[h] = (uint32_t)i;
             end[prev] = (uint32_t)i;
@@ -94,8 +107,8 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
                                                     const float4* __restrict__ pos_s,
                                                     const uint32_t* __restrict__ start,
                                                     const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
-                                                    float* __restrict__ dens, float4* __restrict__ packed) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                    Win win, float* __restrict__ dens, float4* __restrict__ packed) {
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
     if (d_num) effective_num(num, start, g.nb_cells, num);
     if (i >= num) return;
@@ -104,11 +117,12 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
     float4 pi = pos_s[i];
     if (SCALED) { pi.x *= scale; pi.y *= scale; pi.z *= scale; }
     const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
+    const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
     float sum = 0.0f;
-    for (int z = c.z - 1; z <= c.z + 1; ++z) {
-        for (int y = c.y - 1; y <= c.y + 1; ++y) {
+    for (int dz = -1; dz <= 1; ++dz) {
+        for (int dy = -1; dy <= 1; ++dy) {
             uint32_t a, b;
-            row_range(c.x, y, z, g, start, end, a, b);
+            row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
             for (uint32_t j = a; j < b; ++j) {
                 float4 pj = pos_s[j];
                 if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
@@ -146,13 +160,14 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
                                                   const float4* __restrict__ ve_in,
                                                   const uint32_t* __restrict__ start,
                                                   const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
-                                                  float4* __restrict__ force_out, float4* __restrict__ xsph_out,
+                                                  Win win, float4* __restrict__ force_out,
+                                                  float4* __restrict__ xsph_out,
                                                   // FUSED only:
                                                   const float4* __restrict__ vel_s, const uint32_t* __restrict__ idx,
                                                   GravitySources gsrc, int collision, rtps_float4 bmin,
                                                   rtps_float4 bmax, float dt, float4* __restrict__ pos_u,
                                                   float4* __restrict__ vel_u, float4* __restrict__ veleval_u) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
     bool deleted = false;
     if (d_num) deleted = effective_num(num, start, g.nb_cells, num);
@@ -168,6 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
     if (PACKED) { vi = ve_in[i]; di = vi.w; } else { vi = ve_in[i]; di = dens_in[i]; }
     const float Pi = K * (di - rho0);
     const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
+    const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
     ForceAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 
     auto pair = [&](uint32_t j, float rx, float ry, float rz, float r2) {
@@ -183,10 +199,10 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
     };
 
     if constexpr (!COMPACT) {
-        for (int z = c.z - 1; z <= c.z + 1; ++z) {
-            for (int y = c.y - 1; y <= c.y + 1; ++y) {
+        for (int dz = -1; dz <= 1; ++dz) {
+            for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
-                row_range(c.x, y, z, g, start, end, a, b);
+                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
                 for (uint32_t j = a; j < b; ++j) {
                     float rx, ry, rz;
                     float r2 = sep(j, rx, ry, rz);
@@ -209,10 +225,10 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
             }
             cnt = 0;
         };
-        for (int z = c.z - 1; z <= c.z + 1; ++z) {
-            for (int y = c.y - 1; y <= c.y + 1; ++y) {
+        for (int dz = -1; dz <= 1; ++dz) {
+            for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
-                row_range(c.x, y, z, g, start, end, a, b);
+                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
                 for (uint32_t c0 = a; __ballot(c0 < b) != 0ull; c0 += kChunk) {
                     if (__ballot(cnt > kNbrCap - kChunk) != 0ull) drain();
                     const uint32_t c1 = min(c0 + (uint32_t)kChunk, b);
@@ -288,27 +304,31 @@ __global__ __launch_bounds__(kBlock) void k_leapfrog(int num_host, const int* __
     veleval_u[o] = eo;
 }
 
-// Candidate (K) and accepted (M, self included) pair totals of the
-// neighbour loop on the current sorted state, for the FLOP model.
+// Pair totals of the neighbour loop on the current sorted state: out[0] =
+// candidates of the reference's 27-cell loop (K), out[1] = accepted pairs
+// (M, self included), out[2] = candidates the windowed rows actually test.
 __global__ __launch_bounds__(kBlock) void k_count_pairs(int num_host, const int* __restrict__ d_num,
                                                         const float4* __restrict__ pos_s,
                                                         const uint32_t* __restrict__ start,
                                                         const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
-                                                        unsigned long long* __restrict__ out) {
+                                                        Win win, unsigned long long* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
     if (d_num) effective_num(num, start, g.nb_cells, num);
-    unsigned long long k = 0, m = 0;
+    unsigned long long k = 0, m = 0, kw = 0;
     if (i < num) {
         const float scale = sp.simulation_scale, h = sp.smoothing_distance, h2 = h * h;
         float4 pi = pos_s[i];
         pi.x *= scale; pi.y *= scale; pi.z *= scale;
         const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
-        for (int z = c.z - 1; z <= c.z + 1; ++z)
-            for (int y = c.y - 1; y <= c.y + 1; ++y) {
-                uint32_t a, b;
-                row_range(c.x, y, z, g, start, end, a, b);
+        const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy) {
+                uint32_t a, b, wa, wb;
+                row_range(c.x, c.y + dy, c.z + dz, g, start, end, a, b);
+                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, wa, wb);
                 k += b - a;
+                kw += wb - wa;
                 for (uint32_t j = a; j < b; ++j) {
                     float4 pj = pos_s[j];
                     float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
@@ -319,19 +339,21 @@ __global__ __launch_bounds__(kBlock) void k_count_pairs(int num_host, const int*
     for (int o = 32; o > 0; o >>= 1) {
         k += __shfl_down(k, o, 64);
         m += __shfl_down(m, o, 64);
+        kw += __shfl_down(kw, o, 64);
     }
-    if ((threadIdx.x & 63) == 0 && (k | m)) {
+    if ((threadIdx.x & 63) == 0 && (k | m | kw)) {
         atomicAdd(&out[0], k);
         atomicAdd(&out[1], m);
+        atomicAdd(&out[2], kw);
     }
 }
 
 hipError_t launch_count_pairs(int num, const int* d_num, const float4* pos_s, const uint32_t* start,
-                              const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
+                              const uint32_t* end, const rtps_sph_params& sp, const Grid& g, const Win& win,
                               unsigned long long* out, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
-    k_count_pairs<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, out);
+    k_count_pairs<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, out);
     return hipGetLastError();
 }
 
@@ -348,22 +370,24 @@ __global__ void k_commit_num(int* __restrict__ d_num, const uint32_t* __restrict
 // Launchers shared by the C ABI and the system.
 
 // Force-loop variant: 0 = one branch per candidate, 1 = compacted LDS lists.
-int g_force_variant = 1;
+int g_force_variant = 0;
 
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
-                       uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, hipStream_t st) {
+                       uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, int sub_bits,
+                       hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     const int n = n_pad > n_clear ? n_pad : n_clear;
     if (n <= 0) return hipSuccess;
-    k_hash<<<blocks_for(n), kBlock, 0, st>>>(num, d_num, n_pad, pad_key, pos_u, keys, idx, g, start_clear, n_clear);
+    k_hash<<<blocks_for(n), kBlock, 0, st>>>(num, d_num, n_pad, pad_key, pos_u, keys, idx, g, start_clear, n_clear,
+                                             sub_bits);
     return hipGetLastError();
 }
 
 hipError_t launch_cellindices(int num, const int* d_num, const uint32_t* keys, int nb_cells, uint32_t* start,
-                              uint32_t* end, hipStream_t st) {
+                              uint32_t* end, int sub_bits, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
-    k_cellindices<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, keys, (uint32_t)nb_cells, start, end);
+    k_cellindices<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, keys, (uint32_t)nb_cells, start, end, sub_bits);
     return hipGetLastError();
 }
 
@@ -384,24 +408,25 @@ hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const 
 }
 
 hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
-                          const rtps_sph_params& sp, const Grid& g, float* dens, float4* packed, hipStream_t st) {
+                          const rtps_sph_params& sp, const Grid& g, const Win& win, float* dens, float4* packed,
+                          hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
     const int nb = blocks_for(num);
     if (packed) {
-        if (scaled) k_density<true, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, dens, packed);
-        else k_density<false, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, dens, packed);
+        if (scaled) k_density<true, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
+        else k_density<false, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
     } else {
-        if (scaled) k_density<true, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, dens, packed);
-        else k_density<false, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, dens, packed);
+        if (scaled) k_density<true, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
+        else k_density<false, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const float* dens, const float4* ve,
                         bool packed, const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp,
-                        const Grid& g, float4* force, float4* xsph, hipStream_t st) {
+                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
@@ -410,11 +435,11 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
     rtps_float4 z{0, 0, 0, 0};
 #define RTPS_FORCE(S, P)                                                                                            \
     if (g_force_variant)                                                                                            \
-        k_force<S, P, false, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, force,   \
+        k_force<S, P, false, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force,   \
                                                           xsph, nullptr, nullptr, none, 0, z, z, 0.f, nullptr,     \
                                                           nullptr, nullptr);                                       \
     else                                                                                                            \
-        k_force<S, P, false, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, force,  \
+        k_force<S, P, false, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force,  \
                                                            xsph, nullptr, nullptr, none, 0, z, z, 0.f, nullptr,    \
                                                            nullptr, nullptr)
     if (packed) { if (scaled) RTPS_FORCE(true, true); else RTPS_FORCE(false, true); }
@@ -425,7 +450,7 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
 
 hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, const float4* ve_packed,
                               const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
-                              float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
+                              const Win& win, float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
                               const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,
                               float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
@@ -433,7 +458,7 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
     const bool scaled = sp.simulation_scale != 1.0f;
     const int nb = blocks_for(num);
 #define RTPS_FFUSED(S, C)                                                                                           \
-    k_force<S, true, true, C><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, force, \
+    k_force<S, true, true, C><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
                                                      xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt, \
                                                      pos_u, vel_u, veleval_u)
     if (scaled) { if (g_force_variant) RTPS_FFUSED(true, true); else RTPS_FFUSED(true, false); }

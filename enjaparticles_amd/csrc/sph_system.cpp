@@ -66,6 +66,10 @@ struct rtps_sph {
 
 namespace {
 
+Win win(const rtps_sph* s) {
+    return Win{s->keys_sorted, s->gps.grid_res.x, s->gps.grid_res.y, s->gps.grid_res.z};
+}
+
 template <typename T>
 hipError_t dalloc(T** p, size_t count) {
     return hipMalloc((void**)p, count * sizeof(T) + 16);
@@ -122,17 +126,18 @@ int launch_front(rtps_sph* s, StageTimer& t) {
     const int n = s->num_host;
     t.mark(ST_HASH);
     RTPS_TRY_HIP(launch_hash(n, s->d_num, n, s->pad_key, s->pos_u, s->keys, s->idx, s->g, s->start,
-                             s->nb_cells + 1, s->stream));
+                             s->nb_cells + 1, kSubBits, s->stream));
     t.mark(ST_SORT);
     RTPS_TRY_HIP(sort_pairs_pingpong(n, s->key_bits, s->keys, s->idx, s->sort_tmp, s->stream, &s->keys_sorted,
                                      &s->idx_sorted));
     t.mark(ST_CELL);
-    RTPS_TRY_HIP(launch_cellindices(n, s->d_num, s->keys_sorted, s->nb_cells, s->start, s->end, s->stream));
+    RTPS_TRY_HIP(launch_cellindices(n, s->d_num, s->keys_sorted, s->nb_cells, s->start, s->end, kSubBits, s->stream));
     t.mark(ST_PERMUTE);
     RTPS_TRY_HIP(launch_permute(n, s->d_num, s->idx_sorted, s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s,
                                 s->ve_s, s->stream));
     t.mark(ST_DENSITY);
-    RTPS_TRY_HIP(launch_density(n, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, nullptr, s->ve_s, s->stream));
+    RTPS_TRY_HIP(launch_density(n, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, win(s), nullptr, s->ve_s,
+                                s->stream));
     return RTPS_OK;
 }
 
@@ -142,7 +147,7 @@ int launch_step(rtps_sph* s, StageTimer& t) {
     if (rc) return rc;
     t.mark(ST_FORCE);
     const bool fo = s->opt_force_out != 0;
-    RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs,
+    RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, win(s),
                                     fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s, s->idx_sorted,
                                     s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u, s->veleval_u,
                                     s->stream));
@@ -223,7 +228,7 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
         s->g = make_grid(s->gp);
         s->gs = make_grid(s->gps);
         s->nb_cells = s->gp.nb_cells;
-        s->key_bits = sort_key_bits(s->nb_cells);
+        s->key_bits = sort_key_bits(s->nb_cells) + kSubBits;
         s->pad_key = (s->key_bits >= 32) ? 0xFFFFFFFFu : ((1u << s->key_bits) - 1u);
         apply_settings(s.get());
 
@@ -306,7 +311,7 @@ int rtps_sph_update(rtps_sph* s) {
     int rc = launch_front(s, t);
     if (rc) return rc;
     RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp, s->gs,
-                              s->force_s, s->xsph_s, s->stream));
+                              win(s), s->force_s, s->xsph_s, s->stream));
     RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                    s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
     s->update_pending = true;
@@ -374,6 +379,10 @@ int rtps_sph_read(rtps_sph* s, int field, void* host, int count) {
     if (field == RTPS_FIELD_KEYS || field == RTPS_FIELD_IDX) {
         RTPS_TRY_HIP(hipMemcpy(host, field == RTPS_FIELD_KEYS ? s->keys_sorted : s->idx_sorted, (size_t)count * 4,
                                hipMemcpyDeviceToHost));
+        if (field == RTPS_FIELD_KEYS) {   // internal keys carry kSubBits of in-cell x; return the cell keys
+            uint32_t* k = (uint32_t*)host;
+            for (int i = 0; i < count; ++i) k[i] >>= kSubBits;
+        }
         return RTPS_OK;
     }
     if (field == RTPS_FIELD_DENSITY) {
@@ -470,20 +479,21 @@ int rtps_sph_reset_timers(rtps_sph* s) {
     return RTPS_OK;
 }
 
-int rtps_sph_count_pairs(rtps_sph* s, long long* K, long long* M) {
+int rtps_sph_count_pairs(rtps_sph* s, long long* K, long long* M, long long* K_tested) {
     if (!s || !K || !M) return set_error(RTPS_ERR_ARG, "rtps_sph_count_pairs: null argument");
     unsigned long long* d = nullptr;
-    RTPS_TRY_HIP(hipMalloc((void**)&d, 2 * sizeof(unsigned long long)));
-    hipError_t e = hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), s->stream);
+    RTPS_TRY_HIP(hipMalloc((void**)&d, 3 * sizeof(unsigned long long)));
+    hipError_t e = hipMemsetAsync(d, 0, 3 * sizeof(unsigned long long), s->stream);
     if (e == hipSuccess)
-        e = launch_count_pairs(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, d, s->stream);
-    unsigned long long h[2] = {0, 0};
+        e = launch_count_pairs(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, win(s), d, s->stream);
+    unsigned long long h[3] = {0, 0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return hip_error(e, "rtps_sph_count_pairs");
     *K = (long long)h[0];
     *M = (long long)h[1];
+    if (K_tested) *K_tested = (long long)h[2];
     return RTPS_OK;
 }
 

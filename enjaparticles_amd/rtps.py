@@ -187,10 +187,11 @@ class SPH:
         return p.value
 
     def countPairs(self):
-        """(K, M): candidate and accepted (self included) neighbour pairs of the last step."""
-        K, M = C.c_longlong(), C.c_longlong()
-        N.check(N.lib().rtps_sph_count_pairs(self._h, C.byref(K), C.byref(M)))
-        return K.value, M.value
+        """(K, M, K_tested): 27-cell candidates, accepted pairs (self included) and
+        the candidates the windowed rows test, on the last step's state."""
+        K, M, Kt = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        N.check(N.lib().rtps_sph_count_pairs(self._h, C.byref(K), C.byref(M), C.byref(Kt)))
+        return K.value, M.value, Kt.value
 
     def stream(self) -> int:
         p = C.c_void_p()
@@ -233,3 +234,90 @@ class RTPS:
         if s.GetSettingAs(str, "system") != "sph":
             return None
         return SPH(s)
+
+
+class BatchedSPH:
+    """Many independent SPH scenes stepped together (rtps_batch in include/rtps_c.h).
+
+    ``scenes`` is a list of settings dicts (one per scene); particles are pushed
+    per scene; ``step(n)`` advances every scene n steps with one launch per
+    stage.  Each scene computes what an ``SPH`` system with its settings does.
+    """
+
+    def __init__(self, scene_settings: list):
+        flat_k, flat_v, counts = [], [], []
+        for st in scene_settings:
+            st = st.as_dict() if isinstance(st, RTPSSettings) else dict(st)
+            counts.append(len(st))
+            flat_k += [str(k).encode() for k in st]
+            flat_v += [str(v).encode() for v in st.values()]
+        ka = (C.c_char_p * max(1, len(flat_k)))(*flat_k)
+        va = (C.c_char_p * max(1, len(flat_v)))(*flat_v)
+        ca = (C.c_int * len(counts))(*counts)
+        h = C.c_void_p()
+        N.check(N.lib().rtps_batch_create(len(counts), ca, C.cast(ka, C.c_void_p), C.cast(va, C.c_void_p),
+                                          C.byref(h)))
+        self._h = h
+        self.caps = [int(dict(st.as_dict() if isinstance(st, RTPSSettings) else st)["max_num_particles"])
+                     for st in scene_settings]
+        self.num_scenes = len(counts)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().rtps_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pushParticles(self, scene: int, pos, vel=None):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+        vptr = None
+        if vel is not None:
+            vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 4)
+            vptr = vel.ctypes.data
+        N.check(N.lib().rtps_batch_push_particles(self._h, scene, pos.ctypes.data, vptr, pos.shape[0]))
+
+    def step(self, nsteps: int = 1):
+        N.check(N.lib().rtps_batch_step(self._h, int(nsteps)))
+
+    def synchronize(self):
+        N.check(N.lib().rtps_batch_synchronize(self._h))
+
+    def getNum(self, scene: int) -> int:
+        v = C.c_int()
+        N.check(N.lib().rtps_batch_get_num(self._h, scene, C.byref(v)))
+        return v.value
+
+    def totalSlots(self) -> int:
+        t = C.c_int()
+        N.check(N.lib().rtps_batch_num_scenes(self._h, None, C.byref(t)))
+        return t.value
+
+    def setOption(self, option: int, value: int):
+        N.check(N.lib().rtps_batch_set_option(self._h, option, int(value)))
+
+    def read(self, scene: int, field: str, count: int | None = None) -> np.ndarray:
+        if count is None:
+            count = self.getNum(scene)
+        if field in ("keys", "idx"):
+            out = np.empty(count, np.uint32)
+        elif field == "density":
+            out = np.empty(count, np.float32)
+        else:
+            out = np.empty((count, 4), np.float32)
+        if count:
+            N.check(N.lib().rtps_batch_read(self._h, scene, N.FIELD[field], out.ctypes.data, count))
+        return out
+
+    def stageTimes(self) -> dict:
+        ms = (C.c_float * N.NUM_STAGES)()
+        cnt = (C.c_int * N.NUM_STAGES)()
+        N.check(N.lib().rtps_batch_stage_times(self._h, ms, cnt, N.NUM_STAGES))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(N.STAGE_NAMES)}
+
+    def resetTimers(self):
+        N.check(N.lib().rtps_batch_reset_timers(self._h))

@@ -178,7 +178,7 @@ enum rtps_field {
     RTPS_FIELD_DENSITY = 6,    /* float, sorted (getDensityBuffer) */
     RTPS_FIELD_FORCE = 7,      /* float4, sorted (when force output is enabled) */
     RTPS_FIELD_XSPH = 8,       /* float4, sorted (when force output is enabled) */
-    RTPS_FIELD_KEYS = 9,       /* uint32, sorted cell keys */
+    RTPS_FIELD_KEYS = 9,       /* uint32, sorted cell keys (device pointer: cell << 4 | in-cell x sixteenth) */
     RTPS_FIELD_IDX = 10,       /* uint32, sorted -> original index */
     RTPS_FIELD_CELL_START = 11, /* uint32[nb_cells+1] (getCellStartBuffer) */
     RTPS_FIELD_CELL_END = 12   /* uint32[nb_cells+1] (getCellEndBuffer) */
@@ -201,8 +201,8 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *   RTPS_OPT_GRAPH       1 = capture rtps_sph_step's inner loop in a hipGraph (default 1)
  *   RTPS_OPT_FORCE_OUT   1 = also store force/xsph per step (default 0)
  *   RTPS_OPT_TIMERS      1 = per-stage hipEvent timers (forces eager launches)
- *   RTPS_OPT_FORCE_VARIANT  force-loop kernel, process-wide: 1 = compacted
- *                        neighbour lists in LDS (default), 0 = one branch per candidate */
+ *   RTPS_OPT_FORCE_VARIANT  force-loop kernel, process-wide: 0 = one branch per
+ *                        candidate (default), 1 = compacted neighbour lists in LDS */
 #define RTPS_OPT_COLLISION 0
 #define RTPS_OPT_GRAPH 1
 #define RTPS_OPT_FORCE_OUT 2
@@ -217,12 +217,39 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value);
 int rtps_sph_stage_times(rtps_sph* s, float* avg_ms, int* counts, int n);
 int rtps_sph_reset_timers(rtps_sph* s);
 
-/* Candidate (K) and accepted (M, self included) pair totals of the 27-cell
- * neighbour loop on the last step's sorted state (the FLOP model's inputs). */
-int rtps_sph_count_pairs(rtps_sph* s, long long* K, long long* M);
+/* Pair totals on the last step's sorted state: K = candidates of the 27-cell
+ * neighbour loop, M = accepted pairs (self included) -- the FLOP model's
+ * inputs -- and K_tested (may be NULL) = candidates the windowed rows test. */
+int rtps_sph_count_pairs(rtps_sph* s, long long* K, long long* M, long long* K_tested);
 
 /* The handle's stream (hipStream_t as void*). */
 int rtps_sph_stream(rtps_sph* s, void** stream);
+
+/* ------------------------------------------------------------------------ */
+/* a14. Batched scenes (no reference counterpart): num_scenes independent SPH
+ * systems (<= 256), each with its own settings, stepped together with one
+ * launch per stage (segmented keys (scene << 24) | cell, persistent
+ * density/force kernels over a queue of (scene, tile) items) and one hipGraph
+ * per step.  Settings of scene s are the n_settings[s] key/value pairs that
+ * follow those of scene s-1 in keys/vals.  Every scene computes what an
+ * rtps_sph system with the same settings and particles computes. */
+typedef struct rtps_batch rtps_batch;
+int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* keys, const char* const* vals,
+                      rtps_batch** out);
+int rtps_batch_destroy(rtps_batch* b);
+int rtps_batch_num_scenes(rtps_batch* b, int* num_scenes, int* total_slots);
+int rtps_batch_push_particles(rtps_batch* b, int scene, const rtps_float4* pos, const rtps_float4* vel, int n);
+int rtps_batch_step(rtps_batch* b, int nsteps);
+int rtps_batch_synchronize(rtps_batch* b);
+int rtps_batch_get_num(rtps_batch* b, int scene, int* num);
+/* Fields as for rtps_sph_read, per scene: unsorted fields in the scene's own
+ * index order; KEYS are the segmented keys (scene << 24) | cell; IDX are
+ * scene-local original indices. */
+int rtps_batch_read(rtps_batch* b, int scene, int field, void* host, int count);
+/* RTPS_OPT_GRAPH, RTPS_OPT_TIMERS, RTPS_OPT_COLLISION (all scenes). */
+int rtps_batch_set_option(rtps_batch* b, int option, int value);
+int rtps_batch_stage_times(rtps_batch* b, float* avg_ms, int* counts, int n);
+int rtps_batch_reset_timers(rtps_batch* b);
 
 #ifdef __cplusplus
 }

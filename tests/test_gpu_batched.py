@@ -1,0 +1,90 @@
+"""Batched scenes (rtps_batch_*): every scene of a batch must compute what a
+plain single-scene system computes from the same settings and particles
+(segmented keys bit-exact, positions/densities within 1e-6 relative), and
+match the CPU oracle at the plain system's 1e-4 bar."""
+import numpy as np
+import pytest
+
+import oracle as O
+from enjaparticles_amd import scenes
+from enjaparticles_amd._native import RTPSError
+from enjaparticles_amd.rtps import SPH, BatchedSPH
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(scs):
+    b = BatchedSPH([sc.settings for sc in scs])
+    for s, sc in enumerate(scs):
+        b.pushParticles(s, sc.pos, sc.vel)
+    return b
+
+
+def test_each_scene_equals_plain_system(cuda):
+    scs = scenes.batched(6, 600, 6000, seed=77)
+    b = _batch(scs)
+    b.step(5)
+    for s, sc in enumerate(scs):
+        p = SPH(sc.settings)
+        p.pushParticles(sc.pos, sc.vel)
+        p.step(5)
+        assert b.getNum(s) == p.getNum() == sc.n
+        kb = b.read(s, "keys")
+        kp = p.read("keys")
+        np.testing.assert_array_equal(kb, (np.uint32(s) << np.uint32(24)) | kp)
+        np.testing.assert_array_equal(b.read(s, "idx"), p.read("idx"))
+        np.testing.assert_allclose(b.read(s, "pos")[:, :3], p.read("pos")[:, :3], rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(b.read(s, "density"), p.read("density"), rtol=1e-6)
+        p.close()
+    b.close()
+
+
+def test_scene_matches_oracle(cuda):
+    scs = scenes.batched(3, 2000, 4000, seed=5)
+    b = _batch(scs)
+    for _ in range(5):
+        b.step(1)
+    for s, sc in enumerate(scs):
+        sp, gp, gs = O.derive_params(sc.settings)
+        st = O.State(sc.pos, sc.vel, sp, gp, gs)
+        for _ in range(5):
+            st.step(sc.dt, O.FLAG_COLLISION)
+        np.testing.assert_allclose(b.read(s, "pos")[:, :3], st.pos_u[:, :3], rtol=1e-4)
+        # densities are stored in sorted order; compare them by original index
+        # (the within-cell order of the two sorts differs)
+        d_gpu = np.empty(sc.n, np.float32)
+        d_gpu[b.read(s, "idx")] = b.read(s, "density")
+        d_ref = np.empty(sc.n, np.float32)
+        d_ref[st.idx[: st.num]] = st.density[: st.num]
+        np.testing.assert_allclose(d_gpu, d_ref, rtol=1e-4)
+    b.close()
+
+
+def test_graph_equals_eager_and_deletion(cuda):
+    scs = scenes.batched(4, 800, 3000, seed=9)
+    pos = scs[2].pos.copy()
+    pos[::50, 2] = 60.0                       # out of scene 2's grid: deleted
+    scs[2] = scenes.Scene(scs[2].name, scs[2].settings, pos, scs[2].vel, 3, scs[2].dt)
+    a, e = _batch(scs), _batch(scs)
+    e.setOption(1, 0)                         # eager launches
+    a.step(3)
+    e.step(3)
+    for s in range(4):
+        assert a.getNum(s) == e.getNum(s)
+        np.testing.assert_array_equal(a.read(s, "pos"), e.read(s, "pos"))
+    assert a.getNum(2) == len(pos) - len(pos[::50])
+    assert a.getNum(1) == scs[1].n
+    a.close()
+    e.close()
+
+
+def test_capacity_and_arguments(cuda):
+    scs = scenes.batched(2, 500, 600, seed=1)
+    b = _batch(scs)
+    with pytest.raises(RTPSError):
+        b.pushParticles(0, scs[0].pos[:1])      # scene full
+    with pytest.raises(RTPSError):
+        b.pushParticles(5, scs[0].pos[:1])      # no such scene
+    b.close()
+    with pytest.raises(RTPSError):
+        BatchedSPH([{"system": "flock", "max_num_particles": "10"}])

@@ -45,12 +45,17 @@ def test_ten_steps_match_fp32_oracle(cuda):
         s.step(1)
         st.step(sc.dt, O.FLAG_COLLISION)
         pos, vel = s.read("pos"), s.read("vel")
-        dens = s.read("density")
+        # densities are in sorted order and the within-cell order of the two
+        # sorts differs: compare by original index
+        dens = np.empty(sc.n, np.float32)
+        dens[s.read("idx")] = s.read("density")
+        ref = np.empty(sc.n, np.float32)
+        ref[st.idx[: st.num]] = st.density[: st.num]
         # per-element 1e-4 relative on positions; velocities and densities
         # max-norm 1e-4 relative of their fields
         np.testing.assert_allclose(pos[:, :3], st.pos_u[:, :3], rtol=1e-4, atol=0)
         assert _rel_max(vel[:, :3], st.vel_u[:, :3]) <= 1e-4, k
-        np.testing.assert_allclose(dens, st.density[: st.num], rtol=1e-4)
+        np.testing.assert_allclose(dens, ref, rtol=1e-4)
     assert s.getNum() == st.num == sc.n
     s.close()
 
@@ -110,7 +115,15 @@ def test_deletion_path_matches_oracle(cuda):
         st.step(sc.dt, O.FLAG_COLLISION)
     n = s.getNum()
     assert n == st.num == sc.n - len(pos[5::211])
-    np.testing.assert_allclose(s.read("pos")[:, :3], st.pos_u[:n, :3], rtol=1e-4)
+    # The deletion path renumbers the survivors in sorted order, and the
+    # within-cell order of the two sorts differs: match particles by position
+    # (a bijection, every pair within 1e-5).
+    from scipy.spatial import cKDTree
+    a = s.read("pos")[:, :3].astype(np.float64)
+    b = st.pos_u[:n, :3].astype(np.float64)
+    d, j = cKDTree(b).query(a)
+    assert d.max() <= 1e-5
+    assert len(np.unique(j)) == n
     s.close()
 
 
