@@ -104,6 +104,10 @@ SIGNATURES = {
     "rtps_batch_set_option": (_i, [P, _i, _i]),
     "rtps_batch_stage_times": (_i, [P, P, P, _i]),
     "rtps_batch_reset_timers": (_i, [P]),
+    "rtps_adj_create": (_i, [P, P, _i, _i, _i, _i, P]),
+    "rtps_adj_destroy": (_i, [P]),
+    "rtps_adj_forward": (_i, [P, P, P, _i, _f, _f, _i, P, P, P]),
+    "rtps_adj_backward": (_i, [P, P, P, P, P, P, P, P]),
 }
 
 

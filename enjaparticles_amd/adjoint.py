@@ -1,0 +1,94 @@
+"""Differentiable SPH rollout: torch.autograd.Function over the hand-written
+adjoint of include/rtps_c.h (rtps_adj_*).  torch provides the device memory
+and the stream; forward and backward are HIP kernels.
+
+    sim = AdjointSPH(settings, num_particles, checkpoint_every=20, max_steps=200)
+    pos_T = sph_rollout(sim, pos0, vel0, viscosity, gas_constant, nsteps)
+    loss = torch.nn.functional.mse_loss(pos_T[:, :3], target[:, :3])
+    loss.backward()   # -> vel0.grad, viscosity.grad, gas_constant.grad
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _native as N
+
+
+class AdjointSPH:
+    """Owns an rtps_adj handle (checkpoints and per-segment records)."""
+
+    def __init__(self, settings: dict, num_particles: int, checkpoint_every: int = 20, max_steps: int = 200):
+        ka, va, n = N.settings_arrays(settings)
+        h = C.c_void_p()
+        N.check(N.lib().rtps_adj_create(C.cast(ka, C.c_void_p), C.cast(va, C.c_void_p), n, num_particles,
+                                        checkpoint_every, max_steps, C.byref(h)))
+        self._h = h
+        self.num = num_particles
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().rtps_adj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, t: torch.Tensor, name: str):
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and tuple(t.shape) == (self.num, 4)):
+            raise ValueError(f"{name} must be a contiguous float32 CUDA tensor of shape ({self.num}, 4)")
+
+    def forward(self, pos0, vel0, nsteps: int, viscosity: float, gas_constant: float, collision: bool = True):
+        self._check(pos0, "pos0")
+        self._check(vel0, "vel0")
+        pos_t = torch.empty_like(pos0)
+        vel_t = torch.empty_like(pos0)
+        st = torch.cuda.current_stream().cuda_stream
+        N.check(N.lib().rtps_adj_forward(self._h, pos0.data_ptr(), vel0.data_ptr(), int(nsteps), float(viscosity),
+                                         float(gas_constant), int(collision), pos_t.data_ptr(), vel_t.data_ptr(),
+                                         st))
+        return pos_t, vel_t
+
+    def backward(self, grad_pos_t, grad_vel_t=None):
+        self._check(grad_pos_t, "grad_pos_T")
+        dv0 = torch.empty_like(grad_pos_t)
+        dx0 = torch.empty_like(grad_pos_t)
+        dmu, dK = C.c_double(), C.c_double()
+        st = torch.cuda.current_stream().cuda_stream
+        gv = None
+        if grad_vel_t is not None:
+            self._check(grad_vel_t, "grad_vel_T")
+            gv = grad_vel_t.data_ptr()
+        N.check(N.lib().rtps_adj_backward(self._h, grad_pos_t.data_ptr(), gv, dv0.data_ptr(), dx0.data_ptr(),
+                                          C.byref(dmu), C.byref(dK), st))
+        return dv0, dx0, dmu.value, dK.value
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vel0, viscosity, gas_constant, sim, pos0, nsteps, collision):
+        pos_t, _ = sim.forward(pos0.detach().contiguous(), vel0.detach().contiguous(), nsteps,
+                               float(viscosity), float(gas_constant), collision)
+        ctx.sim = sim
+        ctx.save_for_backward(viscosity, gas_constant)
+        return pos_t
+
+    @staticmethod
+    def backward(ctx, grad_pos_t):
+        visc, gas = ctx.saved_tensors
+        g = grad_pos_t.detach().contiguous().clone()
+        g[:, 3] = 0.0
+        dv0, _dx0, dmu, dK = ctx.sim.backward(g)
+        dv0[:, 3] = 0.0
+        return (dv0, torch.tensor(dmu, dtype=visc.dtype, device=visc.device),
+                torch.tensor(dK, dtype=gas.dtype, device=gas.device), None, None, None, None)
+
+
+def sph_rollout(sim: AdjointSPH, pos0: torch.Tensor, vel0: torch.Tensor, viscosity: torch.Tensor,
+                gas_constant: torch.Tensor, nsteps: int, collision: bool = True) -> torch.Tensor:
+    """Final positions after nsteps, differentiable in vel0, viscosity and gas_constant."""
+    return _Rollout.apply(vel0, viscosity, gas_constant, sim, pos0, int(nsteps), bool(collision))

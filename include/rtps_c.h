@@ -251,6 +251,28 @@ int rtps_batch_set_option(rtps_batch* b, int option, int value);
 int rtps_batch_stage_times(rtps_batch* b, float* avg_ms, int* counts, int n);
 int rtps_batch_reset_timers(rtps_batch* b);
 
+/* ------------------------------------------------------------------------ */
+/* a15. Adjoint (no reference counterpart): gradients of a loss on the final
+ * state with respect to the initial velocities, the viscosity and the gas
+ * constant, through `nsteps` steps of the SPH system (density, force/XSPH,
+ * walls, leapfrog; neighbour pairs of each step held fixed; simulation_scale
+ * 1, no point sources, no deletion).  The forward keeps the state every
+ * checkpoint_every steps; the backward replays each segment from its
+ * checkpoint and runs the hand-written adjoint kernels through it.
+ * All arrays are device float4[num] in original particle order; the calls
+ * run on `stream`. */
+typedef struct rtps_adj rtps_adj;
+int rtps_adj_create(const char* const* keys, const char* const* vals, int n, int num_particles,
+                    int checkpoint_every, int max_steps, rtps_adj** out);
+int rtps_adj_destroy(rtps_adj* a);
+/* RTPS_ERR_STATE if particles left the grid (the adjoint has no deletion path). */
+int rtps_adj_forward(rtps_adj* a, const rtps_float4* pos0, const rtps_float4* vel0, int nsteps, float viscosity,
+                     float gas_constant, int collision, rtps_float4* pos_T, rtps_float4* vel_T /* may be NULL */,
+                     void* stream);
+int rtps_adj_backward(rtps_adj* a, const rtps_float4* dL_dpos_T, const rtps_float4* dL_dvel_T /* may be NULL */,
+                      rtps_float4* dL_dvel0, rtps_float4* dL_dpos0 /* may be NULL */, double* dL_dviscosity,
+                      double* dL_dgas_constant, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
