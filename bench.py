@@ -96,17 +96,65 @@ def batched_leg(steps: int, num_scenes: int = 256, seed: int = 1003):
             "ms_per_step": 1e3 * tb / steps, "stages_ms": stages}
 
 
+def adjoint_leg(steps: int = 200, checkpoint_every: int = 20):
+    """Forward + adjoint of the 65,536-particle rollout (200 steps, dt 0.002,
+    checkpoints every 20 steps): the forward keeps only checkpoints, the
+    backward replays each segment and runs the adjoint steps.  Also times the
+    same scene as a plain system (graph replay) for the cost ratio."""
+    import numpy as np
+    import torch
+    from enjaparticles_amd import scenes
+    from enjaparticles_amd.adjoint import AdjointSPH
+    from enjaparticles_amd.rtps import SPH
+    sc = scenes.adjoint64k()
+    dev = torch.device("cuda:0")
+    sim = AdjointSPH(sc.settings, sc.n, checkpoint_every=checkpoint_every, max_steps=steps)
+    p0 = torch.from_numpy(sc.pos).to(dev)
+    v0 = torch.from_numpy(sc.vel).to(dev)
+    g = torch.from_numpy(np.random.default_rng(5).normal(size=sc.pos.shape).astype(np.float32)).to(dev)
+    g[:, 3] = 0.0
+    mu, gas = 0.01, 3.0
+    sim.forward(p0, v0, steps, mu, gas)          # untimed first pass
+    sim.backward(g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.forward(p0, v0, steps, mu, gas)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sim.backward(g)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    sim.close()
+    p = SPH(sc.settings)
+    p.pushParticles(sc.pos, sc.vel)
+    p.step(2)
+    p.synchronize()
+    t3 = time.perf_counter()
+    p.step(steps)
+    p.synchronize()
+    t4 = time.perf_counter()
+    p.close()
+    tf, tb, tp = t1 - t0, t2 - t1, t4 - t3
+    return {"particles": sc.n, "steps": steps, "checkpoint_every": checkpoint_every,
+            "value": sc.n * steps / (tf + tb), "unit": "particle-updates/s (forward + adjoint)",
+            "forward_ms_per_step": 1e3 * tf / steps, "adjoint_ms_per_step": 1e3 * tb / steps,
+            "plain_ms_per_step": 1e3 * tp / steps, "adjoint_over_forward": tb / tf,
+            "adjoint_over_plain": tb / tp}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--timer-steps", type=int, default=20, help="eager steps with per-stage events")
+    ap.add_argument("--chunk", type=int, default=100, help="timed steps between pair counts")
+    ap.add_argument("--no-stage-timers", action="store_true", help="time the graph without per-stage events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--profile", action="store_true", help="timed loop only (for rocprofv3 runs)")
     ap.add_argument("--force-variant", type=int, default=0, help="1 = compacted neighbour lists, 0 = direct")
     ap.add_argument("--batched-steps", type=int, default=30, help="steps of the batched leg (0 = skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=200, help="steps of the fwd+adjoint leg (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -126,18 +174,30 @@ def main():
     s.pushParticles(sc.pos, sc.vel)
     s.setOption(N.OPT_GRAPH, 1)
     s.setOption(N.OPT_FORCE_VARIANT, args.force_variant)
+    s.setOption(N.OPT_TIMERS, 0 if args.no_stage_timers else 1)
     s.step(args.warmup)
     s.synchronize()
     print(f"# warmup {args.warmup} steps done", file=sys.stderr, flush=True)
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s.step(args.steps)
-    s.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    # Timed region: `steps` graph replays in chunks; each step records its
+    # per-stage HIP events on the handle's stream inside the graph.  The pair
+    # counts of the FLOP model are taken between chunks, outside the timing.
+    s.resetTimers()
+    counts = [s.countPairs()]
+    elapsed, done = 0.0, 0
+    while done < args.steps:
+        k = min(args.chunk, args.steps - done)
+        s.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.step(k)
+        s.synchronize()
+        torch.cuda.synchronize()
+        elapsed += time.perf_counter() - t0
+        done += k
+        counts.append(s.countPairs())
+        print(f"# {done} steps, {elapsed:.3f} s", file=sys.stderr, flush=True)
     n = s.getNum()
-    print(f"# timed {args.steps} steps: {elapsed:.3f} s", file=sys.stderr, flush=True)
     if args.profile:
         # a plain 256 MiB device copy of known size, to calibrate the HBM
         # byte counters (FETCH_SIZE / WRITE_SIZE) of the same profiling run
@@ -147,25 +207,16 @@ def main():
         torch.cuda.synchronize()
         print("# reference copy: 268435456 bytes read + written", file=sys.stderr, flush=True)
         return
-
-    # per-stage times with HIP events on the handle's stream (eager launches)
-    K, M, Kt = s.countPairs()
-    s.setOption(N.OPT_TIMERS, 1)
-    s.resetTimers()
-    s.step(args.timer_steps)
-    s.synchronize()
-    stages = s.stageTimes()
-    print("# stages (ms): " + ", ".join(f"{k} {v[0]:.4f}" for k, v in stages.items()), file=sys.stderr, flush=True)
-    K2, M2, Kt2 = s.countPairs()
+    stages = s.stageTimes() if not args.no_stage_timers else {}
     s.setOption(N.OPT_TIMERS, 0)
-    Kp, Mp, Ktp = 0.5 * (K + K2) / n, 0.5 * (M + M2) / n, 0.5 * (Kt + Kt2) / n
+    # pair counts per particle, averaged over the chunk boundaries (trapezoid)
+    sizes = [min(args.chunk, args.steps - c * args.chunk) for c in range(len(counts) - 1)]
+    avg = [sum(0.5 * (counts[c][q] + counts[c + 1][q]) * sizes[c] for c in range(len(sizes))) / args.steps / n
+           for q in range(3)]
+    Kp, Mp, Ktp = avg
     # force FLOP model: 8 per candidate pair + 35 per accepted pair (self excluded)
     force_flop = n * (8.0 * Kp + 35.0 * (Mp - 1.0))
-    force_ms = stages["force"][0]
-    achieved = force_flop / (force_ms * 1e-3) / 1e12
     dens_flop = n * (8.0 * Kp + 4.0 * Mp)
-    dens_ms = stages["density"][0]
-
     ms_per_step = 1e3 * elapsed / args.steps
     out = {
         "metric": METRIC,
@@ -182,9 +233,13 @@ def main():
         "data": "synthetic (seeded jittered lattice, SplitMix64)",
         "config": {"workload": "plain SPH dam break, 1,048,576 particles (128x128x64), h=0.04, dt=0.003, "
                                "walls on, hipGraph per step", "particles": n, "steps_per_s": 1e3 / ms_per_step},
-        "stages_ms": {k: round(v[0], 5) for k, v in stages.items()},
         "pairs_per_particle": {"candidates_27cell": Kp, "accepted_incl_self": Mp, "candidates_tested": Ktp},
-        "roofline": {
+    }
+    if stages:
+        force_ms, dens_ms = stages["force"][0], stages["density"][0]
+        achieved = force_flop / (force_ms * 1e-3) / 1e12
+        out["stages_ms"] = {k: round(v[0], 5) for k, v in stages.items()}
+        out["roofline"] = {
             "bound": "valu",
             "kernel": "k_force (fused force + walls + leapfrog)",
             "achieved": achieved,
@@ -194,18 +249,24 @@ def main():
             "traffic": None,
             "flop_per_launch": force_flop,
             "avg_launch_ms": force_ms,
-        },
-        "density_roofline": {
+            "launches": stages["force"][1],
+        }
+        out["density_roofline"] = {
             "achieved_tflops": dens_flop / (dens_ms * 1e-3) / 1e12,
             "frac": dens_flop / (dens_ms * 1e-3) / 1e12 / PEAK_VALU_FP32_TFLOPS,
-        },
-    }
+            "avg_launch_ms": dens_ms,
+        }
     s.close()
     if args.batched_steps > 0:
         print("# batched leg ...", file=sys.stderr, flush=True)
         out["batched"] = batched_leg(args.batched_steps)
         print(f"# batched: {out['batched']['value']:.3e} p-u/s, x{out['batched']['ratio']:.2f} vs sequential plain",
               file=sys.stderr, flush=True)
+    if args.adjoint_steps > 0:
+        print("# fwd+adjoint leg ...", file=sys.stderr, flush=True)
+        out["fwd_adjoint"] = adjoint_leg(args.adjoint_steps)
+        print(f"# fwd+adjoint: {out['fwd_adjoint']['value']:.3e} p-u/s, adjoint/forward "
+              f"{out['fwd_adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)

@@ -114,6 +114,7 @@ constexpr int kSubCells = 1 << kSubBits;
 struct Win {
     const uint32_t* keys;
     float rx, ry, rz;
+    int remap;   // XCD-aware tile order (xcd_tile) for the neighbour kernels
 };
 
 struct WindowCtx {

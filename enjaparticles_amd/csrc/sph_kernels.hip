@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
                                                     const uint32_t* __restrict__ start,
                                                     const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
                                                     Win win, float* __restrict__ dens, float4* __restrict__ packed) {
-    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int i = (win.remap ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
     if (d_num) effective_num(num, start, g.nb_cells, num);
     if (i >= num) return;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
                                                   GravitySources gsrc, int collision, rtps_float4 bmin,
                                                   rtps_float4 bmax, float dt, float4* __restrict__ pos_u,
                                                   float4* __restrict__ vel_u, float4* __restrict__ veleval_u) {
-    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int i = (win.remap ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
     bool deleted = false;
     if (d_num) deleted = effective_num(num, start, g.nb_cells, num);

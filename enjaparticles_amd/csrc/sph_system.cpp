@@ -43,10 +43,15 @@ struct rtps_sph {
     int* d_num = nullptr;
     void* sort_tmp = nullptr;
     GravitySources gsrc{};
-    int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0;
+    int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0;
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
-    hipEvent_t ev[ST_N + 1] = {};
+    // Timers: two event sets alternate between steps (set k & 1 for step k),
+    // each recorded by its own captured step graph when graphs are on.
+    hipGraphExec_t gexec_t[2] = {};
+    bool timed_graph_valid = false;
+    hipEvent_t ev[2][ST_N + 1] = {};
+    bool ev_pending[2] = {};
     double stage_ms[ST_N] = {};
     long stage_cnt[ST_N] = {};
     bool update_pending = false;
@@ -54,12 +59,15 @@ struct rtps_sph {
     ~rtps_sph() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto& g : gexec_t)
+            if (g) (void)hipGraphExecDestroy(g);
         void* bufs[] = {pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s, force_s, xsph_s, keys, idx, start, end, d_num,
                         sort_tmp};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
+        for (auto& set : ev)
+            for (auto& e : set)
+                if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -67,7 +75,8 @@ struct rtps_sph {
 namespace {
 
 Win win(const rtps_sph* s) {
-    return Win{s->keys_sorted, s->gps.grid_res.x, s->gps.grid_res.y, s->gps.grid_res.z};
+    return Win{s->opt_window ? s->keys_sorted : nullptr, s->gps.grid_res.x, s->gps.grid_res.y, s->gps.grid_res.z,
+               s->opt_remap};
 }
 
 template <typename T>
@@ -105,6 +114,7 @@ void apply_settings(rtps_sph* s) {
     load_gravity_sources(s);
     s->settings.updated();
     s->graph_valid = false;
+    s->timed_graph_valid = false;
 }
 
 int maybe_update_params(rtps_sph* s) {
@@ -115,9 +125,13 @@ int maybe_update_params(rtps_sph* s) {
 struct StageTimer {
     rtps_sph* s;
     bool on;
-    explicit StageTimer(rtps_sph* s_) : s(s_), on(s_->opt_timers != 0) {}
+    int set = 0;
+    hipError_t err = hipSuccess;
+    explicit StageTimer(rtps_sph* s_, int set_ = 0) : s(s_), on(s_->opt_timers != 0), set(set_) {}
     void mark(int i) {
-        if (on) (void)hipEventRecord(s->ev[i], s->stream);
+        if (!on) return;
+        hipError_t e = hipEventRecord(s->ev[set][i], s->stream);
+        if (e != hipSuccess && err == hipSuccess) err = e;
     }
 };
 
@@ -156,37 +170,80 @@ int launch_step(rtps_sph* s, StageTimer& t) {
     return RTPS_OK;
 }
 
-int collect_timers(rtps_sph* s) {
-    RTPS_TRY_HIP(hipEventSynchronize(s->ev[ST_N]));
+// Adds the elapsed times of event set `set` (a finished or finishing step).
+int collect_timers(rtps_sph* s, int set) {
+    if (!s->ev_pending[set]) return RTPS_OK;
+    s->ev_pending[set] = false;
+    RTPS_TRY_HIP(hipEventSynchronize(s->ev[set][ST_N]));
     for (int i = 0; i < ST_N; ++i) {
         float ms = 0.f;
-        RTPS_TRY_HIP(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+        RTPS_TRY_HIP(hipEventElapsedTime(&ms, s->ev[set][i], s->ev[set][i + 1]));
         s->stage_ms[i] += ms;
         s->stage_cnt[i] += 1;
     }
     return RTPS_OK;
 }
 
-int capture_graph(rtps_sph* s) {
-    if (s->gexec) {
-        (void)hipGraphExecDestroy(s->gexec);
-        s->gexec = nullptr;
+// Captures one step into *out; with timers, event set `set` is recorded
+// around the stages inside the graph.
+int capture_step(rtps_sph* s, bool timed, int set, hipGraphExec_t* out) {
+    if (*out) {
+        (void)hipGraphExecDestroy(*out);
+        *out = nullptr;
     }
     hipGraph_t graph = nullptr;
     RTPS_TRY_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    StageTimer off(s);
-    off.on = false;
-    int rc = launch_step(s, off);
+    StageTimer t(s, set);
+    t.on = timed;
+    int rc = launch_step(s, t);
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
-    if (rc) {
+    if (rc || t.err != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
-        return rc;
+        return rc ? rc : hip_error(t.err, "hipEventRecord (graph capture)");
     }
     if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
-    e = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
+    return RTPS_OK;
+}
+
+int capture_graph(rtps_sph* s) {
+    int rc = capture_step(s, false, 0, &s->gexec);
+    if (rc) return rc;
     s->graph_valid = true;
+    return RTPS_OK;
+}
+
+// Timed steps: step k records event set k & 1; the set is read back just
+// before it is recorded again (two steps later), so the host never waits for
+// the step the GPU is running and the timers add no gap between steps.
+int step_timed(rtps_sph* s, int nsteps) {
+    if (s->opt_graph && !s->timed_graph_valid) {
+        for (int k = 0; k < 2; ++k) {
+            int rc = capture_step(s, true, k, &s->gexec_t[k]);
+            if (rc) return rc;
+        }
+        s->timed_graph_valid = true;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        const int set = k & 1;
+        int rc = collect_timers(s, set);
+        if (rc) return rc;
+        if (s->opt_graph) {
+            RTPS_TRY_HIP(hipGraphLaunch(s->gexec_t[set], s->stream));
+        } else {
+            StageTimer t(s, set);
+            rc = launch_step(s, t);
+            if (rc) return rc;
+            if (t.err != hipSuccess) return hip_error(t.err, "hipEventRecord");
+        }
+        s->ev_pending[set] = true;
+    }
+    for (int set = 0; set < 2; ++set) {
+        int rc = collect_timers(s, set);
+        if (rc) return rc;
+    }
     return RTPS_OK;
 }
 
@@ -259,7 +316,8 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
         RTPS_TRY_HIP(sort_clear_temp(s->sort_tmp, (int)N, s->stream));
         s->keys_sorted = s->keys;
         s->idx_sorted = s->idx;
-        for (auto& e : s->ev) RTPS_TRY_HIP(hipEventCreate(&e));
+        for (auto& set : s->ev)
+            for (auto& e : set) RTPS_TRY_HIP(hipEventCreate(&e));
         RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
         *out = s.release();
         return RTPS_OK;
@@ -333,16 +391,7 @@ int rtps_sph_step(rtps_sph* s, int nsteps) {
     if (s->update_pending) return set_error(RTPS_ERR_STATE, "rtps_sph_step after update() without integrate()");
     maybe_update_params(s);
     if (s->num_host == 0) return RTPS_OK;
-    if (s->opt_timers) {
-        for (int k = 0; k < nsteps; ++k) {
-            StageTimer t(s);
-            int rc = launch_step(s, t);
-            if (rc) return rc;
-            rc = collect_timers(s);
-            if (rc) return rc;
-        }
-        return RTPS_OK;
-    }
+    if (s->opt_timers) return step_timed(s, nsteps);
     if (s->opt_graph) {
         if (!s->graph_valid) {
             int rc = capture_graph(s);
@@ -455,9 +504,12 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_FORCE_OUT: s->opt_force_out = value ? 1 : 0; break;
         case RTPS_OPT_TIMERS: s->opt_timers = value ? 1 : 0; break;
         case RTPS_OPT_FORCE_VARIANT: g_force_variant = value ? 1 : 0; break;
+        case RTPS_OPT_WINDOW: s->opt_window = value ? 1 : 0; break;
+        case RTPS_OPT_TILE_REMAP: s->opt_remap = value ? 1 : 0; break;
         default: return set_error(RTPS_ERR_ARG, "unknown option");
     }
     s->graph_valid = false;
+    s->timed_graph_valid = false;
     return RTPS_OK;
 }
 

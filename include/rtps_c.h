@@ -200,14 +200,24 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *   RTPS_OPT_COLLISION   1 = wall collision on (default; the shipped SPH.cpp:297 has it off)
  *   RTPS_OPT_GRAPH       1 = capture rtps_sph_step's inner loop in a hipGraph (default 1)
  *   RTPS_OPT_FORCE_OUT   1 = also store force/xsph per step (default 0)
- *   RTPS_OPT_TIMERS      1 = per-stage hipEvent timers (forces eager launches)
+ *   RTPS_OPT_TIMERS      1 = per-stage hipEvent timers, recorded inside the step
+ *                        graph (or around eager launches); two event sets
+ *                        alternate, so the host reads step k-2's times while
+ *                        step k-1 runs and rtps_sph_step returns after the
+ *                        last step finished
  *   RTPS_OPT_FORCE_VARIANT  force-loop kernel, process-wide: 0 = one branch per
- *                        candidate (default), 1 = compacted neighbour lists in LDS */
+ *                        candidate (default), 1 = compacted neighbour lists in LDS
+ *   RTPS_OPT_WINDOW      1 = scan only the x window of each neighbour row (default)
+ *   RTPS_OPT_TILE_REMAP  1 = XCD-contiguous tile order in the neighbour kernels
+ *                        (default 0: the interleaved hardware order balances
+ *                        dense regions across XCDs better) */
 #define RTPS_OPT_COLLISION 0
 #define RTPS_OPT_GRAPH 1
 #define RTPS_OPT_FORCE_OUT 2
 #define RTPS_OPT_TIMERS 3
 #define RTPS_OPT_FORCE_VARIANT 4
+#define RTPS_OPT_WINDOW 5
+#define RTPS_OPT_TILE_REMAP 6
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Per-stage GPU times (SPH::printTimers, SPH.cpp:664-672): average ms per

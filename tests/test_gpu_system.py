@@ -170,3 +170,24 @@ def test_addbox_matches_addrect(cuda):
     assert np.all(p[:, 3] == 1.0)
     assert p[1, 0] == np.float32(np.float32(0.1) + np.float32(0.02))
     s.close()
+
+
+def test_stage_timers_do_not_change_the_step(cuda):
+    """Per-stage timers (events inside the step graph, two alternating sets)
+    time every step and leave the results bit-identical."""
+    sc = scenes.dam_custom(24, 24, 12, seed=78)
+    a = _system(sc, graph=True)
+    b = _system(sc, graph=True)
+    b.setOption(3, 1)
+    c = _system(sc, graph=False)
+    c.setOption(3, 1)
+    for s in (a, b, c):
+        s.step(7)
+    np.testing.assert_array_equal(b.read("pos"), a.read("pos"))
+    np.testing.assert_array_equal(c.read("pos"), a.read("pos"))
+    for s in (b, c):
+        st = s.stageTimes()
+        assert all(cnt == 7 for _, cnt in st.values()), st
+        assert all(ms > 0.0 for ms, _ in st.values()), st
+    for s in (a, b, c):
+        s.close()
