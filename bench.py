@@ -142,6 +142,57 @@ def adjoint_leg(steps: int = 200, checkpoint_every: int = 20):
             "adjoint_over_plain": tb / tp}
 
 
+def batched_adjoint_leg(steps: int = 100, num_scenes: int = 64, checkpoint_every: int = 20, seed: int = 1005):
+    """Batched + differentiable: 64 scenes with counts log-uniform in
+    [16384, 131072] (per-scene viscosity and gas constant), forward and
+    per-scene adjoint of `steps` steps in one batched launch per stage, next
+    to the same scenes' plain batched forward."""
+    import numpy as np
+    import torch
+    from enjaparticles_amd import scenes
+    from enjaparticles_amd.adjoint import BatchedAdjointSPH
+    from enjaparticles_amd.rtps import BatchedSPH
+    scs = scenes.batched(num_scenes, 16384, 131072, seed=seed)
+    total = sum(sc.n for sc in scs)
+    dev = torch.device("cuda:0")
+    pos0 = torch.from_numpy(np.concatenate([sc.pos for sc in scs])).to(dev)
+    vel = np.zeros((total, 4), np.float32)
+    vel[:, :3] = scenes.normal(seed + 1, 3 * total, 0.05).reshape(total, 3)
+    vel0 = torch.from_numpy(vel).to(dev)
+    mus = [sc.meta["viscosity"] for sc in scs]
+    gases = [sc.meta["gas_constant"] for sc in scs]
+    b = BatchedAdjointSPH([sc.settings for sc in scs], checkpoint_every=checkpoint_every, max_steps=steps)
+    g = torch.from_numpy(np.random.default_rng(6).normal(size=(total, 4)).astype(np.float32)).to(dev)
+    g[:, 3] = 0.0
+    b.forward(pos0, vel0, checkpoint_every, mus, gases)      # untimed first touch (one segment)
+    b.backward(g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.forward(pos0, vel0, steps, mus, gases)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _, _, dmu, dK = b.backward(g)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    b.close()
+    p = BatchedSPH([sc.settings for sc in scs])
+    for i, sc in enumerate(scs):
+        p.pushParticles(i, sc.pos, vel[sum(x.n for x in scs[:i]):][:sc.n])
+    p.step(2)
+    p.synchronize()
+    t3 = time.perf_counter()
+    p.step(steps)
+    p.synchronize()
+    t4 = time.perf_counter()
+    p.close()
+    tf, tb, tp = t1 - t0, t2 - t1, t4 - t3
+    return {"scenes": num_scenes, "particles": total, "steps": steps, "checkpoint_every": checkpoint_every,
+            "value": total * steps / (tf + tb), "unit": "particle-updates/s (forward + adjoint)",
+            "forward_ms_per_step": 1e3 * tf / steps, "adjoint_ms_per_step": 1e3 * tb / steps,
+            "plain_batched_ms_per_step": 1e3 * tp / steps, "adjoint_over_forward": tb / tf,
+            "adjoint_over_plain": tb / tp, "finite_gradients": bool(np.all(np.isfinite(dmu + dK)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +206,7 @@ def main():
     ap.add_argument("--force-variant", type=int, default=0, help="1 = compacted neighbour lists, 0 = direct")
     ap.add_argument("--batched-steps", type=int, default=30, help="steps of the batched leg (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="steps of the fwd+adjoint leg (0 = skip)")
+    ap.add_argument("--badj-steps", type=int, default=100, help="steps of the batched fwd+adjoint leg (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -175,7 +227,8 @@ def main():
     s.setOption(N.OPT_GRAPH, 1)
     s.setOption(N.OPT_FORCE_VARIANT, args.force_variant)
     s.setOption(N.OPT_TIMERS, 0 if args.no_stage_timers else 1)
-    s.step(args.warmup)
+    tune = s.autotune(measure=True)          # the first warmup step tunes the neighbour launches
+    s.step(max(0, args.warmup - 1))
     s.synchronize()
     print(f"# warmup {args.warmup} steps done", file=sys.stderr, flush=True)
 
@@ -234,6 +287,7 @@ def main():
         "config": {"workload": "plain SPH dam break, 1,048,576 particles (128x128x64), h=0.04, dt=0.003, "
                                "walls on, hipGraph per step", "particles": n, "steps_per_s": 1e3 / ms_per_step},
         "pairs_per_particle": {"candidates_27cell": Kp, "accepted_incl_self": Mp, "candidates_tested": Ktp},
+        "autotune": tune,
     }
     if stages:
         force_ms, dens_ms = stages["force"][0], stages["density"][0]
@@ -267,6 +321,11 @@ def main():
         out["fwd_adjoint"] = adjoint_leg(args.adjoint_steps)
         print(f"# fwd+adjoint: {out['fwd_adjoint']['value']:.3e} p-u/s, adjoint/forward "
               f"{out['fwd_adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
+    if args.badj_steps > 0:
+        print("# batched fwd+adjoint leg ...", file=sys.stderr, flush=True)
+        out["batched_fwd_adjoint"] = batched_adjoint_leg(args.badj_steps)
+        print(f"# batched fwd+adjoint: {out['batched_fwd_adjoint']['value']:.3e} p-u/s, adjoint/forward "
+              f"{out['batched_fwd_adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)

@@ -23,8 +23,8 @@ FIELD = {
     "pos": 0, "vel": 1, "veleval": 2, "pos_s": 3, "vel_s": 4, "veleval_s": 5, "density": 6,
     "force": 7, "xsph": 8, "keys": 9, "idx": 10, "cell_start": 11, "cell_end": 12,
 }
-OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS, OPT_FORCE_VARIANT, OPT_WINDOW, OPT_TILE_REMAP = (
-    0, 1, 2, 3, 4, 5, 6)
+OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS, OPT_FORCE_VARIANT, OPT_WINDOW, OPT_TILE_REMAP, OPT_AUTOTUNE = (
+    0, 1, 2, 3, 4, 5, 6, 7)
 NUM_STAGES = 6
 STAGE_NAMES = ("hash", "sort", "cellindices", "permute", "density", "force")
 MAX_GRAVITY_SOURCES = 16
@@ -90,6 +90,7 @@ SIGNATURES = {
     "rtps_sph_get_setting": (_i, [P, C.c_char_p, P, _i]),
     "rtps_sph_get_params": (_i, [P, P, P, P]),
     "rtps_sph_set_option": (_i, [P, _i, _i]),
+    "rtps_sph_autotune": (_i, [P, _i, P, P, P]),
     "rtps_sph_stage_times": (_i, [P, P, P, _i]),
     "rtps_sph_reset_timers": (_i, [P]),
     "rtps_sph_stream": (_i, [P, P]),
@@ -109,6 +110,11 @@ SIGNATURES = {
     "rtps_adj_destroy": (_i, [P]),
     "rtps_adj_forward": (_i, [P, P, P, _i, _f, _f, _i, P, P, P]),
     "rtps_adj_backward": (_i, [P, P, P, P, P, P, P, P]),
+    "rtps_badj_create": (_i, [_i, P, P, P, _i, _i, P]),
+    "rtps_badj_destroy": (_i, [P]),
+    "rtps_badj_layout": (_i, [P, P, P, P]),
+    "rtps_badj_forward": (_i, [P, P, P, _i, P, P, _i, P, P, P]),
+    "rtps_badj_backward": (_i, [P, P, P, P, P, P, P, P]),
 }
 
 

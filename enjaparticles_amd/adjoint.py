@@ -6,6 +6,9 @@ and the stream; forward and backward are HIP kernels.
     pos_T = sph_rollout(sim, pos0, vel0, viscosity, gas_constant, nsteps)
     loss = torch.nn.functional.mse_loss(pos_T[:, :3], target[:, :3])
     loss.backward()   # -> vel0.grad, viscosity.grad, gas_constant.grad
+
+Many scenes at once (rtps_badj_*): BatchedAdjointSPH and batched_rollout,
+with one viscosity and gas constant per scene.
 """
 from __future__ import annotations
 
@@ -92,3 +95,108 @@ def sph_rollout(sim: AdjointSPH, pos0: torch.Tensor, vel0: torch.Tensor, viscosi
                 gas_constant: torch.Tensor, nsteps: int, collision: bool = True) -> torch.Tensor:
     """Final positions after nsteps, differentiable in vel0, viscosity and gas_constant."""
     return _Rollout.apply(vel0, viscosity, gas_constant, sim, pos0, int(nsteps), bool(collision))
+
+
+class BatchedAdjointSPH:
+    """Owns an rtps_badj handle: forward and adjoint of many scenes at once.
+
+    Scene s holds exactly the ``max_num_particles`` of its settings; the
+    particle arrays are the scenes' arrays concatenated (``offsets[s]`` to
+    ``offsets[s + 1]``).  Viscosity and gas constant are per scene.
+    """
+
+    def __init__(self, scene_settings: list, checkpoint_every: int = 20, max_steps: int = 100):
+        flat_k, flat_v, counts = [], [], []
+        for st in scene_settings:
+            st = dict(st)
+            counts.append(len(st))
+            flat_k += [str(k).encode() for k in st]
+            flat_v += [str(v).encode() for v in st.values()]
+        ka = (C.c_char_p * max(1, len(flat_k)))(*flat_k)
+        va = (C.c_char_p * max(1, len(flat_v)))(*flat_v)
+        ca = (C.c_int * len(counts))(*counts)
+        h = C.c_void_p()
+        N.check(N.lib().rtps_badj_create(len(counts), ca, C.cast(ka, C.c_void_p), C.cast(va, C.c_void_p),
+                                         checkpoint_every, max_steps, C.byref(h)))
+        self._h = h
+        ns, tot = C.c_int(), C.c_int()
+        offs = (C.c_int * (len(counts) + 1))()
+        N.check(N.lib().rtps_badj_layout(h, C.byref(ns), C.byref(tot), offs))
+        self.num_scenes = ns.value
+        self.num = tot.value
+        self.offsets = list(offs)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().rtps_badj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, t: torch.Tensor, name: str):
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and tuple(t.shape) == (self.num, 4)):
+            raise ValueError(f"{name} must be a contiguous float32 CUDA tensor of shape ({self.num}, 4)")
+
+    def _scalars(self, x, name: str):
+        a = [float(v) for v in (x.tolist() if isinstance(x, torch.Tensor) else x)]
+        if len(a) != self.num_scenes:
+            raise ValueError(f"{name} needs one value per scene ({self.num_scenes})")
+        return (C.c_float * self.num_scenes)(*a)
+
+    def forward(self, pos0, vel0, nsteps: int, viscosity, gas_constant, collision: bool = True):
+        self._check(pos0, "pos0")
+        self._check(vel0, "vel0")
+        pos_t = torch.empty_like(pos0)
+        vel_t = torch.empty_like(pos0)
+        st = torch.cuda.current_stream().cuda_stream
+        N.check(N.lib().rtps_badj_forward(self._h, pos0.data_ptr(), vel0.data_ptr(), int(nsteps),
+                                          self._scalars(viscosity, "viscosity"),
+                                          self._scalars(gas_constant, "gas_constant"), int(collision),
+                                          pos_t.data_ptr(), vel_t.data_ptr(), st))
+        return pos_t, vel_t
+
+    def backward(self, grad_pos_t, grad_vel_t=None):
+        self._check(grad_pos_t, "grad_pos_T")
+        dv0 = torch.empty_like(grad_pos_t)
+        dx0 = torch.empty_like(grad_pos_t)
+        dmu = (C.c_double * self.num_scenes)()
+        dK = (C.c_double * self.num_scenes)()
+        st = torch.cuda.current_stream().cuda_stream
+        gv = None
+        if grad_vel_t is not None:
+            self._check(grad_vel_t, "grad_vel_T")
+            gv = grad_vel_t.data_ptr()
+        N.check(N.lib().rtps_badj_backward(self._h, grad_pos_t.data_ptr(), gv, dv0.data_ptr(), dx0.data_ptr(),
+                                           dmu, dK, st))
+        return dv0, dx0, list(dmu), list(dK)
+
+
+class _BatchedRollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vel0, viscosity, gas_constant, sim, pos0, nsteps, collision):
+        pos_t, _ = sim.forward(pos0.detach().contiguous(), vel0.detach().contiguous(), nsteps,
+                               viscosity.detach().cpu(), gas_constant.detach().cpu(), collision)
+        ctx.sim = sim
+        ctx.save_for_backward(viscosity, gas_constant)
+        return pos_t
+
+    @staticmethod
+    def backward(ctx, grad_pos_t):
+        visc, gas = ctx.saved_tensors
+        g = grad_pos_t.detach().contiguous().clone()
+        g[:, 3] = 0.0
+        dv0, _dx0, dmu, dK = ctx.sim.backward(g)
+        dv0[:, 3] = 0.0
+        return (dv0, torch.tensor(dmu, dtype=visc.dtype, device=visc.device),
+                torch.tensor(dK, dtype=gas.dtype, device=gas.device), None, None, None, None)
+
+
+def batched_rollout(sim: BatchedAdjointSPH, pos0: torch.Tensor, vel0: torch.Tensor, viscosity: torch.Tensor,
+                    gas_constant: torch.Tensor, nsteps: int, collision: bool = True) -> torch.Tensor:
+    """Final positions of every scene after nsteps, differentiable in vel0 and
+    in the per-scene viscosity and gas constant (tensors of num_scenes)."""
+    return _BatchedRollout.apply(vel0, viscosity, gas_constant, sim, pos0, int(nsteps), bool(collision))

@@ -1,27 +1,11 @@
-// Hand-written adjoint of the SPH step (no reference counterpart).
-//
-// Forward step n (simulation_scale 1, no point sources), with sorted
-// neighbour pairs held fixed:
-//   rho_i = c_rho sum_j (h^2 - q^2)^3                           (density.cl)
-//   a_i   = sum_j [ c_ij phi(q) r_ij + beta_ij psi(q) (e_j - e_i) ]   (force.cl)
-//           c_ij = m cp K (rho_i + rho_j - 2 rho0) / (rho_i rho_j), phi = (h-q)^2/q
-//           beta_ij = m mu cv / (rho_i rho_j),                 psi = h - q
-//   X_i   = cx sum_j (h^2 - q^2)^3 (e_j - e_i) / (rho_i + rho_j)  (XSPH)
-//   f_i   = clamp(a_i + walls(x_i, v_i) + g)                     (collision_wall.cl, leapfrog.cl)
-//   v' = v + dt f + xi X,  e' = (v + v')/2,  x' = x + dt v'       (leapfrog.cl)
-// c_ij, beta_ij and the XSPH weight are symmetric in (i, j), so every pair
-// term of the backward pass is a gather over the same neighbour rows:
-//   B1  per particle: leapfrog, velocity clamp and walls backwards ->
-//       adjoints of a (ga) and X (gX), lambda_v of step n, the direct part of
-//       lambda_x
-//   B2  pair loop: d/dx, d/de and d/drho of the pressure, viscosity and XSPH
-//       terms, plus the per-block partial sums of dL/dmu and dL/dK
-//   B3  pair loop: density backwards, d rho / dx with the grho of both ends
-// then lambda_x, lambda_e of step n are scattered to the original order.
+// Single-system launches of the hand-written adjoint of the SPH step (no
+// reference counterpart).  The per-particle bodies and the derivation live in
+// adjoint_device.h; here each kernel maps thread i to sorted particle i.
 
 #include <hip/hip_runtime.h>
 
 #include "adjoint.h"
+#include "adjoint_device.h"
 #include "rtps_internal.h"
 #include "sph_device.h"
 
@@ -31,11 +15,10 @@ constexpr int kABlock = 256;
 
 static inline int ablocks(long n) { return (int)((n + kABlock - 1) / kABlock); }
 
-__device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
-__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline AdjRows rows_of(const uint32_t* start, const uint32_t* end, const Win& win) {
+    return AdjRows{start, end, win.keys, win.rx, win.ry, win.rz, 0u};
+}
 
-// B1.  lam_x/v/e: adjoints of step n+1 (original order).  Writes the sorted
-// ga/gX, lam_v of step n (original order) and the direct part of lam_x (sorted).
 __global__ __launch_bounds__(kABlock) void k_adj_b1(int num, const uint32_t* __restrict__ idx,
                                                     const float4* __restrict__ pos_s,
                                                     const float4* __restrict__ vel_s,
@@ -47,218 +30,39 @@ __global__ __launch_bounds__(kABlock) void k_adj_b1(int num, const uint32_t* __r
                                                     float4* __restrict__ lxp_s) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= num) return;
-    const uint32_t o = idx[i];
-    const float3 lx = f3(lam_x[o]), lv = f3(lam_v[o]), le = f3(lam_e[o]);
-    // adjoint of v' (x' = x + dt v', e' = (v + v')/2)
-    const float3 lvb = make_float3(lv.x + dt * lx.x + 0.5f * le.x, lv.y + dt * lx.y + 0.5f * le.y,
-                                   lv.z + dt * lx.z + 0.5f * le.z);
-    const float4 p = pos_s[i], v4 = vel_s[i];
-    const float3 v = f3(v4);
-    // f = force + walls + g, then the velocity clamp
-    float4 F = force_s[i];
-    if (collision) collision_wall(F, p, v4, sp, bmin, bmax);
-    const float3 f = make_float3(F.x + sp.gravity.x, F.y + sp.gravity.y, F.z + sp.gravity.z);
-    float3 gfc = make_float3(dt * lvb.x, dt * lvb.y, dt * lvb.z);   // adjoint of the clamped force
-    float3 gf = gfc;
-    const float speed = sqrtf(dot3(f, f));
-    if (speed > sp.velocity_limit) {
-        const float k = sp.velocity_limit / speed;
-        const float3 u = make_float3(f.x / speed, f.y / speed, f.z / speed);
-        const float ug = dot3(u, gfc);
-        gf = make_float3(k * (gfc.x - u.x * ug), k * (gfc.y - u.y * ug), k * (gfc.z - u.z * ug));
-    }
-    const float xi = sp.xsph_factor;
-    ga_s[i] = make_float4(gf.x, gf.y, gf.z, 0.f);
-    gX_s[i] = make_float4(xi * lvb.x, xi * lvb.y, xi * lvb.z, 0.f);
-    float3 gv = make_float3(lvb.x + 0.5f * le.x, lvb.y + 0.5f * le.y, lvb.z + 0.5f * le.z);
-    float3 gx = lx;
-    if (collision) {
-        // active walls: d w / d x = -k s n n^T, d w / d v = -d n n^T, friction -v_t (v_t = v (1 - n.v))
-        const float s = sp.simulation_scale, bd = sp.rest_distance * 0.5f;
-        const float px = p.x * s, py = p.y * s, pz = p.z * s;
-        const float diffs[6] = {bd - (pz - bmin.z), bd - (bmax.z - pz), bd - (py - bmin.y),
-                                bd - (bmax.y - py), bd - (px - bmin.x), bd - (bmax.x - px)};
-        const float3 ns[6] = {make_float3(0, 0, 1), make_float3(0, 0, -1), make_float3(0, 1, 0),
-                              make_float3(0, -1, 0), make_float3(1, 0, 0), make_float3(-1, 0, 0)};
-#pragma unroll
-        for (int w = 0; w < 6; ++w) {
-            if (!(diffs[w] > sp.EPSILON)) continue;
-            const float3 n = ns[w];
-            const float ng = dot3(n, gf);
-            const float kx = -sp.boundary_stiffness * s * ng, kv = -sp.boundary_dampening * ng;
-            gx.x += kx * n.x; gx.y += kx * n.y; gx.z += kx * n.z;
-            gv.x += kv * n.x; gv.y += kv * n.y; gv.z += kv * n.z;
-            const float ndv = dot3(n, v);
-            const float3 vt = make_float3(v.x - v.x * ndv, v.y - v.y * ndv, v.z - v.z * ndv);
-            if ((vt.x + vt.y + vt.z) / 3.0f > 0.0f) {
-                // friction f = -v_t:  (d f / d v)^T g = -(1 - n.v) g + n (v . g)
-                const float vg = dot3(v, gf);
-                gv.x += -(1.f - ndv) * gf.x + n.x * vg;
-                gv.y += -(1.f - ndv) * gf.y + n.y * vg;
-                gv.z += -(1.f - ndv) * gf.z + n.z * vg;
-            }
-        }
-    }
-    lam_v_out[o] = make_float4(gv.x, gv.y, gv.z, 0.f);
-    lxp_s[i] = make_float4(gx.x, gx.y, gx.z, 0.f);
+    adj_b1_body(i, idx, pos_s, vel_s, force_s, sp, bmin, bmax, collision, dt, lam_x, lam_v, lam_e, ga_s, gX_s,
+                lam_v_out, lxp_s);
 }
 
-// B2: pair terms.  ve_s = veleval.xyz | density (sorted).  Writes gx2 (sorted,
-// d/dx of the pair terms), ge (sorted, d/de), grho (sorted) and per-block
-// partial sums of dL/dmu, dL/dK.
+// B2 plus the per-block partial sums of dL/dmu, dL/dK (partial[blockIdx.x]).
 __global__ __launch_bounds__(kABlock) void k_adj_b2(int num, const float4* __restrict__ pos_s,
                                                     const float4* __restrict__ ve_s,
                                                     const float4* __restrict__ ga_s,
-                                                    const float4* __restrict__ gX_s,
-                                                    const uint32_t* __restrict__ start,
-                                                    const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
-                                                    Win win, float4* __restrict__ gx2_s, float4* __restrict__ ge_s,
-                                                    float* __restrict__ grho_s, float2* __restrict__ partial) {
-    __shared__ float2 red[kABlock / 64];
+                                                    const float4* __restrict__ gX_s, AdjRows rows,
+                                                    rtps_sph_params sp, Grid g, float4* __restrict__ gx2_s,
+                                                    float4* __restrict__ ge_s, float* __restrict__ grho_s,
+                                                    float2* __restrict__ partial) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float gmu = 0.f, gK = 0.f;
-    if (i < num) {
-        const float h = sp.smoothing_distance, h2 = h * h, eps = sp.EPSILON;
-        const float m = sp.mass, rho0 = sp.rest_density, K = sp.K, mu = sp.viscosity;
-        const float cp = -0.5f * sp.wspiky_d_coef, cv = sp.wvisc_dd_coef;
-        const float cx = 2.0f * sp.mass * sp.wpoly6_coef;
-        const float4 pi = pos_s[i], vei = ve_s[i];
-        const float3 ei = f3(vei), gai = f3(ga_s[i]), gXi = f3(gX_s[i]);
-        const float ri = vei.w;
-        float3 gx = make_float3(0, 0, 0), ge = make_float3(0, 0, 0);
-        float grho = 0.f;
-        const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
-        const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy) {
-                uint32_t a, b;
-                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
-                for (uint32_t j = a; j < b; ++j) {
-                    const float4 pj = pos_s[j];
-                    const float3 r = make_float3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-                    const float q2 = dot3(r, r);
-                    if (!(q2 <= h2) || j == (uint32_t)i) continue;
-                    const float4 vej = ve_s[j];
-                    const float3 ej = f3(vej), gaj = f3(ga_s[j]), gXj = f3(gX_s[j]);
-                    const float rj = vej.w;
-                    const float qr = sqrtf(q2);
-                    const float q = fmaxf(qr, eps);
-                    const float dq = qr > eps ? 1.f : 0.f;          // d q / d |r|
-                    const float inv_rr = 1.f / (ri * rj);
-                    const float3 dga = make_float3(gai.x - gaj.x, gai.y - gaj.y, gai.z - gaj.z);
-                    const float3 de = make_float3(ej.x - ei.x, ej.y - ei.y, ej.z - ei.z);
-                    const float3 rh = qr > 0.f ? make_float3(r.x / qr, r.y / qr, r.z / qr) : make_float3(0, 0, 0);
-                    // pressure: T_ij = c phi r
-                    const float hq = h - q;
-                    const float phi = hq * hq / q;
-                    const float dphi = -hq * (h + q) / (q * q) * dq;   // d phi / d |r|
-                    const float cK = m * cp * (ri + rj - 2.f * rho0) * inv_rr;   // c / K
-                    const float cc = K * cK;
-                    const float rdga = dot3(r, dga);
-                    const float sig = phi * rdga;
-                    gx.x += cc * (phi * dga.x + dphi * rdga * rh.x);
-                    gx.y += cc * (phi * dga.y + dphi * rdga * rh.y);
-                    gx.z += cc * (phi * dga.z + dphi * rdga * rh.z);
-                    grho += sig * m * cp * K * (2.f * rho0 - rj) * inv_rr / ri;
-                    gK += cK * phi * dot3(r, gai);
-                    // viscosity: V_ij = beta psi (e_j - e_i), psi = h - q
-                    const float bmu = m * cv * inv_rr;           // beta / mu
-                    const float beta = mu * bmu;
-                    const float psi = hq;
-                    const float3 nga = make_float3(-dga.x, -dga.y, -dga.z);   // ga_j - ga_i
-                    ge.x += beta * psi * nga.x;
-                    ge.y += beta * psi * nga.y;
-                    ge.z += beta * psi * nga.z;
-                    const float deg = dot3(de, nga);
-                    gx.x += beta * deg * dq * rh.x;
-                    gx.y += beta * deg * dq * rh.y;
-                    gx.z += beta * deg * dq * rh.z;
-                    grho += -(beta / ri) * psi * dot3(de, dga);
-                    gmu += bmu * psi * dot3(de, gai);
-                    // XSPH: Y_ij = cx w6 (e_j - e_i) / D
-                    const float t = h2 - q2;
-                    const float w6 = t * t * t;
-                    const float D = ri + rj;
-                    const float3 dgX = make_float3(gXi.x - gXj.x, gXi.y - gXj.y, gXi.z - gXj.z);
-                    const float yk = cx * w6 / D;
-                    ge.x -= yk * dgX.x;
-                    ge.y -= yk * dgX.y;
-                    ge.z -= yk * dgX.z;
-                    const float dedgX = dot3(de, dgX);
-                    const float yr = cx / D * (-6.f) * t * t * dedgX;
-                    gx.x += yr * r.x;
-                    gx.y += yr * r.y;
-                    gx.z += yr * r.z;
-                    grho += -yk / D * dedgX;
-                }
-            }
-        gx2_s[i] = make_float4(gx.x, gx.y, gx.z, 0.f);
-        ge_s[i] = make_float4(ge.x, ge.y, ge.z, 0.f);
-        grho_s[i] = grho;
-    }
-    // deterministic per-block sums of dL/dmu, dL/dK
-    for (int off = 32; off > 0; off >>= 1) {
-        gmu += __shfl_down(gmu, off, 64);
-        gK += __shfl_down(gK, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_float2(gmu, gK);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float2 t = make_float2(0.f, 0.f);
-        for (int w = 0; w < kABlock / 64; ++w) {
-            t.x += red[w].x;
-            t.y += red[w].y;
-        }
-        partial[blockIdx.x] = t;
-    }
+    if (i < num) adj_b2_body(i, pos_s, ve_s, ga_s, gX_s, rows, sp, g, gx2_s, ge_s, grho_s, gmu, gK);
+    adj_block_partial<kABlock>(gmu, gK, partial, blockIdx.x);
 }
 
-// B3: density backwards and the scatter of lambda_x, lambda_e of step n.
 __global__ __launch_bounds__(kABlock) void k_adj_b3(int num, const uint32_t* __restrict__ idx,
                                                     const float4* __restrict__ pos_s,
                                                     const float* __restrict__ grho_s,
                                                     const float4* __restrict__ lxp_s,
                                                     const float4* __restrict__ gx2_s,
-                                                    const float4* __restrict__ ge_s,
-                                                    const uint32_t* __restrict__ start,
-                                                    const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
-                                                    Win win, float4* __restrict__ lam_x_out,
+                                                    const float4* __restrict__ ge_s, AdjRows rows,
+                                                    rtps_sph_params sp, Grid g, float4* __restrict__ lam_x_out,
                                                     float4* __restrict__ lam_e_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= num) return;
-    const float h = sp.smoothing_distance, h2 = h * h;
-    const float crho = sp.wpoly6_coef * sp.mass;
-    const float4 pi = pos_s[i];
-    const float gri = grho_s[i];
-    float3 gx = make_float3(0, 0, 0);
-    const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
-    const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy) {
-            uint32_t a, b;
-            row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
-            for (uint32_t j = a; j < b; ++j) {
-                const float4 pj = pos_s[j];
-                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                const float q2 = rx * rx + ry * ry + rz * rz;
-                if (!(q2 <= h2) || j == (uint32_t)i) continue;
-                const float t = h2 - q2;
-                const float k = crho * (-6.f) * t * t * (gri + grho_s[j]);
-                gx.x += k * rx;
-                gx.y += k * ry;
-                gx.z += k * rz;
-            }
-        }
-    const float4 lp = lxp_s[i], g2 = gx2_s[i];
-    const uint32_t o = idx[i];
-    lam_x_out[o] = make_float4(lp.x + g2.x + gx.x, lp.y + g2.y + gx.y, lp.z + g2.z + gx.z, 0.f);
-    lam_e_out[o] = ge_s[i];
+    adj_b3_body(i, idx, pos_s, grho_s, lxp_s, gx2_s, ge_s, rows, sp, g, lam_x_out, lam_e_out);
 }
 
-// Forward integration of the adjoint's recorded step: walls (on a copy of the
-// stored SPH force, which B1 needs unchanged) and leapfrog, scattered to the
-// next state.  Flags particles that left the grid (the adjoint does not
-// follow the deletion path).
+// Walls + leapfrog of a recorded step.  Flags particles that left the grid
+// (the adjoint does not follow the deletion path).
 __global__ __launch_bounds__(kABlock) void k_adj_integrate(int num, const float4* __restrict__ pos_s,
                                                            const float4* __restrict__ vel_s,
                                                            const float4* __restrict__ force_s,
@@ -272,39 +76,13 @@ __global__ __launch_bounds__(kABlock) void k_adj_integrate(int num, const float4
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0 && start[nb_cells] != kEmpty) *left = 1;
     if (i >= num) return;
-    const float4 p = pos_s[i], v = vel_s[i];
-    float4 F = force_s[i];
-    if (collision) collision_wall(F, p, v, sp, bmin, bmax);
-    float4 po, vo, eo;
-    leapfrog(p, v, F, xsph_s[i], sp, dt, po, vo, eo);
-    const uint32_t o = idx[i];
-    pos_u[o] = po;
-    vel_u[o] = vo;
-    veleval_u[o] = eo;
+    adj_integrate_body(i, pos_s, vel_s, force_s, xsph_s, idx, sp, bmin, bmax, collision, dt, pos_u, vel_u,
+                       veleval_u);
 }
 
 // Adds the per-block partial sums (fixed order) to the running dL/dmu, dL/dK.
 __global__ void k_adj_reduce(const float2* __restrict__ partial, int n, double* __restrict__ acc) {
-    __shared__ double s0[256], s1[256];
-    double a = 0.0, b = 0.0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        a += (double)partial[k].x;
-        b += (double)partial[k].y;
-    }
-    s0[threadIdx.x] = a;
-    s1[threadIdx.x] = b;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            s0[threadIdx.x] += s0[threadIdx.x + w];
-            s1[threadIdx.x] += s1[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        acc[0] += s0[0];
-        acc[1] += s1[0];
-    }
+    adj_reduce_body(partial, 0, n, acc);
 }
 
 int adjoint_blocks(int num) { return ablocks(num); }
@@ -338,8 +116,8 @@ hipError_t launch_adj_b2(int num, const float4* pos_s, const float4* ve_s, const
                          hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
-    k_adj_b2<<<ablocks(num), kABlock, 0, st>>>(num, pos_s, ve_s, ga_s, gX_s, start, end, sp, g, win, gx2_s, ge_s,
-                                               grho_s, partial);
+    k_adj_b2<<<ablocks(num), kABlock, 0, st>>>(num, pos_s, ve_s, ga_s, gX_s, rows_of(start, end, win), sp, g, gx2_s,
+                                               ge_s, grho_s, partial);
     return hipGetLastError();
 }
 
@@ -349,8 +127,8 @@ hipError_t launch_adj_b3(int num, const uint32_t* idx, const float4* pos_s, cons
                          float4* lam_e_out, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
-    k_adj_b3<<<ablocks(num), kABlock, 0, st>>>(num, idx, pos_s, grho_s, lxp_s, gx2_s, ge_s, start, end, sp, g, win,
-                                               lam_x_out, lam_e_out);
+    k_adj_b3<<<ablocks(num), kABlock, 0, st>>>(num, idx, pos_s, grho_s, lxp_s, gx2_s, ge_s, rows_of(start, end, win),
+                                               sp, g, lam_x_out, lam_e_out);
     return hipGetLastError();
 }
 

@@ -81,7 +81,7 @@ namespace {
     } while (0)
 
 Win win_of(const rtps_adj* a, const Record& r) {
-    return Win{r.keys, a->gps.grid_res.x, a->gps.grid_res.y, a->gps.grid_res.z, 0};
+    return Win{r.keys, a->gps.grid_res.x, a->gps.grid_res.y, a->gps.grid_res.z, 0, 0, -1};
 }
 
 // One forward step from state slot `in` to slot `out`, recording into r.

@@ -90,7 +90,10 @@ __global__ __launch_bounds__(kBBlock) void k_cellindices_b(int T, const uint32_t
     if (last) end[g] = (uint32_t)(i + 1);
 }
 
-template <bool FORCE>
+// MODE: kPairsDensity, kPairsForceStep (force + walls + leapfrog, scattered to
+// the unsorted arrays) or kPairsForceStore (force and XSPH stored in sorted
+// order for the adjoint's recorded step).
+template <int MODE>
 __global__ __launch_bounds__(kBBlock) void k_pairs_b(const BatchItem* __restrict__ items, int n_items,
                                                      int* __restrict__ queue, const SceneDev* __restrict__ scenes,
                                                      const int* __restrict__ d_num, const float4* __restrict__ pos_s,
@@ -99,7 +102,8 @@ __global__ __launch_bounds__(kBBlock) void k_pairs_b(const BatchItem* __restrict
                                                      const uint32_t* __restrict__ keys, float4* __restrict__ ve,
                                                      const float4* __restrict__ vel_s,
                                                      const uint32_t* __restrict__ idx, float4* __restrict__ pos_u,
-                                                     float4* __restrict__ vel_u, float4* __restrict__ veleval_u) {
+                                                     float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
+                                                     float4* __restrict__ force_out, float4* __restrict__ xsph_out) {
     __shared__ int sh_item;
     for (;;) {
         if (threadIdx.x == 0) sh_item = atomicAdd(queue, 1);
@@ -128,7 +132,7 @@ __global__ __launch_bounds__(kBBlock) void k_pairs_b(const BatchItem* __restrict
         const uint32_t* st = start + S.cell_base;
         const uint32_t* en = end + S.cell_base;
         const uint32_t prefix = (uint32_t)s << kSceneShift;
-        if (!FORCE) {
+        if (MODE == kPairsDensity) {
             float sum = 0.0f;
             for (int dz = -1; dz <= 1; ++dz)
                 for (int dy = -1; dy <= 1; ++dy) {
@@ -165,6 +169,11 @@ __global__ __launch_bounds__(kBBlock) void k_pairs_b(const BatchItem* __restrict
                 }
             float4 F, X;
             force_epilogue(acc, di, sp, F, X);
+            if (MODE == kPairsForceStore) {
+                force_out[i] = F;
+                xsph_out[i] = X;
+                continue;
+            }
             const float4 v = vel_s[i];
             if (S.collision) collision_wall(F, p_unscaled, v, sp, S.bmin, S.bmax);
             float4 po, vo, eo;
@@ -200,17 +209,31 @@ hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* sce
     return hipGetLastError();
 }
 
-hipError_t launch_pairs_b(bool force, int grid, const BatchItem* items, int n_items, int* queue,
+hipError_t launch_pairs_b(int mode, int grid, const BatchItem* items, int n_items, int* queue,
                           const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
                           const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,
-                          const uint32_t* idx, float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st) {
+                          const uint32_t* idx, float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
+                          float4* force_out, float4* xsph_out) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
-    if (force)
-        k_pairs_b<true><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start, end, keys, ve,
-                                                  vel_s, idx, pos_u, vel_u, veleval_u);
-    else
-        k_pairs_b<false><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start, end, keys, ve,
-                                                   vel_s, idx, pos_u, vel_u, veleval_u);
+    if (mode == kPairsForceStore && (!force_out || !xsph_out)) return hipErrorInvalidValue;
+    switch (mode) {
+        case kPairsDensity:
+            k_pairs_b<kPairsDensity><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start,
+                                                               end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
+                                                               force_out, xsph_out);
+            break;
+        case kPairsForceStep:
+            k_pairs_b<kPairsForceStep><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start,
+                                                                 end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
+                                                                 force_out, xsph_out);
+            break;
+        case kPairsForceStore:
+            k_pairs_b<kPairsForceStore><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start,
+                                                                  end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
+                                                                  force_out, xsph_out);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "batch_layout.h"
 #include "batch_types.h"
 #include "rtps_internal.h"
 #include "sph_device.h"
@@ -89,11 +90,11 @@ int launch_step(rtps_batch* b) {
     RTPS_TRY_HIP(launch_permute(b->T, nullptr, b->idx_sorted, b->pos_u, b->vel_u, b->veleval_u, b->pos_s, b->vel_s,
                                 b->ve_s, b->stream));
     mark(b, B_DENSITY);
-    RTPS_TRY_HIP(launch_pairs_b(false, b->persistent_grid, b->items, b->n_items, b->queues, b->scenes, b->d_num,
+    RTPS_TRY_HIP(launch_pairs_b(kPairsDensity, b->persistent_grid, b->items, b->n_items, b->queues, b->scenes, b->d_num,
                                 b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted,
                                 b->pos_u, b->vel_u, b->veleval_u, b->stream));
     mark(b, B_FORCE);
-    RTPS_TRY_HIP(launch_pairs_b(true, b->persistent_grid, b->items, b->n_items, b->queues + 1, b->scenes, b->d_num,
+    RTPS_TRY_HIP(launch_pairs_b(kPairsForceStep, b->persistent_grid, b->items, b->n_items, b->queues + 1, b->scenes, b->d_num,
                                 b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted,
                                 b->pos_u, b->vel_u, b->veleval_u, b->stream));
     RTPS_TRY_HIP(launch_commit_b(b->S, b->scenes, b->d_num, b->start, b->stream));
@@ -127,73 +128,100 @@ int capture(rtps_batch* b) {
 
 }  // namespace
 
+namespace rtps {
+
+int build_batch_layout(int num_scenes, const int* n_settings, const char* const* keys, const char* const* vals,
+                       BatchLayout& L) {
+    if (num_scenes <= 0 || num_scenes > kMaxScenes || !n_settings)
+        return set_error(RTPS_ERR_ARG, "1..256 scenes and their settings are required");
+    L.settings.assign(num_scenes, RTPSSettings());
+    L.scenes.assign(num_scenes, SceneDev());
+    long k = 0, T = 0, cells = 0;
+    int max_cells = 0;
+    for (int s = 0; s < num_scenes; ++s) {
+        if (n_settings[s] < 0) return set_error(RTPS_ERR_ARG, "negative settings count");
+        RTPSSettings& st = L.settings[s];
+        for (int q = 0; q < n_settings[s]; ++q, ++k) {
+            if (!keys[k] || !vals[k]) return set_error(RTPS_ERR_ARG, "null key or value");
+            st.SetSetting<std::string>(keys[k], vals[k]);
+        }
+        if (st.GetSettingAs<std::string>("system") != "sph")
+            return set_error(RTPS_ERR_UNKNOWN_SYSTEM, "scene " + std::to_string(s) + ": system is not sph");
+        const int cap = st.GetSettingAs<int>("max_num_particles");
+        if (cap <= 0) return set_error(RTPS_ERR_ARG, "scene " + std::to_string(s) + ": max_num_particles <= 0");
+        SceneDev& d = L.scenes[s];
+        std::memset(&d, 0, sizeof(d));
+        rtps_grid_params gp, gps;
+        derive_params(st, d.sp, gp, gps);
+        if (gp.nb_cells <= 0 || gp.nb_cells >= (1 << 19))
+            return set_error(RTPS_ERR_ARG, "scene " + std::to_string(s) + ": grid too large for segmented keys");
+        d.g = make_grid(gp);
+        d.gs = make_grid(gps);
+        d.bmin = gps.bnd_min;
+        d.bmax = gps.bnd_max;
+        d.res_x = gps.grid_res.x;
+        d.res_y = gps.grid_res.y;
+        d.res_z = gps.grid_res.z;
+        d.dt = st.GetSettingAs<float>("time_step");
+        d.off = (int)T;
+        d.cap = cap;
+        d.cell_base = (int)cells;
+        d.collision = 1;
+        T += cap;
+        cells += gp.nb_cells + 1;
+        max_cells = std::max(max_cells, gp.nb_cells);
+        if (T >= (1L << 31) - 1) return set_error(RTPS_ERR_ARG, "too many particle slots");
+    }
+    L.T = T;
+    L.total_cells = cells;
+    // sort plan: the low local bits (cell << 4 | xq; pads are all ones) then the scene bits
+    const int low = sort_key_bits(max_cells) + kSubBits;
+    int np = sort_plan(low, L.shifts, L.widths);
+    const int sb = bits_for((unsigned)num_scenes);
+    if (sb > 0) {
+        L.shifts[np] = (int)kSceneShift;
+        L.widths[np] = sb;
+        ++np;
+    }
+    L.npasses = np;
+    // work items: 256-slot tiles of every scene
+    L.items.clear();
+    L.item_first.assign(num_scenes + 1, 0);
+    for (int s = 0; s < num_scenes; ++s) {
+        const SceneDev& d = L.scenes[s];
+        L.item_first[s] = (int)L.items.size();
+        for (int f = 0; f < d.cap; f += 256) L.items.push_back(BatchItem{s, d.off + f, std::min(256, d.cap - f), 0});
+    }
+    L.item_first[num_scenes] = (int)L.items.size();
+    return RTPS_OK;
+}
+
+}  // namespace rtps
+
 extern "C" {
 
 int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* keys, const char* const* vals,
                       rtps_batch** out) {
     try {
-        if (!out || num_scenes <= 0 || num_scenes > kMaxScenes || !n_settings)
-            return set_error(RTPS_ERR_ARG, "rtps_batch_create: 1..256 scenes and their settings are required");
+        if (!out) return set_error(RTPS_ERR_ARG, "rtps_batch_create: null output");
         *out = nullptr;
+        BatchLayout L;
+        int rc = build_batch_layout(num_scenes, n_settings, keys, vals, L);
+        if (rc) return rc;
         std::unique_ptr<rtps_batch> b(new rtps_batch());
         b->S = num_scenes;
-        b->settings.resize(num_scenes);
-        b->scenes_h.resize(num_scenes);
+        b->settings = L.settings;
+        b->scenes_h = L.scenes;
         b->num_h.assign(num_scenes, 0);
-        long k = 0, T = 0, cells = 0;
-        int max_cells = 0;
-        for (int s = 0; s < num_scenes; ++s) {
-            if (n_settings[s] < 0) return set_error(RTPS_ERR_ARG, "negative settings count");
-            RTPSSettings& st = b->settings[s];
-            for (int q = 0; q < n_settings[s]; ++q, ++k) {
-                if (!keys[k] || !vals[k]) return set_error(RTPS_ERR_ARG, "null key or value");
-                st.SetSetting<std::string>(keys[k], vals[k]);
-            }
-            if (st.GetSettingAs<std::string>("system") != "sph")
-                return set_error(RTPS_ERR_UNKNOWN_SYSTEM, "scene " + std::to_string(s) + ": system is not sph");
-            const int cap = st.GetSettingAs<int>("max_num_particles");
-            if (cap <= 0) return set_error(RTPS_ERR_ARG, "scene " + std::to_string(s) + ": max_num_particles <= 0");
-            SceneDev& d = b->scenes_h[s];
-            std::memset(&d, 0, sizeof(d));
-            rtps_grid_params gp, gps;
-            derive_params(st, d.sp, gp, gps);
-            if (gp.nb_cells <= 0 || gp.nb_cells >= (1 << 19))
-                return set_error(RTPS_ERR_ARG, "scene " + std::to_string(s) + ": grid too large for segmented keys");
-            d.g = make_grid(gp);
-            d.gs = make_grid(gps);
-            d.bmin = gps.bnd_min;
-            d.bmax = gps.bnd_max;
-            d.res_x = gps.grid_res.x;
-            d.res_y = gps.grid_res.y;
-            d.res_z = gps.grid_res.z;
-            d.dt = st.GetSettingAs<float>("time_step");
-            d.off = (int)T;
-            d.cap = cap;
-            d.cell_base = (int)cells;
-            d.collision = 1;
-            T += cap;
-            cells += gp.nb_cells + 1;
-            max_cells = std::max(max_cells, gp.nb_cells);
-            if (T >= (1L << 31) - 1) return set_error(RTPS_ERR_ARG, "too many particle slots");
-        }
+        const long T = L.T, cells = L.total_cells;
         b->T = (int)T;
         b->total_cells = (int)cells;
-        // sort plan: the low local bits (cell << 4 | xq; pads are all ones) then the scene bits
-        const int low = sort_key_bits(max_cells) + kSubBits;
-        int np = sort_plan(low, b->shifts, b->widths);
-        const int sb = bits_for((unsigned)num_scenes);
-        if (sb > 0) {
-            b->shifts[np] = (int)kSceneShift;
-            b->widths[np] = sb;
-            ++np;
+        for (int q = 0; q < 4; ++q) {
+            b->shifts[q] = L.shifts[q];
+            b->widths[q] = L.widths[q];
         }
-        b->npasses = np;
-        // work items: 256-slot tiles of every scene
-        std::vector<BatchItem> items;
-        for (int s = 0; s < num_scenes; ++s) {
-            const SceneDev& d = b->scenes_h[s];
-            for (int f = 0; f < d.cap; f += 256) items.push_back(BatchItem{s, d.off + f, std::min(256, d.cap - f), 0});
-        }
+        b->npasses = L.npasses;
+        const std::vector<BatchItem>& items = L.items;
         b->n_items = (int)items.size();
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) {

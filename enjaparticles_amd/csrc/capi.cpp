@@ -113,7 +113,7 @@ int rtps_density(int num, const rtps_float4* pos_s, const uint32_t* start, const
     RTPS_CHECK(num >= 0 && sphp && gps, "rtps_density: bad arguments");
     if (num == 0) return RTPS_OK;
     RTPS_CHECK(pos_s && start && end && density, "rtps_density: null buffer");
-    const Win plain{nullptr, 0.f, 0.f, 0.f, 0};
+    const Win plain{nullptr, 0.f, 0.f, 0.f, 0, 0, -1};
     RTPS_HIP(launch_density(num, nullptr, (const float4*)pos_s, start, end, *sphp, make_grid(*gps), plain, density,
                             nullptr, S(stream)));
     return RTPS_OK;
@@ -125,7 +125,7 @@ int rtps_force(int num, const rtps_float4* pos_s, const float* density, const rt
     RTPS_CHECK(num >= 0 && sphp && gps, "rtps_force: bad arguments");
     if (num == 0) return RTPS_OK;
     RTPS_CHECK(pos_s && density && veleval_s && start && end && force && xsph, "rtps_force: null buffer");
-    const Win plain{nullptr, 0.f, 0.f, 0.f, 0};
+    const Win plain{nullptr, 0.f, 0.f, 0.f, 0, 0, -1};
     RTPS_HIP(launch_force(num, nullptr, (const float4*)pos_s, density, (const float4*)veleval_s, false, start, end,
                           *sphp, make_grid(*gps), plain, (float4*)force, (float4*)xsph, S(stream)));
     return RTPS_OK;

@@ -114,7 +114,9 @@ constexpr int kSubCells = 1 << kSubBits;
 struct Win {
     const uint32_t* keys;
     float rx, ry, rz;
-    int remap;   // XCD-aware tile order (xcd_tile) for the neighbour kernels
+    int remap;     // XCD-aware tile order (xcd_tile) for the neighbour kernels
+    int block;     // threads per block of the neighbour launches (0 = 256)
+    int variant;   // force loop (-1 = the process default)
 };
 
 struct WindowCtx {
@@ -199,7 +201,9 @@ struct ForceAcc {
 __device__ __forceinline__ void force_pair(ForceAcc& acc, float rx, float ry, float rz, float r2, float h, float h2,
                                            float eps, float K, float rho0, float Pi, float di, float4 vi, float4 vj,
                                            float dj) {
-    float rlen = fmaxf(__builtin_sqrtf(r2), eps);   // force.cl:73
+    // force.cl:73.  The hardware square root (<= 1 ulp) stands in for OpenCL's
+    // length(), which the reference builds without correctly rounded sqrt.
+    float rlen = fmaxf(__builtin_amdgcn_sqrtf(r2), eps);
     float hr = h - rlen;
     float dW = hr * hr * fast_rcp(rlen);            // Wspiky_dr, cl_sph_kernels.h:81-97
     float idj = fast_rcp(dj);

@@ -17,6 +17,8 @@
 namespace rtps {
 
 constexpr int kBlock = 256;
+// Candidates per group in the neighbour loops (loads in flight per lane).
+constexpr int kGroup = 4;
 
 static inline int blocks_for(long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }
 
@@ -119,18 +121,27 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
     const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
     const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
     float sum = 0.0f;
+    auto term = [&](float4 pj) {
+        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+        float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        float r2 = rx * rx + ry * ry + rz * rz;
+        float t = h2 - r2;
+        return (r2 <= h2) ? t * t * t : 0.0f;
+    };
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t a, b;
             row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
-            for (uint32_t j = a; j < b; ++j) {
-                float4 pj = pos_s[j];
-                if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
-                float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                float r2 = rx * rx + ry * ry + rz * rz;
-                float t = h2 - r2;
-                sum += (r2 <= h2) ? t * t * t : 0.0f;
+            // groups of kGroup candidates: all loads issued before the first use
+            uint32_t j = a;
+            for (; j + kGroup <= b; j += kGroup) {
+                float4 q[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) q[k] = pos_s[j + k];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) sum += term(q[k]);
             }
+            for (; j < b; ++j) sum += term(pos_s[j]);
         }
     }
     const float rho = sp.wpoly6_coef * sp.mass * sum;
@@ -203,7 +214,34 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
                 row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
-                for (uint32_t j = a; j < b; ++j) {
+                // groups of kGroup candidates: the position loads, then the
+                // veleval loads of the accepted ones, are issued together
+                uint32_t j = a;
+                for (; j + kGroup <= b; j += kGroup) {
+                    float4 q[kGroup], v[kGroup];
+#pragma unroll
+                    for (int k = 0; k < kGroup; ++k) q[k] = pos_s[j + k];
+                    float rx[kGroup], ry[kGroup], rz[kGroup], r2[kGroup];
+                    bool ok[kGroup];
+#pragma unroll
+                    for (int k = 0; k < kGroup; ++k) {
+                        float4 pj = q[k];
+                        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                        rx[k] = pi.x - pj.x; ry[k] = pi.y - pj.y; rz[k] = pi.z - pj.z;
+                        r2[k] = rx[k] * rx[k] + ry[k] * ry[k] + rz[k] * rz[k];
+                        ok[k] = r2[k] <= h2 && j + k != (uint32_t)i;
+                    }
+#pragma unroll
+                    for (int k = 0; k < kGroup; ++k)
+                        if (ok[k]) v[k] = ve_in[j + k];
+#pragma unroll
+                    for (int k = 0; k < kGroup; ++k)
+                        if (ok[k]) {
+                            const float dj = PACKED ? v[k].w : dens_in[j + k];
+                            force_pair(acc, rx[k], ry[k], rz[k], r2[k], h, h2, eps, K, rho0, Pi, di, vi, v[k], dj);
+                        }
+                }
+                for (; j < b; ++j) {
                     float rx, ry, rz;
                     float r2 = sep(j, rx, ry, rz);
                     if (r2 <= h2 && j != (uint32_t)i) pair(j, rx, ry, rz, r2);
@@ -372,6 +410,11 @@ __global__ void k_commit_num(int* __restrict__ d_num, const uint32_t* __restrict
 // Force-loop variant: 0 = one branch per candidate, 1 = compacted LDS lists.
 int g_force_variant = 0;
 
+// Threads per block of a neighbour launch (a multiple of 64, at most kBlock)
+// and its force-loop variant.
+static inline int nbr_block(const Win& w) { return w.block > 0 && w.block <= kBlock && w.block % 64 == 0 ? w.block : kBlock; }
+static inline int nbr_variant(const Win& w) { return w.variant >= 0 ? w.variant : g_force_variant; }
+
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
                        uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, int sub_bits,
                        hipStream_t st) {
@@ -413,13 +456,14 @@ hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const 
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
-    const int nb = blocks_for(num);
+    const int bs = nbr_block(win);
+    const int nb = blocks_for(num, bs);
     if (packed) {
-        if (scaled) k_density<true, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
-        else k_density<false, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
+        if (scaled) k_density<true, true><<<nb, bs, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
+        else k_density<false, true><<<nb, bs, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
     } else {
-        if (scaled) k_density<true, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
-        else k_density<false, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
+        if (scaled) k_density<true, false><<<nb, bs, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
+        else k_density<false, false><<<nb, bs, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
     }
     return hipGetLastError();
 }
@@ -430,18 +474,19 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
-    const int nb = blocks_for(num);
+    const int bs = nbr_block(win), variant = nbr_variant(win);
+    const int nb = blocks_for(num, bs);
     GravitySources none{};
     rtps_float4 z{0, 0, 0, 0};
 #define RTPS_FORCE(S, P)                                                                                            \
-    if (g_force_variant)                                                                                            \
-        k_force<S, P, false, true><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force,   \
-                                                          xsph, nullptr, nullptr, none, 0, z, z, 0.f, nullptr,     \
-                                                          nullptr, nullptr);                                       \
+    if (variant)                                                                                                    \
+        k_force<S, P, false, true><<<nb, bs, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force,   \
+                                                      xsph, nullptr, nullptr, none, 0, z, z, 0.f, nullptr,         \
+                                                      nullptr, nullptr);                                           \
     else                                                                                                            \
-        k_force<S, P, false, false><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force,  \
-                                                           xsph, nullptr, nullptr, none, 0, z, z, 0.f, nullptr,    \
-                                                           nullptr, nullptr)
+        k_force<S, P, false, false><<<nb, bs, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force,  \
+                                                       xsph, nullptr, nullptr, none, 0, z, z, 0.f, nullptr,        \
+                                                       nullptr, nullptr)
     if (packed) { if (scaled) RTPS_FORCE(true, true); else RTPS_FORCE(false, true); }
     else { if (scaled) RTPS_FORCE(true, false); else RTPS_FORCE(false, false); }
 #undef RTPS_FORCE
@@ -456,13 +501,14 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
-    const int nb = blocks_for(num);
+    const int bs = nbr_block(win), variant = nbr_variant(win);
+    const int nb = blocks_for(num, bs);
 #define RTPS_FFUSED(S, C)                                                                                           \
-    k_force<S, true, true, C><<<nb, kBlock, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
-                                                     xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt, \
-                                                     pos_u, vel_u, veleval_u)
-    if (scaled) { if (g_force_variant) RTPS_FFUSED(true, true); else RTPS_FFUSED(true, false); }
-    else { if (g_force_variant) RTPS_FFUSED(false, true); else RTPS_FFUSED(false, false); }
+    k_force<S, true, true, C><<<nb, bs, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
+                                                 xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt,  \
+                                                 pos_u, vel_u, veleval_u)
+    if (scaled) { if (variant) RTPS_FFUSED(true, true); else RTPS_FFUSED(true, false); }
+    else { if (variant) RTPS_FFUSED(false, true); else RTPS_FFUSED(false, false); }
 #undef RTPS_FFUSED
     return hipGetLastError();
 }

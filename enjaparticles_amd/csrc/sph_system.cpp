@@ -15,7 +15,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <vector>
@@ -44,12 +48,17 @@ struct rtps_sph {
     void* sort_tmp = nullptr;
     GravitySources gsrc{};
     int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0;
+    int opt_autotune = 1;
+    // launch configuration of the neighbour kernels (0 / -1 = defaults), set by the autotuner
+    int block_density = 0, block_force = 0, force_variant = -1;
+    bool tuned = false;
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
     // Timers: two event sets alternate between steps (set k & 1 for step k),
     // each recorded by its own captured step graph when graphs are on.
     hipGraphExec_t gexec_t[2] = {};
     bool timed_graph_valid = false;
+    int graph_events_ok = -1;   // captured external event records give elapsed times (-1 = not probed)
     hipEvent_t ev[2][ST_N + 1] = {};
     bool ev_pending[2] = {};
     double stage_ms[ST_N] = {};
@@ -74,9 +83,10 @@ struct rtps_sph {
 
 namespace {
 
-Win win(const rtps_sph* s) {
+// Launch inputs of the density (force = false) or force kernel.
+Win win(const rtps_sph* s, bool force = false) {
     return Win{s->opt_window ? s->keys_sorted : nullptr, s->gps.grid_res.x, s->gps.grid_res.y, s->gps.grid_res.z,
-               s->opt_remap};
+               s->opt_remap, force ? s->block_force : s->block_density, force ? s->force_variant : -1};
 }
 
 template <typename T>
@@ -126,11 +136,13 @@ struct StageTimer {
     rtps_sph* s;
     bool on;
     int set = 0;
+    bool external = false;   // inside a graph capture: record as external event nodes
     hipError_t err = hipSuccess;
     explicit StageTimer(rtps_sph* s_, int set_ = 0) : s(s_), on(s_->opt_timers != 0), set(set_) {}
     void mark(int i) {
         if (!on) return;
-        hipError_t e = hipEventRecord(s->ev[set][i], s->stream);
+        hipError_t e = external ? hipEventRecordWithFlags(s->ev[set][i], s->stream, hipEventRecordExternal)
+                                : hipEventRecord(s->ev[set][i], s->stream);
         if (e != hipSuccess && err == hipSuccess) err = e;
     }
 };
@@ -161,7 +173,7 @@ int launch_step(rtps_sph* s, StageTimer& t) {
     if (rc) return rc;
     t.mark(ST_FORCE);
     const bool fo = s->opt_force_out != 0;
-    RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, win(s),
+    RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, win(s, true),
                                     fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s, s->idx_sorted,
                                     s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u, s->veleval_u,
                                     s->stream));
@@ -195,6 +207,7 @@ int capture_step(rtps_sph* s, bool timed, int set, hipGraphExec_t* out) {
     RTPS_TRY_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     StageTimer t(s, set);
     t.on = timed;
+    t.external = true;
     int rc = launch_step(s, t);
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
     if (rc || t.err != hipSuccess) {
@@ -215,11 +228,34 @@ int capture_graph(rtps_sph* s) {
     return RTPS_OK;
 }
 
+// Whether events recorded by a replayed graph (external event nodes) give
+// elapsed times: a graph of two event records is replayed once.
+bool probe_graph_events(rtps_sph* s) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t ex = nullptr;
+    bool ok = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        bool rec = hipEventRecordWithFlags(s->ev[0][0], s->stream, hipEventRecordExternal) == hipSuccess &&
+                   hipEventRecordWithFlags(s->ev[0][1], s->stream, hipEventRecordExternal) == hipSuccess;
+        ok = hipStreamEndCapture(s->stream, &graph) == hipSuccess && rec;
+    }
+    ok = ok && hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0) == hipSuccess;
+    ok = ok && hipGraphLaunch(ex, s->stream) == hipSuccess && hipStreamSynchronize(s->stream) == hipSuccess;
+    float ms = -1.f;
+    ok = ok && hipEventElapsedTime(&ms, s->ev[0][0], s->ev[0][1]) == hipSuccess && ms >= 0.f;
+    if (ex) (void)hipGraphExecDestroy(ex);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return ok;
+}
+
 // Timed steps: step k records event set k & 1; the set is read back just
 // before it is recorded again (two steps later), so the host never waits for
 // the step the GPU is running and the timers add no gap between steps.
 int step_timed(rtps_sph* s, int nsteps) {
-    if (s->opt_graph && !s->timed_graph_valid) {
+    if (s->opt_graph && s->graph_events_ok < 0) s->graph_events_ok = probe_graph_events(s) ? 1 : 0;
+    const bool graph = s->opt_graph && s->graph_events_ok == 1;
+    if (graph && !s->timed_graph_valid) {
         for (int k = 0; k < 2; ++k) {
             int rc = capture_step(s, true, k, &s->gexec_t[k]);
             if (rc) return rc;
@@ -230,7 +266,7 @@ int step_timed(rtps_sph* s, int nsteps) {
         const int set = k & 1;
         int rc = collect_timers(s, set);
         if (rc) return rc;
-        if (s->opt_graph) {
+        if (graph) {
             RTPS_TRY_HIP(hipGraphLaunch(s->gexec_t[set], s->stream));
         } else {
             StageTimer t(s, set);
@@ -259,6 +295,163 @@ float4* field_f4(rtps_sph* s, int field) {
         case RTPS_FIELD_XSPH: return s->xsph_s;
         default: return nullptr;
     }
+}
+
+
+// ---------------------------------------------------------------------------
+// Autotuner: threads per block of the density and force launches and the
+// force-loop variant, timed on the live sorted state.  Both kernels are
+// idempotent there (density reads positions and writes veleval_s.w; the fused
+// force kernel reads the sorted arrays and writes only the unsorted ones), so
+// timing repeated launches and finishing with the chosen configuration is
+// exactly one ordinary step.  Choices are cached per process, keyed by device
+// and size class, and optionally in the text file named by RTPS_TUNE_CACHE
+// ("key block_density block_force variant" per line).
+struct Tune {
+    int bd, bf, var;
+};
+
+std::mutex g_tune_mu;
+std::map<std::string, Tune> g_tune_cache;
+bool g_tune_file_loaded = false;
+
+const char* tune_file() {
+    const char* f = std::getenv("RTPS_TUNE_CACHE");
+    return f && *f ? f : nullptr;
+}
+
+void tune_load_file_locked() {
+    if (g_tune_file_loaded) return;
+    g_tune_file_loaded = true;
+    const char* f = tune_file();
+    if (!f) return;
+    std::ifstream in(f);
+    std::string k;
+    Tune t;
+    while (in >> k >> t.bd >> t.bf >> t.var) g_tune_cache[k] = t;
+}
+
+bool tune_get(const std::string& key, Tune& t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_file_locked();
+    auto it = g_tune_cache.find(key);
+    if (it == g_tune_cache.end()) return false;
+    t = it->second;
+    return true;
+}
+
+void tune_put(const std::string& key, const Tune& t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_file_locked();
+    g_tune_cache[key] = t;
+    if (const char* f = tune_file()) {
+        std::ofstream out(f, std::ios::app);
+        out << key << ' ' << t.bd << ' ' << t.bf << ' ' << t.var << '\n';
+    }
+}
+
+std::string tune_key(const rtps_sph* s) {
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    std::string arch = "unknown";
+    int cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        arch = prop.gcnArchName;
+        cus = prop.multiProcessorCount;
+        for (char& ch : arch)
+            if (ch == ' ' || ch == ':') ch = '_';
+    }
+    unsigned long n2 = 1;
+    while (n2 < (unsigned long)s->num_host) n2 <<= 1;
+    return arch + "/cu" + std::to_string(cus) + "/n" + std::to_string(n2) + "/c" + std::to_string(s->nb_cells) +
+           "/w" + std::to_string(s->opt_window) + "/r" + std::to_string(s->opt_remap) + "/s" +
+           std::to_string(s->sp.simulation_scale != 1.0f ? 1 : 0);
+}
+
+void apply_tune(rtps_sph* s, const Tune& t) {
+    s->block_density = t.bd;
+    s->block_force = t.bf;
+    s->force_variant = t.var;
+    s->tuned = true;
+    s->graph_valid = false;
+    s->timed_graph_valid = false;
+}
+
+// One step of the system, tuning the launch configuration on the way
+// (measure = false: take a cached choice when there is one).
+int autotune_step(rtps_sph* s, bool measure) {
+    const std::string key = tune_key(s);
+    Tune t{};
+    StageTimer off(s);
+    off.on = false;
+    if (!measure && tune_get(key, t)) {
+        apply_tune(s, t);
+        return launch_step(s, off);
+    }
+    int rc = launch_front(s, off);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    RTPS_TRY_HIP(hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_error(e, "hipEventCreate");
+    }
+    constexpr int kReps = 3;
+    // average ms of kReps launches after one untimed launch; < 0 on error
+    auto time_it = [&](auto&& launch) -> float {
+        if (launch() != hipSuccess) return -1.f;
+        if (hipEventRecord(e0, s->stream) != hipSuccess) return -1.f;
+        for (int r = 0; r < kReps; ++r)
+            if (launch() != hipSuccess) return -1.f;
+        if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.f;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.f;
+        return ms / kReps;
+    };
+    const int blocks[] = {64, 128, 256};
+    float best_d = 1e30f, best_f = 1e30f;
+    Tune best{256, 256, 0};
+    bool failed = false;
+    for (int bs : blocks) {
+        Win w = win(s, false);
+        w.block = bs;
+        const float ms = time_it([&] {
+            return launch_density(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, w, nullptr, s->ve_s,
+                                  s->stream);
+        });
+        if (ms < 0.f) failed = true;
+        else if (ms < best_d) { best_d = ms; best.bd = bs; }
+    }
+    const bool fo = s->opt_force_out != 0;
+    for (int var = 0; var < 2 && !failed; ++var)
+        for (int bs : blocks) {
+            Win w = win(s, true);
+            w.block = bs;
+            w.variant = var;
+            const float ms = time_it([&] {
+                return launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, w,
+                                          fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
+                                          s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u,
+                                          s->vel_u, s->veleval_u, s->stream);
+            });
+            if (ms < 0.f) failed = true;
+            else if (ms < best_f) { best_f = ms; best.bf = bs; best.var = var; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (failed) return hip_error(hipGetLastError(), "autotune launch");
+    apply_tune(s, best);
+    tune_put(key, best);
+    // finish the step with the chosen configuration
+    RTPS_TRY_HIP(launch_density(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, win(s), nullptr,
+                                s->ve_s, s->stream));
+    RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs,
+                                    win(s, true), fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
+                                    s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u,
+                                    s->veleval_u, s->stream));
+    RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
+    return RTPS_OK;
 }
 
 }  // namespace
@@ -369,7 +562,7 @@ int rtps_sph_update(rtps_sph* s) {
     int rc = launch_front(s, t);
     if (rc) return rc;
     RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp, s->gs,
-                              win(s), s->force_s, s->xsph_s, s->stream));
+                              win(s, true), s->force_s, s->xsph_s, s->stream));
     RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                    s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
     s->update_pending = true;
@@ -391,6 +584,11 @@ int rtps_sph_step(rtps_sph* s, int nsteps) {
     if (s->update_pending) return set_error(RTPS_ERR_STATE, "rtps_sph_step after update() without integrate()");
     maybe_update_params(s);
     if (s->num_host == 0) return RTPS_OK;
+    if (s->opt_autotune && !s->tuned && nsteps > 0) {   // the first step also tunes the launches
+        int rc = autotune_step(s, false);
+        if (rc) return rc;
+        if (--nsteps == 0) return RTPS_OK;
+    }
     if (s->opt_timers) return step_timed(s, nsteps);
     if (s->opt_graph) {
         if (!s->graph_valid) {
@@ -506,10 +704,29 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_FORCE_VARIANT: g_force_variant = value ? 1 : 0; break;
         case RTPS_OPT_WINDOW: s->opt_window = value ? 1 : 0; break;
         case RTPS_OPT_TILE_REMAP: s->opt_remap = value ? 1 : 0; break;
+        case RTPS_OPT_AUTOTUNE:
+            s->opt_autotune = value ? 1 : 0;
+            if (!value) apply_tune(s, Tune{0, 0, -1});
+            s->tuned = false;
+            break;
         default: return set_error(RTPS_ERR_ARG, "unknown option");
     }
     s->graph_valid = false;
     s->timed_graph_valid = false;
+    return RTPS_OK;
+}
+
+int rtps_sph_autotune(rtps_sph* s, int measure, int* block_density, int* block_force, int* force_variant) {
+    if (!s) return set_error(RTPS_ERR_ARG, "null handle");
+    if (s->update_pending) return set_error(RTPS_ERR_STATE, "rtps_sph_autotune after update() without integrate()");
+    maybe_update_params(s);
+    if (s->num_host > 0) {
+        int rc = autotune_step(s, measure != 0);
+        if (rc) return rc;
+    }
+    if (block_density) *block_density = s->block_density;
+    if (block_force) *block_force = s->block_force;
+    if (force_variant) *force_variant = s->force_variant;
     return RTPS_OK;
 }
 

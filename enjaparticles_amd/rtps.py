@@ -186,6 +186,12 @@ class SPH:
         N.check(N.lib().rtps_sph_device_ptr(self._h, N.FIELD[field], C.byref(p)))
         return p.value
 
+    def autotune(self, measure: bool = True) -> dict:
+        """One step that (re)tunes the neighbour launches; returns the choice."""
+        bd, bf, var = C.c_int(), C.c_int(), C.c_int()
+        N.check(N.lib().rtps_sph_autotune(self._h, int(measure), C.byref(bd), C.byref(bf), C.byref(var)))
+        return {"block_density": bd.value, "block_force": bf.value, "force_variant": var.value}
+
     def countPairs(self):
         """(K, M, K_tested): 27-cell candidates, accepted pairs (self included) and
         the candidates the windowed rows test, on the last step's state."""

@@ -208,6 +208,8 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *   RTPS_OPT_FORCE_VARIANT  force-loop kernel, process-wide: 0 = one branch per
  *                        candidate (default), 1 = compacted neighbour lists in LDS
  *   RTPS_OPT_WINDOW      1 = scan only the x window of each neighbour row (default)
+ *   RTPS_OPT_AUTOTUNE    1 = the first rtps_sph_step tunes the neighbour launches
+ *                        (default; see rtps_sph_autotune), 0 = fixed defaults
  *   RTPS_OPT_TILE_REMAP  1 = XCD-contiguous tile order in the neighbour kernels
  *                        (default 0: the interleaved hardware order balances
  *                        dense regions across XCDs better) */
@@ -218,7 +220,19 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
 #define RTPS_OPT_FORCE_VARIANT 4
 #define RTPS_OPT_WINDOW 5
 #define RTPS_OPT_TILE_REMAP 6
+#define RTPS_OPT_AUTOTUNE 7
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
+
+/* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))
+ * during which the threads per block of the density and force launches (64,
+ * 128 or 256) and the force-loop variant are timed on the live state and the
+ * fastest kept.  The two kernels are idempotent on one sorted state, so the
+ * step's result does not depend on the choice.  Choices are cached per
+ * process by device and size class (and in the file named by the
+ * RTPS_TUNE_CACHE environment variable, when set); measure = 0 takes a cached
+ * choice when there is one.  With RTPS_OPT_AUTOTUNE on (default) the first
+ * rtps_sph_step does this by itself.  Reports the configuration in use. */
+int rtps_sph_autotune(rtps_sph* s, int measure, int* block_density, int* block_force, int* force_variant);
 
 /* Per-stage GPU times (SPH::printTimers, SPH.cpp:664-672): average ms per
  * call since the last reset, in the order hash, sort, cellindices, permute,
@@ -282,6 +296,28 @@ int rtps_adj_forward(rtps_adj* a, const rtps_float4* pos0, const rtps_float4* ve
 int rtps_adj_backward(rtps_adj* a, const rtps_float4* dL_dpos_T, const rtps_float4* dL_dvel_T /* may be NULL */,
                       rtps_float4* dL_dvel0, rtps_float4* dL_dpos0 /* may be NULL */, double* dL_dviscosity,
                       double* dL_dgas_constant, void* stream);
+
+/* a14 + a15. Batched adjoint (no reference counterpart): the adjoint of
+ * rtps_adj_* for num_scenes independent scenes (<= 256) at once, one launch
+ * per stage for the whole batch, with per-scene viscosity and gas constant
+ * and their gradients.  Scene settings are passed as for rtps_batch_create;
+ * scene s holds exactly its max_num_particles particles, in slots
+ * [offsets[s], offsets[s + 1]) of the concatenated device float4 arrays.
+ * viscosity, gas_constant, dL_dviscosity and dL_dgas_constant are host
+ * arrays of num_scenes entries.  Each scene gets what rtps_adj_* gives for
+ * that scene alone. */
+typedef struct rtps_badj rtps_badj;
+int rtps_badj_create(int num_scenes, const int* n_settings, const char* const* keys, const char* const* vals,
+                     int checkpoint_every, int max_steps, rtps_badj** out);
+int rtps_badj_destroy(rtps_badj* a);
+/* offsets: num_scenes + 1 entries (may be NULL). */
+int rtps_badj_layout(rtps_badj* a, int* num_scenes, int* total, int* offsets);
+int rtps_badj_forward(rtps_badj* a, const rtps_float4* pos0, const rtps_float4* vel0, int nsteps,
+                      const float* viscosity, const float* gas_constant, int collision, rtps_float4* pos_T,
+                      rtps_float4* vel_T /* may be NULL */, void* stream);
+int rtps_badj_backward(rtps_badj* a, const rtps_float4* dL_dpos_T, const rtps_float4* dL_dvel_T /* may be NULL */,
+                       rtps_float4* dL_dvel0, rtps_float4* dL_dpos0 /* may be NULL */, double* dL_dviscosity,
+                       double* dL_dgas_constant, void* stream);
 
 #ifdef __cplusplus
 }

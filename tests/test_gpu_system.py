@@ -182,6 +182,8 @@ def test_stage_timers_do_not_change_the_step(cuda):
     c = _system(sc, graph=False)
     c.setOption(3, 1)
     for s in (a, b, c):
+        s.step(1)          # the first step tunes the launches (not timed)
+        s.resetTimers()
         s.step(7)
     np.testing.assert_array_equal(b.read("pos"), a.read("pos"))
     np.testing.assert_array_equal(c.read("pos"), a.read("pos"))
@@ -190,4 +192,25 @@ def test_stage_timers_do_not_change_the_step(cuda):
         assert all(cnt == 7 for _, cnt in st.values()), st
         assert all(ms > 0.0 for ms, _ in st.values()), st
     for s in (a, b, c):
+        s.close()
+
+
+def test_autotune_choice_does_not_change_the_step(cuda):
+    """The autotuned step (launch configuration timed on the live state) gives
+    the same state as the fixed default configuration, and reports a valid
+    choice."""
+    sc = scenes.dam_custom(24, 24, 12, seed=79)
+    a = _system(sc)
+    a.setOption(7, 0)          # autotune off: 256 threads, direct force loop
+    b = _system(sc)
+    tune = b.autotune(measure=True)
+    assert tune["block_density"] in (64, 128, 256) and tune["block_force"] in (64, 128, 256)
+    assert tune["force_variant"] in (0, 1)
+    a.step(1)
+    np.testing.assert_array_equal(b.read("pos"), a.read("pos"))
+    a.step(4)
+    b.step(4)
+    np.testing.assert_array_equal(b.read("pos"), a.read("pos"))
+    np.testing.assert_array_equal(b.read("vel"), a.read("vel"))
+    for s in (a, b):
         s.close()
