@@ -14,6 +14,7 @@ namespace rtps {
 struct Grid;
 struct GravitySources;
 struct Win;
+struct RowMasks;
 
 // ---- error reporting (capi.cpp) ----
 int set_error(int code, const std::string& msg);
@@ -28,7 +29,7 @@ void read_sph_params(const RTPSSettings& s, rtps_sph_params& sp);
 int sort_key_bits(int nb_cells);
 
 // ---- kernels (sph_kernels.hip) ----
-extern int g_force_variant;   // 0 = direct branch per candidate, 1 = compacted neighbour lists
+extern int g_force_variant;   // 0 = direct branch per candidate, 1 = compacted lists, 2 = recorded masks
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
                        uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, int sub_bits,
                        hipStream_t st);
@@ -39,15 +40,17 @@ hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const 
                           const float4* eu, float4* ps, float4* vs, float4* es, hipStream_t st);
 hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
                           const rtps_sph_params& sp, const Grid& g, const Win& win, float* dens, float4* packed,
-                          hipStream_t st);
+                          hipStream_t st, const RowMasks* masks = nullptr);
 hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const float* dens, const float4* ve,
                         bool packed, const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp,
-                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st);
+                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st,
+                        const RowMasks* masks = nullptr);
 hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, const float4* ve_packed,
                               const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
                               const Win& win, float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
                               const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,
-                              float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st);
+                              float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
+                              const RowMasks* masks = nullptr);
 hipError_t launch_post_force(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
                              const float4* vel_s, float4* force, const rtps_sph_params& sp,
                              const GravitySources& gsrc, int collision, const rtps_grid_params& gps, hipStream_t st);

@@ -114,9 +114,32 @@ constexpr int kSubCells = 1 << kSubBits;
 struct Win {
     const uint32_t* keys;
     float rx, ry, rz;
-    int remap;     // XCD-aware tile order (xcd_tile) for the neighbour kernels
+    int remap;     // tile order of the neighbour kernels (remap_tile)
     int block;     // threads per block of the neighbour launches (0 = 256)
     int variant;   // force loop (-1 = the process default)
+};
+
+// Neighbour rows recorded by the density kernel for the force kernel: per
+// sorted particle i and row r (r = 3 (dz + 1) + (dy + 1)), rows[r * stride + i]
+// = (first candidate a, candidate count n) of the row's window, and
+//  * bit form: bit t of words[(r * kMaskWords + w) * stride + i] = candidate
+//    a + 32 w + t passed r <= h (self excluded).  Candidates past
+//    32 kMaskWords of a row have no bits; the force kernel tests those itself.
+//  * list form: the accepted candidates in row / ascending order as 16-bit
+//    codes (r << 12) | (j - a_r), four per 8-byte slot, slot s of particle i at
+//    codes[s * stride + i] (slot-major, so a wave's loads of slot s are
+//    contiguous); count[i] = their number, or kListCap + 1 when the list
+//    overflowed (the force kernel then runs the direct loop for i).
+constexpr int kMaskWords = 4;
+constexpr int kListCap = 192;          // codes per particle
+constexpr uint32_t kCodeOffBits = 12;  // offset inside the row window
+struct RowMasks {
+    uint2* rows;
+    uint32_t* words;
+    uint2* codes;
+    uint32_t* count;
+    int stride;
+    int form;   // 1 = bit form, 2 = list form
 };
 
 struct WindowCtx {
@@ -184,6 +207,25 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     const int q = nb / X, r = nb % X;
     const int x = b % X, k = b / X;
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// Chunked XCD order: the k-th block an XCD receives takes tile
+// (chunk * 8 + xcd) * C + (k % C) with chunk = k / C, so every XCD works on
+// runs of C consecutive tiles (shared neighbour rows stay in one L2) while the
+// runs of the eight XCDs interleave (dense regions stay spread over all of
+// them).  Blocks past the last whole group of 8 C tiles keep their index.
+__device__ __forceinline__ int xcd_chunk_tile(int b, int nb, int C) {
+    constexpr int X = 8;
+    const int F = nb / (X * C) * (X * C);
+    if (b >= F) return b;
+    const int x = b % X, k = b / X;
+    return ((k / C) * X + x) * C + k % C;
+}
+
+// Tile of block b under the remap option: 0 = hardware order, 1 = one
+// contiguous range per XCD, C >= 2 = chunks of C tiles.
+__device__ __forceinline__ int remap_tile(int remap, int b, int nb) {
+    return remap == 0 ? b : (remap == 1 ? xcd_tile(b, nb) : xcd_chunk_tile(b, nb, remap));
 }
 
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }

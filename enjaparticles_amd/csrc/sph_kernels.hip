@@ -104,13 +104,17 @@ __global__ __launch_bounds__(kBlock) void k_permute(int num_host, const int* __r
 // ---------------------------------------------------------------------------
 // a7: density.cl:86-126.  PACKED: the density is written to veleval_s[i].w
 // (the system layout, so the force loop loads one float4 per neighbour).
-template <bool SCALED, bool PACKED>
+// REC: also record every row's window and the accepted candidates for the
+// force kernel (RowMasks): kRecBits (bit form) or kRecList (list form).
+constexpr int kRecNone = 0, kRecBits = 1, kRecList = 2;
+template <bool SCALED, bool PACKED, int REC>
 __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __restrict__ d_num,
                                                     const float4* __restrict__ pos_s,
                                                     const uint32_t* __restrict__ start,
                                                     const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
-                                                    Win win, float* __restrict__ dens, float4* __restrict__ packed) {
-    const int i = (win.remap ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
+                                                    Win win, float* __restrict__ dens, float4* __restrict__ packed,
+                                                    RowMasks masks) {
+    const int i = remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
     if (d_num) effective_num(num, start, g.nb_cells, num);
     if (i >= num) return;
@@ -121,28 +125,85 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
     const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
     const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
     float sum = 0.0f;
-    auto term = [&](float4 pj) {
+    auto sep2 = [&](float4 pj) {
         if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        float r2 = rx * rx + ry * ry + rz * rz;
+        return rx * rx + ry * ry + rz * rz;
+    };
+    auto term = [&](float r2) {
         float t = h2 - r2;
         return (r2 <= h2) ? t * t * t : 0.0f;
     };
+    const size_t st = (size_t)masks.stride;
+    constexpr bool MASKS = REC == kRecBits, LIST = REC == kRecList;
+    // list form: codes packed four to a slot, flushed as one 8-byte store
+    uint32_t lo = 0, hi = 0, cnt = 0;
+    bool over = false;
+    auto append = [&](uint32_t code) {
+        if (cnt >= (uint32_t)kListCap) {
+            over = true;
+            return;
+        }
+        const uint32_t k = cnt & 3u;
+        if (k == 0u) lo = code;
+        else if (k == 1u) lo |= code << 16;
+        else if (k == 2u) hi = code;
+        else hi |= code << 16;
+        if (k == 3u) masks.codes[(size_t)(cnt >> 2) * st + i] = make_uint2(lo, hi);
+        ++cnt;
+    };
+    int r = 0;
     for (int dz = -1; dz <= 1; ++dz) {
-        for (int dy = -1; dy <= 1; ++dy) {
+        for (int dy = -1; dy <= 1; ++dy, ++r) {
             uint32_t a, b;
             row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
+            // accepted-candidate bits of this row, flushed every 32 candidates
+            uint32_t word = 0, q = 0;
+            int wq = 0;
+            auto flush = [&]() {
+                if (wq < kMaskWords) masks.words[(size_t)(r * kMaskWords + wq) * st + i] = word;
+                word = 0;
+                ++wq;
+            };
+            if (REC != kRecNone) masks.rows[(size_t)r * st + i] = make_uint2(a, b - a);
+            if (LIST && b - a >= (1u << kCodeOffBits)) over = true;   // offsets would not fit a code
+            const uint32_t rtag = (uint32_t)r << kCodeOffBits;
             // groups of kGroup candidates: all loads issued before the first use
             uint32_t j = a;
-            for (; j + kGroup <= b; j += kGroup) {
-                float4 q[kGroup];
+            for (; j + kGroup <= b; j += kGroup, q += kGroup) {
+                float4 p[kGroup];
 #pragma unroll
-                for (int k = 0; k < kGroup; ++k) q[k] = pos_s[j + k];
+                for (int k = 0; k < kGroup; ++k) p[k] = pos_s[j + k];
+                uint32_t nib = 0;
 #pragma unroll
-                for (int k = 0; k < kGroup; ++k) sum += term(q[k]);
+                for (int k = 0; k < kGroup; ++k) {
+                    const float r2 = sep2(p[k]);
+                    sum += term(r2);
+                    const bool acc_k = r2 <= h2 && j + k != (uint32_t)i;
+                    if (MASKS) nib |= (uint32_t)acc_k << k;
+                    if (LIST && acc_k) append(rtag | (j + k - a));
+                }
+                if (MASKS) {
+                    word |= nib << (q & 31u);
+                    if ((q & 31u) == 32u - kGroup) flush();
+                }
             }
-            for (; j < b; ++j) sum += term(pos_s[j]);
+            for (; j < b; ++j, ++q) {
+                const float r2 = sep2(pos_s[j]);
+                sum += term(r2);
+                const bool acc_k = r2 <= h2 && j != (uint32_t)i;
+                if (MASKS) {
+                    word |= (uint32_t)acc_k << (q & 31u);
+                    if ((q & 31u) == 31u) flush();
+                }
+                if (LIST && acc_k) append(rtag | (j - a));
+            }
+            if (MASKS && (q & 31u) != 0u) flush();
         }
+    }
+    if (LIST) {
+        if ((cnt & 3u) != 0u && !over) masks.codes[(size_t)(cnt >> 2) * st + i] = make_uint2(lo, hi);
+        masks.count[i] = over ? (uint32_t)kListCap + 1u : cnt;
     }
     const float rho = sp.wpoly6_coef * sp.mass * sum;
     if (PACKED) packed[i].w = rho; else dens[i] = rho;
@@ -161,10 +222,22 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
 // accepted j to a per-thread list in LDS (kNbrCap entries, entry k of thread t
 // at word k * kBlock + t: conflict-free), then drains the lists with one pair
 // per lane per iteration.  Pairs are visited in the same ascending-j order.
+//
+// LOOP (the force-loop variant): kLoopDirect, kLoopCompact, or kLoopMasked:
+// walk the accepted-candidate bits the density kernel recorded for this step
+// (RowMasks), kGroup accepted neighbours per iteration with their loads
+// issued together; no row searches and no position tests.  Also ascending j.
+//
+// kLoopList: walk the list of accepted neighbours the density kernel recorded
+// (16-bit codes, RowMasks list form), four per iteration; each lane runs
+// through its own list, so a wave costs about its largest neighbour count
+// rather than, row by row, the largest per-row count (the row-by-row walk of
+// kLoopMasked keeps only ~40% of the lanes busy on a dam-break state).
+constexpr int kLoopDirect = 0, kLoopCompact = 1, kLoopMasked = 2, kLoopList = 3;
 constexpr int kNbrCap = 32;
 constexpr int kChunk = 16;
 
-template <bool SCALED, bool PACKED, bool FUSED, bool COMPACT>
+template <bool SCALED, bool PACKED, bool FUSED, int LOOP>
 __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __restrict__ d_num,
                                                   const float4* __restrict__ pos_s,
                                                   const float* __restrict__ dens_in,
@@ -177,8 +250,9 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
                                                   const float4* __restrict__ vel_s, const uint32_t* __restrict__ idx,
                                                   GravitySources gsrc, int collision, rtps_float4 bmin,
                                                   rtps_float4 bmax, float dt, float4* __restrict__ pos_u,
-                                                  float4* __restrict__ vel_u, float4* __restrict__ veleval_u) {
-    const int i = (win.remap ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
+                                                  float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
+                                                  RowMasks masks) {
+    const int i = remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
     bool deleted = false;
     if (d_num) deleted = effective_num(num, start, g.nb_cells, num);
@@ -193,8 +267,6 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
     float di;
     if (PACKED) { vi = ve_in[i]; di = vi.w; } else { vi = ve_in[i]; di = dens_in[i]; }
     const float Pi = K * (di - rho0);
-    const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
-    const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
     ForceAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 
     auto pair = [&](uint32_t j, float rx, float ry, float rz, float r2) {
@@ -209,7 +281,110 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
         return rx * rx + ry * ry + rz * rz;
     };
 
-    if constexpr (!COMPACT) {
+    bool direct = false;
+    if constexpr (LOOP == kLoopMasked) {
+        const size_t st = (size_t)masks.stride;
+        for (int r = 0; r < 9; ++r) {
+            const uint2 rw = masks.rows[(size_t)r * st + i];
+            const uint32_t a = rw.x, n = rw.y;
+            const uint32_t nw = min((n + 31u) >> 5, (uint32_t)kMaskWords);
+            uint32_t mw[kMaskWords];
+#pragma unroll
+            for (int w = 0; w < kMaskWords; ++w)
+                mw[w] = (uint32_t)w < nw ? masks.words[(size_t)(r * kMaskWords + w) * st + i] : 0u;
+            uint32_t m = mw[0], base = a;
+            int kk = 0;
+            // next accepted candidate of this row, ascending
+            auto next = [&](uint32_t& j) {
+                while (m == 0u && kk < kMaskWords - 1) {
+                    ++kk;
+                    m = kk == 1 ? mw[1] : (kk == 2 ? mw[2] : mw[3]);
+                    base += 32u;
+                }
+                if (m == 0u) return false;
+                j = base + (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                return true;
+            };
+            for (;;) {
+                uint32_t jj[kGroup];
+                bool v[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) v[k] = next(jj[k]);
+                if (__ballot(v[0]) == 0ull) break;
+                float4 q[kGroup], e[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (v[k]) q[k] = pos_s[jj[k]];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (v[k]) e[k] = ve_in[jj[k]];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (v[k]) {
+                        float4 pj = q[k];
+                        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                        const float r2 = rx * rx + ry * ry + rz * rz;
+                        const float dj = PACKED ? e[k].w : dens_in[jj[k]];
+                        force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, e[k], dj);
+                    }
+            }
+            // candidates past the recorded bits (rows longer than 32 kMaskWords)
+            for (uint32_t j = a + 32u * kMaskWords; j < a + n; ++j) {
+                float rx, ry, rz;
+                float r2 = sep(j, rx, ry, rz);
+                if (r2 <= h2 && j != (uint32_t)i) pair(j, rx, ry, rz, r2);
+            }
+        }
+    } else if constexpr (LOOP == kLoopList) {
+        const size_t st = (size_t)masks.stride;
+        const uint32_t cnt = masks.count[i];
+        direct = cnt > (uint32_t)kListCap;   // overflowed list: the direct loop below
+        if (!direct) {
+            // row starts of this particle, indexed by the codes' row field
+            __shared__ uint32_t abase[9 * kBlock];
+            const int bs = (int)blockDim.x, t = (int)threadIdx.x;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) abase[r * bs + t] = masks.rows[(size_t)r * st + i].x;
+            const uint32_t ng = (cnt + 3u) >> 2;
+            uint2 pk = ng > 0u ? masks.codes[i] : make_uint2(0u, 0u);
+            for (uint32_t gi = 0; gi < ng; ++gi) {
+                const uint2 cur = pk;
+                if (gi + 1u < ng) pk = masks.codes[(size_t)(gi + 1u) * st + i];   // next slot, one ahead
+                const uint32_t code[kGroup] = {cur.x & 0xFFFFu, cur.x >> 16, cur.y & 0xFFFFu, cur.y >> 16};
+                uint32_t jj[kGroup];
+                bool v[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) {
+                    v[k] = gi * 4u + (uint32_t)k < cnt;
+                    // entries past cnt in the last slot are stale: index row 0 for them (unused)
+                    const uint32_t row = v[k] ? min(code[k] >> kCodeOffBits, 8u) : 0u;
+                    jj[k] = abase[(int)row * bs + t] + (code[k] & ((1u << kCodeOffBits) - 1u));
+                }
+                float4 q[kGroup], e[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (v[k]) q[k] = pos_s[jj[k]];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (v[k]) e[k] = ve_in[jj[k]];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (v[k]) {
+                        float4 pj = q[k];
+                        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                        const float r2 = rx * rx + ry * ry + rz * rz;
+                        const float dj = PACKED ? e[k].w : dens_in[jj[k]];
+                        force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, e[k], dj);
+                    }
+            }
+        }
+    }
+    if (LOOP == kLoopDirect || (LOOP == kLoopList && direct)) {
+        const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
+        const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
         for (int dz = -1; dz <= 1; ++dz) {
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
@@ -248,7 +423,10 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
                 }
             }
         }
-    } else {
+    }
+    if constexpr (LOOP == kLoopCompact) {
+        const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
+        const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
         __shared__ uint32_t nbr[kNbrCap * kBlock];
         uint32_t* mine = nbr + threadIdx.x;
         int cnt = 0;
@@ -413,7 +591,10 @@ int g_force_variant = 0;
 // Threads per block of a neighbour launch (a multiple of 64, at most kBlock)
 // and its force-loop variant.
 static inline int nbr_block(const Win& w) { return w.block > 0 && w.block <= kBlock && w.block % 64 == 0 ? w.block : kBlock; }
-static inline int nbr_variant(const Win& w) { return w.variant >= 0 ? w.variant : g_force_variant; }
+static inline int nbr_variant(const Win& w) {
+    const int v = w.variant >= 0 ? w.variant : g_force_variant;
+    return v >= kLoopDirect && v <= kLoopList ? v : kLoopDirect;
+}
 
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
                        uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, int sub_bits,
@@ -452,44 +633,59 @@ hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const 
 
 hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
                           const rtps_sph_params& sp, const Grid& g, const Win& win, float* dens, float4* packed,
-                          hipStream_t st) {
+                          hipStream_t st, const RowMasks* masks) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
     const int bs = nbr_block(win);
     const int nb = blocks_for(num, bs);
-    if (packed) {
-        if (scaled) k_density<true, true><<<nb, bs, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
-        else k_density<false, true><<<nb, bs, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
-    } else {
-        if (scaled) k_density<true, false><<<nb, bs, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
-        else k_density<false, false><<<nb, bs, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);
-    }
+    const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
+#define RTPS_DENS(S, P, M) k_density<S, P, M><<<nb, bs, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed, m)
+#define RTPS_DENS_R(R)                                                                                              \
+    if (packed) { if (scaled) RTPS_DENS(true, true, R); else RTPS_DENS(false, true, R); }                             \
+    else { if (scaled) RTPS_DENS(true, false, R); else RTPS_DENS(false, false, R); }
+    if (m.rows && m.form == kRecList) { RTPS_DENS_R(kRecList) }
+    else if (m.rows && m.form == kRecBits) { RTPS_DENS_R(kRecBits) }
+    else { RTPS_DENS_R(kRecNone) }
+#undef RTPS_DENS_R
+#undef RTPS_DENS
     return hipGetLastError();
+}
+
+// The masked loop needs the rows the density kernel of this step recorded;
+// without them it falls back to the direct loop.
+static inline int loop_of(const Win& w, const RowMasks& m) {
+    const int v = nbr_variant(w);
+    if (v == kLoopMasked && !(m.rows && m.form == kRecBits)) return kLoopDirect;
+    if (v == kLoopList && !(m.rows && m.form == kRecList)) return kLoopDirect;
+    return v;
 }
 
 hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const float* dens, const float4* ve,
                         bool packed, const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp,
-                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st) {
+                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st,
+                        const RowMasks* masks) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
-    const int bs = nbr_block(win), variant = nbr_variant(win);
+    const int bs = nbr_block(win);
     const int nb = blocks_for(num, bs);
+    const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
+    const int loop = loop_of(win, m);
     GravitySources none{};
     rtps_float4 z{0, 0, 0, 0};
+#define RTPS_FORCE_L(S, P, L)                                                                                       \
+    k_force<S, P, false, L><<<nb, bs, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force, xsph,   \
+                                               nullptr, nullptr, none, 0, z, z, 0.f, nullptr, nullptr, nullptr, m)
 #define RTPS_FORCE(S, P)                                                                                            \
-    if (variant)                                                                                                    \
-        k_force<S, P, false, true><<<nb, bs, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force,   \
-                                                      xsph, nullptr, nullptr, none, 0, z, z, 0.f, nullptr,         \
-                                                      nullptr, nullptr);                                           \
-    else                                                                                                            \
-        k_force<S, P, false, false><<<nb, bs, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force,  \
-                                                       xsph, nullptr, nullptr, none, 0, z, z, 0.f, nullptr,        \
-                                                       nullptr, nullptr)
-    if (packed) { if (scaled) RTPS_FORCE(true, true); else RTPS_FORCE(false, true); }
-    else { if (scaled) RTPS_FORCE(true, false); else RTPS_FORCE(false, false); }
+    if (loop == kLoopList) RTPS_FORCE_L(S, P, kLoopList);                                                       \
+    else if (loop == kLoopMasked) RTPS_FORCE_L(S, P, kLoopMasked);                                                  \
+    else if (loop == kLoopCompact) RTPS_FORCE_L(S, P, kLoopCompact);                                                \
+    else RTPS_FORCE_L(S, P, kLoopDirect)
+    if (packed) { if (scaled) { RTPS_FORCE(true, true); } else { RTPS_FORCE(false, true); } }
+    else { if (scaled) { RTPS_FORCE(true, false); } else { RTPS_FORCE(false, false); } }
 #undef RTPS_FORCE
+#undef RTPS_FORCE_L
     return hipGetLastError();
 }
 
@@ -497,19 +693,27 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
                               const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
                               const Win& win, float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
                               const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,
-                              float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st) {
+                              float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
+                              const RowMasks* masks) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
-    const int bs = nbr_block(win), variant = nbr_variant(win);
+    const int bs = nbr_block(win);
     const int nb = blocks_for(num, bs);
-#define RTPS_FFUSED(S, C)                                                                                           \
-    k_force<S, true, true, C><<<nb, bs, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
+    const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
+    const int loop = loop_of(win, m);
+#define RTPS_FFUSED_L(S, L)                                                                                         \
+    k_force<S, true, true, L><<<nb, bs, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
                                                  xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt,  \
-                                                 pos_u, vel_u, veleval_u)
-    if (scaled) { if (variant) RTPS_FFUSED(true, true); else RTPS_FFUSED(true, false); }
-    else { if (variant) RTPS_FFUSED(false, true); else RTPS_FFUSED(false, false); }
+                                                 pos_u, vel_u, veleval_u, m)
+#define RTPS_FFUSED(S)                                                                                              \
+    if (loop == kLoopList) RTPS_FFUSED_L(S, kLoopList);                                                         \
+    else if (loop == kLoopMasked) RTPS_FFUSED_L(S, kLoopMasked);                                                    \
+    else if (loop == kLoopCompact) RTPS_FFUSED_L(S, kLoopCompact);                                                  \
+    else RTPS_FFUSED_L(S, kLoopDirect)
+    if (scaled) { RTPS_FFUSED(true); } else { RTPS_FFUSED(false); }
 #undef RTPS_FFUSED
+#undef RTPS_FFUSED_L
     return hipGetLastError();
 }
 

@@ -46,6 +46,8 @@ struct rtps_sph {
     uint32_t *keys_sorted = nullptr, *idx_sorted = nullptr;
     int* d_num = nullptr;
     void* sort_tmp = nullptr;
+    // neighbour rows recorded by density for the force loops 2 (bit form) and 3 (list form)
+    RowMasks rm{nullptr, nullptr, nullptr, nullptr, 0, 1}, rl{nullptr, nullptr, nullptr, nullptr, 0, 2};
     GravitySources gsrc{};
     int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0;
     int opt_autotune = 1;
@@ -71,7 +73,7 @@ struct rtps_sph {
         for (auto& g : gexec_t)
             if (g) (void)hipGraphExecDestroy(g);
         void* bufs[] = {pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s, force_s, xsph_s, keys, idx, start, end, d_num,
-                        sort_tmp};
+                        sort_tmp, rm.rows, rm.words, rm.codes, rm.count};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         for (auto& set : ev)
@@ -82,6 +84,13 @@ struct rtps_sph {
 };
 
 namespace {
+
+// Force-loop variant in use, and the recorded rows when it is the masked loop.
+int loop_variant(const rtps_sph* s) { return s->force_variant >= 0 ? s->force_variant : g_force_variant; }
+const RowMasks* masks_of(const rtps_sph* s) {
+    const int v = loop_variant(s);
+    return v == 2 ? &s->rm : (v == 3 ? &s->rl : nullptr);
+}
 
 // Launch inputs of the density (force = false) or force kernel.
 Win win(const rtps_sph* s, bool force = false) {
@@ -163,7 +172,7 @@ int launch_front(rtps_sph* s, StageTimer& t) {
                                 s->ve_s, s->stream));
     t.mark(ST_DENSITY);
     RTPS_TRY_HIP(launch_density(n, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, win(s), nullptr, s->ve_s,
-                                s->stream));
+                                s->stream, masks_of(s)));
     return RTPS_OK;
 }
 
@@ -176,7 +185,7 @@ int launch_step(rtps_sph* s, StageTimer& t) {
     RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, win(s, true),
                                     fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s, s->idx_sorted,
                                     s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u, s->veleval_u,
-                                    s->stream));
+                                    s->stream, masks_of(s)));
     RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
     t.mark(ST_N);
     return RTPS_OK;
@@ -365,7 +374,7 @@ std::string tune_key(const rtps_sph* s) {
     while (n2 < (unsigned long)s->num_host) n2 <<= 1;
     return arch + "/cu" + std::to_string(cus) + "/n" + std::to_string(n2) + "/c" + std::to_string(s->nb_cells) +
            "/w" + std::to_string(s->opt_window) + "/r" + std::to_string(s->opt_remap) + "/s" +
-           std::to_string(s->sp.simulation_scale != 1.0f ? 1 : 0);
+           std::to_string(s->sp.simulation_scale != 1.0f ? 1 : 0) + "/a" + std::to_string(s->opt_autotune == 2 ? 2 : 1);
 }
 
 void apply_tune(rtps_sph* s, const Tune& t) {
@@ -410,21 +419,29 @@ int autotune_step(rtps_sph* s, bool measure) {
         return ms / kReps;
     };
     const int blocks[] = {64, 128, 256};
-    float best_d = 1e30f, best_f = 1e30f;
-    Tune best{256, 256, 0};
+    // density: plain, recording bits, recording lists (each left recorded for the force loops timed next);
+    // the recorded forms (force loops 2 and 3) are candidates only with RTPS_OPT_AUTOTUNE = 2
+    const bool full = s->opt_autotune == 2;
+    const int nmodes = full ? 3 : 1, nvars = full ? 4 : 2;
+    float best_d[3] = {1e30f, 1e30f, 1e30f};
+    int bd[3] = {256, 256, 256};
     bool failed = false;
-    for (int bs : blocks) {
-        Win w = win(s, false);
-        w.block = bs;
-        const float ms = time_it([&] {
-            return launch_density(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, w, nullptr, s->ve_s,
-                                  s->stream);
-        });
-        if (ms < 0.f) failed = true;
-        else if (ms < best_d) { best_d = ms; best.bd = bs; }
-    }
+    const RowMasks* recs[3] = {nullptr, &s->rm, &s->rl};
+    for (int mode = 0; mode < nmodes; ++mode)
+        for (int bs : blocks) {
+            Win w = win(s, false);
+            w.block = bs;
+            const float ms = time_it([&] {
+                return launch_density(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, w, nullptr,
+                                      s->ve_s, s->stream, recs[mode]);
+            });
+            if (ms < 0.f) failed = true;
+            else if (ms < best_d[mode]) { best_d[mode] = ms; bd[mode] = bs; }
+        }
     const bool fo = s->opt_force_out != 0;
-    for (int var = 0; var < 2 && !failed; ++var)
+    float best_total = 1e30f;
+    Tune best{256, 256, 0};
+    for (int var = 0; var < nvars && !failed; ++var)
         for (int bs : blocks) {
             Win w = win(s, true);
             w.block = bs;
@@ -433,10 +450,14 @@ int autotune_step(rtps_sph* s, bool measure) {
                 return launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, w,
                                           fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
                                           s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u,
-                                          s->vel_u, s->veleval_u, s->stream);
+                                          s->vel_u, s->veleval_u, s->stream, recs[var >= 2 ? var - 1 : 0]);
             });
+            const int mode = var >= 2 ? var - 1 : 0;
             if (ms < 0.f) failed = true;
-            else if (ms < best_f) { best_f = ms; best.bf = bs; best.var = var; }
+            else if (ms + best_d[mode] < best_total) {
+                best_total = ms + best_d[mode];
+                best = Tune{bd[mode], bs, var};
+            }
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -445,11 +466,11 @@ int autotune_step(rtps_sph* s, bool measure) {
     tune_put(key, best);
     // finish the step with the chosen configuration
     RTPS_TRY_HIP(launch_density(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, win(s), nullptr,
-                                s->ve_s, s->stream));
+                                s->ve_s, s->stream, masks_of(s)));
     RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs,
                                     win(s, true), fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
                                     s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u,
-                                    s->veleval_u, s->stream));
+                                    s->veleval_u, s->stream, masks_of(s)));
     RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
     return RTPS_OK;
 }
@@ -497,6 +518,13 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
         RTPS_TRY_HIP(dalloc(&s->start, C));
         RTPS_TRY_HIP(dalloc(&s->end, C));
         RTPS_TRY_HIP(dalloc(&s->d_num, 1));
+        RTPS_TRY_HIP(dalloc(&s->rm.rows, 9 * N));
+        RTPS_TRY_HIP(dalloc(&s->rm.words, 9 * (size_t)kMaskWords * N));
+        RTPS_TRY_HIP(dalloc(&s->rm.codes, (size_t)(kListCap / 4) * N));
+        RTPS_TRY_HIP(dalloc(&s->rm.count, N));
+        s->rm.stride = (int)N;
+        s->rl = s->rm;
+        s->rl.form = 2;
         RTPS_TRY_HIP(hipMalloc(&s->sort_tmp, sort_temp_bytes((int)N, s->key_bits)));
         float4* zf[] = {s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s, s->ve_s, s->force_s, s->xsph_s};
         for (float4* p : zf) RTPS_TRY_HIP(hipMemsetAsync(p, 0, N * sizeof(float4), s->stream));
@@ -562,7 +590,7 @@ int rtps_sph_update(rtps_sph* s) {
     int rc = launch_front(s, t);
     if (rc) return rc;
     RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp, s->gs,
-                              win(s, true), s->force_s, s->xsph_s, s->stream));
+                              win(s, true), s->force_s, s->xsph_s, s->stream, masks_of(s)));
     RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                    s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
     s->update_pending = true;
@@ -701,11 +729,18 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_GRAPH: s->opt_graph = value ? 1 : 0; break;
         case RTPS_OPT_FORCE_OUT: s->opt_force_out = value ? 1 : 0; break;
         case RTPS_OPT_TIMERS: s->opt_timers = value ? 1 : 0; break;
-        case RTPS_OPT_FORCE_VARIANT: g_force_variant = value ? 1 : 0; break;
+        case RTPS_OPT_FORCE_VARIANT:
+            if (value < 0 || value > 3) return set_error(RTPS_ERR_ARG, "force variant must be 0, 1, 2 or 3");
+            g_force_variant = value;
+            break;
         case RTPS_OPT_WINDOW: s->opt_window = value ? 1 : 0; break;
-        case RTPS_OPT_TILE_REMAP: s->opt_remap = value ? 1 : 0; break;
+        case RTPS_OPT_TILE_REMAP:
+            if (value < 0 || value > 4096) return set_error(RTPS_ERR_ARG, "tile remap must be 0, 1 or a chunk of 2..4096");
+            s->opt_remap = value;
+            break;
         case RTPS_OPT_AUTOTUNE:
-            s->opt_autotune = value ? 1 : 0;
+            if (value < 0 || value > 2) return set_error(RTPS_ERR_ARG, "autotune must be 0, 1 or 2");
+            s->opt_autotune = value;
             if (!value) apply_tune(s, Tune{0, 0, -1});
             s->tuned = false;
             break;

@@ -1,0 +1,75 @@
+"""Per-kernel summary of a rocprofv3 kernel trace, split by launch shape.
+
+    python3 -m enjaparticles_amd.rocprof_summary RESULTS.db [--top N] [--match REGEX]
+    python3 -m enjaparticles_amd.rocprof_summary COUNTERS.csv [--match REGEX]
+
+rocprofv3 --stats averages every launch of a kernel name together; a run that
+steps systems of different sizes (the plain 1M-particle step, the batched
+scenes' sequential comparison, the adjoint's 65,536-particle rollout ...)
+launches the same kernel with different grids.  This prints, per kernel name
+and grid size, the launch count and the average / min / max duration (us),
+read from the rocpd SQLite database that rocprofv3 writes (-d DIR -o NAME
+gives DIR/NAME_results.db).  Given the counter_collection.csv of a --pmc run
+(--output-format csv), it prints the mean of each counter per kernel and
+launch shape instead.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import re
+import sqlite3
+from collections import defaultdict
+
+
+def summarize(db_path: str, top: int = 40, match: str | None = None):
+    db = sqlite3.connect(db_path)
+    cur = db.cursor()
+    rows = cur.execute(
+        "select s.display_name, d.grid_size_x, d.workgroup_size_x, count(*), avg(d.end - d.start), "
+        "min(d.end - d.start), max(d.end - d.start), sum(d.end - d.start) "
+        "from rocpd_kernel_dispatch d join rocpd_info_kernel_symbol s on d.kernel_id = s.id "
+        "group by s.display_name, d.grid_size_x, d.workgroup_size_x order by sum(d.end - d.start) desc").fetchall()
+    total = sum(r[7] for r in rows) or 1
+    out = []
+    for name, grid, wg, n, avg, mn, mx, tot in rows:
+        short = name.split("(")[0]
+        if match and not re.search(match, short):
+            continue
+        out.append((short, grid, wg, n, avg / 1e3, mn / 1e3, mx / 1e3, 100.0 * tot / total))
+    return out[:top]
+
+
+def counters(csv_path: str, match: str | None = None):
+    acc = defaultdict(lambda: [0.0, 0])
+    with open(csv_path, newline="") as f:
+        for row in csv.DictReader(f):
+            name = row["Kernel_Name"].split("(")[0]
+            if match and not re.search(match, name):
+                continue
+            key = (name, int(row["Grid_Size"]), int(row["Workgroup_Size"]), row["Counter_Name"])
+            a = acc[key]
+            a[0] += float(row["Counter_Value"])
+            a[1] += 1
+    return sorted(((k[0], k[1], k[2], k[3], v[1], v[0] / v[1]) for k, v in acc.items()),
+                  key=lambda r: (r[0], r[1], r[2], r[3]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("db")
+    ap.add_argument("--top", type=int, default=40)
+    ap.add_argument("--match", default=None)
+    a = ap.parse_args()
+    if a.db.endswith(".csv"):
+        print(f"{'kernel':58s} {'grid':>9s} {'wg':>4s} {'counter':>28s} {'calls':>6s} {'mean':>16s}")
+        for r in counters(a.db, a.match):
+            print(f"{r[0][:58]:58s} {r[1]:9d} {r[2]:4d} {r[3]:>28s} {r[4]:6d} {r[5]:16.2f}")
+        return
+    print(f"{'kernel':58s} {'grid':>9s} {'wg':>4s} {'calls':>7s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s} {'%':>6s}")
+    for r in summarize(a.db, a.top, a.match):
+        print(f"{r[0][:58]:58s} {r[1]:9d} {r[2]:4d} {r[3]:7d} {r[4]:10.2f} {r[5]:10.2f} {r[6]:10.2f} {r[7]:6.2f}")
+
+
+if __name__ == "__main__":
+    main()

@@ -212,5 +212,40 @@ def test_autotune_choice_does_not_change_the_step(cuda):
     b.step(4)
     np.testing.assert_array_equal(b.read("pos"), a.read("pos"))
     np.testing.assert_array_equal(b.read("vel"), a.read("vel"))
-    for s in (a, b):
+    c = _system(sc)
+    c.setOption(7, 2)          # all four force loops are candidates
+    tune = c.autotune(measure=True)
+    assert tune["force_variant"] in (0, 1, 2, 3)
+    c.step(4)
+    np.testing.assert_allclose(c.read("pos"), a.read("pos"), rtol=1e-6, atol=1e-7)
+    for s in (a, b, c):
         s.close()
+
+
+@pytest.mark.parametrize("scene", ["dam", "clump"])
+def test_force_loop_variants_agree(cuda, scene):
+    """The four force loops (direct, compacted LDS lists, bits recorded by the
+    density kernel, neighbour-code lists recorded by the density kernel) visit
+    the same pairs in the same order.  The clump packs 300 particles into one
+    cell, so its rows hold more candidates than the recorded bits cover and
+    its particles overflow the recorded lists: both fallbacks are exercised."""
+    sc = scenes.dam_custom(20, 20, 10, seed=80)
+    if scene == "clump":
+        rng = np.random.default_rng(81)
+        pos = sc.pos.copy()
+        pos[:300, :3] = 0.3 + 0.012 * rng.random((300, 3), dtype=np.float32)
+        sc = scenes.Scene("clump", sc.settings, pos, sc.vel, 2, sc.dt)
+    out = []
+    for v in (0, 1, 2, 3):
+        s = _system(sc)
+        s.setOption(7, 0)      # autotune off: the process-wide variant applies
+        s.setOption(4, v)
+        s.step(2)
+        out.append((s.read("pos"), s.read("vel")))
+        s.close()
+    s = _system(sc)            # the variant option is process-wide: restore the default
+    s.setOption(4, 0)
+    s.close()
+    for v in (1, 2, 3):
+        np.testing.assert_allclose(out[v][0], out[0][0], rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(out[v][1], out[0][1], rtol=1e-5, atol=1e-6)

@@ -258,6 +258,28 @@ def test_wall_repulsion_and_friction_quirk():
     assert np.all(f[2] == 0)                               # far from every wall
 
 
+def test_gravity_point_sources():
+    """gravity.cl:9-36 closed forms: outside the cutoff (0.5 scale) a source
+    pulls with alpha M / d^2 (d in scaled units) along the unit direction;
+    inside it pushes away with alpha M / (0.5 scale)^2; sources superpose
+    onto the incoming acceleration.  scale = 2 catches a missing factor."""
+    src = np.array([[1.0, 0.0, 0.0, 1.0], [0.0, 0.0, 5.0, 1.0]], np.float32)
+    mass = np.array([3.0, 2.0], np.float32)
+    alpha = np.array([0.5, 0.25], np.float32)
+    pos = np.array([[0.0, 0.0, 0.0, 1.0], [0.9, 0.0, 0.0, 1.0]], np.float32)
+    f0 = np.array([[0.1, 0.2, 0.3, 0.0], [0.0, 0.0, 0.0, 0.0]], np.float32)
+    s = 2.0
+    for dt in (np.float32, np.float64):
+        f = O.gravity(pos.astype(dt), f0.astype(dt), src, mass, alpha, s)
+        # particle 0: source 0 at scaled distance 2 (outside), source 1 at 10 (outside)
+        e0 = f0[0, :3] + 1.5 / 4.0 * np.array([1, 0, 0]) + 0.5 / 100.0 * np.array([0, 0, 1])
+        # particle 1: source 0 at scaled distance 0.2 < cutoff 1 -> pushed away with 1.5 / 1
+        d1 = 2.0 * (src[1, :3] - pos[1, :3])
+        e1 = -1.5 * np.array([1, 0, 0]) + 0.5 / np.dot(d1, d1) * d1 / np.linalg.norm(d1)
+        np.testing.assert_allclose(f[0, :3], e0, rtol=1e-6)
+        np.testing.assert_allclose(f[1, :3], e1, rtol=1e-6)
+
+
 def test_free_fall_and_clamp():
     st = scenes.dam_settings(1.0, 16)
     sp, gp, gs = O.derive_params(st)
