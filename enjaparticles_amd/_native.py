@@ -23,8 +23,8 @@ FIELD = {
     "pos": 0, "vel": 1, "veleval": 2, "pos_s": 3, "vel_s": 4, "veleval_s": 5, "density": 6,
     "force": 7, "xsph": 8, "keys": 9, "idx": 10, "cell_start": 11, "cell_end": 12,
 }
-OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS, OPT_FORCE_VARIANT, OPT_WINDOW, OPT_TILE_REMAP, OPT_AUTOTUNE = (
-    0, 1, 2, 3, 4, 5, 6, 7)
+(OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS, OPT_FORCE_VARIANT, OPT_WINDOW, OPT_TILE_REMAP, OPT_AUTOTUNE,
+ OPT_SORT_TILE) = (0, 1, 2, 3, 4, 5, 6, 7, 8)
 NUM_STAGES = 6
 STAGE_NAMES = ("hash", "sort", "cellindices", "permute", "density", "force")
 MAX_GRAVITY_SOURCES = 16

@@ -141,9 +141,7 @@ __global__ __launch_bounds__(kBBlock) void k_pairs_b(const BatchItem* __restrict
                     for (uint32_t j = a; j < b; ++j) {
                         float4 pj = pos_s[j];
                         float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
-                        float r2 = rx * rx + ry * ry + rz * rz;
-                        float t = h2 - r2;
-                        sum += (r2 <= h2) ? t * t * t : 0.0f;
+                        poly6_add(sum, rx * rx + ry * ry + rz * rz, h2);
                     }
                 }
             ve[i].w = sp.wpoly6_coef * sp.mass * sum;

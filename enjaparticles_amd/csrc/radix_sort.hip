@@ -4,7 +4,8 @@
 // ordered (the cell keys are < 2^key_bits); digits are at most 8 bits wide and
 // the bits are split evenly over ceil(key_bits / 8) passes.
 //
-// Per pass (R = 2^bits buckets, tiles of 4096 keys = 256 threads x 16):
+// Per pass (R = 2^bits buckets, tiles of 256 threads x ITEMS keys, ITEMS =
+// 4, 8 or 16; g_sort_items picks it, 8 by default):
 //   k_radix_hist    per-tile digit histogram, digit-major in hist[d * ntiles + t];
 //                   lanes of a wave holding the same digit are found with
 //                   `bits` ballots and add their count with one LDS atomic;
@@ -28,19 +29,20 @@
 namespace rtps {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;   // 4096
+constexpr int kSortMinItems = 4;    // smallest tile: sizes the tile-histogram buffer
 constexpr int kSortWaves = kSortThreads / 64;
+
+int g_sort_items = 8;   // keys per thread of a sort tile (4, 8 or 16); 8 measured fastest at 1M keys
 constexpr int kMaxRadix = 256;
 constexpr int kMaxPasses = 4;
 
 int sort_passes(int key_bits) { return key_bits <= 0 ? 0 : (key_bits + 7) / 8; }
 
-static inline int ntiles_for(int n) { return (n + kSortTile - 1) / kSortTile; }
+static inline int ntiles_for(int n, int items) { return (n + kSortThreads * items - 1) / (kSortThreads * items); }
 
 size_t sort_temp_bytes(int n, int key_bits) {
     (void)key_bits;
-    const size_t nt = (size_t)ntiles_for(n > 0 ? n : 1);
+    const size_t nt = (size_t)ntiles_for(n > 0 ? n : 1, kSortMinItems);
     size_t b = (size_t)kMaxPasses * kMaxRadix * sizeof(uint32_t);   // digit totals per pass (fixed offset 0)
     b += 2 * (size_t)n * sizeof(uint32_t);                           // alternate keys / values
     b = (b + 255) & ~(size_t)255;
@@ -79,6 +81,7 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, bool valid
     return peers;
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ kin, int n, int shift,
                                                              int bits, uint32_t* __restrict__ hist,
                                                              uint32_t* __restrict__ gtot, int ntiles) {
@@ -89,8 +92,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
     const uint64_t lt = (1ull << lane) - 1ull;
     for (int d = threadIdx.x; d < R; d += kSortThreads) h[d] = 0;
     __syncthreads();
-    const long base = (long)blockIdx.x * kSortTile;
-    for (int k = 0; k < kSortItems; ++k) {
+    const long base = (long)blockIdx.x * (kSortThreads * ITEMS);
+    for (int k = 0; k < ITEMS; ++k) {
         const long e = base + k * kSortThreads + threadIdx.x;
         const bool valid = e < n;
         const uint32_t d = valid ? (kin[e] >> shift) & mask : 0u;
@@ -159,13 +162,14 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scan(uint32_t* __restric
     }
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout,
                                                                 uint32_t* __restrict__ vout, int n, int shift,
                                                                 int bits, const uint32_t* __restrict__ hist,
                                                                 int ntiles, uint32_t* __restrict__ gtot_clear) {
-    constexpr int kQuarter = kSortTile / kSortWaves;   // 1024 keys per wave, in rounds of 64
+    constexpr int kQuarter = kSortThreads * ITEMS / kSortWaves;   // keys per wave, in rounds of 64
     __shared__ uint32_t cnt[kSortWaves][kMaxRadix];    // running count per (wave, digit), then wave offsets
     __shared__ uint32_t goff[kMaxRadix];
     const int R = 1 << bits;
@@ -178,10 +182,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
         for (int d = tid; d < R; d += kSortThreads) gtot_clear[d] = 0;
     __syncthreads();
 
-    const long base = (long)blockIdx.x * kSortTile + (long)w * kQuarter;
-    uint32_t key[kSortItems], val[kSortItems], pos[kSortItems];
+    const long base = (long)blockIdx.x * (kSortThreads * ITEMS) + (long)w * kQuarter;
+    uint32_t key[ITEMS], val[ITEMS], pos[ITEMS];
 #pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         const long e = base + k * 64 + lane;
         const bool valid = e < n;
         key[k] = valid ? kin[e] : 0xFFFFFFFFu;
@@ -204,7 +208,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         const long e = base + k * 64 + lane;
         if (e < n) {
             const uint32_t d = (key[k] >> shift) & mask;
@@ -255,15 +259,21 @@ hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int np
     if (n <= 1 || npasses == 0) return hipSuccess;
     if (npasses > kMaxPasses) return hipErrorInvalidValue;
     SortTemp t = carve(temp, n);
-    const int ntiles = ntiles_for(n);
+    const int items = g_sort_items == 4 || g_sort_items == 16 ? g_sort_items : 8;
+    const int ntiles = ntiles_for(n, items);
     uint32_t *ki = keys, *vi = vals, *ko = t.kalt, *vo = t.valt;
     for (int p = 0; p < npasses; ++p) {
         const int bits = widths[p], shift = shifts[p];
         if (bits < 1 || bits > 8) return hipErrorInvalidValue;
         uint32_t* gtot = t.gtot + p * kMaxRadix;
-        k_radix_hist<<<ntiles, kSortThreads, 0, st>>>(ki, n, shift, bits, t.hist, gtot, ntiles);
-        k_radix_scan<<<1 << bits, kSortThreads, 0, st>>>(t.hist, gtot, ntiles);
-        k_radix_scatter<<<ntiles, kSortThreads, 0, st>>>(ki, vi, ko, vo, n, shift, bits, t.hist, ntiles, gtot);
+#define RTPS_SORT_PASS(I)                                                                                           \
+        k_radix_hist<I><<<ntiles, kSortThreads, 0, st>>>(ki, n, shift, bits, t.hist, gtot, ntiles);                   \
+        k_radix_scan<<<1 << bits, kSortThreads, 0, st>>>(t.hist, gtot, ntiles);                                      \
+        k_radix_scatter<I><<<ntiles, kSortThreads, 0, st>>>(ki, vi, ko, vo, n, shift, bits, t.hist, ntiles, gtot)
+        if (items == 4) { RTPS_SORT_PASS(4); }
+        else if (items == 8) { RTPS_SORT_PASS(8); }
+        else { RTPS_SORT_PASS(16); }
+#undef RTPS_SORT_PASS
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         uint32_t* tk = ki; ki = ko; ko = tk;

@@ -65,6 +65,7 @@ hipError_t launch_commit_num(int* d_num, const uint32_t* start, int nb_cells, hi
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st);
 
 // ---- radix sort (radix_sort.hip) ----
+extern int g_sort_items;   // keys per thread of a sort tile: 4, 8 or 16 (process-wide)
 int sort_passes(int key_bits);
 size_t sort_temp_bytes(int n, int key_bits);
 hipError_t sort_clear_temp(void* temp, int n, hipStream_t st);

@@ -230,6 +230,13 @@ __device__ __forceinline__ int remap_tile(int remap, int b, int nb) {
 
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// density.cl:44-78 poly6 term of one candidate: sum += (h^2 - r^2)^3 for
+// r <= h, else nothing; t = max(h^2 - r^2, 0) and one fused multiply-add.
+__device__ __forceinline__ void poly6_add(float& sum, float r2, float h2) {
+    const float t = fmaxf(h2 - r2, 0.0f);
+    sum = fmaf(t * t, t, sum);
+}
+
 // Per-particle pair-sum accumulator of force.cl:43-140 (pt.force, pt.viscosity, pt.xsph).
 struct ForceAcc {
     float fx, fy, fz;

@@ -130,10 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         return rx * rx + ry * ry + rz * rz;
     };
-    auto term = [&](float r2) {
-        float t = h2 - r2;
-        return (r2 <= h2) ? t * t * t : 0.0f;
-    };
+    auto add_term = [&](float r2) { poly6_add(sum, r2, h2); };
     const size_t st = (size_t)masks.stride;
     constexpr bool MASKS = REC == kRecBits, LIST = REC == kRecList;
     // list form: codes packed four to a slot, flushed as one 8-byte store
@@ -178,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
 #pragma unroll
                 for (int k = 0; k < kGroup; ++k) {
                     const float r2 = sep2(p[k]);
-                    sum += term(r2);
+                    add_term(r2);
                     const bool acc_k = r2 <= h2 && j + k != (uint32_t)i;
                     if (MASKS) nib |= (uint32_t)acc_k << k;
                     if (LIST && acc_k) append(rtag | (j + k - a));
@@ -190,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
             }
             for (; j < b; ++j, ++q) {
                 const float r2 = sep2(pos_s[j]);
-                sum += term(r2);
+                add_term(r2);
                 const bool acc_k = r2 <= h2 && j != (uint32_t)i;
                 if (MASKS) {
                     word |= (uint32_t)acc_k << (q & 31u);

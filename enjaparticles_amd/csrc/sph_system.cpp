@@ -738,6 +738,11 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
             if (value < 0 || value > 4096) return set_error(RTPS_ERR_ARG, "tile remap must be 0, 1 or a chunk of 2..4096");
             s->opt_remap = value;
             break;
+        case RTPS_OPT_SORT_TILE:
+            if (value != 4 && value != 8 && value != 16)
+                return set_error(RTPS_ERR_ARG, "sort tile must be 4, 8 or 16 keys per thread");
+            g_sort_items = value;
+            break;
         case RTPS_OPT_AUTOTUNE:
             if (value < 0 || value > 2) return set_error(RTPS_ERR_ARG, "autotune must be 0, 1 or 2");
             s->opt_autotune = value;

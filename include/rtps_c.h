@@ -218,6 +218,8 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        rtps_sph_autotune), 2 = over all four loops (2 and 3
  *                        win while the neighbourhoods are sparse, lose once
  *                        they fill), 0 = fixed defaults
+ *   RTPS_OPT_SORT_TILE   keys per thread of a radix-sort tile (256 threads),
+ *                        process-wide: 4, 8 or 16 (default 8)
  *   RTPS_OPT_TILE_REMAP  tile order of the neighbour kernels over the 8 XCDs:
  *                        0 = hardware round-robin (default), 1 = one
  *                        contiguous range per XCD, C in 2..4096 = runs of C
@@ -230,6 +232,7 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
 #define RTPS_OPT_WINDOW 5
 #define RTPS_OPT_TILE_REMAP 6
 #define RTPS_OPT_AUTOTUNE 7
+#define RTPS_OPT_SORT_TILE 8
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))

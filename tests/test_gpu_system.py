@@ -249,3 +249,22 @@ def test_force_loop_variants_agree(cuda, scene):
     for v in (1, 2, 3):
         np.testing.assert_allclose(out[v][0], out[0][0], rtol=1e-6, atol=1e-7)
         np.testing.assert_allclose(out[v][1], out[0][1], rtol=1e-5, atol=1e-6)
+
+
+def test_sort_tile_sizes_give_the_same_step(cuda):
+    """Radix-sort tiles of 1024, 2048 and 4096 keys (RTPS_OPT_SORT_TILE) sort
+    identically (stable), so the steps are bit-identical."""
+    sc = scenes.dam_custom(32, 32, 16, seed=82)
+    out = {}
+    for items in (16, 8, 4):
+        s = _system(sc)
+        s.setOption(8, items)
+        s.step(3)
+        out[items] = (s.read("pos"), s.read("keys"), s.read("idx"))
+        s.close()
+    s = _system(sc)
+    s.setOption(8, 8)           # process-wide: restore the default
+    s.close()
+    for items in (16, 4):
+        for a, b in zip(out[items], out[8]):
+            np.testing.assert_array_equal(a, b)
