@@ -250,11 +250,14 @@ struct ForceAcc {
 __device__ __forceinline__ void force_pair(ForceAcc& acc, float rx, float ry, float rz, float r2, float h, float h2,
                                            float eps, float K, float rho0, float Pi, float di, float4 vi, float4 vj,
                                            float dj) {
-    // force.cl:73.  The hardware square root (<= 1 ulp) stands in for OpenCL's
-    // length(), which the reference builds without correctly rounded sqrt.
-    float rlen = fmaxf(__builtin_amdgcn_sqrtf(r2), eps);
+    // force.cl:73, rlen = max(|r|, eps), and 1/rlen from one hardware
+    // reciprocal square root (~1 ulp; the reference builds OpenCL's length()
+    // without correctly rounded sqrt): r2 clamped to eps^2 gives the same max.
+    const float r2c = fmaxf(r2, eps * eps);
+    const float rinv = __builtin_amdgcn_rsqf(r2c);
+    float rlen = r2c * rinv;
     float hr = h - rlen;
-    float dW = hr * hr * fast_rcp(rlen);            // Wspiky_dr, cl_sph_kernels.h:81-97
+    float dW = hr * hr * rinv;                      // Wspiky_dr, cl_sph_kernels.h:81-97
     float idj = fast_rcp(dj);
     float Pj = K * (dj - rho0);                     // force.cl:84-85
     float kern = dW * (Pi + Pj) * idj;              // :97
