@@ -198,6 +198,51 @@ def batched_adjoint_leg(steps: int = 100, num_scenes: int = 64, checkpoint_every
             "adjoint_over_plain": tb / tp, "finite_gradients": bool(np.all(np.isfinite(dmu + dK)))}
 
 
+def compare4m_leg(steps: int = 100):
+    """Plain and batched forward at 4,194,304 particles: one 256x128x128 dam
+    break, and 64 scenes of 64x64x16, stepped `steps` steps each (hipGraph),
+    with per-stage times.  HBM and L2 counters come from profiling this mode."""
+    from enjaparticles_amd import scenes
+    from enjaparticles_amd.rtps import RTPS, BatchedSPH
+    import enjaparticles_amd._native as N
+    res = {}
+    sc = scenes.dam4m()
+    s = RTPS.generateSystemInstance(sc.settings)
+    s.pushParticles(sc.pos, sc.vel)
+    s.autotune(measure=True)
+    s.step(9)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    tp = time.perf_counter() - t0
+    s.setOption(N.OPT_TIMERS, 1)
+    s.resetTimers()
+    s.step(5)
+    res["plain"] = {"particles": sc.n, "value": sc.n * steps / tp, "ms_per_step": 1e3 * tp / steps,
+                    "stages_ms": {k: round(v[0], 4) for k, v in s.stageTimes().items()}}
+    s.close()
+    scs = scenes.batched_equal(64, 64, 64, 16, seed=1007)
+    total = sum(x.n for x in scs)
+    b = BatchedSPH([x.settings for x in scs])
+    for i, x in enumerate(scs):
+        b.pushParticles(i, x.pos, x.vel)
+    b.step(10)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    tb = time.perf_counter() - t0
+    b.setOption(3, 1)
+    b.resetTimers()
+    b.step(5)
+    res["batched"] = {"scenes": 64, "particles": total, "value": total * steps / tb, "ms_per_step": 1e3 * tb / steps,
+                      "stages_ms": {k: round(v[0], 4) for k, v in b.stageTimes().items()}}
+    b.close()
+    res["steps"] = steps
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +257,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=30, help="steps of the batched leg (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="steps of the fwd+adjoint leg (0 = skip)")
     ap.add_argument("--badj-steps", type=int, default=100, help="steps of the batched fwd+adjoint leg (0 = skip)")
+    ap.add_argument("--compare-4m", type=int, default=0, metavar="STEPS",
+                    help="only run the plain vs batched comparison at 4,194,304 particles for STEPS steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -222,6 +269,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.compare_4m > 0:
+        print(json.dumps({"compare_4m": compare4m_leg(args.compare_4m)}))
+        return
     from enjaparticles_amd import scenes
     from enjaparticles_amd.rtps import RTPS
     import enjaparticles_amd._native as N
@@ -295,6 +345,30 @@ def main():
         "autotune": tune,
     }
     if stages:
+        # HBM side: algorithmic (minimum) bytes of each stage over its measured time
+        import numpy as np
+        cells = s.nb_cells + 1
+        occupied = int(np.count_nonzero(s.read("cell_start") != np.uint32(0xFFFFFFFF)))
+        key_bits = (s.nb_cells + 1).bit_length() + 4      # cell key bits (incl. pad) + 4 sub-cell bits
+        passes = (key_bits + 7) // 8
+        alg = {
+            "hash": 24.0 * n + 4.0 * cells,                 # pos read, key + idx written, start[] cleared
+            "sort": passes * 20.0 * n,                      # per pass: keys read (histogram), pairs read + written
+            "cellindices": 4.0 * n + 8.0 * occupied,        # keys read, start/end of occupied cells written
+            "permute": 100.0 * n,                           # idx + 3 float4 gathered, 3 float4 written
+            "density": 20.0 * n,                            # own position read, density written (neighbours: cache)
+            "force": 100.0 * n,                             # pos, veleval|density, vel, idx read; 3 float4 written
+        }
+        hbm = {}
+        for k, b in alg.items():
+            ms = stages[k][0]
+            gbs = b / (ms * 1e-3) / 1e9
+            hbm[k] = {"bytes": b, "ms": round(ms, 5), "GBps": round(gbs, 1), "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4)}
+        step_bytes = sum(alg.values())
+        hbm["step"] = {"bytes": step_bytes, "ms": round(ms_per_step, 5),
+                       "GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                       "frac_of_peak": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+        out["hbm"] = {"peak_GBps": PEAK_HBM_GBS, "occupied_cells": occupied, "stages": hbm}
         force_ms, dens_ms = stages["force"][0], stages["density"][0]
         achieved = force_flop / (force_ms * 1e-3) / 1e12
         out["stages_ms"] = {k: round(v[0], 5) for k, v in stages.items()}

@@ -88,7 +88,7 @@ int forward_step(rtps_badj* a, int in, int out, BRecord& r, hipStream_t st) {
     RTPS_TRY_HIP(launch_pairs_b(kPairsDensity, a->persistent_grid, a->items, a->n_items, a->queues, a->scenes,
                                 a->d_num, r.pos_s, r.start, r.end, r.keys, r.ve_s, r.vel_s, r.idx, nullptr, nullptr,
                                 nullptr, st));
-    RTPS_TRY_HIP(launch_pairs_b(kPairsForceStore, a->persistent_grid, a->items, a->n_items, a->queues + 1,
+    RTPS_TRY_HIP(launch_pairs_b(kPairsForceStore, a->persistent_grid, a->items, a->n_items, a->queues + kXcdQueues,
                                 a->scenes, a->d_num, r.pos_s, r.start, r.end, r.keys, r.ve_s, r.vel_s, r.idx, nullptr,
                                 nullptr, nullptr, st, r.force_s, a->xsph));
     RTPS_TRY_HIP(launch_badj_integrate(a->n_items, a->items, a->scenes, r.pos_s, r.vel_s, r.force_s, a->xsph, r.idx,
@@ -134,7 +134,7 @@ int rtps_badj_create(int num_scenes, const int* n_settings, const char* const* k
         const size_t N = (size_t)a->T, C = (size_t)a->C;
         RTPS_TRY_HIP(a->alloc(&a->scenes, (size_t)num_scenes));
         RTPS_TRY_HIP(a->alloc(&a->d_num, (size_t)num_scenes));
-        RTPS_TRY_HIP(a->alloc(&a->queues, 2));
+        RTPS_TRY_HIP(a->alloc(&a->queues, (size_t)kQueueInts));
         RTPS_TRY_HIP(a->alloc(&a->item_first, (size_t)num_scenes + 1));
         RTPS_TRY_HIP(a->alloc(&a->left_grid, 1));
         RTPS_TRY_HIP(a->alloc(&a->sid, N));

@@ -41,10 +41,7 @@ __global__ __launch_bounds__(kBBlock) void k_hash_b(int T, const uint16_t* __res
                                                     int n_clear, int* __restrict__ queues) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_clear) start_clear[i] = kEmpty;
-    if (i == 0) {
-        queues[0] = 0;
-        queues[1] = 0;
-    }
+    if (i < kQueueInts) queues[i] = 0;
     if (i >= T) return;
     const int s = sid[i];
     const SceneDev& S = scenes[s];
@@ -90,6 +87,113 @@ __global__ __launch_bounds__(kBBlock) void k_cellindices_b(int T, const uint32_t
     if (last) end[g] = (uint32_t)(i + 1);
 }
 
+// One particle of a work item (sorted slot i of scene s).  Kept out of the
+// persistent loop so that the loop has no divergent exits: every lane reaches
+// the loop's barriers together (a divergent continue there lets the compiler
+// retire lanes one by one, and a parked lane 0 would stop fetching items).
+template <int MODE>
+__device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool deleted,
+                                          const float4* __restrict__ pos_s, const uint32_t* __restrict__ start,
+                                          const uint32_t* __restrict__ end, const uint32_t* __restrict__ keys,
+                                          float4* __restrict__ ve, const float4* __restrict__ vel_s,
+                                          const uint32_t* __restrict__ idx, float4* __restrict__ pos_u,
+                                          float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
+                                          float4* __restrict__ force_out, float4* __restrict__ xsph_out) {
+    const rtps_sph_params& sp = S.sp;
+    const float scale = sp.simulation_scale;
+    const float h = sp.smoothing_distance, h2 = h * h;
+    float4 pi = pos_s[i];
+    const float4 p_unscaled = pi;
+    pi.x *= scale;
+    pi.y *= scale;
+    pi.z *= scale;
+    const Grid& g = S.gs;
+    const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
+    const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, keys, S.res_x, S.res_y, S.res_z);
+    const uint32_t* st = start + S.cell_base;
+    const uint32_t* en = end + S.cell_base;
+    const uint32_t prefix = (uint32_t)s << kSceneShift;
+    if (MODE == kPairsDensity) {
+        float sum = 0.0f;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy) {
+                uint32_t a, b;
+                row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix);
+                auto term = [&](float4 pj) {
+                    float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
+                    poly6_add(sum, rx * rx + ry * ry + rz * rz, h2);
+                };
+                uint32_t j = a;
+                for (; j + kGroup <= b; j += kGroup) {   // loads of a group issued together
+                    float4 p[kGroup];
+#pragma unroll
+                    for (int k = 0; k < kGroup; ++k) p[k] = pos_s[j + k];
+#pragma unroll
+                    for (int k = 0; k < kGroup; ++k) term(p[k]);
+                }
+                for (; j < b; ++j) term(pos_s[j]);
+            }
+        ve[i].w = sp.wpoly6_coef * sp.mass * sum;
+    } else {
+        const float eps = sp.EPSILON, K = sp.K, rho0 = sp.rest_density;
+        const float4 vi = ve[i];
+        const float di = vi.w;
+        const float Pi = K * (di - rho0);
+        ForceAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy) {
+                uint32_t a, b;
+                row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix);
+                uint32_t j = a;
+                for (; j + kGroup <= b; j += kGroup) {   // positions, then accepted velocities, loaded together
+                    float4 p[kGroup], v[kGroup];
+#pragma unroll
+                    for (int k = 0; k < kGroup; ++k) p[k] = pos_s[j + k];
+                    float rx[kGroup], ry[kGroup], rz[kGroup], r2[kGroup];
+                    bool ok[kGroup];
+#pragma unroll
+                    for (int k = 0; k < kGroup; ++k) {
+                        rx[k] = pi.x - p[k].x * scale;
+                        ry[k] = pi.y - p[k].y * scale;
+                        rz[k] = pi.z - p[k].z * scale;
+                        r2[k] = rx[k] * rx[k] + ry[k] * ry[k] + rz[k] * rz[k];
+                        ok[k] = r2[k] <= h2 && j + k != (uint32_t)i;
+                    }
+#pragma unroll
+                    for (int k = 0; k < kGroup; ++k)
+                        if (ok[k]) v[k] = ve[j + k];
+#pragma unroll
+                    for (int k = 0; k < kGroup; ++k)
+                        if (ok[k]) force_pair(acc, rx[k], ry[k], rz[k], r2[k], h, h2, eps, K, rho0, Pi, di, vi, v[k], v[k].w);
+                }
+                for (; j < b; ++j) {
+                    float4 pj = pos_s[j];
+                    float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
+                    float r2 = rx * rx + ry * ry + rz * rz;
+                    if (r2 <= h2 && j != (uint32_t)i) {
+                        const float4 vj = ve[j];
+                        force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, vj.w);
+                    }
+                }
+            }
+        float4 F, X;
+        force_epilogue(acc, di, sp, F, X);
+        if (MODE == kPairsForceStore) {
+            force_out[i] = F;
+            xsph_out[i] = X;
+            return;
+        }
+        const float4 v = vel_s[i];
+        if (S.collision) collision_wall(F, p_unscaled, v, sp, S.bmin, S.bmax);
+        float4 po, vo, eo;
+        leapfrog(p_unscaled, v, F, X, sp, S.dt, po, vo, eo);
+        const uint32_t o = deleted ? (uint32_t)i : idx[i];
+        pos_u[o] = po;
+        vel_u[o] = vo;
+        veleval_u[o] = eo;
+    }
+}
+
 // MODE: kPairsDensity, kPairsForceStep (force + walls + leapfrog, scattered to
 // the unsorted arrays) or kPairsForceStore (force and XSPH stored in sorted
 // order for the adjoint's recorded step).
@@ -105,82 +209,37 @@ __global__ __launch_bounds__(kBBlock) void k_pairs_b(const BatchItem* __restrict
                                                      float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
                                                      float4* __restrict__ force_out, float4* __restrict__ xsph_out) {
     __shared__ int sh_item;
+    const int q0 = (int)(blockIdx.x % kXcdQueues);
+    int q = q0;   // thread 0's current queue
     for (;;) {
-        if (threadIdx.x == 0) sh_item = atomicAdd(queue, 1);
+        if (threadIdx.x == 0) {
+            int item = -1;
+            for (;;) {
+                const int lo = (int)((long)n_items * q / kXcdQueues);
+                const int hi = (int)((long)n_items * (q + 1) / kXcdQueues);
+                const int t = atomicAdd(&queue[q], 1);
+                if (lo + t < hi) {
+                    item = lo + t;
+                    break;
+                }
+                q = (q + 1) % kXcdQueues;   // drained: the next queue (drained queues stay drained)
+                if (q == q0) break;         // all eight drained
+            }
+            sh_item = item;
+        }
         __syncthreads();
         const int it = __builtin_amdgcn_readfirstlane(sh_item);
         __syncthreads();
-        if (it >= n_items) return;   // every block leaves once the queue is drained
+        if (it < 0) break;   // block-uniform: every block leaves once all queues are drained
         const BatchItem I = items[it];
         const int s = __builtin_amdgcn_readfirstlane(I.scene);
         const SceneDev& S = scenes[s];
         int num;
         const bool deleted = scene_num(S, d_num[s], start, num);
         const int i = I.first + (int)threadIdx.x;
-        if ((int)threadIdx.x >= I.count || i - S.off >= num) continue;
-        const rtps_sph_params& sp = S.sp;
-        const float scale = sp.simulation_scale;
-        const float h = sp.smoothing_distance, h2 = h * h;
-        float4 pi = pos_s[i];
-        const float4 p_unscaled = pi;
-        pi.x *= scale;
-        pi.y *= scale;
-        pi.z *= scale;
-        const Grid& g = S.gs;
-        const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
-        const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, keys, S.res_x, S.res_y, S.res_z);
-        const uint32_t* st = start + S.cell_base;
-        const uint32_t* en = end + S.cell_base;
-        const uint32_t prefix = (uint32_t)s << kSceneShift;
-        if (MODE == kPairsDensity) {
-            float sum = 0.0f;
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy) {
-                    uint32_t a, b;
-                    row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix);
-                    for (uint32_t j = a; j < b; ++j) {
-                        float4 pj = pos_s[j];
-                        float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
-                        poly6_add(sum, rx * rx + ry * ry + rz * rz, h2);
-                    }
-                }
-            ve[i].w = sp.wpoly6_coef * sp.mass * sum;
-        } else {
-            const float eps = sp.EPSILON, K = sp.K, rho0 = sp.rest_density;
-            const float4 vi = ve[i];
-            const float di = vi.w;
-            const float Pi = K * (di - rho0);
-            ForceAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy) {
-                    uint32_t a, b;
-                    row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix);
-                    for (uint32_t j = a; j < b; ++j) {
-                        float4 pj = pos_s[j];
-                        float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
-                        float r2 = rx * rx + ry * ry + rz * rz;
-                        if (r2 <= h2 && j != (uint32_t)i) {
-                            const float4 vj = ve[j];
-                            force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, vj.w);
-                        }
-                    }
-                }
-            float4 F, X;
-            force_epilogue(acc, di, sp, F, X);
-            if (MODE == kPairsForceStore) {
-                force_out[i] = F;
-                xsph_out[i] = X;
-                continue;
-            }
-            const float4 v = vel_s[i];
-            if (S.collision) collision_wall(F, p_unscaled, v, sp, S.bmin, S.bmax);
-            float4 po, vo, eo;
-            leapfrog(p_unscaled, v, F, X, sp, S.dt, po, vo, eo);
-            const uint32_t o = deleted ? (uint32_t)i : idx[i];
-            pos_u[o] = po;
-            vel_u[o] = vo;
-            veleval_u[o] = eo;
-        }
+        if ((int)threadIdx.x < I.count && i - S.off < num)
+            pairs_one<MODE>(i, s, S, deleted, pos_s, start, end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
+                            force_out, xsph_out);
     }
 }
 
@@ -196,7 +255,8 @@ hipError_t launch_hash_b(int T, const uint16_t* sid, const SceneDev* scenes, con
                          uint32_t* keys, uint32_t* idx, uint32_t* start, int n_cells, int* queues, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     const int n = T > n_cells ? T : n_cells;
-    k_hash_b<<<bblocks(n), kBBlock, 0, st>>>(T, sid, scenes, d_num, pos_u, keys, idx, start, n_cells, queues);
+    k_hash_b<<<bblocks(n > kQueueInts ? n : kQueueInts), kBBlock, 0, st>>>(T, sid, scenes, d_num, pos_u, keys, idx,
+                                                                         start, n_cells, queues);
     return hipGetLastError();
 }
 

@@ -94,7 +94,7 @@ int launch_step(rtps_batch* b) {
                                 b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted,
                                 b->pos_u, b->vel_u, b->veleval_u, b->stream));
     mark(b, B_FORCE);
-    RTPS_TRY_HIP(launch_pairs_b(kPairsForceStep, b->persistent_grid, b->items, b->n_items, b->queues + 1, b->scenes, b->d_num,
+    RTPS_TRY_HIP(launch_pairs_b(kPairsForceStep, b->persistent_grid, b->items, b->n_items, b->queues + kXcdQueues, b->scenes, b->d_num,
                                 b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted,
                                 b->pos_u, b->vel_u, b->veleval_u, b->stream));
     RTPS_TRY_HIP(launch_commit_b(b->S, b->scenes, b->d_num, b->start, b->stream));
@@ -251,7 +251,7 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
         RTPS_TRY_HIP(hipMalloc((void**)&b->start, (size_t)cells * 4));
         RTPS_TRY_HIP(hipMalloc((void**)&b->end, (size_t)cells * 4));
         RTPS_TRY_HIP(hipMalloc((void**)&b->items, sizeof(BatchItem) * items.size()));
-        RTPS_TRY_HIP(hipMalloc((void**)&b->queues, 2 * sizeof(int)));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->queues, kQueueInts * sizeof(int)));
         RTPS_TRY_HIP(hipMalloc(&b->sort_tmp, sort_temp_bytes((int)T, 32)));
         RTPS_TRY_HIP(hipMemcpyAsync(b->scenes, b->scenes_h.data(), sizeof(SceneDev) * num_scenes,
                                     hipMemcpyHostToDevice, b->stream));
@@ -259,7 +259,7 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
         RTPS_TRY_HIP(hipMemcpyAsync(b->sid, sid.data(), NT * sizeof(uint16_t), hipMemcpyHostToDevice, b->stream));
         RTPS_TRY_HIP(hipMemcpyAsync(b->items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice,
                                     b->stream));
-        RTPS_TRY_HIP(hipMemsetAsync(b->queues, 0, 2 * sizeof(int), b->stream));
+        RTPS_TRY_HIP(hipMemsetAsync(b->queues, 0, kQueueInts * sizeof(int), b->stream));
         RTPS_TRY_HIP(launch_fill_u32(b->start, 666u, cells, b->stream));
         RTPS_TRY_HIP(launch_fill_u32(b->end, 666u, cells, b->stream));
         RTPS_TRY_HIP(sort_clear_temp(b->sort_tmp, (int)T, b->stream));

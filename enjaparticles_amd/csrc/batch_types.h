@@ -10,6 +10,10 @@ namespace rtps {
 
 constexpr uint32_t kSceneShift = 24;   // segmented key: (scene << 24) | local key
 constexpr int kMaxScenes = 256;
+// Work-item queues of the persistent neighbour kernels: one per XCD (items
+// split into 8 contiguous ranges), for each of the two kernels of a step.
+constexpr int kXcdQueues = 8;
+constexpr int kQueueInts = 2 * kXcdQueues;
 
 struct SceneDev {
     rtps_sph_params sp;          // SPHParams of the scene

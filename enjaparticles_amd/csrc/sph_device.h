@@ -131,6 +131,8 @@ struct Win {
 //    contiguous); count[i] = their number, or kListCap + 1 when the list
 //    overflowed (the force kernel then runs the direct loop for i).
 constexpr int kMaskWords = 4;
+// Candidates per group in the neighbour loops (loads in flight per lane).
+constexpr int kGroup = 4;
 constexpr int kListCap = 192;          // codes per particle
 constexpr uint32_t kCodeOffBits = 12;  // offset inside the row window
 struct RowMasks {

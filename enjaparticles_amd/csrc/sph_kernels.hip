@@ -17,8 +17,6 @@
 namespace rtps {
 
 constexpr int kBlock = 256;
-// Candidates per group in the neighbour loops (loads in flight per lane).
-constexpr int kGroup = 4;
 
 static inline int blocks_for(long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }
 

@@ -159,6 +159,16 @@ def dam_custom(nx: int, ny: int, nz: int, seed: int = 1000, box: float | None = 
     return Scene(f"dam{nx}x{ny}x{nz}", dam_settings(box, pos.shape[0]), pos, np.zeros_like(pos), 10, 0.003)
 
 
+def dam4m(seed: int = 1006) -> Scene:
+    """4,194,304 particles (256x128x128 lattice) in its quarter-volume box."""
+    return dam_custom(256, 128, 128, seed=seed)
+
+
+def batched_equal(num_scenes: int, nx: int, ny: int, nz: int, seed: int) -> list[Scene]:
+    """num_scenes dam-break blocks of nx*ny*nz particles each (a uniform batch)."""
+    return [dam_custom(nx, ny, nz, seed=seed * 1000 + k) for k in range(num_scenes)]
+
+
 def adjoint64k(seed: int = 1004, viscosity: float = 0.01, gas_constant: float = 3.0) -> Scene:
     box = quarter_box(64, 64, 16)
     pos = lattice(64, 64, 16, seed)
