@@ -110,84 +110,115 @@ __device__ __forceinline__ void adj_b1_body(int i, const uint32_t* __restrict__ 
 
 // B2 for sorted particle i: pair terms.  ve_s = veleval.xyz | density
 // (sorted).  Writes gx2 (d/dx of the pair terms), ge (d/de), grho (sorted);
-// adds particle i's terms of dL/dmu and dL/dK to gmu, gK.
+// adds particle i's terms of dL/dmu and dL/dK to gmu, gK.  Hardware rsq / rcp
+// (~1 ulp) replace the square root and the divisions.
 __device__ __forceinline__ void adj_b2_body(int i, const float4* __restrict__ pos_s, const float4* __restrict__ ve_s,
                                             const float4* __restrict__ ga_s, const float4* __restrict__ gX_s,
                                             const AdjRows& rows, const rtps_sph_params& sp, const Grid& g,
                                             float4* __restrict__ gx2_s, float4* __restrict__ ge_s,
                                             float* __restrict__ grho_s, float& gmu, float& gK) {
-    const float h = sp.smoothing_distance, h2 = h * h, eps = sp.EPSILON;
+    const float h = sp.smoothing_distance, h2 = h * h, eps = sp.EPSILON, eps2 = eps * eps;
     const float m = sp.mass, rho0 = sp.rest_density, K = sp.K, mu = sp.viscosity;
     const float cp = -0.5f * sp.wspiky_d_coef, cv = sp.wvisc_dd_coef;
     const float cx = 2.0f * sp.mass * sp.wpoly6_coef;
     const float4 pi = pos_s[i], vei = ve_s[i];
     const float3 ei = f3(vei), gai = f3(ga_s[i]), gXi = f3(gX_s[i]);
     const float ri = vei.w;
+    const float inv_ri = __builtin_amdgcn_rcpf(ri);
     float3 gx = make_float3(0, 0, 0), ge = make_float3(0, 0, 0);
     float grho = 0.f;
+    // one accepted pair (q2 <= h2, j != i)
+    auto pair = [&](const float3 r, const float q2, const float4 vej, const float4 gaj4, const float4 gXj4) {
+        const float3 ej = f3(vej), gaj = f3(gaj4), gXj = f3(gXj4);
+        const float rj = vej.w;
+        const float q2c = fmaxf(q2, eps2);
+        const float rinv = __builtin_amdgcn_rsqf(q2c);   // 1 / max(|r|, eps)
+        const float q = q2c * rinv;                        // max(|r|, eps)
+        const float dq = q2 > eps2 ? 1.f : 0.f;            // d q / d |r|
+        const float inv_rr = __builtin_amdgcn_rcpf(ri * rj);
+        const float3 dga = make_float3(gai.x - gaj.x, gai.y - gaj.y, gai.z - gaj.z);
+        const float3 de = make_float3(ej.x - ei.x, ej.y - ei.y, ej.z - ei.z);
+        const float3 rh = make_float3(r.x * rinv, r.y * rinv, r.z * rinv);   // used only where dq = 1
+        // pressure: T_ij = c phi r
+        const float hq = h - q;
+        const float phi = hq * hq * rinv;
+        const float dphi = -hq * (h + q) * rinv * rinv * dq;   // d phi / d |r|
+        const float cK = m * cp * (ri + rj - 2.f * rho0) * inv_rr;   // c / K
+        const float cc = K * cK;
+        const float rdga = dot3(r, dga);
+        const float sig = phi * rdga;
+        gx.x += cc * (phi * dga.x + dphi * rdga * rh.x);
+        gx.y += cc * (phi * dga.y + dphi * rdga * rh.y);
+        gx.z += cc * (phi * dga.z + dphi * rdga * rh.z);
+        grho += sig * m * cp * K * (2.f * rho0 - rj) * inv_rr * inv_ri;
+        gK += cK * phi * dot3(r, gai);
+        // viscosity: V_ij = beta psi (e_j - e_i), psi = h - q
+        const float bmu = m * cv * inv_rr;           // beta / mu
+        const float beta = mu * bmu;
+        const float psi = hq;
+        const float3 nga = make_float3(-dga.x, -dga.y, -dga.z);   // ga_j - ga_i
+        ge.x += beta * psi * nga.x;
+        ge.y += beta * psi * nga.y;
+        ge.z += beta * psi * nga.z;
+        const float deg = dot3(de, nga);
+        gx.x += beta * deg * dq * rh.x;
+        gx.y += beta * deg * dq * rh.y;
+        gx.z += beta * deg * dq * rh.z;
+        grho += -(beta * inv_ri) * psi * dot3(de, dga);
+        gmu += bmu * psi * dot3(de, gai);
+        // XSPH: Y_ij = cx w6 (e_j - e_i) / D
+        const float t = h2 - q2;
+        const float w6 = t * t * t;
+        const float inv_D = __builtin_amdgcn_rcpf(ri + rj);
+        const float3 dgX = make_float3(gXi.x - gXj.x, gXi.y - gXj.y, gXi.z - gXj.z);
+        const float yk = cx * w6 * inv_D;
+        ge.x -= yk * dgX.x;
+        ge.y -= yk * dgX.y;
+        ge.z -= yk * dgX.z;
+        const float dedgX = dot3(de, dgX);
+        const float yr = cx * inv_D * (-6.f) * t * t * dedgX;
+        gx.x += yr * r.x;
+        gx.y += yr * r.y;
+        gx.z += yr * r.z;
+        grho += -yk * inv_D * dedgX;
+    };
     const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
     const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, rows.keys, rows.rx, rows.ry, rows.rz);
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t a, b;
             row_window(wc, c, dy, dz, h2, g, rows.start, rows.end, rows.keys, a, b, rows.prefix);
-            for (uint32_t j = a; j < b; ++j) {
+            uint32_t j = a;
+            for (; j + kGroup <= b; j += kGroup) {   // positions, then the accepted records, loaded together
+                float4 p[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) p[k] = pos_s[j + k];
+                float3 r[kGroup];
+                float q2[kGroup];
+                bool ok[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) {
+                    r[k] = make_float3(pi.x - p[k].x, pi.y - p[k].y, pi.z - p[k].z);
+                    q2[k] = dot3(r[k], r[k]);
+                    ok[k] = q2[k] <= h2 && j + k != (uint32_t)i;
+                }
+                float4 v[kGroup], ga[kGroup], gX[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (ok[k]) {
+                        v[k] = ve_s[j + k];
+                        ga[k] = ga_s[j + k];
+                        gX[k] = gX_s[j + k];
+                    }
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (ok[k]) pair(r[k], q2[k], v[k], ga[k], gX[k]);
+            }
+            for (; j < b; ++j) {
                 const float4 pj = pos_s[j];
                 const float3 r = make_float3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
                 const float q2 = dot3(r, r);
-                if (!(q2 <= h2) || j == (uint32_t)i) continue;
-                const float4 vej = ve_s[j];
-                const float3 ej = f3(vej), gaj = f3(ga_s[j]), gXj = f3(gX_s[j]);
-                const float rj = vej.w;
-                const float qr = sqrtf(q2);
-                const float q = fmaxf(qr, eps);
-                const float dq = qr > eps ? 1.f : 0.f;          // d q / d |r|
-                const float inv_rr = 1.f / (ri * rj);
-                const float3 dga = make_float3(gai.x - gaj.x, gai.y - gaj.y, gai.z - gaj.z);
-                const float3 de = make_float3(ej.x - ei.x, ej.y - ei.y, ej.z - ei.z);
-                const float3 rh = qr > 0.f ? make_float3(r.x / qr, r.y / qr, r.z / qr) : make_float3(0, 0, 0);
-                // pressure: T_ij = c phi r
-                const float hq = h - q;
-                const float phi = hq * hq / q;
-                const float dphi = -hq * (h + q) / (q * q) * dq;   // d phi / d |r|
-                const float cK = m * cp * (ri + rj - 2.f * rho0) * inv_rr;   // c / K
-                const float cc = K * cK;
-                const float rdga = dot3(r, dga);
-                const float sig = phi * rdga;
-                gx.x += cc * (phi * dga.x + dphi * rdga * rh.x);
-                gx.y += cc * (phi * dga.y + dphi * rdga * rh.y);
-                gx.z += cc * (phi * dga.z + dphi * rdga * rh.z);
-                grho += sig * m * cp * K * (2.f * rho0 - rj) * inv_rr / ri;
-                gK += cK * phi * dot3(r, gai);
-                // viscosity: V_ij = beta psi (e_j - e_i), psi = h - q
-                const float bmu = m * cv * inv_rr;           // beta / mu
-                const float beta = mu * bmu;
-                const float psi = hq;
-                const float3 nga = make_float3(-dga.x, -dga.y, -dga.z);   // ga_j - ga_i
-                ge.x += beta * psi * nga.x;
-                ge.y += beta * psi * nga.y;
-                ge.z += beta * psi * nga.z;
-                const float deg = dot3(de, nga);
-                gx.x += beta * deg * dq * rh.x;
-                gx.y += beta * deg * dq * rh.y;
-                gx.z += beta * deg * dq * rh.z;
-                grho += -(beta / ri) * psi * dot3(de, dga);
-                gmu += bmu * psi * dot3(de, gai);
-                // XSPH: Y_ij = cx w6 (e_j - e_i) / D
-                const float t = h2 - q2;
-                const float w6 = t * t * t;
-                const float D = ri + rj;
-                const float3 dgX = make_float3(gXi.x - gXj.x, gXi.y - gXj.y, gXi.z - gXj.z);
-                const float yk = cx * w6 / D;
-                ge.x -= yk * dgX.x;
-                ge.y -= yk * dgX.y;
-                ge.z -= yk * dgX.z;
-                const float dedgX = dot3(de, dgX);
-                const float yr = cx / D * (-6.f) * t * t * dedgX;
-                gx.x += yr * r.x;
-                gx.y += yr * r.y;
-                gx.z += yr * r.z;
-                grho += -yk / D * dedgX;
+                if (q2 <= h2 && j != (uint32_t)i) pair(r, q2, ve_s[j], ga_s[j], gX_s[j]);
             }
         }
     gx2_s[i] = make_float4(gx.x, gx.y, gx.z, 0.f);
@@ -213,17 +244,29 @@ __device__ __forceinline__ void adj_b3_body(int i, const uint32_t* __restrict__ 
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t a, b;
             row_window(wc, c, dy, dz, h2, g, rows.start, rows.end, rows.keys, a, b, rows.prefix);
-            for (uint32_t j = a; j < b; ++j) {
-                const float4 pj = pos_s[j];
+            auto term = [&](float4 pj, float grj, bool ok) {
                 const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                 const float q2 = rx * rx + ry * ry + rz * rz;
-                if (!(q2 <= h2) || j == (uint32_t)i) continue;
+                if (!(ok && q2 <= h2)) return;
                 const float t = h2 - q2;
-                const float k = crho * (-6.f) * t * t * (gri + grho_s[j]);
+                const float k = crho * (-6.f) * t * t * (gri + grj);
                 gx.x += k * rx;
                 gx.y += k * ry;
                 gx.z += k * rz;
+            };
+            uint32_t j = a;
+            for (; j + kGroup <= b; j += kGroup) {   // loads of a group issued together
+                float4 p[kGroup];
+                float gr[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) {
+                    p[k] = pos_s[j + k];
+                    gr[k] = grho_s[j + k];
+                }
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) term(p[k], gr[k], j + k != (uint32_t)i);
             }
+            for (; j < b; ++j) term(pos_s[j], grho_s[j], j != (uint32_t)i);
         }
     const float4 lp = lxp_s[i], g2 = gx2_s[i];
     const uint32_t o = idx[i];
