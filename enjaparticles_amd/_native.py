@@ -95,6 +95,16 @@ SIGNATURES = {
     "rtps_sph_reset_timers": (_i, [P]),
     "rtps_sph_stream": (_i, [P, P]),
     "rtps_sph_count_pairs": (_i, [P, P, P, P]),
+    "rtps_sph_add_hose": (_i, [P, _i, F4, F4, _f, P]),
+    "rtps_sph_update_hose": (_i, [P, _i, F4, F4, _f]),
+    "rtps_sph_refill_hose": (_i, [P, _i, _i]),
+    "rtps_sph_test_delete": (_i, [P]),
+    "rtps_hose_create": (_i, [_f, _i, F4, F4, _f, _f, C.c_ulonglong, P]),
+    "rtps_hose_destroy": (_i, [P]),
+    "rtps_hose_update": (_i, [P, F4, F4, _f, _f]),
+    "rtps_hose_refill": (_i, [P, _i]),
+    "rtps_hose_spray": (_i, [P, P, _i, P]),
+    "rtps_hose_state": (_i, [P, P, P, P, P, P, P]),
     "rtps_batch_create": (_i, [_i, P, P, P, P]),
     "rtps_batch_destroy": (_i, [P]),
     "rtps_batch_num_scenes": (_i, [P, P, P]),

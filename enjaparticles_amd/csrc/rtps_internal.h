@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "rtps_c.h"
 #include "rtps/RTPSSettings.h"
@@ -63,6 +64,9 @@ hipError_t launch_count_pairs(int num, const int* d_num, const float4* pos_s, co
                               unsigned long long* out, hipStream_t st);
 hipError_t launch_commit_num(int* d_num, const uint32_t* start, int nb_cells, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st);
+
+// ---- hoses (hose.cpp) ----
+std::vector<rtps_float4> hose_spray(rtps_hose* h);
 
 // ---- radix sort (radix_sort.hip) ----
 extern int g_sort_items;   // keys per thread of a sort tile: 4, 8 or 16 (process-wide)

@@ -66,8 +66,10 @@ struct rtps_sph {
     double stage_ms[ST_N] = {};
     long stage_cnt[ST_N] = {};
     bool update_pending = false;
+    std::vector<rtps_hose*> hoses;   // SPH::hoses (SPH.h), sprayed at the start of every update
 
     ~rtps_sph() {
+        for (rtps_hose* h : hoses) rtps_hose_destroy(h);
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& g : gexec_t)
@@ -582,9 +584,83 @@ int rtps_sph_push_particles(rtps_sph* s, const rtps_float4* pos, const rtps_floa
     return RTPS_OK;
 }
 
+namespace {
+
+// SPH::sprayHoses (SPH.cpp:547-557): each hose's spray is pushed with the
+// hose velocity.  A push that would exceed max_num is dropped, as
+// SPH::pushParticles does (SPH.cpp:563-567); the hose still counts it sprayed.
+int spray_hoses(rtps_sph* s) {
+    for (rtps_hose* h : s->hoses) {
+        std::vector<rtps_float4> p = hose_spray(h);
+        if (p.empty()) continue;
+        rtps_float4 v{};
+        rtps_hose_state(h, nullptr, nullptr, nullptr, nullptr, nullptr, &v);
+        int num = 0;
+        int rc = rtps_sph_get_num(s, &num);
+        if (rc) return rc;
+        if ((long)num + (long)p.size() > s->max_num) continue;
+        std::vector<rtps_float4> vel(p.size(), v);
+        rc = rtps_sph_push_particles(s, p.data(), vel.data(), (int)p.size());
+        if (rc) return rc;
+    }
+    return RTPS_OK;
+}
+
+int hose_at(rtps_sph* s, int index, rtps_hose** h) {
+    if (!s || index < 0 || index >= (int)s->hoses.size())
+        return set_error(RTPS_ERR_ARG, "no hose with index " + std::to_string(index));
+    *h = s->hoses[index];
+    return RTPS_OK;
+}
+
+}  // namespace
+
+int rtps_sph_add_hose(rtps_sph* s, int total_n, rtps_float4 center, rtps_float4 velocity, float radius, int* index) {
+    if (!s || !index) return set_error(RTPS_ERR_ARG, "rtps_sph_add_hose: null argument");
+    // SPH::addHose (SPH.cpp:523-532): radius in units of the particle spacing
+    const float spacing = s->settings.GetSettingAs<float>("spacing");
+    rtps_hose* h = nullptr;
+    const unsigned long long seed = 0x5EEDull + 0x9E3779B97F4A7C15ull * (unsigned long long)s->hoses.size();
+    int rc = rtps_hose_create(s->settings.GetSettingAs<float>("time_step"), total_n, center, velocity,
+                              radius * spacing, spacing, seed, &h);
+    if (rc) return rc;
+    s->hoses.push_back(h);
+    *index = (int)s->hoses.size() - 1;
+    return RTPS_OK;
+}
+
+int rtps_sph_update_hose(rtps_sph* s, int index, rtps_float4 center, rtps_float4 velocity, float radius) {
+    rtps_hose* h = nullptr;
+    int rc = hose_at(s, index, &h);
+    if (rc) return rc;
+    const float spacing = s->settings.GetSettingAs<float>("spacing");   // SPH::updateHose (SPH.cpp:533-541)
+    return rtps_hose_update(h, center, velocity, radius * spacing, spacing);
+}
+
+int rtps_sph_refill_hose(rtps_sph* s, int index, int refill) {
+    rtps_hose* h = nullptr;
+    int rc = hose_at(s, index, &h);
+    if (rc) return rc;
+    return rtps_hose_refill(h, refill);   // SPH::refillHose (SPH.cpp:542-545)
+}
+
+int rtps_sph_test_delete(rtps_sph* s) {
+    if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_test_delete: null handle");
+    // System::testDelete (System.cpp:366-376): 40 particles moved outside the grid
+    if (s->max_num < 40) return set_error(RTPS_ERR_CAPACITY, "testDelete needs max_num >= 40");
+    std::vector<rtps_float4> far(40, rtps_float4{100.f, 100.f, 100.f, 1.f});
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    RTPS_TRY_HIP(hipMemcpy(s->pos_u, far.data(), far.size() * sizeof(rtps_float4), hipMemcpyHostToDevice));
+    return RTPS_OK;
+}
+
 int rtps_sph_update(rtps_sph* s) {
     if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_update: null handle");
     maybe_update_params(s);
+    if (!s->hoses.empty()) {
+        int rc = spray_hoses(s);   // SPH.cpp:166
+        if (rc) return rc;
+    }
     StageTimer t(s);
     t.on = false;
     int rc = launch_front(s, t);
@@ -607,10 +683,9 @@ int rtps_sph_integrate(rtps_sph* s) {
     return RTPS_OK;
 }
 
-int rtps_sph_step(rtps_sph* s, int nsteps) {
-    if (!s || nsteps < 0) return set_error(RTPS_ERR_ARG, "rtps_sph_step: bad arguments");
-    if (s->update_pending) return set_error(RTPS_ERR_STATE, "rtps_sph_step after update() without integrate()");
-    maybe_update_params(s);
+namespace {
+
+int run_steps(rtps_sph* s, int nsteps) {
     if (s->num_host == 0) return RTPS_OK;
     if (s->opt_autotune && !s->tuned && nsteps > 0) {   // the first step also tunes the launches
         int rc = autotune_step(s, false);
@@ -630,6 +705,22 @@ int rtps_sph_step(rtps_sph* s, int nsteps) {
         StageTimer t(s);
         t.on = false;
         int rc = launch_step(s, t);
+        if (rc) return rc;
+    }
+    return RTPS_OK;
+}
+
+}  // namespace
+
+int rtps_sph_step(rtps_sph* s, int nsteps) {
+    if (!s || nsteps < 0) return set_error(RTPS_ERR_ARG, "rtps_sph_step: bad arguments");
+    if (s->update_pending) return set_error(RTPS_ERR_STATE, "rtps_sph_step after update() without integrate()");
+    maybe_update_params(s);
+    if (s->hoses.empty()) return run_steps(s, nsteps);
+    for (int k = 0; k < nsteps; ++k) {   // hoses spray on the host before every update
+        int rc = spray_hoses(s);
+        if (rc) return rc;
+        rc = run_steps(s, 1);
         if (rc) return rc;
     }
     return RTPS_OK;

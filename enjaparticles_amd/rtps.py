@@ -121,6 +121,29 @@ class SPH:
             self.pushParticles(pos)
         return len(pos)
 
+    def addHose(self, total_n: int, center, velocity, radius: float, color=None, mass=None) -> int:
+        """SPH::addHose (SPH.cpp:523-532): a hose sprayed at the start of every
+        update; radius is in units of the particle spacing.  Returns its index.
+        Colour and mass are accepted for the reference's signature and unused
+        (particles carry no colour; every particle has the system's mass)."""
+        idx = C.c_int()
+        N.check(N.lib().rtps_sph_add_hose(self._h, int(total_n), _f4(center), _f4(velocity), float(radius),
+                                          C.byref(idx)))
+        return idx.value
+
+    def updateHose(self, index: int, center, velocity, radius: float, color=None) -> None:
+        """SPH::updateHose (SPH.cpp:533-541)."""
+        N.check(N.lib().rtps_sph_update_hose(self._h, int(index), _f4(center), _f4(velocity), float(radius)))
+
+    def refillHose(self, index: int, refill: int) -> None:
+        """SPH::refillHose (SPH.cpp:542-545)."""
+        N.check(N.lib().rtps_sph_refill_hose(self._h, int(index), int(refill)))
+
+    def testDelete(self) -> None:
+        """System::testDelete (System.cpp:366-376): moves the first 40 particles
+        to (100, 100, 100), outside the grid; the next update deletes them."""
+        N.check(N.lib().rtps_sph_test_delete(self._h))
+
     def update(self) -> None:
         N.check(N.lib().rtps_sph_update(self._h))
 
@@ -230,6 +253,62 @@ class SPH:
         text = "\n".join(lines)
         print(text)
         return text
+
+
+def _f4(v) -> N.F4:
+    v = [float(c) for c in v] + [0.0] * (4 - len(v))
+    return N.F4(*v[:4])
+
+
+class Hose:
+    """Hose (rtpslib/system/common/Hose.h, Hose.cpp:30-128): sprays a perturbed
+    disc of particles (addDiscRandom, IV.cpp:313-365) perpendicular to its
+    velocity every ``em`` calls of ``spray``, until ``total_n`` have left.
+    Host code; random numbers come from a SplitMix64 stream started at
+    ``seed`` (the reference uses libc rand())."""
+
+    def __init__(self, dt, total_n, center, velocity, radius, spacing, seed: int = 0x5EED):
+        h = C.c_void_p()
+        N.check(N.lib().rtps_hose_create(float(dt), int(total_n), _f4(center), _f4(velocity), float(radius),
+                                         float(spacing), int(seed), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            N.lib().rtps_hose_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update(self, center, velocity, radius, spacing) -> None:
+        N.check(N.lib().rtps_hose_update(self._h, _f4(center), _f4(velocity), float(radius), float(spacing)))
+
+    def refill(self, num: int) -> None:
+        N.check(N.lib().rtps_hose_refill(self._h, int(num)))
+
+    def spray(self) -> np.ndarray:
+        """The particles sprayed by this call, (n, 4) float32 (w = 1)."""
+        left = self.state()["n_left"]
+        out = np.zeros((max(left, 1), 4), np.float32)
+        n = C.c_int()
+        N.check(N.lib().rtps_hose_spray(self._h, out.ctypes.data, left, C.byref(n)))
+        return out[:n.value].copy()
+
+    def state(self) -> dict:
+        n, emc, em = C.c_int(), C.c_int(), C.c_int()
+        u, w, v = N.F4(), N.F4(), N.F4()
+        N.check(N.lib().rtps_hose_state(self._h, C.byref(n), C.byref(emc), C.byref(em), C.byref(u), C.byref(w),
+                                         C.byref(v)))
+        f = lambda q: np.array([q.x, q.y, q.z, q.w], np.float32)   # noqa: E731
+        return {"n_left": n.value, "em_count": emc.value, "em": em.value, "u": f(u), "w": f(w),
+                "velocity": f(v)}
+
+    def getVelocity(self) -> np.ndarray:
+        return self.state()["velocity"]
 
 
 class RTPS:

@@ -108,6 +108,23 @@ public:
         return (int)rvec.size();
     }
 
+    // Hoses (SPH.cpp:523-557): sprayed at the start of every update; radius in
+    // units of the particle spacing.  Colour and mass are not stored.
+    int addHose(int total_n, float4 center, float4 velocity, float radius, float4 = float4{1, 0, 0, 1},
+                float = 0.0f) {
+        flushSettings();
+        int index = -1;
+        check(rtps_sph_add_hose(h_, total_n, center, velocity, radius, &index));
+        return index;
+    }
+    void updateHose(int index, float4 center, float4 velocity, float radius, float4 = float4{1, 0, 0, 1}) {
+        flushSettings();
+        check(rtps_sph_update_hose(h_, index, center, velocity, radius));
+    }
+    void refillHose(int index, int refill) { check(rtps_sph_refill_hose(h_, index, refill)); }
+    // System::testDelete (System.cpp:366-376)
+    void testDelete() { check(rtps_sph_test_delete(h_)); }
+
     RTPSSettings* getSettings() override { return settings_.get(); }
 
     // Buffer access (System.h:168-177, SPH.h:71): host copies of num entries.

@@ -263,6 +263,40 @@ int rtps_sph_count_pairs(rtps_sph* s, long long* K, long long* M, long long* K_t
 int rtps_sph_stream(rtps_sph* s, void** stream);
 
 /* ------------------------------------------------------------------------ */
+/* Hoses: Hose (rtpslib/system/common/Hose.cpp:30-128) with the disc emitter
+ * addDiscRandom (rtpslib/domain/IV.cpp:313-365), host code.  A hose sprays up
+ * to total_n particles, a perturbed disc of the given radius perpendicular to
+ * its velocity every em = (int)(1 + spacing / dt / |velocity| / 10) calls.
+ * The reference draws from libc rand(); a hose here draws from its own
+ * SplitMix64 stream started at `seed`.  rtps_hose_spray writes the sprayed
+ * positions (w = 1) to out (capacity >= the particles left) and their number
+ * to *n. */
+typedef struct rtps_hose rtps_hose;
+int rtps_hose_create(float dt, int total_n, rtps_float4 center, rtps_float4 velocity, float radius, float spacing,
+                     unsigned long long seed, rtps_hose** out);
+int rtps_hose_destroy(rtps_hose* h);
+int rtps_hose_update(rtps_hose* h, rtps_float4 center, rtps_float4 velocity, float radius, float spacing);
+int rtps_hose_refill(rtps_hose* h, int num);
+int rtps_hose_spray(rtps_hose* h, rtps_float4* out, int capacity, int* n);
+/* Emission state: particles left, calls since the last spray, the cadence em,
+ * the disc axes u, w and the velocity (after the reference's zero guard). */
+int rtps_hose_state(rtps_hose* h, int* n_left, int* em_count, int* em, rtps_float4* u, rtps_float4* w,
+                    rtps_float4* velocity);
+
+/* SPH::addHose / updateHose / refillHose (SPH.cpp:523-545): hoses of the
+ * system, sprayed at the start of every update (SPH::sprayHoses, SPH.cpp:166,
+ * 547-557) with the hose velocity for the new particles.  radius is in units
+ * of the settings' "spacing" (SPH.cpp:526).  Sprays are pushed like
+ * rtps_sph_push_particles, so rtps_sph_step takes one host round trip per
+ * step while a hose has particles left. */
+int rtps_sph_add_hose(rtps_sph* s, int total_n, rtps_float4 center, rtps_float4 velocity, float radius, int* index);
+int rtps_sph_update_hose(rtps_sph* s, int index, rtps_float4 center, rtps_float4 velocity, float radius);
+int rtps_sph_refill_hose(rtps_sph* s, int index, int refill);
+/* System::testDelete (System.cpp:366-376): moves the first 40 particles to
+ * (100, 100, 100), outside the grid, so the next update deletes them. */
+int rtps_sph_test_delete(rtps_sph* s);
+
+/* ------------------------------------------------------------------------ */
 /* a14. Batched scenes (no reference counterpart): num_scenes independent SPH
  * systems (<= 256), each with its own settings, stepped together with one
  * launch per stage (segmented keys (scene << 24) | cell, persistent

@@ -268,3 +268,81 @@ def test_sort_tile_sizes_give_the_same_step(cuda):
     for items in (16, 4):
         for a, b in zip(out[items], out[8]):
             np.testing.assert_array_equal(a, b)
+
+
+def _append(st, pos, vel):
+    """Oracle state with particles pushed after the live ones (veleval 0)."""
+    n = st.num
+    return O.State(np.concatenate([st.pos_u[:n], pos]), np.concatenate([st.vel_u[:n], vel]), st.sphp, st.gp,
+                   st.gps, veleval=np.concatenate([st.veleval_u[:n], np.zeros_like(pos)]))
+
+
+def test_hose_sprays_match_oracle(cuda):
+    """Hoses (SPH::addHose / sprayHoses, SPH.cpp:166, 523-557) spray before
+    every update: after each step the system holds the particles the hose
+    restatement sprays (same seed), moved as the oracle moves them (positions
+    1e-4 relative per element, as over the first steps of the dam-break)."""
+    import hose_oracle as H
+    sc = scenes.dam_custom(16, 16, 8, seed=41)
+    settings = {**sc.settings, "max_num_particles": str(sc.n + 600)}
+    box = float(sc.settings["domain_max"].split()[0])
+    center, vel, radius, total = (0.5 * box, 0.5 * box, 0.8 * box, 1.0), (0.1, 0.05, -1.0, 0.0), 3.0, 500
+    s = RTPS.generateSystemInstance(settings)
+    s.pushParticles(sc.pos, sc.vel)
+    assert s.addHose(total, center, vel, radius) == 0
+    spacing = np.float32(s.getSetting("spacing"))
+    dt = np.float32(s.getSetting("time_step"))
+    hose = H.Hose(dt, total, center, vel, np.float32(radius) * spacing, spacing, 0x5EED)
+    sp, gp, gs = O.derive_params(settings)
+    st = O.State(sc.pos, sc.vel, sp, gp, gs)
+    sprays = 0
+    for k in range(24):
+        p = np.array(hose.spray(), np.float32).reshape(-1, 4)
+        if len(p):
+            sprays += 1
+            st = _append(st, p, np.tile(np.array(hose.velocity, np.float32), (len(p), 1)))
+        st.step(sc.dt, O.FLAG_COLLISION)
+        s.step(1)
+        assert s.getNum() == st.num, k
+        np.testing.assert_allclose(s.read("pos")[:, :3], st.pos_u[: st.num, :3], rtol=1e-4, atol=1e-6)
+    assert sprays >= 3 and st.num > sc.n + 100
+    # the split update()/integrate() path sprays at the start of update()
+    b = RTPS.generateSystemInstance(settings)
+    b.pushParticles(sc.pos, sc.vel)
+    b.addHose(total, center, vel, radius)
+    for _ in range(24):
+        b.update()
+        b.integrate()
+    assert b.getNum() == st.num
+    np.testing.assert_allclose(b.read("pos")[:, :3], st.pos_u[: st.num, :3], rtol=1e-4, atol=1e-6)
+    # refill and move
+    b.refillHose(0, 10)
+    b.updateHose(0, (0.4 * box, 0.5 * box, 0.8 * box, 1.0), vel, 2.0)
+    n0 = b.getNum()
+    for _ in range(10):
+        b.step(1)
+    assert b.getNum() == n0 + 10
+    with pytest.raises(Exception):
+        b.refillHose(3, 10)
+    for x in (s, b):
+        x.close()
+
+
+def test_test_delete_removes_forty(cuda):
+    """System::testDelete (System.cpp:366-376) moves the first 40 particles
+    out of the grid; the next update deletes them (SPH.cpp:220-250)."""
+    sc = scenes.dam16k()
+    s = _system(sc)
+    s.testDelete()
+    pos = sc.pos.copy()
+    pos[:40] = (100.0, 100.0, 100.0, 1.0)
+    st = _oracle(scenes.Scene("del40", sc.settings, pos, sc.vel, 2, sc.dt))
+    for _ in range(2):
+        s.step(1)
+        st.step(sc.dt, O.FLAG_COLLISION)
+    n = s.getNum()
+    assert n == st.num == sc.n - 40
+    from scipy.spatial import cKDTree
+    d, j = cKDTree(st.pos_u[:n, :3].astype(np.float64)).query(s.read("pos")[:, :3].astype(np.float64))
+    assert d.max() <= 1e-5 and len(np.unique(j)) == n
+    s.close()
