@@ -255,7 +255,9 @@ __device__ __forceinline__ void force_pair(ForceAcc& acc, float rx, float ry, fl
     // force.cl:73, rlen = max(|r|, eps), and 1/rlen from one hardware
     // reciprocal square root (~1 ulp; the reference builds OpenCL's length()
     // without correctly rounded sqrt): r2 clamped to eps^2 gives the same max.
-    const float r2c = fmaxf(r2, eps * eps);
+    // max on the bit patterns: both are non-negative floats, whose order is
+    // their integer order (fmaxf would also canonicalise both operands)
+    const float r2c = __int_as_float(max(__float_as_int(r2), __float_as_int(eps * eps)));
     const float rinv = __builtin_amdgcn_rsqf(r2c);
     float rlen = r2c * rinv;
     float hr = h - rlen;

@@ -105,6 +105,8 @@ __global__ __launch_bounds__(kBlock) void k_permute(int num_host, const int* __r
 // REC: also record every row's window and the accepted candidates for the
 // force kernel (RowMasks): kRecBits (bit form) or kRecList (list form).
 constexpr int kRecNone = 0, kRecBits = 1, kRecList = 2;
+// candidates per density group (a divisor of 32: the bit words fill in whole groups)
+constexpr int kGroupDensity = 4;
 template <bool SCALED, bool PACKED, int REC>
 __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __restrict__ d_num,
                                                     const float4* __restrict__ pos_s,
@@ -163,15 +165,16 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
             if (REC != kRecNone) masks.rows[(size_t)r * st + i] = make_uint2(a, b - a);
             if (LIST && b - a >= (1u << kCodeOffBits)) over = true;   // offsets would not fit a code
             const uint32_t rtag = (uint32_t)r << kCodeOffBits;
-            // groups of kGroup candidates: all loads issued before the first use
+            // groups of kGroupDensity candidates: all loads issued before the first use
             uint32_t j = a;
-            for (; j + kGroup <= b; j += kGroup, q += kGroup) {
-                float4 p[kGroup];
+            for (; j + kGroupDensity <= b; j += kGroupDensity, q += kGroupDensity) {
+                const float4* __restrict__ pg = pos_s + j;   // immediate offsets for the group's loads
+                float4 p[kGroupDensity];
 #pragma unroll
-                for (int k = 0; k < kGroup; ++k) p[k] = pos_s[j + k];
+                for (int k = 0; k < kGroupDensity; ++k) p[k] = pg[k];
                 uint32_t nib = 0;
 #pragma unroll
-                for (int k = 0; k < kGroup; ++k) {
+                for (int k = 0; k < kGroupDensity; ++k) {
                     const float r2 = sep2(p[k]);
                     add_term(r2);
                     const bool acc_k = r2 <= h2 && j + k != (uint32_t)i;
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
                 }
                 if (MASKS) {
                     word |= nib << (q & 31u);
-                    if ((q & 31u) == 32u - kGroup) flush();
+                    if ((q & 31u) == 32u - kGroupDensity) flush();
                 }
             }
             for (; j < b; ++j, ++q) {
@@ -211,12 +214,14 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
 // (collision_wall.cl) and the leapfrog step (leapfrog.cl), scattering to the
 // unsorted arrays; force/xsph are then stored only if force_out != nullptr.
 //
-// COMPACT: only ~15% of the candidates pass the r <= h test, so a branch on
-// it keeps the whole wave in the pair math for almost every candidate.  The
-// compact variant first tests candidates in chunks of kChunk and appends the
-// accepted j to a per-thread list in LDS (kNbrCap entries, entry k of thread t
-// at word k * kBlock + t: conflict-free), then drains the lists with one pair
-// per lane per iteration.  Pairs are visited in the same ascending-j order.
+// COMPACT: about a third of the windowed candidates pass the r <= h test, so
+// a branch on it keeps the whole wave in the pair math for almost every
+// candidate.  The compact variant tests kGroup candidates per lane per
+// iteration and appends the accepted j to a per-thread list in dynamic LDS
+// (kNbrCap entries, entry k of thread t at word k * blockDim + t:
+// conflict-free); when a lane's list could overflow, every lane drains its
+// list, kGroup pairs per iteration.  Pairs are visited in the same
+// ascending-j order as the direct loop.
 //
 // LOOP (the force-loop variant): kLoopDirect, kLoopCompact, or kLoopMasked:
 // walk the accepted-candidate bits the density kernel recorded for this step
@@ -230,7 +235,6 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
 // kLoopMasked keeps only ~40% of the lanes busy on a dam-break state).
 constexpr int kLoopDirect = 0, kLoopCompact = 1, kLoopMasked = 2, kLoopList = 3;
 constexpr int kNbrCap = 32;
-constexpr int kChunk = 16;
 
 template <bool SCALED, bool PACKED, bool FUSED, int LOOP>
 __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __restrict__ d_num,
@@ -388,9 +392,12 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
                 // veleval loads of the accepted ones, are issued together
                 uint32_t j = a;
                 for (; j + kGroup <= b; j += kGroup) {
+                    // one base address per group: the loads take immediate offsets
+                    const float4* __restrict__ pg = pos_s + j;
+                    const float4* __restrict__ vg = ve_in + j;
                     float4 q[kGroup], v[kGroup];
 #pragma unroll
-                    for (int k = 0; k < kGroup; ++k) q[k] = pos_s[j + k];
+                    for (int k = 0; k < kGroup; ++k) q[k] = pg[k];
                     float rx[kGroup], ry[kGroup], rz[kGroup], r2[kGroup];
                     bool ok[kGroup];
 #pragma unroll
@@ -403,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
                     }
 #pragma unroll
                     for (int k = 0; k < kGroup; ++k)
-                        if (ok[k]) v[k] = ve_in[j + k];
+                        if (ok[k]) v[k] = vg[k];
 #pragma unroll
                     for (int k = 0; k < kGroup; ++k)
                         if (ok[k]) {
@@ -422,17 +429,37 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
     if constexpr (LOOP == kLoopCompact) {
         const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
         const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
-        __shared__ uint32_t nbr[kNbrCap * kBlock];
-        uint32_t* mine = nbr + threadIdx.x;
+        extern __shared__ uint32_t nbr_lds[];   // kNbrCap * blockDim.x words
+        const int bs = (int)blockDim.x;
+        uint32_t* mine = nbr_lds + threadIdx.x;
         int cnt = 0;
+        // kGroup list entries per lane per iteration, their loads issued together
         auto drain = [&]() {
-            for (int k = 0; __ballot(k < cnt) != 0ull; ++k) {
-                if (k < cnt) {
-                    const uint32_t j = mine[k * kBlock];
-                    float rx, ry, rz;
-                    float r2 = sep(j, rx, ry, rz);
-                    pair(j, rx, ry, rz, r2);
+            for (int k = 0; __ballot(k < cnt) != 0ull; k += kGroup) {
+                uint32_t jj[kGroup];
+                bool v[kGroup];
+#pragma unroll
+                for (int q = 0; q < kGroup; ++q) {
+                    v[q] = k + q < cnt;
+                    jj[q] = v[q] ? mine[(k + q) * bs] : 0u;
                 }
+                float4 pq[kGroup], e[kGroup];
+#pragma unroll
+                for (int q = 0; q < kGroup; ++q)
+                    if (v[q]) pq[q] = pos_s[jj[q]];
+#pragma unroll
+                for (int q = 0; q < kGroup; ++q)
+                    if (v[q]) e[q] = ve_in[jj[q]];
+#pragma unroll
+                for (int q = 0; q < kGroup; ++q)
+                    if (v[q]) {
+                        float4 pj = pq[q];
+                        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                        const float r2 = rx * rx + ry * ry + rz * rz;
+                        const float dj = PACKED ? e[q].w : dens_in[jj[q]];
+                        force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, e[q], dj);
+                    }
             }
             cnt = 0;
         };
@@ -440,14 +467,23 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
                 row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
-                for (uint32_t c0 = a; __ballot(c0 < b) != 0ull; c0 += kChunk) {
-                    if (__ballot(cnt > kNbrCap - kChunk) != 0ull) drain();
-                    const uint32_t c1 = min(c0 + (uint32_t)kChunk, b);
-                    for (uint32_t j = c0; j < c1; ++j) {
-                        float rx, ry, rz;
-                        float r2 = sep(j, rx, ry, rz);
-                        if (r2 <= h2 && j != (uint32_t)i) {
-                            mine[cnt * kBlock] = j;
+                for (uint32_t j = a; __ballot(j < b) != 0ull; j += kGroup) {
+                    if (__ballot(cnt > kNbrCap - kGroup) != 0ull) drain();
+                    float4 pq[kGroup];
+                    bool in[kGroup];
+#pragma unroll
+                    for (int q = 0; q < kGroup; ++q) {
+                        in[q] = j + (uint32_t)q < b;
+                        if (in[q]) pq[q] = pos_s[j + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < kGroup; ++q) {
+                        float4 pj = pq[q];
+                        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                        const float r2 = rx * rx + ry * ry + rz * rz;
+                        if (in[q] && r2 <= h2 && j + q != (uint32_t)i) {
+                            mine[cnt * bs] = j + (uint32_t)q;
                             ++cnt;
                         }
                     }
@@ -667,10 +703,11 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
     const int nb = blocks_for(num, bs);
     const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
     const int loop = loop_of(win, m);
+    const size_t lds = loop == kLoopCompact ? (size_t)kNbrCap * bs * sizeof(uint32_t) : 0;
     GravitySources none{};
     rtps_float4 z{0, 0, 0, 0};
 #define RTPS_FORCE_L(S, P, L)                                                                                       \
-    k_force<S, P, false, L><<<nb, bs, 0, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force, xsph,   \
+    k_force<S, P, false, L><<<nb, bs, lds, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force, xsph,   \
                                                nullptr, nullptr, none, 0, z, z, 0.f, nullptr, nullptr, nullptr, m)
 #define RTPS_FORCE(S, P)                                                                                            \
     if (loop == kLoopList) RTPS_FORCE_L(S, P, kLoopList);                                                       \
@@ -697,8 +734,9 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
     const int nb = blocks_for(num, bs);
     const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
     const int loop = loop_of(win, m);
+    const size_t lds = loop == kLoopCompact ? (size_t)kNbrCap * bs * sizeof(uint32_t) : 0;
 #define RTPS_FFUSED_L(S, L)                                                                                         \
-    k_force<S, true, true, L><<<nb, bs, 0, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
+    k_force<S, true, true, L><<<nb, bs, lds, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
                                                  xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt,  \
                                                  pos_u, vel_u, veleval_u, m)
 #define RTPS_FFUSED(S)                                                                                              \

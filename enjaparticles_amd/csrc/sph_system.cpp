@@ -15,6 +15,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -421,6 +422,9 @@ int autotune_step(rtps_sph* s, bool measure) {
         return ms / kReps;
     };
     const int blocks[] = {64, 128, 256};
+    // RTPS_TUNE_VERBOSE=1 prints every timed configuration to stderr
+    const char* verbose_env = std::getenv("RTPS_TUNE_VERBOSE");
+    const bool verbose = verbose_env && verbose_env[0] == '1';
     // density: plain, recording bits, recording lists (each left recorded for the force loops timed next);
     // the recorded forms (force loops 2 and 3) are candidates only with RTPS_OPT_AUTOTUNE = 2
     const bool full = s->opt_autotune == 2;
@@ -437,6 +441,7 @@ int autotune_step(rtps_sph* s, bool measure) {
                 return launch_density(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, w, nullptr,
                                       s->ve_s, s->stream, recs[mode]);
             });
+            if (verbose) std::fprintf(stderr, "rtps autotune: density record %d block %d: %.4f ms\n", mode, bs, ms);
             if (ms < 0.f) failed = true;
             else if (ms < best_d[mode]) { best_d[mode] = ms; bd[mode] = bs; }
         }
@@ -455,6 +460,7 @@ int autotune_step(rtps_sph* s, bool measure) {
                                           s->vel_u, s->veleval_u, s->stream, recs[var >= 2 ? var - 1 : 0]);
             });
             const int mode = var >= 2 ? var - 1 : 0;
+            if (verbose) std::fprintf(stderr, "rtps autotune: force loop %d block %d: %.4f ms\n", var, bs, ms);
             if (ms < 0.f) failed = true;
             else if (ms + best_d[mode] < best_total) {
                 best_total = ms + best_d[mode];
