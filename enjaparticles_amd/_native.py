@@ -22,9 +22,10 @@ RTPS_ERR_STATE = -5
 FIELD = {
     "pos": 0, "vel": 1, "veleval": 2, "pos_s": 3, "vel_s": 4, "veleval_s": 5, "density": 6,
     "force": 7, "xsph": 8, "keys": 9, "idx": 10, "cell_start": 11, "cell_end": 12,
+    "color": 13, "color_s": 14, "mass": 15, "mass_s": 16,
 }
 (OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS, OPT_FORCE_VARIANT, OPT_WINDOW, OPT_TILE_REMAP, OPT_AUTOTUNE,
- OPT_SORT_TILE) = (0, 1, 2, 3, 4, 5, 6, 7, 8)
+ OPT_SORT_TILE, OPT_COLOR) = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9)
 NUM_STAGES = 6
 STAGE_NAMES = ("hash", "sort", "cellindices", "permute", "density", "force")
 MAX_GRAVITY_SOURCES = 16
@@ -78,6 +79,7 @@ SIGNATURES = {
     "rtps_sph_create": (_i, [P, P, _i, P]),
     "rtps_sph_destroy": (_i, [P]),
     "rtps_sph_push_particles": (_i, [P, P, P, _i]),
+    "rtps_sph_push_particles_color": (_i, [P, P, P, _i, F4]),
     "rtps_sph_update": (_i, [P]),
     "rtps_sph_integrate": (_i, [P]),
     "rtps_sph_step": (_i, [P, _i]),
@@ -86,6 +88,9 @@ SIGNATURES = {
     "rtps_sph_read": (_i, [P, _i, P, _i]),
     "rtps_sph_write": (_i, [P, _i, P, _i]),
     "rtps_sph_device_ptr": (_i, [P, _i, P]),
+    "rtps_sph_read_async": (_i, [P, _i, P, _i]),
+    "rtps_host_alloc": (_i, [_sz, P]),
+    "rtps_host_free": (_i, [P]),
     "rtps_sph_set_setting": (_i, [P, C.c_char_p, C.c_char_p]),
     "rtps_sph_get_setting": (_i, [P, C.c_char_p, P, _i]),
     "rtps_sph_get_params": (_i, [P, P, P, P]),
@@ -95,8 +100,8 @@ SIGNATURES = {
     "rtps_sph_reset_timers": (_i, [P]),
     "rtps_sph_stream": (_i, [P, P]),
     "rtps_sph_count_pairs": (_i, [P, P, P, P]),
-    "rtps_sph_add_hose": (_i, [P, _i, F4, F4, _f, P]),
-    "rtps_sph_update_hose": (_i, [P, _i, F4, F4, _f]),
+    "rtps_sph_add_hose": (_i, [P, _i, F4, F4, _f, F4, P]),
+    "rtps_sph_update_hose": (_i, [P, _i, F4, F4, _f, F4]),
     "rtps_sph_refill_hose": (_i, [P, _i, _i]),
     "rtps_sph_test_delete": (_i, [P]),
     "rtps_hose_create": (_i, [_f, _i, F4, F4, _f, _f, C.c_ulonglong, P]),

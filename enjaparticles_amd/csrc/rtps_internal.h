@@ -37,8 +37,10 @@ hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, c
 hipError_t launch_cellindices(int num, const int* d_num, const uint32_t* keys, int nb_cells, uint32_t* start,
                               uint32_t* end, int sub_bits, hipStream_t st);
 hipError_t launch_read_nc(const uint32_t* start, int nb_cells, int* nc, hipStream_t st);
+// cu/cs (optional): particle colours, permuted like the positions
 hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const float4* pu, const float4* vu,
-                          const float4* eu, float4* ps, float4* vs, float4* es, hipStream_t st);
+                          const float4* eu, float4* ps, float4* vs, float4* es, hipStream_t st,
+                          const float4* cu = nullptr, float4* cs = nullptr);
 hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
                           const rtps_sph_params& sp, const Grid& g, const Win& win, float* dens, float4* packed,
                           hipStream_t st, const RowMasks* masks = nullptr);
@@ -51,19 +53,23 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
                               const Win& win, float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
                               const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,
                               float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
-                              const RowMasks* masks = nullptr);
+                              const RowMasks* masks = nullptr, const float4* color_s = nullptr,
+                              float4* color_u = nullptr);
 hipError_t launch_post_force(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
                              const float4* vel_s, float4* force, const rtps_sph_params& sp,
                              const GravitySources& gsrc, int collision, const rtps_grid_params& gps, hipStream_t st);
 hipError_t launch_leapfrog(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
                            const float4* vel_s, const float4* force, const float4* xsph, const uint32_t* idx,
                            const rtps_sph_params& sp, float dt, float4* pos_u, float4* vel_u, float4* veleval_u,
-                           hipStream_t st);
+                           hipStream_t st, const float4* color_s = nullptr, float4* color_u = nullptr);
 hipError_t launch_count_pairs(int num, const int* d_num, const float4* pos_s, const uint32_t* start,
                               const uint32_t* end, const rtps_sph_params& sp, const Grid& g, const Win& win,
                               unsigned long long* out, hipStream_t st);
 hipError_t launch_commit_num(int* d_num, const uint32_t* start, int nb_cells, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st);
+
+// Default particle colour of System::pushParticles (System.h:160).
+constexpr rtps_float4 kDefaultColor{1.0f, 0.0f, 0.0f, 1.0f};
 
 // ---- hoses (hose.cpp) ----
 std::vector<rtps_float4> hose_spray(rtps_hose* h);

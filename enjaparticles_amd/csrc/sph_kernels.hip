@@ -89,7 +89,8 @@ __global__ __launch_bounds__(kBlock) void k_permute(int num_host, const int* __r
                                                     const uint32_t* __restrict__ idx, const float4* __restrict__ pu,
                                                     const float4* __restrict__ vu, const float4* __restrict__ eu,
                                                     float4* __restrict__ ps, float4* __restrict__ vs,
-                                                    float4* __restrict__ es) {
+                                                    float4* __restrict__ es, const float4* __restrict__ cu,
+                                                    float4* __restrict__ cs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int num = d_num ? *d_num : num_host;
     if (i >= num) return;
@@ -97,6 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(int num_host, const int* __r
     ps[i] = pu[s];
     vs[i] = vu[s];
     es[i] = eu[s];
+    if (cu) cs[i] = cu[s];   // permute.cl:41 (colours, when the system keeps them)
 }
 
 // ---------------------------------------------------------------------------
@@ -250,7 +252,8 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
                                                   GravitySources gsrc, int collision, rtps_float4 bmin,
                                                   rtps_float4 bmax, float dt, float4* __restrict__ pos_u,
                                                   float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
-                                                  RowMasks masks) {
+                                                  RowMasks masks, const float4* __restrict__ color_s,
+                                                  float4* __restrict__ color_u) {
     const int i = remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
     bool deleted = false;
@@ -506,6 +509,9 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
         pos_u[o] = po;
         vel_u[o] = vo;
         veleval_u[o] = eo;
+        // leapfrog.cl:107 writes color_u[o] = color_s[i] every step; it changes
+        // something only when the survivors are renumbered
+        if (deleted && color_u) color_u[i] = color_s[i];
     }
 }
 
@@ -537,7 +543,8 @@ __global__ __launch_bounds__(kBlock) void k_leapfrog(int num_host, const int* __
                                                      const float4* __restrict__ xsph,
                                                      const uint32_t* __restrict__ idx, rtps_sph_params sp, float dt,
                                                      float4* __restrict__ pos_u, float4* __restrict__ vel_u,
-                                                     float4* __restrict__ veleval_u) {
+                                                     float4* __restrict__ veleval_u, const float4* __restrict__ color_s,
+                                                     float4* __restrict__ color_u) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
     bool deleted = false;
@@ -549,6 +556,7 @@ __global__ __launch_bounds__(kBlock) void k_leapfrog(int num_host, const int* __
     pos_u[o] = po;
     vel_u[o] = vo;
     veleval_u[o] = eo;
+    if (deleted && color_u) color_u[i] = color_s[i];   // leapfrog.cl:107
 }
 
 // Pair totals of the neighbour loop on the current sorted state: out[0] =
@@ -655,10 +663,11 @@ hipError_t launch_read_nc(const uint32_t* start, int nb_cells, int* nc, hipStrea
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st);
 
 hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const float4* pu, const float4* vu,
-                          const float4* eu, float4* ps, float4* vs, float4* es, hipStream_t st) {
+                          const float4* eu, float4* ps, float4* vs, float4* es, hipStream_t st, const float4* cu,
+                          float4* cs) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
-    k_permute<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, idx, pu, vu, eu, ps, vs, es);
+    k_permute<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, idx, pu, vu, eu, ps, vs, es, cu, cs);
     return hipGetLastError();
 }
 
@@ -708,7 +717,8 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
     rtps_float4 z{0, 0, 0, 0};
 #define RTPS_FORCE_L(S, P, L)                                                                                       \
     k_force<S, P, false, L><<<nb, bs, lds, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force, xsph,   \
-                                               nullptr, nullptr, none, 0, z, z, 0.f, nullptr, nullptr, nullptr, m)
+                                               nullptr, nullptr, none, 0, z, z, 0.f, nullptr, nullptr, nullptr, m,    \
+                                               nullptr, nullptr)
 #define RTPS_FORCE(S, P)                                                                                            \
     if (loop == kLoopList) RTPS_FORCE_L(S, P, kLoopList);                                                       \
     else if (loop == kLoopMasked) RTPS_FORCE_L(S, P, kLoopMasked);                                                  \
@@ -726,7 +736,7 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
                               const Win& win, float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
                               const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,
                               float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
-                              const RowMasks* masks) {
+                              const RowMasks* masks, const float4* color_s, float4* color_u) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
@@ -738,7 +748,7 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
 #define RTPS_FFUSED_L(S, L)                                                                                         \
     k_force<S, true, true, L><<<nb, bs, lds, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
                                                  xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt,  \
-                                                 pos_u, vel_u, veleval_u, m)
+                                                 pos_u, vel_u, veleval_u, m, color_s, color_u)
 #define RTPS_FFUSED(S)                                                                                              \
     if (loop == kLoopList) RTPS_FFUSED_L(S, kLoopList);                                                         \
     else if (loop == kLoopMasked) RTPS_FFUSED_L(S, kLoopMasked);                                                    \
@@ -764,11 +774,11 @@ hipError_t launch_post_force(int num, const int* d_num, const uint32_t* start, i
 hipError_t launch_leapfrog(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
                            const float4* vel_s, const float4* force, const float4* xsph, const uint32_t* idx,
                            const rtps_sph_params& sp, float dt, float4* pos_u, float4* vel_u, float4* veleval_u,
-                           hipStream_t st) {
+                           hipStream_t st, const float4* color_s, float4* color_u) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     k_leapfrog<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, start, nb_cells, pos_s, vel_s, force, xsph, idx, sp, dt,
-                                                    pos_u, vel_u, veleval_u);
+                                                    pos_u, vel_u, veleval_u, color_s, color_u);
     return hipGetLastError();
 }
 

@@ -43,6 +43,7 @@ struct rtps_sph {
     hipStream_t stream = nullptr;
     float4 *pos_u = nullptr, *vel_u = nullptr, *veleval_u = nullptr;
     float4 *pos_s = nullptr, *vel_s = nullptr, *ve_s = nullptr, *force_s = nullptr, *xsph_s = nullptr;
+    float4 *color_u = nullptr, *color_s = nullptr;   // cl_color_u / cl_color_s (System.cpp:236-237)
     uint32_t *keys = nullptr, *idx = nullptr, *start = nullptr, *end = nullptr;
     uint32_t *keys_sorted = nullptr, *idx_sorted = nullptr;
     int* d_num = nullptr;
@@ -52,6 +53,7 @@ struct rtps_sph {
     GravitySources gsrc{};
     int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0;
     int opt_autotune = 1;
+    int opt_color = 0;   // permute colours every step (set by the first non-default colour pushed)
     // launch configuration of the neighbour kernels (0 / -1 = defaults), set by the autotuner
     int block_density = 0, block_force = 0, force_variant = -1;
     bool tuned = false;
@@ -68,6 +70,7 @@ struct rtps_sph {
     long stage_cnt[ST_N] = {};
     bool update_pending = false;
     std::vector<rtps_hose*> hoses;   // SPH::hoses (SPH.h), sprayed at the start of every update
+    std::vector<rtps_float4> hose_color;
 
     ~rtps_sph() {
         for (rtps_hose* h : hoses) rtps_hose_destroy(h);
@@ -75,8 +78,8 @@ struct rtps_sph {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& g : gexec_t)
             if (g) (void)hipGraphExecDestroy(g);
-        void* bufs[] = {pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s, force_s, xsph_s, keys, idx, start, end, d_num,
-                        sort_tmp, rm.rows, rm.words, rm.codes, rm.count};
+        void* bufs[] = {pos_u,   vel_u,    veleval_u, pos_s,   vel_s,   ve_s,  force_s,  xsph_s,  color_u, color_s,
+                        keys,    idx,      start,     end,     d_num,   sort_tmp, rm.rows, rm.words, rm.codes, rm.count};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         for (auto& set : ev)
@@ -94,6 +97,10 @@ const RowMasks* masks_of(const rtps_sph* s) {
     const int v = loop_variant(s);
     return v == 2 ? &s->rm : (v == 3 ? &s->rl : nullptr);
 }
+
+// Colour buffers handed to the kernels (none unless the system keeps colours).
+float4* col_u(const rtps_sph* s) { return s->opt_color ? s->color_u : nullptr; }
+float4* col_s(const rtps_sph* s) { return s->opt_color ? s->color_s : nullptr; }
 
 // Launch inputs of the density (force = false) or force kernel.
 Win win(const rtps_sph* s, bool force = false) {
@@ -172,7 +179,7 @@ int launch_front(rtps_sph* s, StageTimer& t) {
     RTPS_TRY_HIP(launch_cellindices(n, s->d_num, s->keys_sorted, s->nb_cells, s->start, s->end, kSubBits, s->stream));
     t.mark(ST_PERMUTE);
     RTPS_TRY_HIP(launch_permute(n, s->d_num, s->idx_sorted, s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s,
-                                s->ve_s, s->stream));
+                                s->ve_s, s->stream, col_u(s), col_s(s)));
     t.mark(ST_DENSITY);
     RTPS_TRY_HIP(launch_density(n, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, win(s), nullptr, s->ve_s,
                                 s->stream, masks_of(s)));
@@ -188,7 +195,7 @@ int launch_step(rtps_sph* s, StageTimer& t) {
     RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, win(s, true),
                                     fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s, s->idx_sorted,
                                     s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u, s->veleval_u,
-                                    s->stream, masks_of(s)));
+                                    s->stream, masks_of(s), col_s(s), col_u(s)));
     RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
     t.mark(ST_N);
     return RTPS_OK;
@@ -305,6 +312,8 @@ float4* field_f4(rtps_sph* s, int field) {
         case RTPS_FIELD_VELEVAL_S: return s->ve_s;
         case RTPS_FIELD_FORCE: return s->force_s;
         case RTPS_FIELD_XSPH: return s->xsph_s;
+        case RTPS_FIELD_COLOR: return s->color_u;
+        case RTPS_FIELD_COLOR_S: return s->color_s;
         default: return nullptr;
     }
 }
@@ -478,7 +487,7 @@ int autotune_step(rtps_sph* s, bool measure) {
     RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs,
                                     win(s, true), fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
                                     s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u,
-                                    s->veleval_u, s->stream, masks_of(s)));
+                                    s->veleval_u, s->stream, masks_of(s), col_s(s), col_u(s)));
     RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
     return RTPS_OK;
 }
@@ -521,6 +530,8 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
         RTPS_TRY_HIP(dalloc(&s->ve_s, N));
         RTPS_TRY_HIP(dalloc(&s->force_s, N));
         RTPS_TRY_HIP(dalloc(&s->xsph_s, N));
+        RTPS_TRY_HIP(dalloc(&s->color_u, N));
+        RTPS_TRY_HIP(dalloc(&s->color_s, N));
         RTPS_TRY_HIP(dalloc(&s->keys, N));
         RTPS_TRY_HIP(dalloc(&s->idx, N));
         RTPS_TRY_HIP(dalloc(&s->start, C));
@@ -536,6 +547,12 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
         RTPS_TRY_HIP(hipMalloc(&s->sort_tmp, sort_temp_bytes((int)N, s->key_bits)));
         float4* zf[] = {s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s, s->ve_s, s->force_s, s->xsph_s};
         for (float4* p : zf) RTPS_TRY_HIP(hipMemsetAsync(p, 0, N * sizeof(float4), s->stream));
+        {   // every slot starts with the default colour of pushParticles (System.h:160)
+            std::vector<rtps_float4> red(N, kDefaultColor);
+            RTPS_TRY_HIP(hipMemcpyAsync(s->color_u, red.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+            RTPS_TRY_HIP(hipMemcpyAsync(s->color_s, red.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+            RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+        }
         RTPS_TRY_HIP(hipMemsetAsync(s->keys, 0, N * 4, s->stream));
         RTPS_TRY_HIP(hipMemsetAsync(s->idx, 0, N * 4, s->stream));
         RTPS_TRY_HIP(hipMemsetAsync(s->d_num, 0, sizeof(int), s->stream));
@@ -569,7 +586,8 @@ int rtps_sph_get_num(rtps_sph* s, int* num) {
     return RTPS_OK;
 }
 
-int rtps_sph_push_particles(rtps_sph* s, const rtps_float4* pos, const rtps_float4* vel, int n) {
+int rtps_sph_push_particles_color(rtps_sph* s, const rtps_float4* pos, const rtps_float4* vel, int n,
+                                  rtps_float4 color) {
     if (!s || (!pos && n > 0) || n < 0) return set_error(RTPS_ERR_ARG, "rtps_sph_push_particles: bad arguments");
     if (n == 0) return RTPS_OK;
     int num = 0;
@@ -578,7 +596,17 @@ int rtps_sph_push_particles(rtps_sph* s, const rtps_float4* pos, const rtps_floa
     if ((long)num + n > s->max_num)
         return set_error(RTPS_ERR_CAPACITY, "pushParticles: exceeded max nb (" + std::to_string(s->max_num) +
                                                 ") of particles allowed");
+    // SPH::pushParticles (SPH.cpp:560-602): positions, colours, velocities
+    const bool other = color.x != kDefaultColor.x || color.y != kDefaultColor.y || color.z != kDefaultColor.z ||
+                       color.w != kDefaultColor.w;
+    if (other && !s->opt_color) {   // from now on colours follow their particles
+        s->opt_color = 1;
+        s->graph_valid = false;
+        s->timed_graph_valid = false;
+    }
+    std::vector<rtps_float4> cols((size_t)n, color);
     RTPS_TRY_HIP(hipMemcpyAsync(s->pos_u + num, pos, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+    RTPS_TRY_HIP(hipMemcpyAsync(s->color_u + num, cols.data(), (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
     if (vel) RTPS_TRY_HIP(hipMemcpyAsync(s->vel_u + num, vel, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
     else RTPS_TRY_HIP(hipMemsetAsync(s->vel_u + num, 0, (size_t)n * 16, s->stream));
     num += n;
@@ -590,13 +618,18 @@ int rtps_sph_push_particles(rtps_sph* s, const rtps_float4* pos, const rtps_floa
     return RTPS_OK;
 }
 
+int rtps_sph_push_particles(rtps_sph* s, const rtps_float4* pos, const rtps_float4* vel, int n) {
+    return rtps_sph_push_particles_color(s, pos, vel, n, kDefaultColor);
+}
+
 namespace {
 
 // SPH::sprayHoses (SPH.cpp:547-557): each hose's spray is pushed with the
 // hose velocity.  A push that would exceed max_num is dropped, as
 // SPH::pushParticles does (SPH.cpp:563-567); the hose still counts it sprayed.
 int spray_hoses(rtps_sph* s) {
-    for (rtps_hose* h : s->hoses) {
+    for (size_t k = 0; k < s->hoses.size(); ++k) {
+        rtps_hose* h = s->hoses[k];
         std::vector<rtps_float4> p = hose_spray(h);
         if (p.empty()) continue;
         rtps_float4 v{};
@@ -606,7 +639,7 @@ int spray_hoses(rtps_sph* s) {
         if (rc) return rc;
         if ((long)num + (long)p.size() > s->max_num) continue;
         std::vector<rtps_float4> vel(p.size(), v);
-        rc = rtps_sph_push_particles(s, p.data(), vel.data(), (int)p.size());
+        rc = rtps_sph_push_particles_color(s, p.data(), vel.data(), (int)p.size(), s->hose_color[k]);
         if (rc) return rc;
     }
     return RTPS_OK;
@@ -621,7 +654,8 @@ int hose_at(rtps_sph* s, int index, rtps_hose** h) {
 
 }  // namespace
 
-int rtps_sph_add_hose(rtps_sph* s, int total_n, rtps_float4 center, rtps_float4 velocity, float radius, int* index) {
+int rtps_sph_add_hose(rtps_sph* s, int total_n, rtps_float4 center, rtps_float4 velocity, float radius,
+                      rtps_float4 color, int* index) {
     if (!s || !index) return set_error(RTPS_ERR_ARG, "rtps_sph_add_hose: null argument");
     // SPH::addHose (SPH.cpp:523-532): radius in units of the particle spacing
     const float spacing = s->settings.GetSettingAs<float>("spacing");
@@ -631,14 +665,17 @@ int rtps_sph_add_hose(rtps_sph* s, int total_n, rtps_float4 center, rtps_float4 
                               radius * spacing, spacing, seed, &h);
     if (rc) return rc;
     s->hoses.push_back(h);
+    s->hose_color.push_back(color);
     *index = (int)s->hoses.size() - 1;
     return RTPS_OK;
 }
 
-int rtps_sph_update_hose(rtps_sph* s, int index, rtps_float4 center, rtps_float4 velocity, float radius) {
+int rtps_sph_update_hose(rtps_sph* s, int index, rtps_float4 center, rtps_float4 velocity, float radius,
+                         rtps_float4 color) {
     rtps_hose* h = nullptr;
     int rc = hose_at(s, index, &h);
     if (rc) return rc;
+    s->hose_color[index] = color;
     const float spacing = s->settings.GetSettingAs<float>("spacing");   // SPH::updateHose (SPH.cpp:533-541)
     return rtps_hose_update(h, center, velocity, radius * spacing, spacing);
 }
@@ -683,7 +720,8 @@ int rtps_sph_integrate(rtps_sph* s) {
     if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_integrate: null handle");
     if (!s->update_pending) return set_error(RTPS_ERR_STATE, "integrate() needs a preceding update()");
     RTPS_TRY_HIP(launch_leapfrog(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
-                                 s->xsph_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u, s->stream));
+                                 s->xsph_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u, s->stream,
+                                 col_s(s), col_u(s)));
     RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
     s->update_pending = false;
     return RTPS_OK;
@@ -757,6 +795,12 @@ int rtps_sph_read(rtps_sph* s, int field, void* host, int count) {
         }
         return RTPS_OK;
     }
+    if (field == RTPS_FIELD_MASS || field == RTPS_FIELD_MASS_S) {
+        // every particle carries the system mass (SPH.cpp:572-573 fills mass_p with sphp.mass)
+        float* m = (float*)host;
+        for (int i = 0; i < count; ++i) m[i] = s->sp.mass;
+        return RTPS_OK;
+    }
     if (field == RTPS_FIELD_DENSITY) {
         std::vector<float> tmp((size_t)count * 4);
         RTPS_TRY_HIP(hipMemcpy(tmp.data(), s->ve_s, (size_t)count * 16, hipMemcpyDeviceToHost));
@@ -770,10 +814,52 @@ int rtps_sph_read(rtps_sph* s, int field, void* host, int count) {
     return RTPS_OK;
 }
 
+int rtps_host_alloc(size_t bytes, void** host) {
+    if (!host) return set_error(RTPS_ERR_ARG, "rtps_host_alloc: null argument");
+    *host = nullptr;
+    RTPS_TRY_HIP(hipHostMalloc(host, bytes ? bytes : 1, hipHostMallocDefault));
+    return RTPS_OK;
+}
+
+int rtps_host_free(void* host) {
+    if (host) RTPS_TRY_HIP(hipHostFree(host));
+    return RTPS_OK;
+}
+
+int rtps_sph_read_async(rtps_sph* s, int field, void* host, int count) {
+    if (!s || !host || count < 0) return set_error(RTPS_ERR_ARG, "rtps_sph_read_async: bad arguments");
+    if (count == 0) return RTPS_OK;
+    const void* src = nullptr;
+    size_t bytes = 0;
+    if (field == RTPS_FIELD_CELL_START || field == RTPS_FIELD_CELL_END) {
+        if (count > s->nb_cells + 1) return set_error(RTPS_ERR_ARG, "rtps_sph_read_async: count > nb_cells + 1");
+        src = field == RTPS_FIELD_CELL_START ? s->start : s->end;
+        bytes = (size_t)count * 4;
+    } else if (count > s->max_num) {
+        return set_error(RTPS_ERR_ARG, "rtps_sph_read_async: count > max_num");
+    } else if (field == RTPS_FIELD_IDX) {
+        src = s->idx_sorted;
+        bytes = (size_t)count * 4;
+    } else {
+        src = field_f4(s, field);
+        bytes = (size_t)count * 16;
+    }
+    if (!src)
+        return set_error(RTPS_ERR_ARG, "rtps_sph_read_async: only float4 fields, IDX and the cell arrays (KEYS, "
+                                       "DENSITY and MASS need host-side conversion: use rtps_sph_read)");
+    RTPS_TRY_HIP(hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, s->stream));
+    return RTPS_OK;
+}
+
 int rtps_sph_write(rtps_sph* s, int field, const void* host, int count) {
     if (!s || !host || count < 0 || count > s->max_num) return set_error(RTPS_ERR_ARG, "rtps_sph_write: bad arguments");
-    if (field != RTPS_FIELD_POS && field != RTPS_FIELD_VEL && field != RTPS_FIELD_VELEVAL)
-        return set_error(RTPS_ERR_ARG, "rtps_sph_write: only POS, VEL and VELEVAL are writable");
+    if (field != RTPS_FIELD_POS && field != RTPS_FIELD_VEL && field != RTPS_FIELD_VELEVAL && field != RTPS_FIELD_COLOR)
+        return set_error(RTPS_ERR_ARG, "rtps_sph_write: only POS, VEL, VELEVAL and COLOR are writable");
+    if (field == RTPS_FIELD_COLOR && !s->opt_color) {   // written colours are kept with their particles
+        s->opt_color = 1;
+        s->graph_valid = false;
+        s->timed_graph_valid = false;
+    }
     RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
     RTPS_TRY_HIP(hipMemcpy(field_f4(s, field), host, (size_t)count * 16, hipMemcpyHostToDevice));
     return RTPS_OK;
@@ -787,6 +873,8 @@ int rtps_sph_device_ptr(rtps_sph* s, int field, void** dev) {
         case RTPS_FIELD_CELL_START: *dev = s->start; return RTPS_OK;
         case RTPS_FIELD_CELL_END: *dev = s->end; return RTPS_OK;
         case RTPS_FIELD_DENSITY: return set_error(RTPS_ERR_ARG, "density lives in VELEVAL_S .w (packed)");
+        case RTPS_FIELD_MASS:
+        case RTPS_FIELD_MASS_S: return set_error(RTPS_ERR_ARG, "masses are not stored (every particle has sphp.mass)");
         default: break;
     }
     float4* p = field_f4(s, field);
@@ -839,6 +927,10 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
             if (value != 4 && value != 8 && value != 16)
                 return set_error(RTPS_ERR_ARG, "sort tile must be 4, 8 or 16 keys per thread");
             g_sort_items = value;
+            break;
+        case RTPS_OPT_COLOR:
+            if (value < 0 || value > 1) return set_error(RTPS_ERR_ARG, "colour option must be 0 or 1");
+            s->opt_color = value;
             break;
         case RTPS_OPT_AUTOTUNE:
             if (value < 0 || value > 2) return set_error(RTPS_ERR_ARG, "autotune must be 0, 1 or 2");

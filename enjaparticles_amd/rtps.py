@@ -90,16 +90,19 @@ class SPH:
         self.close()
 
     # -- System API -------------------------------------------------------
-    def pushParticles(self, pos, vel=None) -> None:
+    def pushParticles(self, pos, vel=None, color=None) -> None:
+        """SPH::pushParticles (SPH.cpp:560-602); color is one float4 for all
+        pushed particles (default (1, 0, 0, 1), System.h:160)."""
         pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
         vptr = None
         if vel is not None:
             vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 4)
             assert vel.shape == pos.shape
             vptr = vel.ctypes.data
-        N.check(N.lib().rtps_sph_push_particles(self._h, pos.ctypes.data, vptr, pos.shape[0]))
+        col = _f4(DEFAULT_COLOR if color is None else color)
+        N.check(N.lib().rtps_sph_push_particles_color(self._h, pos.ctypes.data, vptr, pos.shape[0], col))
 
-    def addBox(self, nn: int, bmin, bmax, spacing: float | None = None) -> int:
+    def addBox(self, nn: int, bmin, bmax, spacing: float | None = None, color=None) -> int:
         """System::addBox (System.cpp:331-338) with addRect (IV.cpp:165-202):
         float-accumulated lattice, z outer / x inner, at most nn particles."""
         if spacing is None:
@@ -118,22 +121,24 @@ class SPH:
             z = np.float32(z + sp)
         pos = np.array(out, np.float32).reshape(-1, 4)
         if len(pos):
-            self.pushParticles(pos)
+            self.pushParticles(pos, color=color)
         return len(pos)
 
     def addHose(self, total_n: int, center, velocity, radius: float, color=None, mass=None) -> int:
         """SPH::addHose (SPH.cpp:523-532): a hose sprayed at the start of every
         update; radius is in units of the particle spacing.  Returns its index.
-        Colour and mass are accepted for the reference's signature and unused
-        (particles carry no colour; every particle has the system's mass)."""
+        Sprayed particles get the hose colour; mass is accepted for the
+        reference's signature (every particle has the system's mass)."""
         idx = C.c_int()
-        N.check(N.lib().rtps_sph_add_hose(self._h, int(total_n), _f4(center), _f4(velocity), float(radius),
+        col = _f4(DEFAULT_COLOR if color is None else color)
+        N.check(N.lib().rtps_sph_add_hose(self._h, int(total_n), _f4(center), _f4(velocity), float(radius), col,
                                           C.byref(idx)))
         return idx.value
 
     def updateHose(self, index: int, center, velocity, radius: float, color=None) -> None:
         """SPH::updateHose (SPH.cpp:533-541)."""
-        N.check(N.lib().rtps_sph_update_hose(self._h, int(index), _f4(center), _f4(velocity), float(radius)))
+        col = _f4(DEFAULT_COLOR if color is None else color)
+        N.check(N.lib().rtps_sph_update_hose(self._h, int(index), _f4(center), _f4(velocity), float(radius), col))
 
     def refillHose(self, index: int, refill: int) -> None:
         """SPH::refillHose (SPH.cpp:542-545)."""
@@ -192,13 +197,20 @@ class SPH:
             count = self.nb_cells + 1 if field in ("cell_start", "cell_end") else self.getNum()
         if field in ("keys", "idx", "cell_start", "cell_end"):
             out = np.empty(count, np.uint32)
-        elif field == "density":
+        elif field in ("density", "mass", "mass_s"):
             out = np.empty(count, np.float32)
         else:
             out = np.empty((count, 4), np.float32)
         if count:
             N.check(N.lib().rtps_sph_read(self._h, f, out.ctypes.data, count))
         return out
+
+    def readAsync(self, field: str, out: "HostBuffer", count: int | None = None) -> None:
+        """Enqueue a copy of a float4 field (or idx / cell arrays) into a
+        page-locked HostBuffer; it is there after synchronize()."""
+        count = out.count if count is None else count
+        assert count <= out.count
+        N.check(N.lib().rtps_sph_read_async(self._h, N.FIELD[field], out.ptr, int(count)))
 
     def write(self, field: str, arr) -> None:
         a = np.ascontiguousarray(arr, np.float32).reshape(-1, 4)
@@ -233,6 +245,10 @@ class SPH:
     def getPositionBuffer(self): return self.read("pos_s")
     def getVelocityBuffer(self): return self.read("vel_s")
     def getDensityBuffer(self): return self.read("density")
+    def getColorBuffer(self): return self.read("color_s")
+    def getColorBufferUnsorted(self): return self.read("color")
+    def getMassBuffer(self): return self.read("mass_s")
+    def getMassBufferUnsorted(self): return self.read("mass")
     def getCellStartBuffer(self): return self.read("cell_start")
     def getCellEndBuffer(self): return self.read("cell_end")
 
@@ -253,6 +269,36 @@ class SPH:
         text = "\n".join(lines)
         print(text)
         return text
+
+
+DEFAULT_COLOR = (1.0, 0.0, 0.0, 1.0)   # System.h:160
+
+
+class HostBuffer:
+    """Page-locked host memory (rtps_host_alloc) seen as a numpy array of
+    `count` float4 records (or uint32 with dtype=np.uint32)."""
+
+    def __init__(self, count: int, dtype=np.float32):
+        self.count = int(count)
+        shape = (self.count, 4) if np.dtype(dtype) == np.float32 else (self.count,)
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = C.c_void_p()
+        N.check(N.lib().rtps_host_alloc(nbytes, C.byref(p)))
+        self.ptr = p.value
+        raw = (C.c_char * max(nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(raw, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            self.array = None
+            N.lib().rtps_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _f4(v) -> N.F4:

@@ -88,14 +88,15 @@ public:
         return n;
     }
 
-    void pushParticles(const std::vector<float4>& pos, const std::vector<float4>& vel, float4 = float4{1, 0, 0, 1},
-                       float = 0.0f) override {
-        check(rtps_sph_push_particles(h_, pos.data(), vel.empty() ? nullptr : vel.data(), (int)pos.size()));
+    void pushParticles(const std::vector<float4>& pos, const std::vector<float4>& vel,
+                       float4 color = float4{1, 0, 0, 1}, float = 0.0f) override {
+        check(rtps_sph_push_particles_color(h_, pos.data(), vel.empty() ? nullptr : vel.data(), (int)pos.size(),
+                                            color));
     }
 
     // System::addBox + addRect (System.cpp:331-338, IV.cpp:165-202): float-accumulated
     // lattice at the settings' spacing, z outer, x inner, at most nn particles.
-    int addBox(int nn, float4 min, float4 max, bool, float4 = float4{1, 0, 0, 1}, float = 0.0f) override {
+    int addBox(int nn, float4 min, float4 max, bool, float4 color = float4{1, 0, 0, 1}, float = 0.0f) override {
         const float spacing = getSettingFromSystem<float>("spacing");
         std::vector<float4> rvec;
         for (float z = min.z; z <= max.z; z += spacing)
@@ -104,22 +105,22 @@ public:
                     if ((int)rvec.size() >= nn) break;
                     rvec.push_back(float4{x, y, z, 1.0f});
                 }
-        pushParticles(rvec, std::vector<float4>());
+        pushParticles(rvec, std::vector<float4>(), color);
         return (int)rvec.size();
     }
 
-    // Hoses (SPH.cpp:523-557): sprayed at the start of every update; radius in
-    // units of the particle spacing.  Colour and mass are not stored.
-    int addHose(int total_n, float4 center, float4 velocity, float radius, float4 = float4{1, 0, 0, 1},
+    // Hoses (SPH.cpp:523-557): sprayed at the start of every update with the
+    // hose colour; radius in units of the particle spacing.  Mass is not stored.
+    int addHose(int total_n, float4 center, float4 velocity, float radius, float4 color = float4{1, 0, 0, 1},
                 float = 0.0f) {
         flushSettings();
         int index = -1;
-        check(rtps_sph_add_hose(h_, total_n, center, velocity, radius, &index));
+        check(rtps_sph_add_hose(h_, total_n, center, velocity, radius, color, &index));
         return index;
     }
-    void updateHose(int index, float4 center, float4 velocity, float radius, float4 = float4{1, 0, 0, 1}) {
+    void updateHose(int index, float4 center, float4 velocity, float radius, float4 color = float4{1, 0, 0, 1}) {
         flushSettings();
-        check(rtps_sph_update_hose(h_, index, center, velocity, radius));
+        check(rtps_sph_update_hose(h_, index, center, velocity, radius, color));
     }
     void refillHose(int index, int refill) { check(rtps_sph_refill_hose(h_, index, refill)); }
     // System::testDelete (System.cpp:366-376)
@@ -132,6 +133,13 @@ public:
     std::vector<float4> getVelocityBufferUnsorted() { return readF4(RTPS_FIELD_VEL); }
     std::vector<float4> getPositionBuffer() { return readF4(RTPS_FIELD_POS_S); }
     std::vector<float4> getVelocityBuffer() { return readF4(RTPS_FIELD_VEL_S); }
+    std::vector<float4> getColorBufferUnsorted() { return readF4(RTPS_FIELD_COLOR); }
+    std::vector<float4> getColorBuffer() { return readF4(RTPS_FIELD_COLOR_S); }
+    std::vector<float> getMassBuffer() {
+        std::vector<float> out(getNum());
+        if (!out.empty()) check(rtps_sph_read(h_, RTPS_FIELD_MASS_S, out.data(), (int)out.size()));
+        return out;
+    }
     std::vector<float> getDensityBuffer() {
         std::vector<float> out(getNum());
         if (!out.empty()) check(rtps_sph_read(h_, RTPS_FIELD_DENSITY, out.data(), (int)out.size()));

@@ -154,6 +154,13 @@ int rtps_sph_destroy(rtps_sph* s);
  * float4; vel may be NULL for zero).  RTPS_ERR_CAPACITY if num + n > max_num
  * (the reference prints and returns). */
 int rtps_sph_push_particles(rtps_sph* s, const rtps_float4* pos, const rtps_float4* vel, int n);
+/* The same with the colour argument of SPH::pushParticles (every pushed
+ * particle gets `color`; rtps_sph_push_particles uses the default (1,0,0,1)).
+ * From the first push of another colour on, the system keeps colours sorted
+ * and renumbered with their particles every step, as permute.cl / leapfrog.cl
+ * do (RTPS_OPT_COLOR). */
+int rtps_sph_push_particles_color(rtps_sph* s, const rtps_float4* pos, const rtps_float4* vel, int n,
+                                  rtps_float4 color);
 
 /* SPH::update (SPH.cpp:117-125,149-300) and SPH::integrate (SPH.cpp:389-441). */
 int rtps_sph_update(rtps_sph* s);
@@ -181,14 +188,30 @@ enum rtps_field {
     RTPS_FIELD_KEYS = 9,       /* uint32, sorted cell keys (device pointer: cell << 4 | in-cell x sixteenth) */
     RTPS_FIELD_IDX = 10,       /* uint32, sorted -> original index */
     RTPS_FIELD_CELL_START = 11, /* uint32[nb_cells+1] (getCellStartBuffer) */
-    RTPS_FIELD_CELL_END = 12   /* uint32[nb_cells+1] (getCellEndBuffer) */
+    RTPS_FIELD_CELL_END = 12,  /* uint32[nb_cells+1] (getCellEndBuffer) */
+    RTPS_FIELD_COLOR = 13,     /* float4, unsorted (getColorBufferUnsorted) */
+    RTPS_FIELD_COLOR_S = 14,   /* float4, sorted (getColorBuffer); current while colours are kept */
+    RTPS_FIELD_MASS = 15,      /* float, unsorted (getMassBufferUnsorted): the system mass; read only */
+    RTPS_FIELD_MASS_S = 16     /* float, sorted (getMassBuffer); read only */
 };
 /* Blocking copy of the first `count` elements of a field into host memory. */
 int rtps_sph_read(rtps_sph* s, int field, void* host, int count);
 /* Device pointer of a field (zero-copy access on the handle's stream). */
 int rtps_sph_device_ptr(rtps_sph* s, int field, void** dev);
-/* Overwrite the first `count` elements of an unsorted float4 field (POS, VEL, VELEVAL). */
+/* Overwrite the first `count` elements of an unsorted float4 field (POS, VEL,
+ * VELEVAL, COLOR; writing COLOR turns RTPS_OPT_COLOR on). */
 int rtps_sph_write(rtps_sph* s, int field, const void* host, int count);
+
+/* Host export for renderers (in place of the reference's GL buffer sharing,
+ * System.cpp:214-238): rtps_sph_read_async enqueues the copy of the first
+ * `count` elements of a float4 field, IDX or a cell array on the handle's
+ * stream and returns; the data is in `host` after the next
+ * rtps_sph_synchronize (or rtps_sph_read).  Steps enqueued after it do not
+ * disturb it.  `host` should be page-locked memory from rtps_host_alloc, so
+ * the copy overlaps host work. */
+int rtps_host_alloc(size_t bytes, void** host);
+int rtps_host_free(void* host);
+int rtps_sph_read_async(rtps_sph* s, int field, void* host, int count);
 
 /* RTPSSettings::SetSetting / GetSettingAs<string> on the handle's settings;
  * a change is applied at the next update (SPH.cpp:154). */
@@ -220,6 +243,11 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        they fill), 0 = fixed defaults
  *   RTPS_OPT_SORT_TILE   keys per thread of a radix-sort tile (256 threads),
  *                        process-wide: 4, 8 or 16 (default 8)
+ *   RTPS_OPT_COLOR       1 = permute colours every step and renumber them with
+ *                        the survivors of a deletion (permute.cl:41,
+ *                        leapfrog.cl:107); 0 = colours stay where pushed.
+ *                        Default 0; the first push of a non-default colour
+ *                        sets 1.
  *   RTPS_OPT_TILE_REMAP  tile order of the neighbour kernels over the 8 XCDs:
  *                        0 = hardware round-robin (default), 1 = one
  *                        contiguous range per XCD, C in 2..4096 = runs of C
@@ -233,6 +261,7 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
 #define RTPS_OPT_TILE_REMAP 6
 #define RTPS_OPT_AUTOTUNE 7
 #define RTPS_OPT_SORT_TILE 8
+#define RTPS_OPT_COLOR 9
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))
@@ -287,10 +316,12 @@ int rtps_hose_state(rtps_hose* h, int* n_left, int* em_count, int* em, rtps_floa
  * system, sprayed at the start of every update (SPH::sprayHoses, SPH.cpp:166,
  * 547-557) with the hose velocity for the new particles.  radius is in units
  * of the settings' "spacing" (SPH.cpp:526).  Sprays are pushed like
- * rtps_sph_push_particles, so rtps_sph_step takes one host round trip per
- * step while a hose has particles left. */
-int rtps_sph_add_hose(rtps_sph* s, int total_n, rtps_float4 center, rtps_float4 velocity, float radius, int* index);
-int rtps_sph_update_hose(rtps_sph* s, int index, rtps_float4 center, rtps_float4 velocity, float radius);
+ * rtps_sph_push_particles_color with the hose colour, so rtps_sph_step takes
+ * one host round trip per step while a hose exists. */
+int rtps_sph_add_hose(rtps_sph* s, int total_n, rtps_float4 center, rtps_float4 velocity, float radius,
+                      rtps_float4 color, int* index);
+int rtps_sph_update_hose(rtps_sph* s, int index, rtps_float4 center, rtps_float4 velocity, float radius,
+                         rtps_float4 color);
 int rtps_sph_refill_hose(rtps_sph* s, int index, int refill);
 /* System::testDelete (System.cpp:366-376): moves the first 40 particles to
  * (100, 100, 100), outside the grid, so the next update deletes them. */

@@ -289,7 +289,7 @@ def test_hose_sprays_match_oracle(cuda):
     center, vel, radius, total = (0.5 * box, 0.5 * box, 0.8 * box, 1.0), (0.1, 0.05, -1.0, 0.0), 3.0, 500
     s = RTPS.generateSystemInstance(settings)
     s.pushParticles(sc.pos, sc.vel)
-    assert s.addHose(total, center, vel, radius) == 0
+    assert s.addHose(total, center, vel, radius, color=(0.0, 0.0, 1.0, 1.0)) == 0
     spacing = np.float32(s.getSetting("spacing"))
     dt = np.float32(s.getSetting("time_step"))
     hose = H.Hose(dt, total, center, vel, np.float32(radius) * spacing, spacing, 0x5EED)
@@ -306,6 +306,10 @@ def test_hose_sprays_match_oracle(cuda):
         assert s.getNum() == st.num, k
         np.testing.assert_allclose(s.read("pos")[:, :3], st.pos_u[: st.num, :3], rtol=1e-4, atol=1e-6)
     assert sprays >= 3 and st.num > sc.n + 100
+    # sprayed particles carry the hose colour, the pushed ones the default
+    col = s.read("color")
+    np.testing.assert_array_equal(col[: sc.n], np.tile([1.0, 0.0, 0.0, 1.0], (sc.n, 1)).astype(np.float32))
+    np.testing.assert_array_equal(col[sc.n:], np.tile([0.0, 0.0, 1.0, 1.0], (st.num - sc.n, 1)).astype(np.float32))
     # the split update()/integrate() path sprays at the start of update()
     b = RTPS.generateSystemInstance(settings)
     b.pushParticles(sc.pos, sc.vel)
@@ -345,4 +349,78 @@ def test_test_delete_removes_forty(cuda):
     from scipy.spatial import cKDTree
     d, j = cKDTree(st.pos_u[:n, :3].astype(np.float64)).query(s.read("pos")[:, :3].astype(np.float64))
     assert d.max() <= 1e-5 and len(np.unique(j)) == n
+    s.close()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_colors_follow_their_particles(cuda, fused):
+    """Colours are permuted with the particles (permute.cl:41) and renumbered
+    with the survivors of a deletion (leapfrog.cl:107, SPH.cpp:450-457): every
+    particle is labelled by its colour and must keep its label."""
+    sc = scenes.dam16k()
+    pos = sc.pos.copy()
+    gone = np.arange(5, sc.n, 211)
+    pos[gone, 2] = 70.0    # far above the grid: deleted by the first step
+    s = RTPS.generateSystemInstance(sc.settings)
+    s.pushParticles(pos, sc.vel)
+    label = np.zeros((sc.n, 4), np.float32)
+    label[:, 0] = np.arange(sc.n)
+    label[:, 3] = 1.0
+    s.write("color", label)   # written colours are kept with their particles
+    p0 = s.read("pos")
+
+    def one_step():
+        if fused:
+            s.step(1)
+        else:
+            s.update()
+            s.integrate()
+
+    one_step()
+    n = s.getNum()
+    assert n == sc.n - len(gone)
+    idx = s.read("idx", sc.n)
+    cs = s.read("color_s", sc.n)
+    np.testing.assert_array_equal(cs, label[idx])             # sorted colours: colour_u[idx]
+    cu = s.read("color")
+    np.testing.assert_array_equal(cu, cs[:n])                  # survivors renumbered in sorted order
+    ids = cu[:, 0].astype(np.int64)
+    assert not np.isin(ids, gone).any() and len(np.unique(ids)) == n
+    # each surviving slot holds the particle its label names, moved by one step
+    p1 = s.read("pos")
+    assert np.abs(p1[:, :3] - p0[ids, :3]).max() < 0.01
+    # without a deletion colours stay in their slots
+    one_step()
+    np.testing.assert_array_equal(s.read("color"), cu)
+    np.testing.assert_array_equal(s.read("color_s", n), cu[s.read("idx", n)])
+    m = s.read("mass")
+    assert m.shape == (n,) and np.all(m == np.float32(s.getSetting("mass")))
+    s.close()
+
+
+def test_read_async_into_page_locked_memory(cuda):
+    """rtps_sph_read_async: the copy lands in a page-locked buffer at the next
+    synchronisation, holding the state at the point it was enqueued."""
+    from enjaparticles_amd._native import RTPSError
+    from enjaparticles_amd.rtps import HostBuffer
+    sc = scenes.dam_custom(16, 16, 8, seed=9)
+    s = _system(sc)
+    s.step(2)
+    buf, ib = HostBuffer(sc.n), HostBuffer(sc.n, np.uint32)
+    before, idx_before = s.read("pos"), s.read("idx")
+    s.readAsync("pos", buf)
+    s.readAsync("idx", ib)
+    s.step(3)                  # enqueued after the copies
+    s.synchronize()
+    np.testing.assert_array_equal(buf.array, before)
+    np.testing.assert_array_equal(ib.array, idx_before)
+    after = s.read("pos")
+    assert not np.array_equal(after, before)
+    s.readAsync("pos", buf)
+    s.synchronize()
+    np.testing.assert_array_equal(buf.array, after)
+    with pytest.raises(RTPSError):
+        s.readAsync("density", buf)
+    buf.close()
+    ib.close()
     s.close()
