@@ -198,6 +198,27 @@ def batched_adjoint_leg(steps: int = 100, num_scenes: int = 64, checkpoint_every
             "adjoint_over_plain": tb / tp, "finite_gradients": bool(np.all(np.isfinite(dmu + dK)))}
 
 
+def surface_leg(s, res: int, iso: float = 0.5, reps: int = 10):
+    """Colour field and iso-surface of the current state (SPH::postProcess):
+    wall time per call, each ending in a stream synchronisation."""
+    import ctypes as C
+    from enjaparticles_amd import _native as N
+    lib = N.lib()
+    n = C.c_int()
+    N.check(lib.rtps_sph_color_field(s._h, res, None))               # buffers, first launch
+    N.check(lib.rtps_sph_surface(s._h, res, iso, 0, None, None, C.byref(n)))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        N.check(lib.rtps_sph_color_field(s._h, res, None))
+    field_ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        N.check(lib.rtps_sph_surface(s._h, res, iso, 0, None, None, C.byref(n)))
+    surf_ms = (time.perf_counter() - t0) * 1e3 / reps
+    return {"res": res, "nodes": res ** 3, "iso": iso, "triangles": n.value, "field_ms": field_ms,
+            "field_plus_surface_ms": surf_ms, "nodes_per_s": res ** 3 / (field_ms * 1e-3)}
+
+
 def compare4m_leg(steps: int = 100):
     """Plain and batched forward at 4,194,304 particles: one 256x128x128 dam
     break, and 64 scenes of 64x64x16, stepped `steps` steps each (hipGraph),
@@ -254,6 +275,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--profile", action="store_true", help="timed loop only (for rocprofv3 runs)")
     ap.add_argument("--force-variant", type=int, default=0, help="1 = compacted neighbour lists, 0 = direct")
+    ap.add_argument("--surface-res", type=int, default=128,
+                    help="colour-field lattice of the surface leg on the final state (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=30, help="steps of the batched leg (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="steps of the fwd+adjoint leg (0 = skip)")
     ap.add_argument("--badj-steps", type=int, default=100, help="steps of the batched fwd+adjoint leg (0 = skip)")
@@ -390,6 +413,12 @@ def main():
             "frac": dens_flop / (dens_ms * 1e-3) / 1e12 / PEAK_VALU_FP32_TFLOPS,
             "avg_launch_ms": dens_ms,
         }
+    if args.surface_res > 0 and not args.profile:
+        print("# surface leg ...", file=sys.stderr, flush=True)
+        out["surface"] = surface_leg(s, args.surface_res)
+        print(f"# surface: field {out['surface']['field_ms']:.3f} ms, +surface "
+              f"{out['surface']['field_plus_surface_ms']:.3f} ms, {out['surface']['triangles']} triangles",
+              file=sys.stderr, flush=True)
     s.close()
     if args.batched_steps > 0:
         print("# batched leg ...", file=sys.stderr, flush=True)

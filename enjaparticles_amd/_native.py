@@ -89,6 +89,11 @@ SIGNATURES = {
     "rtps_sph_write": (_i, [P, _i, P, _i]),
     "rtps_sph_device_ptr": (_i, [P, _i, P]),
     "rtps_sph_read_async": (_i, [P, _i, P, _i]),
+    "rtps_sph_color_field": (_i, [P, _i, P]),
+    "rtps_sph_surface": (_i, [P, _i, _f, _i, P, P, P]),
+    "rtps_sph_post_process": (_i, [P]),
+    "rtps_sph_mesh": (_i, [P, _i, P, P, P]),
+    "rtps_mc_case_table": (_i, [P, P]),
     "rtps_host_alloc": (_i, [_sz, P]),
     "rtps_host_free": (_i, [P]),
     "rtps_sph_set_setting": (_i, [P, C.c_char_p, C.c_char_p]),

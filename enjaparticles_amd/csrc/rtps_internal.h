@@ -71,6 +71,19 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st);
 // Default particle colour of System::pushParticles (System.h:160).
 constexpr rtps_float4 kDefaultColor{1.0f, 0.0f, 0.0f, 1.0f};
 
+// ---- colour field and iso-surface (surface.hip) ----
+size_t surface_scan_tiles(long ncubes);
+hipError_t launch_color_field(int res, const float4* pos_s, const float4* ve_s, const uint32_t* start,
+                              const uint32_t* end, const rtps_sph_params& sp, const Grid& g, const Win& win,
+                              const rtps_float4& bmin_scaled, const rtps_float4& bmax_scaled, float* field,
+                              hipStream_t st);
+hipError_t launch_mc_count(int res, float iso, const float* field, const uint8_t* ntri, uint8_t* cases,
+                           uint32_t* counts, uint32_t* offs, uint32_t* tile_sums, uint32_t* total, hipStream_t st);
+hipError_t launch_mc_emit(int res, float iso, const float* field, const uint8_t* cases, const uint32_t* counts,
+                          const uint32_t* offs, const uint32_t* tile_offs, const int8_t* tri,
+                          const rtps_float4& wmin, const rtps_float4& wmax, long capacity, float* verts,
+                          float* normals, hipStream_t st);
+
 // ---- hoses (hose.cpp) ----
 std::vector<rtps_float4> hose_spray(rtps_hose* h);
 

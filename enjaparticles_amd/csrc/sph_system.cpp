@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "rtps_internal.h"
+#include "mc_table.h"
 #include "sph_device.h"
 
 using namespace rtps;
@@ -69,6 +70,17 @@ struct rtps_sph {
     double stage_ms[ST_N] = {};
     long stage_cnt[ST_N] = {};
     bool update_pending = false;
+    bool sorted_ready = false;   // a step or update has filled the sorted arrays
+    // colour field and iso-surface (SPH::postProcess)
+    int cf_res = 0;
+    float* cf_field = nullptr;
+    uint8_t* mc_cases = nullptr;
+    uint32_t *mc_counts = nullptr, *mc_offs = nullptr, *mc_tiles = nullptr, *mc_total = nullptr;
+    int8_t* mc_tri = nullptr;
+    uint8_t* mc_ntri = nullptr;
+    float *mesh_v = nullptr, *mesh_n = nullptr;
+    long mesh_cap = 0;
+    int mesh_ntri = 0;
     std::vector<rtps_hose*> hoses;   // SPH::hoses (SPH.h), sprayed at the start of every update
     std::vector<rtps_float4> hose_color;
 
@@ -81,6 +93,9 @@ struct rtps_sph {
         void* bufs[] = {pos_u,   vel_u,    veleval_u, pos_s,   vel_s,   ve_s,  force_s,  xsph_s,  color_u, color_s,
                         keys,    idx,      start,     end,     d_num,   sort_tmp, rm.rows, rm.words, rm.codes, rm.count};
         for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        void* sbufs[] = {cf_field, mc_cases, mc_counts, mc_offs, mc_tiles, mc_total, mc_tri, mc_ntri, mesh_v, mesh_n};
+        for (void* b : sbufs)
             if (b) (void)hipFree(b);
         for (auto& set : ev)
             for (auto& e : set)
@@ -169,6 +184,7 @@ struct StageTimer {
 // hash -> sort -> cellindices -> permute (System::hash_and_sort, SPH.cpp:170-213)
 int launch_front(rtps_sph* s, StageTimer& t) {
     const int n = s->num_host;
+    s->sorted_ready = true;
     t.mark(ST_HASH);
     RTPS_TRY_HIP(launch_hash(n, s->d_num, n, s->pad_key, s->pos_u, s->keys, s->idx, s->g, s->start,
                              s->nb_cells + 1, kSubBits, s->stream));
@@ -812,6 +828,136 @@ int rtps_sph_read(rtps_sph* s, int field, void* host, int count) {
     if (!src) return set_error(RTPS_ERR_ARG, "rtps_sph_read: unknown field");
     RTPS_TRY_HIP(hipMemcpy(host, src, (size_t)count * 16, hipMemcpyDeviceToHost));
     return RTPS_OK;
+}
+
+namespace {
+
+// Buffers of the colour field and the surface for a res^3 lattice.
+int surface_buffers(rtps_sph* s, int res) {
+    if (!s->mc_tri) {   // case table, once per handle
+        const McTable& t = mc_table();
+        RTPS_TRY_HIP(hipMalloc(&s->mc_tri, sizeof(t.tri)));
+        RTPS_TRY_HIP(hipMalloc(&s->mc_ntri, sizeof(t.ntri)));
+        RTPS_TRY_HIP(hipMemcpy(s->mc_tri, t.tri, sizeof(t.tri), hipMemcpyHostToDevice));
+        RTPS_TRY_HIP(hipMemcpy(s->mc_ntri, t.ntri, sizeof(t.ntri), hipMemcpyHostToDevice));
+        RTPS_TRY_HIP(hipMalloc(&s->mc_total, sizeof(uint32_t)));
+    }
+    if (s->cf_res == res) return RTPS_OK;
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    void* old[] = {s->cf_field, s->mc_cases, s->mc_counts, s->mc_offs, s->mc_tiles};
+    for (void* b : old)
+        if (b) RTPS_TRY_HIP(hipFree(b));
+    s->cf_field = nullptr;
+    s->mc_cases = nullptr;
+    s->mc_counts = s->mc_offs = s->mc_tiles = nullptr;
+    s->cf_res = 0;
+    const size_t nodes = (size_t)res * res * res, m = (size_t)(res - 1), cubes = m * m * m;
+    RTPS_TRY_HIP(hipMalloc(&s->cf_field, nodes * sizeof(float)));
+    RTPS_TRY_HIP(hipMalloc(&s->mc_cases, cubes));
+    RTPS_TRY_HIP(hipMalloc(&s->mc_counts, cubes * sizeof(uint32_t)));
+    RTPS_TRY_HIP(hipMalloc(&s->mc_offs, cubes * sizeof(uint32_t)));
+    RTPS_TRY_HIP(hipMalloc(&s->mc_tiles, surface_scan_tiles((long)cubes) * sizeof(uint32_t)));
+    s->cf_res = res;
+    return RTPS_OK;
+}
+
+// ColorField::execute (ColorField.cpp) on the last step's sorted state.
+int color_field(rtps_sph* s, int res) {
+    if (res < 2 || res > 1024) return set_error(RTPS_ERR_ARG, "colour field resolution must be 2..1024");
+    if (!s->sorted_ready) return set_error(RTPS_ERR_STATE, "the colour field needs a preceding update or step");
+    if (s->update_pending) return set_error(RTPS_ERR_STATE, "the colour field needs integrate() after update()");
+    int rc = surface_buffers(s, res);
+    if (rc) return rc;
+    RTPS_TRY_HIP(launch_color_field(res, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, win(s), s->gps.bnd_min,
+                                    s->gps.bnd_max, s->cf_field, s->stream));
+    return RTPS_OK;
+}
+
+// MarchingCubes::execute: the iso-surface of the colour field into the
+// handle's mesh buffers (grown to the triangle count).
+int surface(rtps_sph* s, int res, float iso) {
+    int rc = color_field(s, res);
+    if (rc) return rc;
+    RTPS_TRY_HIP(launch_mc_count(res, iso, s->cf_field, s->mc_ntri, s->mc_cases, s->mc_counts, s->mc_offs,
+                                 s->mc_tiles, s->mc_total, s->stream));
+    uint32_t total = 0;
+    RTPS_TRY_HIP(hipMemcpyAsync(&total, s->mc_total, sizeof(total), hipMemcpyDeviceToHost, s->stream));
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    if ((long)total > s->mesh_cap) {
+        if (s->mesh_v) RTPS_TRY_HIP(hipFree(s->mesh_v));
+        if (s->mesh_n) RTPS_TRY_HIP(hipFree(s->mesh_n));
+        s->mesh_v = s->mesh_n = nullptr;
+        s->mesh_cap = 0;
+        const long cap = (long)total + total / 4 + 64;
+        RTPS_TRY_HIP(hipMalloc(&s->mesh_v, (size_t)cap * 9 * sizeof(float)));
+        RTPS_TRY_HIP(hipMalloc(&s->mesh_n, (size_t)cap * 9 * sizeof(float)));
+        s->mesh_cap = cap;
+    }
+    s->mesh_ntri = (int)total;
+    if (total > 0)
+        RTPS_TRY_HIP(launch_mc_emit(res, iso, s->cf_field, s->mc_cases, s->mc_counts, s->mc_offs, s->mc_tiles,
+                                    s->mc_tri, s->gp.bnd_min, s->gp.bnd_max, s->mesh_cap, s->mesh_v, s->mesh_n,
+                                    s->stream));
+    return RTPS_OK;
+}
+
+int copy_mesh(rtps_sph* s, int capacity, float* vertices, float* normals, int* ntris) {
+    if (ntris) *ntris = s->mesh_ntri;
+    const int n = std::min(capacity, s->mesh_ntri);
+    if (n > 0 && vertices)
+        RTPS_TRY_HIP(hipMemcpyAsync(vertices, s->mesh_v, (size_t)n * 9 * sizeof(float), hipMemcpyDeviceToHost,
+                                    s->stream));
+    if (n > 0 && normals)
+        RTPS_TRY_HIP(hipMemcpyAsync(normals, s->mesh_n, (size_t)n * 9 * sizeof(float), hipMemcpyDeviceToHost,
+                                    s->stream));
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    return RTPS_OK;
+}
+
+}  // namespace
+
+int rtps_mc_case_table(signed char* tri, unsigned char* ntri) {
+    McTable t;
+    if (!build_mc_table(t)) return set_error(RTPS_ERR_ARG, "marching-cubes case table: a case needs too many triangles");
+    if (tri) std::memcpy(tri, t.tri, sizeof(t.tri));
+    if (ntri) std::memcpy(ntri, t.ntri, sizeof(t.ntri));
+    return RTPS_OK;
+}
+
+int rtps_sph_color_field(rtps_sph* s, int res, float* field) {
+    if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_color_field: null handle");
+    int rc = color_field(s, res);
+    if (rc) return rc;
+    if (field)
+        RTPS_TRY_HIP(hipMemcpyAsync(field, s->cf_field, (size_t)res * res * res * sizeof(float), hipMemcpyDeviceToHost,
+                                    s->stream));
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    return RTPS_OK;
+}
+
+int rtps_sph_surface(rtps_sph* s, int res, float iso, int capacity, float* vertices, float* normals, int* ntris) {
+    if (!s || capacity < 0) return set_error(RTPS_ERR_ARG, "rtps_sph_surface: bad arguments");
+    int rc = surface(s, res, iso);
+    if (rc) return rc;
+    return copy_mesh(s, capacity, vertices, normals, ntris);
+}
+
+int rtps_sph_post_process(rtps_sph* s) {
+    if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_post_process: null handle");
+    // SPH::postProcess (SPH.cpp:338-386): only with "use_color_field" and particles
+    const bool use = s->settings.GetSettingAs<bool>("use_color_field", "0");
+    if (!use || s->num_host == 0 || !s->sorted_ready) {
+        s->mesh_ntri = 0;
+        return RTPS_OK;
+    }
+    const int res = std::max(2, (int)s->settings.GetSettingAs<unsigned>("color_field_res", "2"));
+    const float iso = s->settings.GetSettingAs<float>("color_field_iso", "0.0001");   // MarchingCubes.cpp isolevel
+    return surface(s, res, iso);
+}
+
+int rtps_sph_mesh(rtps_sph* s, int capacity, float* vertices, float* normals, int* ntris) {
+    if (!s || capacity < 0) return set_error(RTPS_ERR_ARG, "rtps_sph_mesh: bad arguments");
+    return copy_mesh(s, capacity, vertices, normals, ntris);
 }
 
 int rtps_host_alloc(size_t bytes, void** host) {

@@ -159,7 +159,40 @@ class SPH:
         N.check(N.lib().rtps_sph_integrate(self._h))
 
     def postProcess(self) -> None:
-        """Colour field / surface extraction is out of scope; nothing to do."""
+        """SPH::postProcess (SPH.cpp:338-386): with setting use_color_field = 1,
+        the colour-field iso-surface (color_field_res, color_field_iso),
+        returned by getMCMesh()."""
+        N.check(N.lib().rtps_sph_post_process(self._h))
+
+    def getMCMesh(self):
+        """The mesh of the last postProcess(): (vertices, normals), each
+        (ntris, 3, 3) float32, or None when there is none."""
+        n = C.c_int()
+        N.check(N.lib().rtps_sph_mesh(self._h, 0, None, None, C.byref(n)))
+        if n.value == 0:
+            return None
+        v = np.empty((n.value, 3, 3), np.float32)
+        nn = np.empty((n.value, 3, 3), np.float32)
+        N.check(N.lib().rtps_sph_mesh(self._h, n.value, v.ctypes.data, nn.ctypes.data, C.byref(n)))
+        return v, nn
+
+    def colorField(self, res: int) -> np.ndarray:
+        """ColorField::execute on the last step's sorted state: (res, res, res)
+        float32 indexed [z, y, x] over the boundary box."""
+        out = np.empty((res, res, res), np.float32)
+        N.check(N.lib().rtps_sph_color_field(self._h, int(res), out.ctypes.data))
+        return out
+
+    def surface(self, res: int, iso: float = 0.0001):
+        """Iso-surface of colorField(res) at iso: (vertices, normals), each
+        (ntris, 3, 3) float32, vertices in world coordinates."""
+        n = C.c_int()
+        N.check(N.lib().rtps_sph_surface(self._h, int(res), float(iso), 0, None, None, C.byref(n)))
+        v = np.empty((n.value, 3, 3), np.float32)
+        nn = np.empty((n.value, 3, 3), np.float32)
+        if n.value:
+            N.check(N.lib().rtps_sph_mesh(self._h, n.value, v.ctypes.data, nn.ctypes.data, C.byref(n)))
+        return v, nn
 
     def step(self, nsteps: int = 1) -> None:
         """nsteps x (update + integrate), fused kernels, one hipGraph replay per step."""

@@ -38,6 +38,13 @@ inline void check(int rc) {
     if (rc < 0) throw Error(rc, rtps_last_error());
 }
 
+// Iso-surface of the colour field (rtpslib's Mesh, render/util/Mesh.h): three
+// vertices of 3 floats per triangle, with per-vertex normals.
+struct Mesh {
+    std::vector<float> vertices, normals;
+    int ntris = 0;
+};
+
 class System {
 public:
     virtual ~System() {}
@@ -75,6 +82,23 @@ public:
         check(rtps_sph_update(h_));
     }
     void integrate() override { check(rtps_sph_integrate(h_)); }
+    // SPH::postProcess (SPH.cpp:338-386): the colour-field surface when the
+    // settings ask for it ("use_color_field", "color_field_res").
+    void postProcess() override {
+        flushSettings();
+        check(rtps_sph_post_process(h_));
+    }
+    // SPH::getMCMesh: the last postProcess() surface, or nullptr.
+    Mesh* getMCMesh() {
+        int n = 0;
+        check(rtps_sph_mesh(h_, 0, nullptr, nullptr, &n));
+        if (n == 0) return nullptr;
+        mesh_.ntris = n;
+        mesh_.vertices.resize((size_t)n * 9);
+        mesh_.normals.resize((size_t)n * 9);
+        check(rtps_sph_mesh(h_, n, mesh_.vertices.data(), mesh_.normals.data(), &n));
+        return &mesh_;
+    }
     // n x (update + integrate) replayed from one hipGraph (no per-stage host calls).
     void step(int n) {
         flushSettings();
@@ -167,6 +191,7 @@ public:
 private:
     std::unique_ptr<RTPSSettings> settings_;
     rtps_sph* h_ = nullptr;
+    Mesh mesh_;
     std::map<std::string, std::string> pushed_;
 
     // Settings written through getSettings() since the last step reach the

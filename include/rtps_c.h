@@ -328,6 +328,39 @@ int rtps_sph_refill_hose(rtps_sph* s, int index, int refill);
 int rtps_sph_test_delete(rtps_sph* s);
 
 /* ------------------------------------------------------------------------ */
+/* Colour field and iso-surface: SPH::postProcess (SPH.cpp:338-386) with
+ * ColorField (colorfield.cl, ColorField.cpp) and MarchingCubes
+ * (MarchingCubes.cpp, classifycubes.cl, constructhp.cl, traversehp.cl), on
+ * the sorted state of the last step or update.
+ *
+ * rtps_sph_color_field: the res^3 lattice over the scaled boundary box, node
+ * (s, t, r) = mass * sum_j Wspiky(max(|x - x_j|, eps), h) / rho_j over the
+ * neighbours within h; copied to `field` (may be NULL) x fastest, then y, z.
+ *
+ * rtps_sph_surface: the iso-surface of that field at `iso`, *ntris
+ * triangles, the first `capacity` of them copied as 9 floats each: vertices
+ * in world coordinates (the boundary box; the reference scales to [0, 10]^3)
+ * and unit normals pointing away from the fluid.  Triangles come in cube
+ * order (x fastest); the case table is generated (rtps_mc_case_table) and
+ * closes the surface on every cube face.
+ *
+ * rtps_sph_post_process: SPH::postProcess: with setting "use_color_field"
+ * = 1, the surface for "color_field_res" (default 2) and "color_field_iso"
+ * (default 0.0001, the reference's isolevel), kept in the handle;
+ * rtps_sph_mesh copies it (SPH::getMCMesh). */
+int rtps_sph_color_field(rtps_sph* s, int res, float* field);
+int rtps_sph_surface(rtps_sph* s, int res, float iso, int capacity, float* vertices, float* normals, int* ntris);
+int rtps_sph_post_process(rtps_sph* s);
+int rtps_sph_mesh(rtps_sph* s, int capacity, float* vertices, float* normals, int* ntris);
+/* The generated marching-cubes case table: tri[256 * 16] edge ids (-1
+ * padded, three per triangle), ntri[256] triangles per case.  Corner k of a
+ * cube is at (k & 1, k >> 1 & 1, k >> 2 & 1); edges 0-3 join corners
+ * (0,1) (2,3) (4,5) (6,7), edges 4-7 (0,2) (1,3) (4,6) (5,7), edges 8-11
+ * (0,4) (1,5) (2,6) (3,7); bit k of a case is set when corner k exceeds the
+ * iso level.  Host only. */
+int rtps_mc_case_table(signed char* tri, unsigned char* ntri);
+
+/* ------------------------------------------------------------------------ */
 /* a14. Batched scenes (no reference counterpart): num_scenes independent SPH
  * systems (<= 256), each with its own settings, stepped together with one
  * launch per stage (segmented keys (scene << 24) | cell, persistent
