@@ -89,6 +89,8 @@ SIGNATURES = {
     "rtps_sph_write": (_i, [P, _i, P, _i]),
     "rtps_sph_device_ptr": (_i, [P, _i, P]),
     "rtps_sph_read_async": (_i, [P, _i, P, _i]),
+    "rtps_sph_load_triangles": (_i, [P, P, _i]),
+    "rtps_collision_triangles": (_i, [_i, P, P, P, P, _i, P, P]),
     "rtps_sph_color_field": (_i, [P, _i, P]),
     "rtps_sph_surface": (_i, [P, _i, _f, _i, P, P, P]),
     "rtps_sph_post_process": (_i, [P]),

@@ -7,6 +7,7 @@
 
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "rtps_internal.h"
 #include "sph_device.h"
@@ -159,6 +160,25 @@ int rtps_collision_wall(int num, const rtps_float4* pos_s, const rtps_float4* ve
     GravitySources none{};
     RTPS_HIP(launch_post_force(num, nullptr, nullptr, 0, (const float4*)pos_s, (const float4*)vel_s, (float4*)force,
                                *sphp, none, 1, *gps, S(stream)));
+    return RTPS_OK;
+}
+
+int rtps_collision_triangles(int num, const rtps_float4* pos_s, const rtps_float4* vel_s, rtps_float4* force,
+                             const rtps_triangle* tris, int n, const rtps_sph_params* sphp, void* stream) {
+    RTPS_CHECK(num >= 0 && n >= 0 && sphp && (tris || n == 0), "rtps_collision_triangles: bad arguments");
+    if (num == 0 || n == 0) return RTPS_OK;
+    RTPS_CHECK(pos_s && vel_s && force, "rtps_collision_triangles: null buffer");
+    std::vector<TriPrep> prep = prepare_triangles(tris, n, sphp->simulation_scale);
+    set_triangle_reach(prep, sphp->boundary_distance);
+    TriPrep* dev = nullptr;
+    RTPS_HIP(hipMalloc(&dev, prep.size() * sizeof(TriPrep)));
+    hipError_t e = hipMemcpyAsync(dev, prep.data(), prep.size() * sizeof(TriPrep), hipMemcpyHostToDevice, S(stream));
+    if (e == hipSuccess)
+        e = launch_collision_tri(num, nullptr, nullptr, 0, (const float4*)pos_s, (const float4*)vel_s, (float4*)force,
+                                 dev, n, *sphp, S(stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(S(stream));   // the triangle buffer is freed below
+    (void)hipFree(dev);
+    RTPS_HIP(e);
     return RTPS_OK;
 }
 

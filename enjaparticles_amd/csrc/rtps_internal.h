@@ -71,6 +71,21 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st);
 // Default particle colour of System::pushParticles (System.h:160).
 constexpr rtps_float4 kDefaultColor{1.0f, 0.0f, 0.0f, 1.0f};
 
+// ---- triangle collision (collision_tri.hip) ----
+// One triangle in the simulation scale: vertices a, b, c (q* = c), the
+// unnormalised normal n = (b - a) x (c - a), the unit normal u, and the
+// bounding sphere (centre c*, radius) with reach2 = (radius + dist)^2 widened.
+struct TriPrep {
+    float ax, ay, az, bx, by, bz, qx, qy, qz;
+    float nx, ny, nz, ux, uy, uz;
+    float cx, cy, cz, radius, reach2;
+};
+std::vector<TriPrep> prepare_triangles(const rtps_triangle* tris, int n, float scale);
+void set_triangle_reach(std::vector<TriPrep>& t, float dist);
+hipError_t launch_collision_tri(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
+                                const float4* vel_s, float4* force, const TriPrep* tris, int ntri,
+                                const rtps_sph_params& sp, hipStream_t st);
+
 // ---- colour field and iso-surface (surface.hip) ----
 size_t surface_scan_tiles(long ncubes);
 hipError_t launch_color_field(int res, const float4* pos_s, const float4* ve_s, const uint32_t* start,

@@ -82,6 +82,10 @@ struct rtps_sph {
     long mesh_cap = 0;
     int mesh_ntri = 0;
     std::vector<rtps_hose*> hoses;   // SPH::hoses (SPH.h), sprayed at the start of every update
+    std::vector<rtps_triangle> triangles;   // SPH::loadTriangles, world units
+    TriPrep* tri_dev = nullptr;
+    size_t tri_cap = 0;
+    int ntri = 0;
     std::vector<rtps_float4> hose_color;
 
     ~rtps_sph() {
@@ -94,7 +98,8 @@ struct rtps_sph {
                         keys,    idx,      start,     end,     d_num,   sort_tmp, rm.rows, rm.words, rm.codes, rm.count};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
-        void* sbufs[] = {cf_field, mc_cases, mc_counts, mc_offs, mc_tiles, mc_total, mc_tri, mc_ntri, mesh_v, mesh_n};
+        void* sbufs[] = {cf_field, mc_cases, mc_counts, mc_offs, mc_tiles, mc_total, mc_tri, mc_ntri, mesh_v, mesh_n,
+                         tri_dev};
         for (void* b : sbufs)
             if (b) (void)hipFree(b);
         for (auto& set : ev)
@@ -151,11 +156,35 @@ void load_gravity_sources(rtps_sph* s) {
     s->gsrc.n = any ? n : 0;
 }
 
+// Triangles in the current simulation scale and boundary distance, on the device.
+hipError_t upload_triangles(rtps_sph* s) {
+    s->ntri = 0;
+    if (s->triangles.empty()) return hipSuccess;
+    std::vector<TriPrep> prep = prepare_triangles(s->triangles.data(), (int)s->triangles.size(),
+                                                  s->sp.simulation_scale);
+    set_triangle_reach(prep, s->sp.boundary_distance);
+    if (prep.size() > s->tri_cap) {
+        hipError_t e = hipStreamSynchronize(s->stream);
+        if (e == hipSuccess && s->tri_dev) e = hipFree(s->tri_dev);
+        s->tri_dev = nullptr;
+        s->tri_cap = 0;
+        if (e == hipSuccess) e = hipMalloc(&s->tri_dev, prep.size() * sizeof(TriPrep));
+        if (e != hipSuccess) return e;
+        s->tri_cap = prep.size();
+    }
+    hipError_t e = hipMemcpyAsync(s->tri_dev, prep.data(), prep.size() * sizeof(TriPrep), hipMemcpyHostToDevice,
+                                  s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) s->ntri = (int)prep.size();
+    return e;
+}
+
 void apply_settings(rtps_sph* s) {
     read_sph_params(s->settings, s->sp);
     s->sp.num = s->num_host;
     s->dt = s->settings.GetSettingAs<float>("time_step");
     load_gravity_sources(s);
+    if (!s->triangles.empty()) (void)upload_triangles(s);   // a failure leaves ntri = 0: reported by the next load
     s->settings.updated();
     s->graph_valid = false;
     s->timed_graph_valid = false;
@@ -202,17 +231,38 @@ int launch_front(rtps_sph* s, StageTimer& t) {
     return RTPS_OK;
 }
 
-// One fused step: front + force/gravity/collision/leapfrog + count commit.
+// Force through integration + count commit.  One fused kernel, or, while
+// triangles take part in the collisions, force; gravity and walls; triangles
+// (SPH::collision, SPH.cpp:303-336); leapfrog as separate kernels.
+int launch_back(rtps_sph* s) {
+    const bool fo = s->opt_force_out != 0;
+    if (s->ntri > 0 && s->opt_collision) {
+        RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp,
+                                  s->gs, win(s, true), s->force_s, s->xsph_s, s->stream, masks_of(s)));
+        RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
+                                       s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
+        RTPS_TRY_HIP(launch_collision_tri(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
+                                          s->tri_dev, s->ntri, s->sp, s->stream));
+        RTPS_TRY_HIP(launch_leapfrog(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
+                                     s->xsph_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u,
+                                     s->stream, col_s(s), col_u(s)));
+    } else {
+        RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs,
+                                        win(s, true), fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
+                                        s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u,
+                                        s->veleval_u, s->stream, masks_of(s), col_s(s), col_u(s)));
+    }
+    RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
+    return RTPS_OK;
+}
+
+// One step: front + launch_back.
 int launch_step(rtps_sph* s, StageTimer& t) {
     int rc = launch_front(s, t);
     if (rc) return rc;
     t.mark(ST_FORCE);
-    const bool fo = s->opt_force_out != 0;
-    RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, win(s, true),
-                                    fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s, s->idx_sorted,
-                                    s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u, s->veleval_u,
-                                    s->stream, masks_of(s), col_s(s), col_u(s)));
-    RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
+    rc = launch_back(s);
+    if (rc) return rc;
     t.mark(ST_N);
     return RTPS_OK;
 }
@@ -500,12 +550,7 @@ int autotune_step(rtps_sph* s, bool measure) {
     // finish the step with the chosen configuration
     RTPS_TRY_HIP(launch_density(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, win(s), nullptr,
                                 s->ve_s, s->stream, masks_of(s)));
-    RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs,
-                                    win(s, true), fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
-                                    s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u,
-                                    s->veleval_u, s->stream, masks_of(s), col_s(s), col_u(s)));
-    RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
-    return RTPS_OK;
+    return launch_back(s);
 }
 
 }  // namespace
@@ -703,6 +748,16 @@ int rtps_sph_refill_hose(rtps_sph* s, int index, int refill) {
     return rtps_hose_refill(h, refill);   // SPH::refillHose (SPH.cpp:542-545)
 }
 
+int rtps_sph_load_triangles(rtps_sph* s, const rtps_triangle* tris, int n) {
+    if (!s || n < 0 || (n > 0 && !tris)) return set_error(RTPS_ERR_ARG, "rtps_sph_load_triangles: bad arguments");
+    s->triangles.assign(tris, tris + n);
+    maybe_update_params(s);
+    RTPS_TRY_HIP(upload_triangles(s));
+    s->graph_valid = false;
+    s->timed_graph_valid = false;
+    return RTPS_OK;
+}
+
 int rtps_sph_test_delete(rtps_sph* s) {
     if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_test_delete: null handle");
     // System::testDelete (System.cpp:366-376): 40 particles moved outside the grid
@@ -728,6 +783,9 @@ int rtps_sph_update(rtps_sph* s) {
                               win(s, true), s->force_s, s->xsph_s, s->stream, masks_of(s)));
     RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                    s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
+    if (s->ntri > 0 && s->opt_collision)
+        RTPS_TRY_HIP(launch_collision_tri(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s,
+                                          s->force_s, s->tri_dev, s->ntri, s->sp, s->stream));
     s->update_pending = true;
     return RTPS_OK;
 }

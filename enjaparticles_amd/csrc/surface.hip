@@ -69,17 +69,28 @@ __global__ __launch_bounds__(kBlockSurf) void k_color_field(int res, const float
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t a, b;
             row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
-            for (uint32_t j = a; j < b; ++j) {
-                float4 pj = pos_s[j];
+            // colorfield.cl:63-75 per candidate, kGroup candidates per
+            // iteration with their position loads issued together
+            auto term = [&](float4 pj, uint32_t j) {
                 if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
                 const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
                 const float r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 <= h2) {   // colorfield.cl:63-75
+                if (r2 <= h2) {
                     const float rlen = fmaxf(__builtin_sqrtf(r2), eps);
                     const float hr = h - rlen;
                     sum += alpha * hr * hr * hr / ve_s[j].w;
                 }
+            };
+            uint32_t j = a;
+            for (; j + kGroup <= b; j += kGroup) {
+                const float4* __restrict__ pg = pos_s + j;
+                float4 q[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) q[k] = pg[k];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) term(q[k], j + k);
             }
+            for (; j < b; ++j) term(pos_s[j], j);
         }
     field[idx] = sum * sp.mass;
 }

@@ -130,6 +130,18 @@ def collision_wall(pos_s, vel_s, force_s, sphp, gps):
     return f
 
 
+def collision_triangles(pos_s, vel_s, force_s, triangles, sphp):
+    """CollisionTriangle::execute: force_s plus the triangle collision forces
+    (triangles: (n, 3, 3) world vertex coordinates, host)."""
+    from .rtps import triangles_array
+    num = pos_s.shape[0]
+    f = force_s.clone()
+    t = triangles_array(triangles)
+    N.check(N.lib().rtps_collision_triangles(num, _ptr(pos_s), _ptr(vel_s), _ptr(f),
+                                             t.ctypes.data if len(t) else None, len(t), C.byref(sphp), _stream()))
+    return f
+
+
 def leapfrog(pos_s, vel_s, force_s, xsph_s, idx, sphp, dt, pos_u, vel_u, veleval_u):
     """Returns updated clones of (pos_u, vel_u, veleval_u)."""
     num = pos_s.shape[0]

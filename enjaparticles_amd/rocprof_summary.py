@@ -22,6 +22,11 @@ import sqlite3
 from collections import defaultdict
 
 
+def short_name(name: str) -> str:
+    """Kernel name without its argument list (anonymous namespaces dropped)."""
+    return name.replace("(anonymous namespace)::", "").split("(")[0]
+
+
 def summarize(db_path: str, top: int = 40, match: str | None = None):
     db = sqlite3.connect(db_path)
     cur = db.cursor()
@@ -33,7 +38,7 @@ def summarize(db_path: str, top: int = 40, match: str | None = None):
     total = sum(r[7] for r in rows) or 1
     out = []
     for name, grid, wg, n, avg, mn, mx, tot in rows:
-        short = name.split("(")[0]
+        short = short_name(name)
         if match and not re.search(match, short):
             continue
         out.append((short, grid, wg, n, avg / 1e3, mn / 1e3, mx / 1e3, 100.0 * tot / total))
@@ -44,7 +49,7 @@ def counters(csv_path: str, match: str | None = None):
     acc = defaultdict(lambda: [0.0, 0])
     with open(csv_path, newline="") as f:
         for row in csv.DictReader(f):
-            name = row["Kernel_Name"].split("(")[0]
+            name = short_name(row["Kernel_Name"])
             if match and not re.search(match, name):
                 continue
             key = (name, int(row["Grid_Size"]), int(row["Workgroup_Size"]), row["Counter_Name"])

@@ -144,6 +144,13 @@ class SPH:
         """SPH::refillHose (SPH.cpp:542-545)."""
         N.check(N.lib().rtps_sph_refill_hose(self._h, int(index), int(refill)))
 
+    def loadTriangles(self, triangles) -> None:
+        """SPH::loadTriangles: triangles (n, 3, 3) or (n, 3, 4) of vertex
+        coordinates in world units; particles collide with them after the
+        walls in every update (collision_tri.cl).  An empty array removes them."""
+        t = triangles_array(triangles)
+        N.check(N.lib().rtps_sph_load_triangles(self._h, t.ctypes.data if len(t) else None, len(t)))
+
     def testDelete(self) -> None:
         """System::testDelete (System.cpp:366-376): moves the first 40 particles
         to (100, 100, 100), outside the grid; the next update deletes them."""
@@ -305,6 +312,18 @@ class SPH:
 
 
 DEFAULT_COLOR = (1.0, 0.0, 0.0, 1.0)   # System.h:160
+
+
+def triangles_array(triangles) -> np.ndarray:
+    """(n, 4, 4) float32 rtps_triangle records (three vertices, normal slot 0)
+    from (n, 3, 3) or (n, 3, 4) vertex coordinates."""
+    v = np.asarray(triangles, np.float32)
+    if v.size == 0:
+        return np.zeros((0, 4, 4), np.float32)
+    assert v.ndim == 3 and v.shape[1] == 3 and v.shape[2] in (3, 4)
+    out = np.zeros((len(v), 4, 4), np.float32)
+    out[:, :3, :3] = v[:, :, :3]
+    return out
 
 
 class HostBuffer:

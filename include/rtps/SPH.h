@@ -28,6 +28,7 @@
 namespace rtps {
 
 typedef rtps_float4 float4;
+typedef rtps_triangle Triangle;
 
 struct Error : std::runtime_error {
     int code;
@@ -147,6 +148,10 @@ public:
         check(rtps_sph_update_hose(h_, index, center, velocity, radius, color));
     }
     void refillHose(int index, int refill) { check(rtps_sph_refill_hose(h_, index, refill)); }
+    // SPH::loadTriangles: collision triangles (world units), used with walls.
+    void loadTriangles(std::vector<Triangle>& triangles) {
+        check(rtps_sph_load_triangles(h_, triangles.data(), (int)triangles.size()));
+    }
     // System::testDelete (System.cpp:366-376)
     void testDelete() { check(rtps_sph_test_delete(h_)); }
 

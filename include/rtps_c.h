@@ -132,6 +132,22 @@ int rtps_gravity(int num, int nsrc, const rtps_float4* sources /*host*/, const f
 int rtps_collision_wall(int num, const rtps_float4* pos_s, const rtps_float4* vel_s, rtps_float4* force,
                         const rtps_sph_params* sphp, const rtps_grid_params* gps, void* stream);
 
+/* Triangle record of CollisionTriangle (collision_tri.cl:63-67): three
+ * vertices (w unused) and a normal slot (recomputed from the vertices). */
+typedef struct {
+    rtps_float4 verts[3];
+    rtps_float4 normal;
+} rtps_triangle;
+
+/* a10'. CollisionTriangle::execute (Collision_triangle.cpp,
+ * collision_tri.cl:120-231, 418-470): for every particle and triangle, the
+ * segment pos +- boundary_distance * v/|v| against the triangle (Ericson);
+ * a hit closer than boundary_distance adds the penalty and friction of
+ * cl_sph_collision.h along the triangle's unit normal to force.  tris is a
+ * host array of n triangles in world units. */
+int rtps_collision_triangles(int num, const rtps_float4* pos_s, const rtps_float4* vel_s, rtps_float4* force,
+                             const rtps_triangle* tris /*host*/, int n, const rtps_sph_params* sphp, void* stream);
+
 /* a11. LeapFrog::execute (rtpslib/system/sph/LeapFrog.h:44-63, leapfrog.cl:30-121):
  * integrates sorted particle i and scatters it to pos_u/vel_u/veleval_u[idx[i]]. */
 int rtps_leapfrog(int num, const rtps_float4* pos_s, const rtps_float4* vel_s, const rtps_float4* force,
@@ -323,6 +339,13 @@ int rtps_sph_add_hose(rtps_sph* s, int total_n, rtps_float4 center, rtps_float4 
 int rtps_sph_update_hose(rtps_sph* s, int index, rtps_float4 center, rtps_float4 velocity, float radius,
                          rtps_float4 color);
 int rtps_sph_refill_hose(rtps_sph* s, int index, int refill);
+/* SPH::loadTriangles (SPH.cpp, Collision_triangle.cpp:41-53): triangles
+ * (world units) the particles collide with after the walls in every update
+ * (SPH::collision, with RTPS_OPT_COLLISION on); n = 0 removes them.  While
+ * triangles are loaded a step runs force, gravity and walls, triangles and
+ * the integration as separate kernels. */
+int rtps_sph_load_triangles(rtps_sph* s, const rtps_triangle* tris, int n);
+
 /* System::testDelete (System.cpp:366-376): moves the first 40 particles to
  * (100, 100, 100), outside the grid, so the next update deletes them. */
 int rtps_sph_test_delete(rtps_sph* s);
