@@ -198,6 +198,45 @@ def batched_adjoint_leg(steps: int = 100, num_scenes: int = 64, checkpoint_every
             "adjoint_over_plain": tb / tp, "finite_gradients": bool(np.all(np.isfinite(dmu + dK)))}
 
 
+def triangles_leg(s, ntri: int, steps: int = 50):
+    """Steps of the current state with ntri collision triangles loaded (a
+    wavy sheet over the whole box at a fifth of its height, inside the fluid,
+    facing down) against the same number of steps without: the split
+    collision path and its triangle kernel."""
+    import numpy as np
+    _, gp, _ = s.params()
+    lo = np.array([gp.bnd_min.x, gp.bnd_min.y, gp.bnd_min.z])
+    hi = np.array([gp.bnd_max.x, gp.bnd_max.y, gp.bnd_max.z])
+    side = max(1, int(round((ntri / 2) ** 0.5)))
+    xs = np.linspace(lo[0], hi[0], side + 1)
+    ys = np.linspace(lo[1], hi[1], side + 1)
+    zz = lo[2] + 0.2 * (hi[2] - lo[2]) + 0.01 * (hi[2] - lo[2]) * np.sin(np.add.outer(xs, ys) * 7.0)
+    tris = []
+    for i in range(side):
+        for j in range(side):
+            a = (xs[i], ys[j], zz[i, j])
+            b = (xs[i + 1], ys[j], zz[i + 1, j])
+            c = (xs[i + 1], ys[j + 1], zz[i + 1, j + 1])
+            d = (xs[i], ys[j + 1], zz[i, j + 1])
+            tris += [(a, c, b), (a, d, c)]          # wound to face down
+    tris = np.array(tris, np.float32)
+
+    def timed():
+        s.step(2)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    base = timed()
+    s.loadTriangles(tris)
+    with_tri = timed()
+    s.loadTriangles(np.zeros((0, 3, 3), np.float32))
+    return {"triangles": len(tris), "steps": steps, "ms_per_step_without": base, "ms_per_step_with": with_tri,
+            "extra_ms": with_tri - base}
+
+
 def surface_leg(s, res: int, iso: float = 0.5, reps: int = 10):
     """Colour field and iso-surface of the current state (SPH::postProcess):
     wall time per call, each ending in a stream synchronisation."""
@@ -275,6 +314,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--profile", action="store_true", help="timed loop only (for rocprofv3 runs)")
     ap.add_argument("--force-variant", type=int, default=0, help="1 = compacted neighbour lists, 0 = direct")
+    ap.add_argument("--triangles", type=int, default=2048,
+                    help="collision triangles of the triangle leg on the final state (0 = skip)")
     ap.add_argument("--surface-res", type=int, default=128,
                     help="colour-field lattice of the surface leg on the final state (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=30, help="steps of the batched leg (0 = skip)")
@@ -418,6 +459,12 @@ def main():
         out["surface"] = surface_leg(s, args.surface_res)
         print(f"# surface: field {out['surface']['field_ms']:.3f} ms, +surface "
               f"{out['surface']['field_plus_surface_ms']:.3f} ms, {out['surface']['triangles']} triangles",
+              file=sys.stderr, flush=True)
+    if args.triangles > 0 and not args.profile:
+        print("# triangle leg ...", file=sys.stderr, flush=True)
+        out["triangle_collision"] = triangles_leg(s, args.triangles)
+        print(f"# triangles: {out['triangle_collision']['ms_per_step_with']:.3f} ms/step with "
+              f"{out['triangle_collision']['triangles']} vs {out['triangle_collision']['ms_per_step_without']:.3f}",
               file=sys.stderr, flush=True)
     s.close()
     if args.batched_steps > 0:

@@ -170,12 +170,16 @@ int rtps_collision_triangles(int num, const rtps_float4* pos_s, const rtps_float
     RTPS_CHECK(pos_s && vel_s && force, "rtps_collision_triangles: null buffer");
     std::vector<TriPrep> prep = prepare_triangles(tris, n, sphp->simulation_scale);
     set_triangle_reach(prep, sphp->boundary_distance);
+    const std::vector<float4> reach = triangle_reach(prep);
     TriPrep* dev = nullptr;
-    RTPS_HIP(hipMalloc(&dev, prep.size() * sizeof(TriPrep)));
+    RTPS_HIP(hipMalloc(&dev, prep.size() * (sizeof(TriPrep) + sizeof(float4))));
+    float4* dev_reach = reinterpret_cast<float4*>(dev + prep.size());
     hipError_t e = hipMemcpyAsync(dev, prep.data(), prep.size() * sizeof(TriPrep), hipMemcpyHostToDevice, S(stream));
     if (e == hipSuccess)
+        e = hipMemcpyAsync(dev_reach, reach.data(), reach.size() * sizeof(float4), hipMemcpyHostToDevice, S(stream));
+    if (e == hipSuccess)
         e = launch_collision_tri(num, nullptr, nullptr, 0, (const float4*)pos_s, (const float4*)vel_s, (float4*)force,
-                                 dev, n, *sphp, S(stream));
+                                 dev, dev_reach, n, *sphp, S(stream));
     if (e == hipSuccess) e = hipStreamSynchronize(S(stream));   // the triangle buffer is freed below
     (void)hipFree(dev);
     RTPS_HIP(e);

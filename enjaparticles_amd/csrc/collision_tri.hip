@@ -8,10 +8,10 @@
 //
 // Per triangle the host prepares, in the simulation scale, its vertices
 // a, b, c (= v * scale), n = (b - a) x (c - a), the unit normal n / |n| and a
-// bounding sphere.  A particle
-// farther than radius + dist from the sphere's centre cannot get a hit
-// closer than dist (tp lies in the triangle), so it skips the test; that
-// changes no result.  Blocks stage the triangles through LDS in chunks.
+// bounding sphere.  A particle farther than radius + dist from the sphere's
+// centre cannot get a hit closer than dist (tp lies in the triangle), so it
+// skips the test; that changes no result.  A block keeps only the triangles
+// whose widened sphere meets its particles' bounding box, in LDS, in order.
 
 #include <hip/hip_runtime.h>
 
@@ -26,17 +26,36 @@ namespace rtps {
 namespace {
 
 constexpr int kBlockTri = 256;
-constexpr int kTriChunk = 256;   // triangles per LDS chunk (20 KiB of TriPrep)
+constexpr int kWavesTri = kBlockTri / 64;
+constexpr int kTriChunk = 256;   // triangles tested per chunk (one per thread); 20 KiB of TriPrep in LDS
 
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Blocks of consecutive sorted particles are compact in space: per chunk of
+// triangles, each thread tests one triangle's reach sphere against the
+// block's bounding box, and the candidates are compacted into LDS in triangle
+// order (ballot ranks, then per-wave offsets), so every particle visits the
+// same triangles in the same order as without the cull.
 __global__ __launch_bounds__(kBlockTri) void k_collision_tri(int num_host, const int* __restrict__ d_num,
                                                              const uint32_t* __restrict__ start, int nb_cells,
                                                              const float4* __restrict__ pos_s,
                                                              const float4* __restrict__ vel_s,
                                                              float4* __restrict__ force,
-                                                             const TriPrep* __restrict__ tris, int ntri,
+                                                             const TriPrep* __restrict__ tris,
+                                                             const float4* __restrict__ reach, int ntri,
                                                              rtps_sph_params sp) {
     __shared__ TriPrep lt[kTriChunk];
+    __shared__ float box[kWavesTri][6];
+    __shared__ int wave_n[kWavesTri];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
     int num = d_num ? *d_num : num_host;
     if (d_num) effective_num(num, start, nb_cells, num);
     const bool live = i < num;
@@ -49,6 +68,29 @@ __global__ __launch_bounds__(kBlockTri) void k_collision_tri(int num_host, const
         v = vel_s[i];
     }
     const float px = p.x * s, py = p.y * s, pz = p.z * s;
+    // bounding box of the block's live particles
+    {
+        const float inf = __builtin_inff();
+        const float lo[3] = {live ? px : inf, live ? py : inf, live ? pz : inf};
+        const float hi[3] = {live ? px : -inf, live ? py : -inf, live ? pz : -inf};
+        for (int c = 0; c < 3; ++c) {
+            const float a = wave_min(lo[c]), b = wave_max(hi[c]);
+            if (lane == 0) {
+                box[wave][c] = a;
+                box[wave][3 + c] = b;
+            }
+        }
+    }
+    __syncthreads();
+    float bl[3], bh[3];
+    for (int c = 0; c < 3; ++c) {
+        bl[c] = box[0][c];
+        bh[c] = box[0][3 + c];
+        for (int w = 1; w < kWavesTri; ++w) {
+            bl[c] = fminf(bl[c], box[w][c]);
+            bh[c] = fmaxf(bh[c], box[w][3 + c]);
+        }
+    }
     // v3normalize (collision_tri.cl:46-50): NaN for v = 0, and then no hit
     const float mag = __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
     const float vnx = v.x / mag, vny = v.y / mag, vnz = v.z / mag;
@@ -58,13 +100,30 @@ __global__ __launch_bounds__(kBlockTri) void k_collision_tri(int num_host, const
     const float qpx = sx - qx, qpy = sy - qy, qpz = sz - qz;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     for (int c0 = 0; c0 < ntri; c0 += kTriChunk) {
-        const int nc = min(kTriChunk, ntri - c0);
+        const int t = c0 + (int)threadIdx.x;
+        bool cand = false;
+        if (t < ntri) {
+            const float4 r = reach[t];   // centre, reach squared
+            const float dx = fmaxf(fmaxf(bl[0] - r.x, r.x - bh[0]), 0.f);
+            const float dy = fmaxf(fmaxf(bl[1] - r.y, r.y - bh[1]), 0.f);
+            const float dz = fmaxf(fmaxf(bl[2] - r.z, r.z - bh[2]), 0.f);
+            cand = dx * dx + dy * dy + dz * dz <= r.w;
+        }
+        const unsigned long long m = __ballot(cand);
+        const int rank = __popcll(m & ((1ull << lane) - 1ull));
+        __syncthreads();   // the previous chunk's triangles are no longer read
+        if (lane == 0) wave_n[wave] = __popcll(m);
         __syncthreads();
-        for (int t = threadIdx.x; t < nc; t += blockDim.x) lt[t] = tris[c0 + t];
+        int base = 0, total = 0;
+        for (int w = 0; w < kWavesTri; ++w) {
+            if (w < wave) base += wave_n[w];
+            total += wave_n[w];
+        }
+        if (cand) lt[base + rank] = tris[t];
         __syncthreads();
         if (!live) continue;
-        for (int t = 0; t < nc; ++t) {
-            const TriPrep& T = lt[t];
+        for (int q = 0; q < total; ++q) {
+            const TriPrep& T = lt[q];
             const float cx = px - T.cx, cy = py - T.cy, cz = pz - T.cz;
             if (cx * cx + cy * cy + cz * cz > T.reach2) continue;   // no hit within dist
             // Ericson segment-triangle test (collision_tri.cl:120-170)
@@ -87,15 +146,15 @@ __global__ __launch_bounds__(kBlockTri) void k_collision_tri(int num_host, const
             const float tx = uu * T.ax + vv * T.bx + ww * T.qx;   // tp = u a + v b + w c
             const float ty = uu * T.ay + vv * T.by + ww * T.qy;
             const float tz = uu * T.az + vv * T.bz + ww * T.qz;
-            const float dx = px - tx, dy = py - ty, dz = pz - tz;
-            const float pen = dist - __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+            const float ddx = px - tx, ddy = py - ty, ddz = pz - tz;
+            const float pen = dist - __builtin_sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
             if (!(pen > 1e-6f)) continue;   // collision_tri.cl:196
             // calculateRepulsionForce + calculateFrictionForce (zero coefficients)
             const float ndv = T.ux * v.x + T.uy * v.y + T.uz * v.z;
-            const float m = k * pen - damp * ndv;
-            fx += m * T.ux;
-            fy += m * T.uy;
-            fz += m * T.uz;
+            const float m2 = k * pen - damp * ndv;
+            fx += m2 * T.ux;
+            fy += m2 * T.uy;
+            fz += m2 * T.uz;
             const float vtx = v.x - v.x * ndv, vty = v.y - v.y * ndv, vtz = v.z - v.z * ndv;
             if ((vtx + vty + vtz) / 3.0f > 0.0f) {
                 fx -= vtx;
@@ -155,13 +214,19 @@ void set_triangle_reach(std::vector<TriPrep>& t, float dist) {
     }
 }
 
+std::vector<float4> triangle_reach(const std::vector<TriPrep>& t) {
+    std::vector<float4> out(t.size());
+    for (size_t q = 0; q < t.size(); ++q) out[q] = make_float4(t[q].cx, t[q].cy, t[q].cz, t[q].reach2);
+    return out;
+}
+
 hipError_t launch_collision_tri(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
-                                const float4* vel_s, float4* force, const TriPrep* tris, int ntri,
+                                const float4* vel_s, float4* force, const TriPrep* tris, const float4* reach, int ntri,
                                 const rtps_sph_params& sp, hipStream_t st) {
     (void)hipGetLastError();
     if (num <= 0 || ntri <= 0) return hipSuccess;
     k_collision_tri<<<(num + kBlockTri - 1) / kBlockTri, kBlockTri, 0, st>>>(num, d_num, start, nb_cells, pos_s, vel_s,
-                                                                           force, tris, ntri, sp);
+                                                                           force, tris, reach, ntri, sp);
     return hipGetLastError();
 }
 

@@ -82,8 +82,10 @@ struct TriPrep {
 };
 std::vector<TriPrep> prepare_triangles(const rtps_triangle* tris, int n, float scale);
 void set_triangle_reach(std::vector<TriPrep>& t, float dist);
+// (centre, reach squared) per triangle, for the block cull
+std::vector<float4> triangle_reach(const std::vector<TriPrep>& t);
 hipError_t launch_collision_tri(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
-                                const float4* vel_s, float4* force, const TriPrep* tris, int ntri,
+                                const float4* vel_s, float4* force, const TriPrep* tris, const float4* reach, int ntri,
                                 const rtps_sph_params& sp, hipStream_t st);
 
 // ---- colour field and iso-surface (surface.hip) ----

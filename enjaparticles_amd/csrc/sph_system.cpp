@@ -83,7 +83,8 @@ struct rtps_sph {
     int mesh_ntri = 0;
     std::vector<rtps_hose*> hoses;   // SPH::hoses (SPH.h), sprayed at the start of every update
     std::vector<rtps_triangle> triangles;   // SPH::loadTriangles, world units
-    TriPrep* tri_dev = nullptr;
+    TriPrep* tri_dev = nullptr;   // tri_cap records, then tri_cap reach spheres (tri_reach)
+    float4* tri_reach = nullptr;
     size_t tri_cap = 0;
     int ntri = 0;
     std::vector<rtps_float4> hose_color;
@@ -163,17 +164,23 @@ hipError_t upload_triangles(rtps_sph* s) {
     std::vector<TriPrep> prep = prepare_triangles(s->triangles.data(), (int)s->triangles.size(),
                                                   s->sp.simulation_scale);
     set_triangle_reach(prep, s->sp.boundary_distance);
+    const std::vector<float4> reach = triangle_reach(prep);
     if (prep.size() > s->tri_cap) {
         hipError_t e = hipStreamSynchronize(s->stream);
         if (e == hipSuccess && s->tri_dev) e = hipFree(s->tri_dev);
         s->tri_dev = nullptr;
+        s->tri_reach = nullptr;
         s->tri_cap = 0;
-        if (e == hipSuccess) e = hipMalloc(&s->tri_dev, prep.size() * sizeof(TriPrep));
+        if (e == hipSuccess) e = hipMalloc(&s->tri_dev, prep.size() * (sizeof(TriPrep) + sizeof(float4)));
         if (e != hipSuccess) return e;
         s->tri_cap = prep.size();
+        s->tri_reach = reinterpret_cast<float4*>(s->tri_dev + s->tri_cap);
     }
     hipError_t e = hipMemcpyAsync(s->tri_dev, prep.data(), prep.size() * sizeof(TriPrep), hipMemcpyHostToDevice,
                                   s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->tri_reach, reach.data(), reach.size() * sizeof(float4), hipMemcpyHostToDevice,
+                           s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e == hipSuccess) s->ntri = (int)prep.size();
     return e;
@@ -242,7 +249,7 @@ int launch_back(rtps_sph* s) {
         RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                        s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
         RTPS_TRY_HIP(launch_collision_tri(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
-                                          s->tri_dev, s->ntri, s->sp, s->stream));
+                                          s->tri_dev, s->tri_reach, s->ntri, s->sp, s->stream));
         RTPS_TRY_HIP(launch_leapfrog(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                      s->xsph_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u,
                                      s->stream, col_s(s), col_u(s)));
@@ -785,7 +792,7 @@ int rtps_sph_update(rtps_sph* s) {
                                    s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
     if (s->ntri > 0 && s->opt_collision)
         RTPS_TRY_HIP(launch_collision_tri(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s,
-                                          s->force_s, s->tri_dev, s->ntri, s->sp, s->stream));
+                                          s->force_s, s->tri_dev, s->tri_reach, s->ntri, s->sp, s->stream));
     s->update_pending = true;
     return RTPS_OK;
 }
