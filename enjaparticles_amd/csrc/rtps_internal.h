@@ -34,8 +34,10 @@ extern int g_force_variant;   // 0 = direct branch per candidate, 1 = compacted 
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
                        uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, int sub_bits,
                        hipStream_t st);
+// sub (optional, sub_bits = 4 only): the sub-cell start table of the
+// windowed neighbour rows, 16 entries per cell
 hipError_t launch_cellindices(int num, const int* d_num, const uint32_t* keys, int nb_cells, uint32_t* start,
-                              uint32_t* end, int sub_bits, hipStream_t st);
+                              uint32_t* end, int sub_bits, hipStream_t st, uint32_t* sub = nullptr);
 hipError_t launch_read_nc(const uint32_t* start, int nb_cells, int* nc, hipStream_t st);
 // cu/cs (optional): particle colours, permuted like the positions
 hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const float4* pu, const float4* vu,

@@ -117,6 +117,9 @@ struct Win {
     int remap;     // tile order of the neighbour kernels (remap_tile)
     int block;     // threads per block of the neighbour launches (0 = 256)
     int variant;   // force loop (-1 = the process default)
+    // sub-cell starts (cellindices with a table, kSubCells entries per cell):
+    // the window bounds are looked up instead of searched; nullptr = search
+    const uint32_t* sub = nullptr;
 };
 
 // Neighbour rows recorded by the density kernel for the force kernel: per
@@ -180,8 +183,31 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__
 __device__ __forceinline__ void row_window(const WindowCtx& w, int3 c, int dy, int dz, float h2, const Grid& g,
                                            const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                            const uint32_t* __restrict__ keys, uint32_t& a, uint32_t& b,
-                                           uint32_t key_prefix = 0u) {
-    row_range(c.x, c.y + dy, c.z + dz, g, start, end, a, b);
+                                           uint32_t key_prefix = 0u, const uint32_t* __restrict__ sub = nullptr) {
+    const bool table = sub != nullptr && w.on;
+    // the row's three cells; with the table their starts are kept for the lookups
+    const int hb = grid_hash(c.x - 1, c.y + dy, c.z + dz, g);
+    uint32_t cs[3] = {kEmpty, kEmpty, kEmpty}, hc[3] = {0u, 0u, 0u};
+    if (table) {
+        a = kEmpty;
+        b = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t hk = (uint32_t)grid_hash(c.x - 1 + k, c.y + dy, c.z + dz, g);
+            hc[k] = hk;
+            if (hk < (uint32_t)g.nb_cells) {
+                const uint32_t sk = start[hk];
+                if (sk != kEmpty) {
+                    cs[k] = sk;
+                    a = min(a, sk);
+                    b = max(b, end[hk]);
+                }
+            }
+        }
+        if (a == kEmpty) { a = 0; b = 0; }
+    } else {
+        row_range(c.x, c.y + dy, c.z + dz, g, start, end, a, b);
+    }
     if (!w.on || a >= b) return;
     const float ey = dy < 0 ? (w.uy - (float)c.y) * w.wy : (dy > 0 ? ((float)(c.y + 1) - w.uy) * w.wy : 0.0f);
     const float ez = dz < 0 ? (w.uz - (float)c.z) * w.wz : (dz > 0 ? ((float)(c.z + 1) - w.uz) * w.wz : 0.0f);
@@ -194,7 +220,23 @@ __device__ __forceinline__ void row_window(const WindowCtx& w, int3 c, int dy, i
     const float rel = w.ux - (float)(c.x - 1);
     const int r_lo = max((int)floorf((rel - half) * (float)kSubCells) - 1, 0);
     const int r_hi = min((int)floorf((rel + half) * (float)kSubCells) + 1, 3 * kSubCells - 1);
-    const uint32_t base = key_prefix | ((uint32_t)grid_hash(c.x - 1, c.y + dy, c.z + dz, g) << kSubBits);
+    if (table) {
+        // first index with sub-key >= r: the table entry of its cell when that
+        // cell holds particles, else the start of the next occupied cell
+        const uint32_t row_end = b;
+        auto first_at = [&](int r) -> uint32_t {
+            const int co = r >> kSubBits;
+            if (co >= 3) return row_end;
+            if (cs[co] != kEmpty) return sub[(size_t)hc[co] * kSubCells + (r & (kSubCells - 1))];
+            for (int k = co + 1; k < 3; ++k)
+                if (cs[k] != kEmpty) return cs[k];
+            return row_end;
+        };
+        a = first_at(r_lo);
+        b = first_at(r_hi + 1);
+        return;
+    }
+    const uint32_t base = key_prefix | ((uint32_t)hb << kSubBits);
     const uint32_t na = lower_bound_u32(keys, a, b, base + (uint32_t)r_lo);
     b = lower_bound_u32(keys, na, b, base + (uint32_t)r_hi + 1u);
     a = na;

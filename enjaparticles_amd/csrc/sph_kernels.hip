@@ -62,12 +62,24 @@ __global__ __launch_bounds__(kBlock) void k_hash(int num_host, const int* __rest
 __global__ __launch_bounds__(kBlock) void k_cellindices(int num_host, const int* __restrict__ d_num,
                                                         const uint32_t* __restrict__ keys, uint32_t nb_cells,
                                                         uint32_t* __restrict__ start, uint32_t* __restrict__ end,
-                                                        int sub_bits) {
+                                                        int sub_bits, uint32_t* __restrict__ sub) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int num = d_num ? *d_num : num_host;
     if (i >= num) return;
     uint32_t h = keys[i] >> sub_bits;
     if (h > nb_cells) return;
+    if (sub && h < nb_cells) {
+        // sub-cell starts of an occupied cell: entry q = first index of the
+        // cell with in-cell x sixteenth >= q (the cell's end when none); each
+        // entry is written once, by the particle that starts it
+        const uint32_t xq = keys[i] & (kSubCells - 1);
+        uint32_t q0 = 0;
+        if (i > 0 && (keys[i - 1] >> sub_bits) == h) q0 = (keys[i - 1] & (kSubCells - 1)) + 1;
+        uint32_t* row = sub + (size_t)h * kSubCells;
+        for (uint32_t q = q0; q <= xq; ++q) row[q] = (uint32_t)i;
+        if (i == num - 1 || (keys[i + 1] >> sub_bits) != h)
+            for (uint32_t q = xq + 1; q < (uint32_t)kSubCells; ++q) row[q] = (uint32_t)i + 1;
+    }
     if (i == 0) {
         start[h] = 0;
     } else {
@@ -155,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy, ++r) {
             uint32_t a, b;
-            row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
+            row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
             // accepted-candidate bits of this row, flushed every 32 candidates
             uint32_t word = 0, q = 0;
             int wq = 0;
@@ -390,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
         for (int dz = -1; dz <= 1; ++dz) {
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
-                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
+                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
                 // groups of kGroup candidates: the position loads, then the
                 // veleval loads of the accepted ones, are issued together
                 uint32_t j = a;
@@ -469,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __res
         for (int dz = -1; dz <= 1; ++dz) {
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
-                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
+                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
                 for (uint32_t j = a; __ballot(j < b) != 0ull; j += kGroup) {
                     if (__ballot(cnt > kNbrCap - kGroup) != 0ull) drain();
                     float4 pq[kGroup];
@@ -581,7 +593,7 @@ __global__ __launch_bounds__(kBlock) void k_count_pairs(int num_host, const int*
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b, wa, wb;
                 row_range(c.x, c.y + dy, c.z + dz, g, start, end, a, b);
-                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, wa, wb);
+                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, wa, wb, 0u, win.sub);
                 k += b - a;
                 kw += wb - wa;
                 for (uint32_t j = a; j < b; ++j) {
@@ -647,10 +659,11 @@ hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, c
 }
 
 hipError_t launch_cellindices(int num, const int* d_num, const uint32_t* keys, int nb_cells, uint32_t* start,
-                              uint32_t* end, int sub_bits, hipStream_t st) {
+                              uint32_t* end, int sub_bits, hipStream_t st, uint32_t* sub) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
-    k_cellindices<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, keys, (uint32_t)nb_cells, start, end, sub_bits);
+    k_cellindices<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, keys, (uint32_t)nb_cells, start, end, sub_bits,
+                                                       sub_bits == kSubBits ? sub : nullptr);
     return hipGetLastError();
 }
 

@@ -47,6 +47,9 @@ struct rtps_sph {
     float4 *color_u = nullptr, *color_s = nullptr;   // cl_color_u / cl_color_s (System.cpp:236-237)
     uint32_t *keys = nullptr, *idx = nullptr, *start = nullptr, *end = nullptr;
     uint32_t *keys_sorted = nullptr, *idx_sorted = nullptr;
+    // sub-cell starts of the windowed rows (16 per cell; nullptr when the grid
+    // is too large for it: the kernels then search the sorted keys)
+    uint32_t* sub_start = nullptr;
     int* d_num = nullptr;
     void* sort_tmp = nullptr;
     // neighbour rows recorded by density for the force loops 2 (bit form) and 3 (list form)
@@ -95,8 +98,9 @@ struct rtps_sph {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& g : gexec_t)
             if (g) (void)hipGraphExecDestroy(g);
-        void* bufs[] = {pos_u,   vel_u,    veleval_u, pos_s,   vel_s,   ve_s,  force_s,  xsph_s,  color_u, color_s,
-                        keys,    idx,      start,     end,     d_num,   sort_tmp, rm.rows, rm.words, rm.codes, rm.count};
+        void* bufs[] = {pos_u, vel_u, veleval_u, pos_s,    vel_s,   ve_s,     force_s,  xsph_s,   color_u,  color_s,
+                        keys,  idx,   start,     end,      d_num,   sort_tmp, rm.rows,  rm.words, rm.codes, rm.count,
+                        sub_start};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         void* sbufs[] = {cf_field, mc_cases, mc_counts, mc_offs, mc_tiles, mc_total, mc_tri, mc_ntri, mesh_v, mesh_n,
@@ -119,14 +123,19 @@ const RowMasks* masks_of(const rtps_sph* s) {
     return v == 2 ? &s->rm : (v == 3 ? &s->rl : nullptr);
 }
 
+// Sub-cell start table of the windowed rows, when it is in use.
+uint32_t* sub_table(const rtps_sph* s) { return s->opt_window == 1 ? s->sub_start : nullptr; }
+
 // Colour buffers handed to the kernels (none unless the system keeps colours).
 float4* col_u(const rtps_sph* s) { return s->opt_color ? s->color_u : nullptr; }
 float4* col_s(const rtps_sph* s) { return s->opt_color ? s->color_s : nullptr; }
 
 // Launch inputs of the density (force = false) or force kernel.
 Win win(const rtps_sph* s, bool force = false) {
-    return Win{s->opt_window ? s->keys_sorted : nullptr, s->gps.grid_res.x, s->gps.grid_res.y, s->gps.grid_res.z,
-               s->opt_remap, force ? s->block_force : s->block_density, force ? s->force_variant : -1};
+    Win w{s->opt_window ? s->keys_sorted : nullptr, s->gps.grid_res.x, s->gps.grid_res.y, s->gps.grid_res.z,
+          s->opt_remap, force ? s->block_force : s->block_density, force ? s->force_variant : -1};
+    w.sub = sub_table(s);
+    return w;
 }
 
 template <typename T>
@@ -228,7 +237,8 @@ int launch_front(rtps_sph* s, StageTimer& t) {
     RTPS_TRY_HIP(sort_pairs_pingpong(n, s->key_bits, s->keys, s->idx, s->sort_tmp, s->stream, &s->keys_sorted,
                                      &s->idx_sorted));
     t.mark(ST_CELL);
-    RTPS_TRY_HIP(launch_cellindices(n, s->d_num, s->keys_sorted, s->nb_cells, s->start, s->end, kSubBits, s->stream));
+    RTPS_TRY_HIP(launch_cellindices(n, s->d_num, s->keys_sorted, s->nb_cells, s->start, s->end, kSubBits, s->stream,
+                                    sub_table(s)));
     t.mark(ST_PERMUTE);
     RTPS_TRY_HIP(launch_permute(n, s->d_num, s->idx_sorted, s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s,
                                 s->ve_s, s->stream, col_u(s), col_s(s)));
@@ -605,6 +615,8 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
         RTPS_TRY_HIP(dalloc(&s->start, C));
         RTPS_TRY_HIP(dalloc(&s->end, C));
         RTPS_TRY_HIP(dalloc(&s->d_num, 1));
+        if ((size_t)s->nb_cells * kSubCells * sizeof(uint32_t) <= ((size_t)2 << 30))   // at most 2 GiB
+            RTPS_TRY_HIP(dalloc(&s->sub_start, (size_t)s->nb_cells * kSubCells));
         RTPS_TRY_HIP(dalloc(&s->rm.rows, 9 * N));
         RTPS_TRY_HIP(dalloc(&s->rm.words, 9 * (size_t)kMaskWords * N));
         RTPS_TRY_HIP(dalloc(&s->rm.codes, (size_t)(kListCap / 4) * N));
@@ -1129,7 +1141,10 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
             if (value < 0 || value > 3) return set_error(RTPS_ERR_ARG, "force variant must be 0, 1, 2 or 3");
             g_force_variant = value;
             break;
-        case RTPS_OPT_WINDOW: s->opt_window = value ? 1 : 0; break;
+        case RTPS_OPT_WINDOW:
+            if (value < 0 || value > 2) return set_error(RTPS_ERR_ARG, "window option must be 0, 1 or 2");
+            s->opt_window = value;
+            break;
         case RTPS_OPT_TILE_REMAP:
             if (value < 0 || value > 4096) return set_error(RTPS_ERR_ARG, "tile remap must be 0, 1 or a chunk of 2..4096");
             s->opt_remap = value;

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kBlockSurf) void k_color_field(int res, const float
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t a, b;
-            row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b);
+            row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
             // colorfield.cl:63-75 per candidate, kGroup candidates per
             // iteration with their position loads issued together
             auto term = [&](float4 pj, uint32_t j) {

@@ -251,7 +251,11 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        candidates as bits and the force kernel walks those
  *                        row by row, 3 = the density kernel records a list of
  *                        16-bit neighbour codes and each lane walks its own list
- *   RTPS_OPT_WINDOW      1 = scan only the x window of each neighbour row (default)
+ *   RTPS_OPT_WINDOW      1 = scan only the x window of each neighbour row, its
+ *                        bounds looked up in a table of sub-cell starts that
+ *                        cellindices writes (default; 64 bytes per grid cell),
+ *                        2 = the same window found by binary search of the
+ *                        sorted keys, 0 = whole rows
  *   RTPS_OPT_AUTOTUNE    1 = the first rtps_sph_step tunes the neighbour launches
  *                        over the force loops 0 and 1 (default; see
  *                        rtps_sph_autotune), 2 = over all four loops (2 and 3

@@ -424,3 +424,28 @@ def test_read_async_into_page_locked_memory(cuda):
     buf.close()
     ib.close()
     s.close()
+
+
+@pytest.mark.parametrize("scene", ["dam", "clump_deletion"])
+def test_window_table_equals_key_search(cuda, scene):
+    """The x windows of the neighbour rows looked up in the sub-cell start
+    table (RTPS_OPT_WINDOW 1) are exactly the binary-search windows (2): the
+    steps are bit-identical, also with empty cells, a 300-particle cell and
+    particles leaving the grid."""
+    sc = scenes.dam_custom(20, 20, 10, seed=90)
+    if scene == "clump_deletion":
+        rng = np.random.default_rng(91)
+        pos = sc.pos.copy()
+        pos[:300, :3] = 0.3 + 0.012 * rng.random((300, 3), dtype=np.float32)
+        pos[400::97, 2] = 70.0
+        sc = scenes.Scene("clump", sc.settings, pos, sc.vel, 2, sc.dt)
+    out = []
+    for w in (1, 2):
+        s = _system(sc)
+        s.setOption(5, w)
+        s.step(4)
+        out.append((s.read("pos"), s.read("vel"), s.getNum(), s.countPairs()))
+        s.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2] and out[0][3] == out[1][3]
