@@ -75,7 +75,8 @@ __device__ __forceinline__ bool cell_of(uint32_t key, const SceneDev* __restrict
 // Cell start/end on the segmented keys; start/end hold global sorted indices.
 __global__ __launch_bounds__(kBBlock) void k_cellindices_b(int T, const uint32_t* __restrict__ keys,
                                                            const SceneDev* __restrict__ scenes,
-                                                           uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+                                                           uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                           uint32_t* __restrict__ sub) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= T) return;
     uint32_t g;
@@ -85,6 +86,17 @@ __global__ __launch_bounds__(kBBlock) void k_cellindices_b(int T, const uint32_t
     const bool last = i == T - 1 || !cell_of(keys[i + 1], scenes, gn) || gn != g;
     if (first) start[g] = (uint32_t)i;
     if (last) end[g] = (uint32_t)(i + 1);
+    const uint32_t key = keys[i];
+    const SceneDev& S = scenes[key >> kSceneShift];
+    if (sub && ((key & kLocalMask) >> kSubBits) < (uint32_t)S.g.nb_cells) {
+        // sub-cell starts of the windowed rows (as in the plain cellindices)
+        const uint32_t xq = key & (kSubCells - 1);
+        const uint32_t q0 = first ? 0u : (keys[i - 1] & (kSubCells - 1)) + 1;
+        uint32_t* row = sub + (size_t)g * kSubCells;
+        for (uint32_t q = q0; q <= xq; ++q) row[q] = (uint32_t)i;
+        if (last)
+            for (uint32_t q = xq + 1; q < (uint32_t)kSubCells; ++q) row[q] = (uint32_t)i + 1;
+    }
 }
 
 // One particle of a work item (sorted slot i of scene s).  Kept out of the
@@ -98,8 +110,10 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                                           float4* __restrict__ ve, const float4* __restrict__ vel_s,
                                           const uint32_t* __restrict__ idx, float4* __restrict__ pos_u,
                                           float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
-                                          float4* __restrict__ force_out, float4* __restrict__ xsph_out) {
+                                          float4* __restrict__ force_out, float4* __restrict__ xsph_out,
+                                          const uint32_t* __restrict__ sub_all) {
     const rtps_sph_params& sp = S.sp;
+    const uint32_t* sub = sub_all ? sub_all + (size_t)S.cell_base * kSubCells : nullptr;
     const float scale = sp.simulation_scale;
     const float h = sp.smoothing_distance, h2 = h * h;
     float4 pi = pos_s[i];
@@ -118,7 +132,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
-                row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix);
+                row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix, sub);
                 auto term = [&](float4 pj) {
                     float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
                     poly6_add(sum, rx * rx + ry * ry + rz * rz, h2);
@@ -143,7 +157,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
-                row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix);
+                row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix, sub);
                 uint32_t j = a;
                 for (; j + kGroup <= b; j += kGroup) {   // positions, then accepted velocities, loaded together
                     float4 p[kGroup], v[kGroup];
@@ -207,7 +221,8 @@ __global__ __launch_bounds__(kBBlock) void k_pairs_b(const BatchItem* __restrict
                                                      const float4* __restrict__ vel_s,
                                                      const uint32_t* __restrict__ idx, float4* __restrict__ pos_u,
                                                      float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
-                                                     float4* __restrict__ force_out, float4* __restrict__ xsph_out) {
+                                                     float4* __restrict__ force_out, float4* __restrict__ xsph_out,
+                                                     const uint32_t* __restrict__ sub) {
     __shared__ int sh_item;
     const int q0 = (int)(blockIdx.x % kXcdQueues);
     int q = q0;   // thread 0's current queue
@@ -239,7 +254,7 @@ __global__ __launch_bounds__(kBBlock) void k_pairs_b(const BatchItem* __restrict
         const int i = I.first + (int)threadIdx.x;
         if ((int)threadIdx.x < I.count && i - S.off < num)
             pairs_one<MODE>(i, s, S, deleted, pos_s, start, end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
-                            force_out, xsph_out);
+                            force_out, xsph_out, sub);
     }
 }
 
@@ -261,9 +276,9 @@ hipError_t launch_hash_b(int T, const uint16_t* sid, const SceneDev* scenes, con
 }
 
 hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* scenes, uint32_t* start, uint32_t* end,
-                                hipStream_t st) {
+                                hipStream_t st, uint32_t* sub) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
-    k_cellindices_b<<<bblocks(T), kBBlock, 0, st>>>(T, keys, scenes, start, end);
+    k_cellindices_b<<<bblocks(T), kBBlock, 0, st>>>(T, keys, scenes, start, end, sub);
     return hipGetLastError();
 }
 
@@ -271,24 +286,24 @@ hipError_t launch_pairs_b(int mode, int grid, const BatchItem* items, int n_item
                           const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
                           const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,
                           const uint32_t* idx, float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
-                          float4* force_out, float4* xsph_out) {
+                          float4* force_out, float4* xsph_out, const uint32_t* sub) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (mode == kPairsForceStore && (!force_out || !xsph_out)) return hipErrorInvalidValue;
     switch (mode) {
         case kPairsDensity:
             k_pairs_b<kPairsDensity><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start,
                                                                end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
-                                                               force_out, xsph_out);
+                                                               force_out, xsph_out, sub);
             break;
         case kPairsForceStep:
             k_pairs_b<kPairsForceStep><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start,
                                                                  end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
-                                                                 force_out, xsph_out);
+                                                                 force_out, xsph_out, sub);
             break;
         case kPairsForceStore:
             k_pairs_b<kPairsForceStore><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start,
                                                                   end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
-                                                                  force_out, xsph_out);
+                                                                  force_out, xsph_out, sub);
             break;
         default: return hipErrorInvalidValue;
     }

@@ -38,6 +38,7 @@ struct rtps_batch {
     int n_items = 0, persistent_grid = 0;
     int* queues = nullptr;
     void* sort_tmp = nullptr;
+    uint32_t* sub = nullptr;   // sub-cell starts of the windowed rows, 16 per cell
     int shifts[4] = {}, widths[4] = {}, npasses = 0;
     int opt_graph = 1, opt_timers = 0;
     hipGraphExec_t gexec = nullptr;
@@ -50,7 +51,7 @@ struct rtps_batch {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         void* bufs[] = {scenes, d_num, sid, pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s, keys, idx,
-                        start, end, items, queues, sort_tmp};
+                        start, end, items, queues, sort_tmp, sub};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         for (auto& e : ev)
@@ -85,18 +86,18 @@ int launch_step(rtps_batch* b) {
     RTPS_TRY_HIP(sort_pairs_passes(b->T, b->shifts, b->widths, b->npasses, b->keys, b->idx, b->sort_tmp, b->stream,
                                    &b->keys_sorted, &b->idx_sorted));
     mark(b, B_CELL);
-    RTPS_TRY_HIP(launch_cellindices_b(b->T, b->keys_sorted, b->scenes, b->start, b->end, b->stream));
+    RTPS_TRY_HIP(launch_cellindices_b(b->T, b->keys_sorted, b->scenes, b->start, b->end, b->stream, b->sub));
     mark(b, B_PERMUTE);
     RTPS_TRY_HIP(launch_permute(b->T, nullptr, b->idx_sorted, b->pos_u, b->vel_u, b->veleval_u, b->pos_s, b->vel_s,
                                 b->ve_s, b->stream));
     mark(b, B_DENSITY);
     RTPS_TRY_HIP(launch_pairs_b(kPairsDensity, b->persistent_grid, b->items, b->n_items, b->queues, b->scenes, b->d_num,
                                 b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted,
-                                b->pos_u, b->vel_u, b->veleval_u, b->stream));
+                                b->pos_u, b->vel_u, b->veleval_u, b->stream, nullptr, nullptr, b->sub));
     mark(b, B_FORCE);
     RTPS_TRY_HIP(launch_pairs_b(kPairsForceStep, b->persistent_grid, b->items, b->n_items, b->queues + kXcdQueues, b->scenes, b->d_num,
                                 b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted,
-                                b->pos_u, b->vel_u, b->veleval_u, b->stream));
+                                b->pos_u, b->vel_u, b->veleval_u, b->stream, nullptr, nullptr, b->sub));
     RTPS_TRY_HIP(launch_commit_b(b->S, b->scenes, b->d_num, b->start, b->stream));
     mark(b, B_N);
     return RTPS_OK;
@@ -250,6 +251,8 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
         RTPS_TRY_HIP(hipMalloc((void**)&b->idx, NT * 4));
         RTPS_TRY_HIP(hipMalloc((void**)&b->start, (size_t)cells * 4));
         RTPS_TRY_HIP(hipMalloc((void**)&b->end, (size_t)cells * 4));
+        if ((size_t)cells * kSubCells * 4 <= ((size_t)2 << 30))   // at most 2 GiB; else the kernels search
+            RTPS_TRY_HIP(hipMalloc((void**)&b->sub, (size_t)cells * kSubCells * 4));
         RTPS_TRY_HIP(hipMalloc((void**)&b->items, sizeof(BatchItem) * items.size()));
         RTPS_TRY_HIP(hipMalloc((void**)&b->queues, kQueueInts * sizeof(int)));
         RTPS_TRY_HIP(hipMalloc(&b->sort_tmp, sort_temp_bytes((int)T, 32)));

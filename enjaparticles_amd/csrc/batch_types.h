@@ -34,14 +34,15 @@ struct BatchItem {
 
 hipError_t launch_hash_b(int T, const uint16_t* sid, const SceneDev* scenes, const int* d_num, const float4* pos_u,
                          uint32_t* keys, uint32_t* idx, uint32_t* start, int n_cells, int* queues, hipStream_t st);
+// sub (optional): sub-cell start table, 16 entries per cell of every scene
 hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* scenes, uint32_t* start, uint32_t* end,
-                                hipStream_t st);
+                                hipStream_t st, uint32_t* sub = nullptr);
 constexpr int kPairsDensity = 0, kPairsForceStep = 1, kPairsForceStore = 2;
 hipError_t launch_pairs_b(int mode, int grid, const BatchItem* items, int n_items, int* queue,
                           const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
                           const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,
                           const uint32_t* idx, float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
-                          float4* force_out = nullptr, float4* xsph_out = nullptr);
+                          float4* force_out = nullptr, float4* xsph_out = nullptr, const uint32_t* sub = nullptr);
 hipError_t launch_commit_b(int S, const SceneDev* scenes, int* d_num, const uint32_t* start, hipStream_t st);
 
 }  // namespace rtps
