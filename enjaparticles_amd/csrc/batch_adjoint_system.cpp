@@ -130,7 +130,7 @@ int rtps_badj_create(int num_scenes, const int* n_settings, const char* const* k
             if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
                 cus = prop.multiProcessorCount;
         }
-        a->persistent_grid = std::max(1, std::min(a->n_items, cus * 4));
+        a->persistent_grid = std::max(1, std::min(a->n_items, cus * kPersistentBlocksPerCU));
         const size_t N = (size_t)a->T, C = (size_t)a->C;
         RTPS_TRY_HIP(a->alloc(&a->scenes, (size_t)num_scenes));
         RTPS_TRY_HIP(a->alloc(&a->d_num, (size_t)num_scenes));

@@ -230,7 +230,7 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
             if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
                 cus = prop.multiProcessorCount;
         }
-        b->persistent_grid = std::max(1, std::min(b->n_items, cus * 4));
+        b->persistent_grid = std::max(1, std::min(b->n_items, cus * kPersistentBlocksPerCU));
 
         std::vector<uint16_t> sid((size_t)T);
         for (int s = 0; s < num_scenes; ++s)

@@ -38,6 +38,11 @@ hipError_t launch_hash_b(int T, const uint16_t* sid, const SceneDev* scenes, con
 hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* scenes, uint32_t* start, uint32_t* end,
                                 hipStream_t st, uint32_t* sub = nullptr);
 constexpr int kPairsDensity = 0, kPairsForceStep = 1, kPairsForceStore = 2;
+// Blocks per CU of the persistent pair kernels (their grid is CUs x this):
+// 5 blocks of 4 waves fill the 5 waves per SIMD their 91 VGPRs allow (the
+// 256-scene step measured 5% faster than with 4; 6 with a 6-wave register
+// budget was slower, 8 much slower).
+constexpr int kPersistentBlocksPerCU = 5;
 hipError_t launch_pairs_b(int mode, int grid, const BatchItem* items, int n_items, int* queue,
                           const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
                           const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,

@@ -250,8 +250,13 @@ __global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __r
 constexpr int kLoopDirect = 0, kLoopCompact = 1, kLoopMasked = 2, kLoopList = 3;
 constexpr int kNbrCap = 32;
 
+// At least six waves per SIMD: the compiler keeps the force kernel within 80
+// VGPRs (from 85, with a few spilled bytes), which measured 3-8% faster than
+// five waves on the 1M dam break; eight waves spill too much.
+constexpr int kForceWavesPerSimd = 6;
+
 template <bool SCALED, bool PACKED, bool FUSED, int LOOP>
-__global__ __launch_bounds__(kBlock) void k_force(int num_host, const int* __restrict__ d_num,
+__global__ __launch_bounds__(kBlock, kForceWavesPerSimd) void k_force(int num_host, const int* __restrict__ d_num,
                                                   const float4* __restrict__ pos_s,
                                                   const float* __restrict__ dens_in,
                                                   const float4* __restrict__ ve_in,
