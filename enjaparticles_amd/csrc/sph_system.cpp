@@ -50,6 +50,7 @@ struct rtps_sph {
     // sub-cell starts of the windowed rows (16 per cell; nullptr when the grid
     // is too large for it: the kernels then search the sorted keys)
     uint32_t* sub_start = nullptr;
+    bool sub_valid = false;   // the last cellindices wrote the table for the current sorted state
     int* d_num = nullptr;
     void* sort_tmp = nullptr;
     // neighbour rows recorded by density for the force loops 2 (bit form) and 3 (list form)
@@ -124,7 +125,11 @@ const RowMasks* masks_of(const rtps_sph* s) {
 }
 
 // Sub-cell start table of the windowed rows, when it is in use.
-uint32_t* sub_table(const rtps_sph* s) { return s->opt_window == 1 ? s->sub_start : nullptr; }
+// Written by cellindices while the window option asks for it; read by the
+// neighbour kernels only when the current sorted state's table was written
+// (entries of cells that are empty now may be stale or uninitialised).
+uint32_t* sub_table_to_write(const rtps_sph* s) { return s->opt_window == 1 ? s->sub_start : nullptr; }
+uint32_t* sub_table(const rtps_sph* s) { return s->sub_valid ? s->sub_start : nullptr; }
 
 // Colour buffers handed to the kernels (none unless the system keeps colours).
 float4* col_u(const rtps_sph* s) { return s->opt_color ? s->color_u : nullptr; }
@@ -238,7 +243,8 @@ int launch_front(rtps_sph* s, StageTimer& t) {
                                      &s->idx_sorted));
     t.mark(ST_CELL);
     RTPS_TRY_HIP(launch_cellindices(n, s->d_num, s->keys_sorted, s->nb_cells, s->start, s->end, kSubBits, s->stream,
-                                    sub_table(s)));
+                                    sub_table_to_write(s)));
+    s->sub_valid = sub_table_to_write(s) != nullptr;
     t.mark(ST_PERMUTE);
     RTPS_TRY_HIP(launch_permute(n, s->d_num, s->idx_sorted, s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s,
                                 s->ve_s, s->stream, col_u(s), col_s(s)));
@@ -822,7 +828,9 @@ int rtps_sph_integrate(rtps_sph* s) {
 
 namespace {
 
-int run_steps(rtps_sph* s, int nsteps) {
+// eager: launch the kernels directly even when graphs are on (a step right
+// after a push, whose graph would be captured for one replay only)
+int run_steps(rtps_sph* s, int nsteps, bool eager = false) {
     if (s->num_host == 0) return RTPS_OK;
     if (s->opt_autotune && !s->tuned && nsteps > 0) {   // the first step also tunes the launches
         int rc = autotune_step(s, false);
@@ -830,7 +838,7 @@ int run_steps(rtps_sph* s, int nsteps) {
         if (--nsteps == 0) return RTPS_OK;
     }
     if (s->opt_timers) return step_timed(s, nsteps);
-    if (s->opt_graph) {
+    if (s->opt_graph && !eager) {
         if (!s->graph_valid) {
             int rc = capture_graph(s);
             if (rc) return rc;
@@ -855,9 +863,10 @@ int rtps_sph_step(rtps_sph* s, int nsteps) {
     maybe_update_params(s);
     if (s->hoses.empty()) return run_steps(s, nsteps);
     for (int k = 0; k < nsteps; ++k) {   // hoses spray on the host before every update
+        const int before = s->num_host;
         int rc = spray_hoses(s);
         if (rc) return rc;
-        rc = run_steps(s, 1);
+        rc = run_steps(s, 1, s->num_host != before);
         if (rc) return rc;
     }
     return RTPS_OK;

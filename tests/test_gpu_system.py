@@ -449,3 +449,23 @@ def test_window_table_equals_key_search(cuda, scene):
     np.testing.assert_array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1], out[1][1])
     assert out[0][2] == out[1][2] and out[0][3] == out[1][3]
+
+
+def test_window_table_is_used_only_when_written(cuda):
+    """After steps with the searched windows (RTPS_OPT_WINDOW 2), switching to
+    the table (1) takes effect at the next step: queries on the current
+    sorted state (pair counts, colour field) keep the searched windows, since
+    the table was not written for it."""
+    sc = scenes.dam_custom(16, 16, 8, seed=92)
+    s = _system(sc)
+    s.setOption(5, 2)
+    s.step(3)
+    k2, f2 = s.countPairs(), s.colorField(12)
+    s.setOption(5, 1)
+    assert s.countPairs() == k2
+    np.testing.assert_array_equal(s.colorField(12), f2)
+    s.step(1)                  # writes the table; queries now read it
+    k1 = s.countPairs()
+    s.setOption(5, 2)
+    assert s.countPairs() == k1
+    s.close()
