@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-updates/sec at 1M SPH (plain vs batched vs fwd+adjoint); HBM % peak"
 PEAK_HBM_GBS = 8000.0          # MI355X HBM3E
 PEAK_VALU_FP32_TFLOPS = 78.6   # non-packed FP32 VALU (packed FP32 is disabled in this build)
+PEAK_PACKED_FP32_TFLOPS = 157.3  # published FP32 peak, packed (two lanes per VALU op)
 # HBM bytes per launch of the fused force kernel on this workload (direct loop,
 # 64-thread blocks), from two rocprofv3 --pmc runs over the first 320 steps:
 # FETCH_SIZE 228686 KiB x 2.0 + WRITE_SIZE 90400 KiB x 1.0, the factors read off
@@ -33,26 +34,56 @@ PEAK_VALU_FP32_TFLOPS = 78.6   # non-packed FP32 VALU (packed FP32 is disabled i
 FORCE_HBM_BYTES_PER_LAUNCH = (228686 * 2.0 + 90400 * 1.0) * 1024
 
 
-def cpu_baseline(scene, steps: int = 2):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, steps: int = 2, single_steps: int = 2):
     """Oracle FP32 step (bitonic sort, OpenMP) on a bounded sample: `steps`
-    steps of the same 1M-particle workload from its initial state."""
+    steps of the same 1M-particle workload from its initial state on all
+    host threads, then `single_steps` more on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     sp, gp, gs = O.derive_params(scene.settings)
     st = O.State(scene.pos, scene.vel, sp, gp, gs)
-    st.step(scene.dt, O.FLAG_COLLISION | O.FLAG_BITONIC)   # untimed first touch
+    flags = O.FLAG_COLLISION | O.FLAG_BITONIC
+    st.step(scene.dt, flags)   # untimed first touch
+    threads = O.num_threads()
     t0 = time.perf_counter()
     for _ in range(steps):
-        st.step(scene.dt, O.FLAG_COLLISION | O.FLAG_BITONIC)
+        st.step(scene.dt, flags)
     dt = time.perf_counter() - t0
+    single = None
+    if single_steps > 0:
+        O.set_threads(1)
+        try:
+            t1 = time.perf_counter()
+            for _ in range(single_steps):
+                st.step(scene.dt, flags)
+            d1 = time.perf_counter() - t1
+        finally:
+            O.set_threads(threads)
+        single = {"value": scene.n * single_steps / d1, "ms_per_step": 1e3 * d1 / single_steps,
+                  "steps": single_steps}
     return {
         "value": scene.n * steps / dt,
         "unit": "particle-updates/s",
-        "cores": O.num_threads(),
+        "cores": threads,
         "kind": "port",
         "sample": f"{steps} steps (after 1 untimed) of the {scene.n}-particle dam break, FP32 CPU restatement "
-                  f"of the reference kernels with a bitonic sort, {O.num_threads()} OpenMP threads",
+                  f"of EnjaParticles' OpenCL kernels (hash, bitonic sort, cell indices, permute, density, "
+                  f"force, walls, leapfrog), {threads} OpenMP threads",
         "ms_per_step": 1e3 * dt / steps,
+        "cpu_model": _cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "single_thread": single,
     }
 
 
@@ -312,6 +343,7 @@ def main():
     ap.add_argument("--no-stage-timers", action="store_true", help="time the graph without per-stage events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-single-steps", type=int, default=2)
     ap.add_argument("--profile", action="store_true", help="timed loop only (for rocprofv3 runs)")
     ap.add_argument("--force-variant", type=int, default=0, help="1 = compacted neighbour lists, 0 = direct")
     ap.add_argument("--triangles", type=int, default=2048,
@@ -443,6 +475,8 @@ def main():
             "peak": PEAK_VALU_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_VALU_FP32_TFLOPS,
+            "peak_packed": PEAK_PACKED_FP32_TFLOPS,
+            "frac_packed": achieved / PEAK_PACKED_FP32_TFLOPS,
             "traffic": FORCE_HBM_BYTES_PER_LAUNCH,
             "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
             "flop_per_launch": force_flop,
@@ -452,6 +486,7 @@ def main():
         out["density_roofline"] = {
             "achieved_tflops": dens_flop / (dens_ms * 1e-3) / 1e12,
             "frac": dens_flop / (dens_ms * 1e-3) / 1e12 / PEAK_VALU_FP32_TFLOPS,
+            "frac_packed": dens_flop / (dens_ms * 1e-3) / 1e12 / PEAK_PACKED_FP32_TFLOPS,
             "avg_launch_ms": dens_ms,
         }
     if args.surface_res > 0 and not args.profile:
@@ -484,7 +519,7 @@ def main():
               f"{out['batched_fwd_adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps, args.cpu_single_steps)
     print(json.dumps(out))
 
 

@@ -98,6 +98,7 @@ def _declare(L):
         "oracle_step_f64": (i, [P, P, P, P, P, P, P, d, i, i, P, P, P] + [P] * 10),
         "oracle_derive_params": (i, [P, P, i, P, P, P]),
         "oracle_num_threads": (i, []),
+        "oracle_set_threads": (None, [i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -120,6 +121,11 @@ def _sfx(dtype):
 
 def num_threads() -> int:
     return lib().oracle_num_threads()
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the following oracle calls (n <= 0: all host cores)."""
+    lib().oracle_set_threads(int(n))
 
 
 # ---------------------------------------------------------------------------

@@ -563,6 +563,15 @@ extern "C" int oracle_num_threads(void) {
 #endif
 }
 
+// OpenMP threads of the following oracle calls (n <= 0: all host cores).
+extern "C" void oracle_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
+#else
+    (void)n;
+#endif
+}
+
 
 // ---------------------------------------------------------------------------
 // a1: parameter replay.  SPH::calculate (SPHSettings.cpp:30-144), the settings
