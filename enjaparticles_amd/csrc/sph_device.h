@@ -178,6 +178,73 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__
     return lo;
 }
 
+// row_window with the sub-cell start table, for a particle inside the grid
+// away from its x edges (WindowCtx::on) and grids of fewer than 2^23 cells.
+// The same [a, b) as the general code below, without branches on loaded
+// values: there every lookup waits for the one before it (start, then end,
+// then the table entry), here the row's loads are issued together.
+//  * the cell hashes are computed in integers: for |values| < 2^24 the float
+//    arithmetic of grid_hash is exact, so the two agree;
+//  * start/end of all three cells are loaded (index nb_cells, the sentinel
+//    entry both arrays hold, stands in for hashes outside the grid) and the
+//    unoccupied ones are discarded;
+//  * the table entries at both window ends are loaded whether or not their
+//    cells are occupied (entries of empty cells may be stale; they are
+//    discarded the same way).
+// Rows farther than h return before loading anything, with an empty [0, 0).
+__device__ __forceinline__ void row_window_table(const WindowCtx& w, int3 c, int dy, int dz, float h2,
+                                                 const Grid& g, const uint32_t* __restrict__ start,
+                                                 const uint32_t* __restrict__ end,
+                                                 const uint32_t* __restrict__ sub, uint32_t& a, uint32_t& b) {
+    const float ey = dy < 0 ? (w.uy - (float)c.y) * w.wy : (dy > 0 ? ((float)(c.y + 1) - w.uy) * w.wy : 0.0f);
+    const float ez = dz < 0 ? (w.uz - (float)c.z) * w.wz : (dz > 0 ? ((float)(c.z + 1) - w.uz) * w.wz : 0.0f);
+    const float d2 = ey * ey + ez * ez;
+    if (d2 > h2 * 1.0001f) {
+        a = 0;
+        b = 0;
+        return;
+    }
+    const uint32_t nbc = (uint32_t)g.nb_cells;
+    const int hb = ((c.z + dz) * (int)g.ry + (c.y + dy)) * (int)g.rx + (c.x - 1);
+    // hash of cell k of the row, nb_cells when outside the grid (no arrays:
+    // dynamically indexed ones would live in scratch memory)
+    auto hash_k = [&](int k) {
+        const uint32_t hh = (uint32_t)(hb + k);
+        return hh < nbc ? hh : nbc;
+    };
+    const uint32_t h0 = hash_k(0), h1 = hash_k(1), h2c = hash_k(2);
+    // hardware square root (about 1 ulp): the window is widened by a whole
+    // sub-cell on each side, so the rounding never drops a candidate
+    const float half = __builtin_amdgcn_sqrtf(fmaxf(h2 - d2, 0.0f)) * (w.hx * __builtin_amdgcn_rsqf(h2));
+    const float rel = w.ux - (float)(c.x - 1);
+    const int r_lo = max((int)floorf((rel - half) * (float)kSubCells) - 1, 0);
+    const int r_hi = min((int)floorf((rel + half) * (float)kSubCells) + 1, 3 * kSubCells - 1);
+    // table entry q of the cell holding sub-key r (cell 2 for r past the row)
+    auto entry = [&](int r) {
+        const uint32_t hc = hash_k(min(r >> kSubBits, 2));
+        return sub[(size_t)min(hc, nbc - 1u) * kSubCells + (uint32_t)(r & (kSubCells - 1))];
+    };
+    const uint32_t s0 = start[h0], s1 = start[h1], s2 = start[h2c];
+    const uint32_t e0 = end[h0], e1 = end[h1], e2 = end[h2c];
+    const uint32_t t_lo = entry(r_lo), t_hi = entry(r_hi + 1);
+    const bool o0 = h0 < nbc && s0 != kEmpty, o1 = h1 < nbc && s1 != kEmpty, o2 = h2c < nbc && s2 != kEmpty;
+    const uint32_t lo = min(o0 ? s0 : kEmpty, min(o1 ? s1 : kEmpty, o2 ? s2 : kEmpty));
+    const uint32_t hi = max(o0 ? e0 : 0u, max(o1 ? e1 : 0u, o2 ? e2 : 0u));
+    // first index with sub-key >= r: the table entry when r's cell is
+    // occupied, else the start of the next occupied cell, else the row end
+    const uint32_t occ = (o0 ? 1u : 0u) | (o1 ? 2u : 0u) | (o2 ? 4u : 0u);
+    const uint32_t f2 = o2 ? s2 : hi, f1 = o1 ? s1 : f2;
+    auto first_at = [&](int r, uint32_t t) {
+        const int co = r >> kSubBits;
+        const uint32_t fb = co == 0 ? f1 : (co == 1 ? f2 : hi);
+        return ((occ >> co) & 1u) ? t : fb;
+    };
+    // a row without occupied cells comes out as [0, 0) (hi = f1 = f2 = 0)
+    // with no branch, so no load waits for the start values
+    a = first_at(r_lo, t_lo);
+    b = first_at(r_hi + 1, t_hi);
+}
+
 // Row (y, z) = (c.y + dy, c.z + dz) of particle i: [a, b) of its candidates.
 // key_prefix: high key bits shared by the whole row (the scene id of batched keys).
 __device__ __forceinline__ void row_window(const WindowCtx& w, int3 c, int dy, int dz, float h2, const Grid& g,
@@ -185,6 +252,10 @@ __device__ __forceinline__ void row_window(const WindowCtx& w, int3 c, int dy, i
                                            const uint32_t* __restrict__ keys, uint32_t& a, uint32_t& b,
                                            uint32_t key_prefix = 0u, const uint32_t* __restrict__ sub = nullptr) {
     const bool table = sub != nullptr && w.on;
+    if (table && g.nb_cells < (1 << 23)) {
+        row_window_table(w, c, dy, dz, h2, g, start, end, sub, a, b);
+        return;
+    }
     // the row's three cells; with the table their starts are kept for the lookups
     const int hb = grid_hash(c.x - 1, c.y + dy, c.z + dz, g);
     uint32_t cs[3] = {kEmpty, kEmpty, kEmpty}, hc[3] = {0u, 0u, 0u};
