@@ -211,8 +211,10 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
 // MODE: kPairsDensity, kPairsForceStep (force + walls + leapfrog, scattered to
 // the unsorted arrays) or kPairsForceStore (force and XSPH stored in sorted
 // order for the adjoint's recorded step).
+// (kBBlock = 256: one block per CU is one wave per SIMD, so the register budget
+// of kPersistentBlocksPerCU waves per SIMD keeps all the persistent blocks resident.)
 template <int MODE>
-__global__ __launch_bounds__(kBBlock) void k_pairs_b(const BatchItem* __restrict__ items, int n_items,
+__global__ __launch_bounds__(kBBlock, kPersistentBlocksPerCU) void k_pairs_b(const BatchItem* __restrict__ items, int n_items,
                                                      int* __restrict__ queue, const SceneDev* __restrict__ scenes,
                                                      const int* __restrict__ d_num, const float4* __restrict__ pos_s,
                                                      const uint32_t* __restrict__ start,

@@ -39,7 +39,7 @@ hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* sce
                                 hipStream_t st, uint32_t* sub = nullptr);
 constexpr int kPairsDensity = 0, kPairsForceStep = 1, kPairsForceStore = 2;
 // Blocks per CU of the persistent pair kernels (their grid is CUs x this):
-// 5 blocks of 4 waves fill the 5 waves per SIMD their 91 VGPRs allow (the
+// 5 blocks of 4 waves, compiled for 5 waves per SIMD (at most 96 VGPRs; the
 // 256-scene step measured 5% faster than with 4; 6 with a 6-wave register
 // budget was slower, 8 much slower).
 constexpr int kPersistentBlocksPerCU = 5;

@@ -121,8 +121,10 @@ __global__ __launch_bounds__(kBlock) void k_permute(int num_host, const int* __r
 constexpr int kRecNone = 0, kRecBits = 1, kRecList = 2;
 // candidates per density group (a divisor of 32: the bit words fill in whole groups)
 constexpr int kGroupDensity = 4;
+// Nine waves per SIMD (at most 56 VGPRs, the occupancy of the search-based
+// windows; the table windows would otherwise take 57 and leave eight).
 template <bool SCALED, bool PACKED, int REC>
-__global__ __launch_bounds__(kBlock) void k_density(int num_host, const int* __restrict__ d_num,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k_density(int num_host, const int* __restrict__ d_num,
                                                     const float4* __restrict__ pos_s,
                                                     const uint32_t* __restrict__ start,
                                                     const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
