@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -289,6 +291,42 @@ def surface_leg(s, res: int, iso: float = 0.5, reps: int = 10):
             "field_plus_surface_ms": surf_ms, "nodes_per_s": res ** 3 / (field_ms * 1e-3)}
 
 
+def slab_leg(steps: int = 100, warmup: int = 20, slabs: int = 8):
+    """One GPU's share of the 1M dam break cut into `slabs` z-slabs: the
+    bottom slab's 128x128x8 lattice layers (131,072 particles) and the 2-cell
+    halo above them (4 layers, 65,536 particles, the next slab's particles a
+    sharded step reads), the first layers of the same jittered lattice in the
+    same box, stepped as one plain system.  The halo is stepped too (a sharded
+    step would only read it), so the time is the stage cost of one slab with
+    its halo; particle-updates/s counts the slab's own particles."""
+    import torch
+    from enjaparticles_amd import scenes
+    from enjaparticles_amd.rtps import RTPS
+    import enjaparticles_amd._native as N
+    sc = scenes.dam1m()
+    own_layers, halo_layers = 64 // slabs, 4
+    own = 128 * 128 * own_layers
+    pos = scenes.lattice(128, 128, 64, 1002, count=128 * 128 * (own_layers + halo_layers))
+    s = RTPS.generateSystemInstance(sc.settings)
+    s.pushParticles(pos, np.zeros_like(pos))
+    s.setOption(N.OPT_GRAPH, 1)
+    s.setOption(N.OPT_TIMERS, 1)
+    s.autotune(measure=True)
+    s.step(max(0, warmup - 1))
+    s.synchronize()
+    s.resetTimers()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    stages = {k: round(v[0], 5) for k, v in s.stageTimes().items() if v[1] > 0}
+    n = s.getNum()
+    s.close()
+    return {"own_particles": own, "halo_particles": n - own, "steps": steps, "ms_per_step": 1e3 * dt / steps,
+            "value": own * steps / dt, "unit": "particle-updates/s (own particles)", "stages_ms": stages}
+
+
 def compare4m_leg(steps: int = 100):
     """Plain and batched forward at 4,194,304 particles: one 256x128x128 dam
     break, and 64 scenes of 64x64x16, stepped `steps` steps each (hipGraph),
@@ -353,6 +391,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=30, help="steps of the batched leg (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="steps of the fwd+adjoint leg (0 = skip)")
     ap.add_argument("--badj-steps", type=int, default=100, help="steps of the batched fwd+adjoint leg (0 = skip)")
+    ap.add_argument("--slab-steps", type=int, default=100,
+                    help="steps of the one-z-slab leg (one GPU's share of an 8-way split; 0 = skip)")
     ap.add_argument("--compare-4m", type=int, default=0, metavar="STEPS",
                     help="only run the plain vs batched comparison at 4,194,304 particles for STEPS steps")
     args = ap.parse_args()
@@ -512,6 +552,11 @@ def main():
         out["fwd_adjoint"] = adjoint_leg(args.adjoint_steps)
         print(f"# fwd+adjoint: {out['fwd_adjoint']['value']:.3e} p-u/s, adjoint/forward "
               f"{out['fwd_adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
+    if args.slab_steps > 0:
+        print("# one-slab leg ...", file=sys.stderr, flush=True)
+        out["one_slab"] = slab_leg(args.slab_steps)
+        print(f"# one slab: {out['one_slab']['ms_per_step']:.3f} ms/step, {out['one_slab']['value']:.3e} p-u/s "
+              f"(own particles)", file=sys.stderr, flush=True)
     if args.badj_steps > 0:
         print("# batched fwd+adjoint leg ...", file=sys.stderr, flush=True)
         out["batched_fwd_adjoint"] = batched_adjoint_leg(args.badj_steps)

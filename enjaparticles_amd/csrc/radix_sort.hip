@@ -93,10 +93,15 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
     for (int d = threadIdx.x; d < R; d += kSortThreads) h[d] = 0;
     __syncthreads();
     const long base = (long)blockIdx.x * (kSortThreads * ITEMS);
+    // all ITEMS loads first (index clamped to n - 1, n >= 2: no branch, one wait)
+    uint32_t kk[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) kk[k] = kin[min(base + k * kSortThreads + threadIdx.x, (long)n - 1)];
+#pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const long e = base + k * kSortThreads + threadIdx.x;
         const bool valid = e < n;
-        const uint32_t d = valid ? (kin[e] >> shift) & mask : 0u;
+        const uint32_t d = valid ? (kk[k] >> shift) & mask : 0u;
         const uint64_t peers = match_digit(d, bits, valid);
         if (valid && (peers & lt) == 0ull) atomicAdd(&h[d], (uint32_t)__popcll(peers));
     }
@@ -184,12 +189,18 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
 
     const long base = (long)blockIdx.x * (kSortThreads * ITEMS) + (long)w * kQuarter;
     uint32_t key[ITEMS], val[ITEMS], pos[ITEMS];
+    // all loads first (index clamped to n - 1, n >= 2: no branch, one wait)
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const long e = min(base + k * 64 + lane, (long)n - 1);
+        key[k] = kin[e];
+        val[k] = vin[e];
+    }
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const long e = base + k * 64 + lane;
         const bool valid = e < n;
-        key[k] = valid ? kin[e] : 0xFFFFFFFFu;
-        val[k] = valid ? vin[e] : 0u;
+        if (!valid) key[k] = 0xFFFFFFFFu;
         const uint32_t d = (key[k] >> shift) & mask;
         const uint64_t peers = match_digit(d, bits, valid);
         const uint32_t before = cnt[w][d];                 // this wave's earlier rounds
