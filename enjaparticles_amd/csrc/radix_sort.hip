@@ -4,8 +4,10 @@
 // ordered (the cell keys are < 2^key_bits); digits are at most 8 bits wide and
 // the bits are split evenly over ceil(key_bits / 8) passes.
 //
-// Per pass (R = 2^bits buckets, tiles of 256 threads x ITEMS keys, ITEMS =
-// 4, 8 or 16; g_sort_items picks it, 8 by default):
+// Per pass (R = 2^bits buckets, tiles of 256 x ITEMS keys, ITEMS = 4, 8 or
+// 16; g_sort_items picks it, by default 16 from 2^20 keys up and 8 below; the
+// tile kernels run 1024 threads of ITEMS / 4 keys each, 16 waves per tile for
+// latency hiding):
 //   k_radix_hist    per-tile digit histogram, digit-major in hist[d * ntiles + t];
 //                   lanes of a wave holding the same digit are found with
 //                   `bits` ballots and add their count with one LDS atomic;
@@ -13,10 +15,10 @@
 //   k_radix_scan    one block per digit: exclusive scan of its row plus the
 //                   total of all smaller digits -> global offsets
 //   k_radix_scatter stable ranking inside the tile: wave w owns the tile's
-//                   w-th quarter and walks it in rounds of 64 keys; the rank
+//                   w-th sixteenth and walks it in rounds of 64 keys; the rank
 //                   of a key = popcount of the lower lanes with the same digit
 //                   (ballots) + the wave's running count of that digit (LDS);
-//                   a 4-wave exclusive scan per digit then gives each wave's
+//                   a 16-wave exclusive scan per digit then gives each wave's
 //                   offset and the key goes to digit offset + rank.  Block 0
 //                   clears gtot for the next use.
 // Keys and values ping-pong between the caller's arrays and the alternate
@@ -28,11 +30,15 @@
 
 namespace rtps {
 
-constexpr int kSortThreads = 256;
+constexpr int kSortThreads = 256;   // tile = kSortThreads x ITEMS keys; also the scan kernel's block
 constexpr int kSortMinItems = 4;    // smallest tile: sizes the tile-histogram buffer
 constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kTileThreads = 1024;  // threads of the histogram and scatter kernels (ITEMS / 4 keys each)
+constexpr int kTileWaves = kTileThreads / 64;
 
-int g_sort_items = 8;   // keys per thread of a sort tile (4, 8 or 16); 8 measured fastest at 1M keys
+// 256 x g_sort_items keys per sort tile (4, 8 or 16; 0 = by size: 4096-key
+// tiles measured 77 us for 1M keys against 84 us with 2048 and 115 us with 1024)
+int g_sort_items = 0;
 constexpr int kMaxRadix = 256;
 constexpr int kMaxPasses = 4;
 
@@ -81,8 +87,9 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, bool valid
     return peers;
 }
 
-template <int ITEMS>
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ kin, int n, int shift,
+// IT = keys per thread (tile = kTileThreads x IT keys).
+template <int IT>
+__global__ __launch_bounds__(kTileThreads) void k_radix_hist(const uint32_t* __restrict__ kin, int n, int shift,
                                                              int bits, uint32_t* __restrict__ hist,
                                                              uint32_t* __restrict__ gtot, int ntiles) {
     __shared__ uint32_t h[kMaxRadix];
@@ -90,23 +97,23 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
     const uint32_t mask = (uint32_t)R - 1u;
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
-    for (int d = threadIdx.x; d < R; d += kSortThreads) h[d] = 0;
+    for (int d = threadIdx.x; d < R; d += kTileThreads) h[d] = 0;
     __syncthreads();
-    const long base = (long)blockIdx.x * (kSortThreads * ITEMS);
-    // all ITEMS loads first (index clamped to n - 1, n >= 2: no branch, one wait)
-    uint32_t kk[ITEMS];
+    const long base = (long)blockIdx.x * (kTileThreads * IT);
+    // all IT loads first (index clamped to n - 1, n >= 2: no branch, one wait)
+    uint32_t kk[IT];
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) kk[k] = kin[min(base + k * kSortThreads + threadIdx.x, (long)n - 1)];
+    for (int k = 0; k < IT; ++k) kk[k] = kin[min(base + k * kTileThreads + threadIdx.x, (long)n - 1)];
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const long e = base + k * kSortThreads + threadIdx.x;
+    for (int k = 0; k < IT; ++k) {
+        const long e = base + k * kTileThreads + threadIdx.x;
         const bool valid = e < n;
         const uint32_t d = valid ? (kk[k] >> shift) & mask : 0u;
         const uint64_t peers = match_digit(d, bits, valid);
         if (valid && (peers & lt) == 0ull) atomicAdd(&h[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < R; d += kSortThreads) {
+    for (int d = threadIdx.x; d < R; d += kTileThreads) {
         const uint32_t c = h[d];
         hist[(long)d * ntiles + blockIdx.x] = c;
         if (c) atomicAdd(&gtot[d], c);
@@ -167,37 +174,37 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scan(uint32_t* __restric
     }
 }
 
-template <int ITEMS>
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
+template <int IT>
+__global__ __launch_bounds__(kTileThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout,
                                                                 uint32_t* __restrict__ vout, int n, int shift,
                                                                 int bits, const uint32_t* __restrict__ hist,
                                                                 int ntiles, uint32_t* __restrict__ gtot_clear) {
-    constexpr int kQuarter = kSortThreads * ITEMS / kSortWaves;   // keys per wave, in rounds of 64
-    __shared__ uint32_t cnt[kSortWaves][kMaxRadix];    // running count per (wave, digit), then wave offsets
+    constexpr int kPerWave = 64 * IT;                  // keys per wave, in IT rounds of 64
+    __shared__ uint32_t cnt[kTileWaves][kMaxRadix];    // running count per (wave, digit), then wave offsets
     __shared__ uint32_t goff[kMaxRadix];
     const int R = 1 << bits;
     const uint32_t mask = (uint32_t)R - 1u;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint64_t lt = (1ull << lane) - 1ull;
-    for (int e = tid; e < kSortWaves * kMaxRadix; e += kSortThreads) (&cnt[0][0])[e] = 0;
-    for (int d = tid; d < R; d += kSortThreads) goff[d] = hist[(long)d * ntiles + blockIdx.x];
+    for (int e = tid; e < kTileWaves * kMaxRadix; e += kTileThreads) (&cnt[0][0])[e] = 0;
+    for (int d = tid; d < R; d += kTileThreads) goff[d] = hist[(long)d * ntiles + blockIdx.x];
     if (blockIdx.x == 0)
-        for (int d = tid; d < R; d += kSortThreads) gtot_clear[d] = 0;
+        for (int d = tid; d < R; d += kTileThreads) gtot_clear[d] = 0;
     __syncthreads();
 
-    const long base = (long)blockIdx.x * (kSortThreads * ITEMS) + (long)w * kQuarter;
-    uint32_t key[ITEMS], val[ITEMS], pos[ITEMS];
+    const long base = (long)blockIdx.x * (kTileThreads * IT) + (long)w * kPerWave;
+    uint32_t key[IT], val[IT], pos[IT];
     // all loads first (index clamped to n - 1, n >= 2: no branch, one wait)
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
+    for (int k = 0; k < IT; ++k) {
         const long e = min(base + k * 64 + lane, (long)n - 1);
         key[k] = kin[e];
         val[k] = vin[e];
     }
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
+    for (int k = 0; k < IT; ++k) {
         const long e = base + k * 64 + lane;
         const bool valid = e < n;
         if (!valid) key[k] = 0xFFFFFFFFu;
@@ -208,10 +215,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
         if (valid && (peers & lt) == 0ull) cnt[w][d] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    for (int d = tid; d < R; d += kSortThreads) {          // exclusive scan over the 4 waves
+    for (int d = tid; d < R; d += kTileThreads) {          // exclusive scan over the 16 waves
         uint32_t run = 0;
 #pragma unroll
-        for (int q = 0; q < kSortWaves; ++q) {
+        for (int q = 0; q < kTileWaves; ++q) {
             const uint32_t c = cnt[q][d];
             cnt[q][d] = run;
             run += c;
@@ -219,7 +226,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
+    for (int k = 0; k < IT; ++k) {
         const long e = base + k * 64 + lane;
         if (e < n) {
             const uint32_t d = (key[k] >> shift) & mask;
@@ -270,7 +277,8 @@ hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int np
     if (n <= 1 || npasses == 0) return hipSuccess;
     if (npasses > kMaxPasses) return hipErrorInvalidValue;
     SortTemp t = carve(temp, n);
-    const int items = g_sort_items == 4 || g_sort_items == 16 ? g_sort_items : 8;
+    const int items = g_sort_items == 4 || g_sort_items == 8 || g_sort_items == 16 ? g_sort_items
+                      : (n >= (1 << 20) ? 16 : 8);
     const int ntiles = ntiles_for(n, items);
     uint32_t *ki = keys, *vi = vals, *ko = t.kalt, *vo = t.valt;
     for (int p = 0; p < npasses; ++p) {
@@ -278,9 +286,9 @@ hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int np
         if (bits < 1 || bits > 8) return hipErrorInvalidValue;
         uint32_t* gtot = t.gtot + p * kMaxRadix;
 #define RTPS_SORT_PASS(I)                                                                                           \
-        k_radix_hist<I><<<ntiles, kSortThreads, 0, st>>>(ki, n, shift, bits, t.hist, gtot, ntiles);                   \
+        k_radix_hist<I / 4><<<ntiles, kTileThreads, 0, st>>>(ki, n, shift, bits, t.hist, gtot, ntiles);               \
         k_radix_scan<<<1 << bits, kSortThreads, 0, st>>>(t.hist, gtot, ntiles);                                      \
-        k_radix_scatter<I><<<ntiles, kSortThreads, 0, st>>>(ki, vi, ko, vo, n, shift, bits, t.hist, ntiles, gtot)
+        k_radix_scatter<I / 4><<<ntiles, kTileThreads, 0, st>>>(ki, vi, ko, vo, n, shift, bits, t.hist, ntiles, gtot)
         if (items == 4) { RTPS_SORT_PASS(4); }
         else if (items == 8) { RTPS_SORT_PASS(8); }
         else { RTPS_SORT_PASS(16); }

@@ -107,7 +107,7 @@ hipError_t launch_mc_emit(int res, float iso, const float* field, const uint8_t*
 std::vector<rtps_float4> hose_spray(rtps_hose* h);
 
 // ---- radix sort (radix_sort.hip) ----
-extern int g_sort_items;   // keys per thread of a sort tile: 4, 8 or 16 (process-wide)
+extern int g_sort_items;   // 256 x this keys per sort tile: 4, 8 or 16, 0 = by size (process-wide)
 int sort_passes(int key_bits);
 size_t sort_temp_bytes(int n, int key_bits);
 hipError_t sort_clear_temp(void* temp, int n, hipStream_t st);

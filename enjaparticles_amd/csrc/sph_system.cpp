@@ -520,6 +520,10 @@ int autotune_step(rtps_sph* s, bool measure) {
         return ms / kReps;
     };
     const int blocks[] = {64, 128, 256};
+    // a later candidate (larger block, later loop) must be faster by this
+    // factor: the first state is no guide to close calls (64-thread blocks keep
+    // the neighbour rows of consecutive tiles in fewer L2s as the state evolves)
+    constexpr float kMargin = 0.98f;
     // RTPS_TUNE_VERBOSE=1 prints every timed configuration to stderr
     const char* verbose_env = std::getenv("RTPS_TUNE_VERBOSE");
     const bool verbose = verbose_env && verbose_env[0] == '1';
@@ -541,7 +545,7 @@ int autotune_step(rtps_sph* s, bool measure) {
             });
             if (verbose) std::fprintf(stderr, "rtps autotune: density record %d block %d: %.4f ms\n", mode, bs, ms);
             if (ms < 0.f) failed = true;
-            else if (ms < best_d[mode]) { best_d[mode] = ms; bd[mode] = bs; }
+            else if (best_d[mode] >= 1e30f || ms < kMargin * best_d[mode]) { best_d[mode] = ms; bd[mode] = bs; }
         }
     const bool fo = s->opt_force_out != 0;
     float best_total = 1e30f;
@@ -560,7 +564,7 @@ int autotune_step(rtps_sph* s, bool measure) {
             const int mode = var >= 2 ? var - 1 : 0;
             if (verbose) std::fprintf(stderr, "rtps autotune: force loop %d block %d: %.4f ms\n", var, bs, ms);
             if (ms < 0.f) failed = true;
-            else if (ms + best_d[mode] < best_total) {
+            else if (best_total >= 1e30f || ms + best_d[mode] < kMargin * best_total) {
                 best_total = ms + best_d[mode];
                 best = Tune{bd[mode], bs, var};
             }
@@ -1159,8 +1163,8 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
             s->opt_remap = value;
             break;
         case RTPS_OPT_SORT_TILE:
-            if (value != 4 && value != 8 && value != 16)
-                return set_error(RTPS_ERR_ARG, "sort tile must be 4, 8 or 16 keys per thread");
+            if (value != 0 && value != 4 && value != 8 && value != 16)
+                return set_error(RTPS_ERR_ARG, "sort tile must be 0 (by size), 4, 8 or 16");
             g_sort_items = value;
             break;
         case RTPS_OPT_COLOR:

@@ -261,8 +261,8 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        rtps_sph_autotune), 2 = over all four loops (2 and 3
  *                        win while the neighbourhoods are sparse, lose once
  *                        they fill), 0 = fixed defaults
- *   RTPS_OPT_SORT_TILE   keys per thread of a radix-sort tile (256 threads),
- *                        process-wide: 4, 8 or 16 (default 8)
+ *   RTPS_OPT_SORT_TILE   radix-sort tiles of 256 x value keys, process-wide:
+ *                        4, 8 or 16, or 0 (default) = 16 from 2^20 keys up, else 8
  *   RTPS_OPT_COLOR       1 = permute colours every step and renumber them with
  *                        the survivors of a deletion (permute.cl:41,
  *                        leapfrog.cl:107); 0 = colours stay where pushed.

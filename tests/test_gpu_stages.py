@@ -89,7 +89,7 @@ def test_hash_pad_keys(cuda):
 
 
 @pytest.mark.parametrize("n,bits", [(0, 15), (1, 15), (2, 1), (4095, 8), (4097, 15), (16384, 15), (70001, 20),
-                                    (1 << 20, 20), (333333, 24), (5000, 32)])
+                                    (1 << 20, 20), ((1 << 20) + 4097, 24), (333333, 24), (5000, 32)])
 def test_sort_matches_stable_sort(cuda, n, bits):
     from enjaparticles_amd import ops
     rng = np.random.default_rng(n + bits)

@@ -263,7 +263,7 @@ def test_sort_tile_sizes_give_the_same_step(cuda):
         out[items] = (s.read("pos"), s.read("keys"), s.read("idx"))
         s.close()
     s = _system(sc)
-    s.setOption(8, 8)           # process-wide: restore the default
+    s.setOption(8, 0)           # process-wide: restore the default (by size)
     s.close()
     for items in (16, 4):
         for a, b in zip(out[items], out[8]):
