@@ -30,6 +30,7 @@ void read_sph_params(const RTPSSettings& s, rtps_sph_params& sp);
 int sort_key_bits(int nb_cells);
 
 // ---- kernels (sph_kernels.hip) ----
+extern int g_density_stage;   // 1 = density rows staged through LDS (default), 0 = read from global memory
 extern int g_force_variant;   // 0 = direct branch per candidate, 1 = compacted lists, 2 = recorded masks
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
                        uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, int sub_bits,

@@ -121,9 +121,17 @@ __global__ __launch_bounds__(kBlock) void k_permute(int num_host, const int* __r
 constexpr int kRecNone = 0, kRecBits = 1, kRecList = 2;
 // candidates per density group (a divisor of 32: the bit words fill in whole groups)
 constexpr int kGroupDensity = 4;
+// STAGE (REC == kRecNone): each wave copies the union of its lanes' windows
+// of a row (when at most kStageCap candidates) into its own LDS slice with
+// full-line loads, and the lanes read their candidates from there; wider
+// unions (waves straddling two rows of cells) read global memory as before.
+// Same candidates in the same order: the same sums.
+constexpr int kStageCap = 256;
+int g_density_stage = 1;
+
 // Nine waves per SIMD (at most 56 VGPRs, the occupancy of the search-based
 // windows; the table windows would otherwise take 57 and leave eight).
-template <bool SCALED, bool PACKED, int REC>
+template <bool SCALED, bool PACKED, int REC, bool STAGE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k_density(int num_host, const int* __restrict__ d_num,
                                                     const float4* __restrict__ pos_s,
                                                     const uint32_t* __restrict__ start,
@@ -165,6 +173,52 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
         if (k == 3u) masks.codes[(size_t)(cnt >> 2) * st + i] = make_uint2(lo, hi);
         ++cnt;
     };
+    if constexpr (STAGE) {
+        extern __shared__ float4 stage_rows[];   // kStageCap per wave
+        float4* __restrict__ my = stage_rows + (threadIdx.x >> 6) * kStageCap;
+        // the wave's live lanes share the copies (lanes past num have returned)
+        const uint64_t live = __ballot(true);
+        const uint32_t nlive = (uint32_t)__popcll(live);
+        const uint32_t rank = (uint32_t)__popcll(live & ((1ull << (threadIdx.x & 63)) - 1ull));
+        for (int dz = -1; dz <= 1; ++dz) {
+            for (int dy = -1; dy <= 1; ++dy) {
+                uint32_t a, b;
+                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
+                const uint32_t A = __ockl_wfred_min_u32(a < b ? a : 0xFFFFFFFFu);
+                const uint32_t B = __ockl_wfred_max_u32(a < b ? b : 0u);
+                if (B <= A) continue;
+                if (B - A <= (uint32_t)kStageCap) {
+                    for (uint32_t t = rank; t < B - A; t += nlive) my[t] = pos_s[A + t];
+                    __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
+                    uint32_t j = a;
+                    for (; j + kGroupDensity <= b; j += kGroupDensity) {
+                        const float4* __restrict__ pg = my + (j - A);
+                        float4 p[kGroupDensity];
+#pragma unroll
+                        for (int k = 0; k < kGroupDensity; ++k) p[k] = pg[k];
+#pragma unroll
+                        for (int k = 0; k < kGroupDensity; ++k) add_term(sep2(p[k]));
+                    }
+                    for (; j < b; ++j) add_term(sep2(my[j - A]));
+                    __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
+                } else {
+                    uint32_t j = a;
+                    for (; j + kGroupDensity <= b; j += kGroupDensity) {
+                        const float4* __restrict__ pg = pos_s + j;
+                        float4 p[kGroupDensity];
+#pragma unroll
+                        for (int k = 0; k < kGroupDensity; ++k) p[k] = pg[k];
+#pragma unroll
+                        for (int k = 0; k < kGroupDensity; ++k) add_term(sep2(p[k]));
+                    }
+                    for (; j < b; ++j) add_term(sep2(pos_s[j]));
+                }
+            }
+        }
+        const float rho = sp.wpoly6_coef * sp.mass * sum;
+        if (PACKED) packed[i].w = rho; else dens[i] = rho;
+        return;
+    }
     int r = 0;
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy, ++r) {
@@ -700,13 +754,17 @@ hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const 
     const int bs = nbr_block(win);
     const int nb = blocks_for(num, bs);
     const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
-#define RTPS_DENS(S, P, M) k_density<S, P, M><<<nb, bs, 0, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed, m)
-#define RTPS_DENS_R(R)                                                                                              \
-    if (packed) { if (scaled) RTPS_DENS(true, true, R); else RTPS_DENS(false, true, R); }                             \
-    else { if (scaled) RTPS_DENS(true, false, R); else RTPS_DENS(false, false, R); }
-    if (m.rows && m.form == kRecList) { RTPS_DENS_R(kRecList) }
-    else if (m.rows && m.form == kRecBits) { RTPS_DENS_R(kRecBits) }
-    else { RTPS_DENS_R(kRecNone) }
+    const bool stage = g_density_stage != 0 && !m.rows;
+    const size_t lds = stage ? (size_t)(bs / 64) * kStageCap * sizeof(float4) : 0;
+#define RTPS_DENS(S, P, M, T)                                                                                       \
+    k_density<S, P, M, T><<<nb, bs, lds, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed, m)
+#define RTPS_DENS_R(R, T)                                                                                           \
+    if (packed) { if (scaled) RTPS_DENS(true, true, R, T); else RTPS_DENS(false, true, R, T); }                       \
+    else { if (scaled) RTPS_DENS(true, false, R, T); else RTPS_DENS(false, false, R, T); }
+    if (m.rows && m.form == kRecList) { RTPS_DENS_R(kRecList, false) }
+    else if (m.rows && m.form == kRecBits) { RTPS_DENS_R(kRecBits, false) }
+    else if (stage) { RTPS_DENS_R(kRecNone, true) }
+    else { RTPS_DENS_R(kRecNone, false) }
 #undef RTPS_DENS_R
 #undef RTPS_DENS
     return hipGetLastError();

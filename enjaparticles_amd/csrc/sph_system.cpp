@@ -1167,6 +1167,10 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
                 return set_error(RTPS_ERR_ARG, "sort tile must be 0 (by size), 4, 8 or 16");
             g_sort_items = value;
             break;
+        case RTPS_OPT_DENSITY_STAGE:
+            if (value < 0 || value > 1) return set_error(RTPS_ERR_ARG, "density stage option must be 0 or 1");
+            g_density_stage = value;
+            break;
         case RTPS_OPT_COLOR:
             if (value < 0 || value > 1) return set_error(RTPS_ERR_ARG, "colour option must be 0 or 1");
             s->opt_color = value;

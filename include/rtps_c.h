@@ -263,6 +263,10 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        they fill), 0 = fixed defaults
  *   RTPS_OPT_SORT_TILE   radix-sort tiles of 256 x value keys, process-wide:
  *                        4, 8 or 16, or 0 (default) = 16 from 2^20 keys up, else 8
+ *   RTPS_OPT_DENSITY_STAGE  1 = the density kernel copies each row's candidates
+ *                        once per wave into LDS (default), 0 = every lane reads
+ *                        its candidates from global memory; process-wide, the
+ *                        same densities either way
  *   RTPS_OPT_COLOR       1 = permute colours every step and renumber them with
  *                        the survivors of a deletion (permute.cl:41,
  *                        leapfrog.cl:107); 0 = colours stay where pushed.
@@ -282,6 +286,7 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
 #define RTPS_OPT_AUTOTUNE 7
 #define RTPS_OPT_SORT_TILE 8
 #define RTPS_OPT_COLOR 9
+#define RTPS_OPT_DENSITY_STAGE 10
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))

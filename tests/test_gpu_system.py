@@ -270,6 +270,32 @@ def test_sort_tile_sizes_give_the_same_step(cuda):
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("scene", ["dam", "clump"])
+def test_density_stage_gives_the_same_step(cuda, scene):
+    """Density rows copied once per wave into LDS (RTPS_OPT_DENSITY_STAGE 1)
+    or read by every lane from global memory (0): the same candidates in the
+    same order, so the steps are bit-identical.  The clump's rows hold more
+    candidates than one LDS copy, so those waves take the global path."""
+    sc = scenes.dam_custom(32, 32, 16, seed=83)
+    if scene == "clump":
+        rng = np.random.default_rng(84)
+        pos = sc.pos.copy()
+        pos[:600, :3] = 0.3 + 0.012 * rng.random((600, 3), dtype=np.float32)
+        sc = scenes.Scene("clump", sc.settings, pos, sc.vel, 2, sc.dt)
+    out = {}
+    for stage in (1, 0):
+        s = _system(sc)
+        s.setOption(10, stage)
+        s.step(3)
+        out[stage] = (s.read("pos"), s.read("vel"), s.read("density"))
+        s.close()
+    s = _system(sc)
+    s.setOption(10, 1)          # process-wide: restore the default
+    s.close()
+    for a, b in zip(out[1], out[0]):
+        np.testing.assert_array_equal(a, b)
+
+
 def _append(st, pos, vel):
     """Oracle state with particles pushed after the live ones (veleval 0)."""
     n = st.num
