@@ -383,7 +383,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-single-steps", type=int, default=2)
     ap.add_argument("--profile", action="store_true", help="timed loop only (for rocprofv3 runs)")
-    ap.add_argument("--force-variant", type=int, default=0, help="1 = compacted neighbour lists, 0 = direct")
+    ap.add_argument("--force-variant", type=int, default=4,
+                    help="process-default force loop before autotuning: 4 = direct with LDS-staged rows, "
+                         "0 = direct, 1 = compacted neighbour lists")
     ap.add_argument("--triangles", type=int, default=2048,
                     help="collision triangles of the triangle leg on the final state (0 = skip)")
     ap.add_argument("--surface-res", type=int, default=128,

@@ -303,7 +303,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
 // through its own list, so a wave costs about its largest neighbour count
 // rather than, row by row, the largest per-row count (the row-by-row walk of
 // kLoopMasked keeps only ~40% of the lanes busy on a dam-break state).
-constexpr int kLoopDirect = 0, kLoopCompact = 1, kLoopMasked = 2, kLoopList = 3;
+//
+// kLoopStaged: the direct loop, with each row's candidates (positions and
+// veleval|density) copied once per wave into LDS as for the density kernel's
+// STAGE (at most kForceStageCap candidates; wider unions read global memory).
+constexpr int kLoopDirect = 0, kLoopCompact = 1, kLoopMasked = 2, kLoopList = 3, kLoopStaged = 4;
+constexpr int kForceStageCap = 192;   // 6 KB per wave: 24 waves per CU fit the 160 KB of LDS
 constexpr int kNbrCap = 32;
 
 // At least six waves per SIMD: the compiler keeps the force kernel within 80
@@ -498,6 +503,76 @@ __global__ __launch_bounds__(kBlock, kForceWavesPerSimd) void k_force(int num_ho
                     float rx, ry, rz;
                     float r2 = sep(j, rx, ry, rz);
                     if (r2 <= h2 && j != (uint32_t)i) pair(j, rx, ry, rz, r2);
+                }
+            }
+        }
+    }
+    if constexpr (LOOP == kLoopStaged) {
+        const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
+        const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
+        extern __shared__ float4 force_rows[];   // 2 x kForceStageCap per wave: positions, then veleval|density
+        float4* __restrict__ my_p = force_rows + (threadIdx.x >> 6) * (2 * kForceStageCap);
+        float4* __restrict__ my_v = my_p + kForceStageCap;
+        const uint64_t live = __ballot(true);   // lanes past num have returned
+        const uint32_t nlive = (uint32_t)__popcll(live);
+        const uint32_t rank = (uint32_t)__popcll(live & ((1ull << (threadIdx.x & 63)) - 1ull));
+        // candidates [a, b) of one row, candidate j at P[j - off] / V[j - off]
+        auto walk = [&](const float4* __restrict__ P, const float4* __restrict__ V, uint32_t a, uint32_t b,
+                        uint32_t off) {
+            uint32_t j = a;
+            for (; j + kGroup <= b; j += kGroup) {
+                const float4* __restrict__ pg = P + (j - off);
+                const float4* __restrict__ vg = V + (j - off);
+                float4 q[kGroup], v[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) q[k] = pg[k];
+                float rx[kGroup], ry[kGroup], rz[kGroup], r2[kGroup];
+                bool ok[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) {
+                    float4 pj = q[k];
+                    if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                    rx[k] = pi.x - pj.x; ry[k] = pi.y - pj.y; rz[k] = pi.z - pj.z;
+                    r2[k] = rx[k] * rx[k] + ry[k] * ry[k] + rz[k] * rz[k];
+                    ok[k] = r2[k] <= h2 && j + k != (uint32_t)i;
+                }
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (ok[k]) v[k] = vg[k];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (ok[k])
+                        force_pair(acc, rx[k], ry[k], rz[k], r2[k], h, h2, eps, K, rho0, Pi, di, vi, v[k],
+                                   PACKED ? v[k].w : dens_in[j + k]);
+            }
+            for (; j < b; ++j) {
+                float4 pj = P[j - off];
+                if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 <= h2 && j != (uint32_t)i) {
+                    const float4 vj = V[j - off];
+                    force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, PACKED ? vj.w : dens_in[j]);
+                }
+            }
+        };
+        for (int dz = -1; dz <= 1; ++dz) {
+            for (int dy = -1; dy <= 1; ++dy) {
+                uint32_t a, b;
+                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
+                const uint32_t A = __ockl_wfred_min_u32(a < b ? a : 0xFFFFFFFFu);
+                const uint32_t B = __ockl_wfred_max_u32(a < b ? b : 0u);
+                if (B <= A) continue;
+                if (B - A <= (uint32_t)kForceStageCap) {
+                    for (uint32_t t = rank; t < B - A; t += nlive) {
+                        my_p[t] = pos_s[A + t];
+                        my_v[t] = ve_in[A + t];
+                    }
+                    __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
+                    walk(my_p, my_v, a, b, A);
+                    __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
+                } else {
+                    walk(pos_s, ve_in, a, b, 0u);
                 }
             }
         }
@@ -697,15 +772,17 @@ __global__ void k_commit_num(int* __restrict__ d_num, const uint32_t* __restrict
 // ---------------------------------------------------------------------------
 // Launchers shared by the C ABI and the system.
 
-// Force-loop variant: 0 = one branch per candidate, 1 = compacted LDS lists.
-int g_force_variant = 0;
+// Force-loop variant: 0 = one branch per candidate, 1 = compacted LDS lists,
+// 2 / 3 = recorded rows (bits / lists), 4 = 0 with the rows staged through LDS
+// (the default: 21% faster than 0 on the 1M dam).
+int g_force_variant = kLoopStaged;
 
 // Threads per block of a neighbour launch (a multiple of 64, at most kBlock)
 // and its force-loop variant.
 static inline int nbr_block(const Win& w) { return w.block > 0 && w.block <= kBlock && w.block % 64 == 0 ? w.block : kBlock; }
 static inline int nbr_variant(const Win& w) {
     const int v = w.variant >= 0 ? w.variant : g_force_variant;
-    return v >= kLoopDirect && v <= kLoopList ? v : kLoopDirect;
+    return v >= kLoopDirect && v <= kLoopStaged ? v : kLoopDirect;
 }
 
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
@@ -790,7 +867,8 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
     const int nb = blocks_for(num, bs);
     const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
     const int loop = loop_of(win, m);
-    const size_t lds = loop == kLoopCompact ? (size_t)kNbrCap * bs * sizeof(uint32_t) : 0;
+    const size_t lds = loop == kLoopCompact ? (size_t)kNbrCap * bs * sizeof(uint32_t)
+                       : (loop == kLoopStaged ? (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4) : 0);
     GravitySources none{};
     rtps_float4 z{0, 0, 0, 0};
 #define RTPS_FORCE_L(S, P, L)                                                                                       \
@@ -799,6 +877,7 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
                                                nullptr, nullptr)
 #define RTPS_FORCE(S, P)                                                                                            \
     if (loop == kLoopList) RTPS_FORCE_L(S, P, kLoopList);                                                       \
+    else if (loop == kLoopStaged) RTPS_FORCE_L(S, P, kLoopStaged);                                                  \
     else if (loop == kLoopMasked) RTPS_FORCE_L(S, P, kLoopMasked);                                                  \
     else if (loop == kLoopCompact) RTPS_FORCE_L(S, P, kLoopCompact);                                                \
     else RTPS_FORCE_L(S, P, kLoopDirect)
@@ -822,13 +901,15 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
     const int nb = blocks_for(num, bs);
     const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
     const int loop = loop_of(win, m);
-    const size_t lds = loop == kLoopCompact ? (size_t)kNbrCap * bs * sizeof(uint32_t) : 0;
+    const size_t lds = loop == kLoopCompact ? (size_t)kNbrCap * bs * sizeof(uint32_t)
+                       : (loop == kLoopStaged ? (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4) : 0);
 #define RTPS_FFUSED_L(S, L)                                                                                         \
     k_force<S, true, true, L><<<nb, bs, lds, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
                                                  xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt,  \
                                                  pos_u, vel_u, veleval_u, m, color_s, color_u)
 #define RTPS_FFUSED(S)                                                                                              \
     if (loop == kLoopList) RTPS_FFUSED_L(S, kLoopList);                                                         \
+    else if (loop == kLoopStaged) RTPS_FFUSED_L(S, kLoopStaged);                                                    \
     else if (loop == kLoopMasked) RTPS_FFUSED_L(S, kLoopMasked);                                                    \
     else if (loop == kLoopCompact) RTPS_FFUSED_L(S, kLoopCompact);                                                  \
     else RTPS_FFUSED_L(S, kLoopDirect)

@@ -530,7 +530,8 @@ int autotune_step(rtps_sph* s, bool measure) {
     // density: plain, recording bits, recording lists (each left recorded for the force loops timed next);
     // the recorded forms (force loops 2 and 3) are candidates only with RTPS_OPT_AUTOTUNE = 2
     const bool full = s->opt_autotune == 2;
-    const int nmodes = full ? 3 : 1, nvars = full ? 4 : 2;
+    const int nmodes = full ? 3 : 1, nvars = full ? 5 : 3;
+    const int vars[] = {0, 4, 1, 2, 3};   // loops in the order they are tried
     float best_d[3] = {1e30f, 1e30f, 1e30f};
     int bd[3] = {256, 256, 256};
     bool failed = false;
@@ -550,8 +551,9 @@ int autotune_step(rtps_sph* s, bool measure) {
     const bool fo = s->opt_force_out != 0;
     float best_total = 1e30f;
     Tune best{256, 256, 0};
-    for (int var = 0; var < nvars && !failed; ++var)
+    for (int vi = 0; vi < nvars && !failed; ++vi)
         for (int bs : blocks) {
+            const int var = vars[vi];
             Win w = win(s, true);
             w.block = bs;
             w.variant = var;
@@ -559,9 +561,9 @@ int autotune_step(rtps_sph* s, bool measure) {
                 return launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, w,
                                           fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
                                           s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u,
-                                          s->vel_u, s->veleval_u, s->stream, recs[var >= 2 ? var - 1 : 0]);
+                                          s->vel_u, s->veleval_u, s->stream, recs[var == 2 || var == 3 ? var - 1 : 0]);
             });
-            const int mode = var >= 2 ? var - 1 : 0;
+            const int mode = var == 2 || var == 3 ? var - 1 : 0;
             if (verbose) std::fprintf(stderr, "rtps autotune: force loop %d block %d: %.4f ms\n", var, bs, ms);
             if (ms < 0.f) failed = true;
             else if (best_total >= 1e30f || ms + best_d[mode] < kMargin * best_total) {
@@ -1151,7 +1153,7 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_FORCE_OUT: s->opt_force_out = value ? 1 : 0; break;
         case RTPS_OPT_TIMERS: s->opt_timers = value ? 1 : 0; break;
         case RTPS_OPT_FORCE_VARIANT:
-            if (value < 0 || value > 3) return set_error(RTPS_ERR_ARG, "force variant must be 0, 1, 2 or 3");
+            if (value < 0 || value > 4) return set_error(RTPS_ERR_ARG, "force variant must be 0 to 4");
             g_force_variant = value;
             break;
         case RTPS_OPT_WINDOW:

@@ -245,19 +245,21 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        step k-1 runs and rtps_sph_step returns after the
  *                        last step finished
  *   RTPS_OPT_FORCE_VARIANT  force-loop kernel, process-wide default for handles
- *                        that are not autotuned: 0 = one branch per candidate
- *                        (default), 1 = compacted neighbour lists in LDS, 2 = the
+ *                        that are not autotuned: 0 = one branch per candidate,
+ *                        1 = compacted neighbour lists in LDS, 2 = the
  *                        density kernel records each row's window and accepted
  *                        candidates as bits and the force kernel walks those
  *                        row by row, 3 = the density kernel records a list of
- *                        16-bit neighbour codes and each lane walks its own list
+ *                        16-bit neighbour codes and each lane walks its own list,
+ *                        4 = 0 with each row's candidates copied once per wave
+ *                        into LDS (default)
  *   RTPS_OPT_WINDOW      1 = scan only the x window of each neighbour row, its
  *                        bounds looked up in a table of sub-cell starts that
  *                        cellindices writes (default; 64 bytes per grid cell),
  *                        2 = the same window found by binary search of the
  *                        sorted keys, 0 = whole rows
  *   RTPS_OPT_AUTOTUNE    1 = the first rtps_sph_step tunes the neighbour launches
- *                        over the force loops 0 and 1 (default; see
+ *                        over the force loops 0, 4 and 1 (default; see
  *                        rtps_sph_autotune), 2 = over all four loops (2 and 3
  *                        win while the neighbourhoods are sparse, lose once
  *                        they fill), 0 = fixed defaults
@@ -291,9 +293,10 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))
  * during which the threads per block of the density and force launches (64,
- * 128 or 256) and the force-loop variant (0 and 1, or 0 to 3 with
+ * 128 or 256) and the force-loop variant (0, 4 and 1, or 0 to 4 with
  * RTPS_OPT_AUTOTUNE = 2; variants 2 and 3 with the density kernel recording
- * rows) are timed on the live state and the fastest density + force pair kept.  The two kernels are idempotent on one sorted state, so the
+ * rows) are timed on the live state and the fastest density + force pair kept
+ * (a candidate tried later must be 2% faster).  The two kernels are idempotent on one sorted state, so the
  * step's result does not depend on the choice.  Choices are cached per
  * process by device and size class (and in the file named by the
  * RTPS_TUNE_CACHE environment variable, when set); measure = 0 takes a cached

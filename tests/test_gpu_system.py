@@ -205,7 +205,7 @@ def test_autotune_choice_does_not_change_the_step(cuda):
     b = _system(sc)
     tune = b.autotune(measure=True)
     assert tune["block_density"] in (64, 128, 256) and tune["block_force"] in (64, 128, 256)
-    assert tune["force_variant"] in (0, 1)
+    assert tune["force_variant"] in (0, 1, 4)
     a.step(1)
     np.testing.assert_array_equal(b.read("pos"), a.read("pos"))
     a.step(4)
@@ -213,9 +213,9 @@ def test_autotune_choice_does_not_change_the_step(cuda):
     np.testing.assert_array_equal(b.read("pos"), a.read("pos"))
     np.testing.assert_array_equal(b.read("vel"), a.read("vel"))
     c = _system(sc)
-    c.setOption(7, 2)          # all four force loops are candidates
+    c.setOption(7, 2)          # all five force loops are candidates
     tune = c.autotune(measure=True)
-    assert tune["force_variant"] in (0, 1, 2, 3)
+    assert tune["force_variant"] in (0, 1, 2, 3, 4)
     c.step(4)
     np.testing.assert_allclose(c.read("pos"), a.read("pos"), rtol=1e-6, atol=1e-7)
     for s in (a, b, c):
@@ -224,11 +224,13 @@ def test_autotune_choice_does_not_change_the_step(cuda):
 
 @pytest.mark.parametrize("scene", ["dam", "clump"])
 def test_force_loop_variants_agree(cuda, scene):
-    """The four force loops (direct, compacted LDS lists, bits recorded by the
-    density kernel, neighbour-code lists recorded by the density kernel) visit
-    the same pairs in the same order.  The clump packs 300 particles into one
-    cell, so its rows hold more candidates than the recorded bits cover and
-    its particles overflow the recorded lists: both fallbacks are exercised."""
+    """The five force loops (direct, compacted LDS lists, bits recorded by the
+    density kernel, neighbour-code lists recorded by the density kernel,
+    direct with the rows staged through LDS) visit the same pairs in the same
+    order; the staged loop gives the direct loop's bits.  The clump packs 300
+    particles into one cell, so its rows hold more candidates than the
+    recorded bits cover, its particles overflow the recorded lists and its
+    rows exceed one LDS copy: the fallbacks are exercised."""
     sc = scenes.dam_custom(20, 20, 10, seed=80)
     if scene == "clump":
         rng = np.random.default_rng(81)
@@ -236,7 +238,7 @@ def test_force_loop_variants_agree(cuda, scene):
         pos[:300, :3] = 0.3 + 0.012 * rng.random((300, 3), dtype=np.float32)
         sc = scenes.Scene("clump", sc.settings, pos, sc.vel, 2, sc.dt)
     out = []
-    for v in (0, 1, 2, 3):
+    for v in (0, 1, 2, 3, 4):
         s = _system(sc)
         s.setOption(7, 0)      # autotune off: the process-wide variant applies
         s.setOption(4, v)
@@ -249,6 +251,8 @@ def test_force_loop_variants_agree(cuda, scene):
     for v in (1, 2, 3):
         np.testing.assert_allclose(out[v][0], out[0][0], rtol=1e-6, atol=1e-7)
         np.testing.assert_allclose(out[v][1], out[0][1], rtol=1e-5, atol=1e-6)
+    np.testing.assert_array_equal(out[4][0], out[0][0])
+    np.testing.assert_array_equal(out[4][1], out[0][1])
 
 
 def test_sort_tile_sizes_give_the_same_step(cuda):
