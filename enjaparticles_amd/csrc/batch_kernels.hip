@@ -20,6 +20,9 @@
 namespace rtps {
 
 constexpr int kBBlock = 256;
+// Candidates per LDS copy of a row in the pair kernels (force: positions and
+// veleval|density; density: positions, twice as many).
+constexpr int kBStageCap = 192;
 constexpr uint32_t kLocalMask = (1u << kSceneShift) - 1u;
 
 static inline int bblocks(long n) { return (int)((n + kBBlock - 1) / kBBlock); }
@@ -111,7 +114,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                                           const uint32_t* __restrict__ idx, float4* __restrict__ pos_u,
                                           float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
                                           float4* __restrict__ force_out, float4* __restrict__ xsph_out,
-                                          const uint32_t* __restrict__ sub_all) {
+                                          const uint32_t* __restrict__ sub_all, float4* __restrict__ stage) {
     const rtps_sph_params& sp = S.sp;
     const uint32_t* sub = sub_all ? sub_all + (size_t)S.cell_base * kSubCells : nullptr;
     const float scale = sp.simulation_scale;
@@ -127,25 +130,51 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
     const uint32_t* st = start + S.cell_base;
     const uint32_t* en = end + S.cell_base;
     const uint32_t prefix = (uint32_t)s << kSceneShift;
+    // Each wave copies the union of its lanes' windows of a row (when at most
+    // kBStageCap candidates) into its LDS slice `stage` with full-line loads
+    // and the lanes read their candidates from there; wider unions read
+    // global memory.  Same candidates in the same order either way.
+    const uint64_t live = __ballot(true);
+    const uint32_t nlive = (uint32_t)__popcll(live);
+    const uint32_t rank = (uint32_t)__popcll(live & ((1ull << (threadIdx.x & 63)) - 1ull));
+    // the row's wave-wide candidate range [A, B) (A >= B: no lane has candidates)
+    auto wave_range = [&](uint32_t a, uint32_t b, uint32_t& A, uint32_t& B) {
+        A = __ockl_wfred_min_u32(a < b ? a : 0xFFFFFFFFu);
+        B = __ockl_wfred_max_u32(a < b ? b : 0u);
+    };
     if (MODE == kPairsDensity) {
         float sum = 0.0f;
+        auto term = [&](float4 pj) {
+            float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
+            poly6_add(sum, rx * rx + ry * ry + rz * rz, h2);
+        };
+        // candidate j at P[j - off]
+        auto walk = [&](const float4* __restrict__ P, uint32_t a, uint32_t b, uint32_t off) {
+            uint32_t j = a;
+            for (; j + kGroup <= b; j += kGroup) {   // loads of a group issued together
+                const float4* __restrict__ pg = P + (j - off);
+                float4 p[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) p[k] = pg[k];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) term(p[k]);
+            }
+            for (; j < b; ++j) term(P[j - off]);
+        };
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy) {
-                uint32_t a, b;
+                uint32_t a, b, A, B;
                 row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix, sub);
-                auto term = [&](float4 pj) {
-                    float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
-                    poly6_add(sum, rx * rx + ry * ry + rz * rz, h2);
-                };
-                uint32_t j = a;
-                for (; j + kGroup <= b; j += kGroup) {   // loads of a group issued together
-                    float4 p[kGroup];
-#pragma unroll
-                    for (int k = 0; k < kGroup; ++k) p[k] = pos_s[j + k];
-#pragma unroll
-                    for (int k = 0; k < kGroup; ++k) term(p[k]);
+                wave_range(a, b, A, B);
+                if (B <= A) continue;
+                if (B - A <= (uint32_t)(2 * kBStageCap)) {   // density uses the whole slice
+                    for (uint32_t t = rank; t < B - A; t += nlive) stage[t] = pos_s[A + t];
+                    __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
+                    walk(stage, a, b, A);
+                    __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
+                } else {
+                    walk(pos_s, a, b, 0u);
                 }
-                for (; j < b; ++j) term(pos_s[j]);
             }
         ve[i].w = sp.wpoly6_coef * sp.mass * sum;
     } else {
@@ -154,40 +183,60 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
         const float di = vi.w;
         const float Pi = K * (di - rho0);
         ForceAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float4* __restrict__ stage_v = stage + kBStageCap;
+        // candidate j at P[j - off] / V[j - off]
+        auto walk = [&](const float4* __restrict__ P, const float4* __restrict__ V, uint32_t a, uint32_t b,
+                        uint32_t off) {
+            uint32_t j = a;
+            for (; j + kGroup <= b; j += kGroup) {   // positions, then accepted velocities, loaded together
+                const float4* __restrict__ pg = P + (j - off);
+                const float4* __restrict__ vg = V + (j - off);
+                float4 p[kGroup], v[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) p[k] = pg[k];
+                float rx[kGroup], ry[kGroup], rz[kGroup], r2[kGroup];
+                bool ok[kGroup];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k) {
+                    rx[k] = pi.x - p[k].x * scale;
+                    ry[k] = pi.y - p[k].y * scale;
+                    rz[k] = pi.z - p[k].z * scale;
+                    r2[k] = rx[k] * rx[k] + ry[k] * ry[k] + rz[k] * rz[k];
+                    ok[k] = r2[k] <= h2 && j + k != (uint32_t)i;
+                }
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (ok[k]) v[k] = vg[k];
+#pragma unroll
+                for (int k = 0; k < kGroup; ++k)
+                    if (ok[k]) force_pair(acc, rx[k], ry[k], rz[k], r2[k], h, h2, eps, K, rho0, Pi, di, vi, v[k], v[k].w);
+            }
+            for (; j < b; ++j) {
+                float4 pj = P[j - off];
+                float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
+                float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 <= h2 && j != (uint32_t)i) {
+                    const float4 vj = V[j - off];
+                    force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, vj.w);
+                }
+            }
+        };
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy) {
-                uint32_t a, b;
+                uint32_t a, b, A, B;
                 row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix, sub);
-                uint32_t j = a;
-                for (; j + kGroup <= b; j += kGroup) {   // positions, then accepted velocities, loaded together
-                    float4 p[kGroup], v[kGroup];
-#pragma unroll
-                    for (int k = 0; k < kGroup; ++k) p[k] = pos_s[j + k];
-                    float rx[kGroup], ry[kGroup], rz[kGroup], r2[kGroup];
-                    bool ok[kGroup];
-#pragma unroll
-                    for (int k = 0; k < kGroup; ++k) {
-                        rx[k] = pi.x - p[k].x * scale;
-                        ry[k] = pi.y - p[k].y * scale;
-                        rz[k] = pi.z - p[k].z * scale;
-                        r2[k] = rx[k] * rx[k] + ry[k] * ry[k] + rz[k] * rz[k];
-                        ok[k] = r2[k] <= h2 && j + k != (uint32_t)i;
+                wave_range(a, b, A, B);
+                if (B <= A) continue;
+                if (B - A <= (uint32_t)kBStageCap) {
+                    for (uint32_t t = rank; t < B - A; t += nlive) {
+                        stage[t] = pos_s[A + t];
+                        stage_v[t] = ve[A + t];
                     }
-#pragma unroll
-                    for (int k = 0; k < kGroup; ++k)
-                        if (ok[k]) v[k] = ve[j + k];
-#pragma unroll
-                    for (int k = 0; k < kGroup; ++k)
-                        if (ok[k]) force_pair(acc, rx[k], ry[k], rz[k], r2[k], h, h2, eps, K, rho0, Pi, di, vi, v[k], v[k].w);
-                }
-                for (; j < b; ++j) {
-                    float4 pj = pos_s[j];
-                    float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
-                    float r2 = rx * rx + ry * ry + rz * rz;
-                    if (r2 <= h2 && j != (uint32_t)i) {
-                        const float4 vj = ve[j];
-                        force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, vj.w);
-                    }
+                    __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
+                    walk(stage, stage_v, a, b, A);
+                    __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
+                } else {
+                    walk(pos_s, ve, a, b, 0u);
                 }
             }
         float4 F, X;
@@ -226,6 +275,7 @@ __global__ __launch_bounds__(kBBlock, kPersistentBlocksPerCU) void k_pairs_b(con
                                                      float4* __restrict__ force_out, float4* __restrict__ xsph_out,
                                                      const uint32_t* __restrict__ sub) {
     __shared__ int sh_item;
+    __shared__ float4 stage_b[kBBlock / 64][2 * kBStageCap];   // one slice per wave (24 KB per block)
     const int q0 = (int)(blockIdx.x % kXcdQueues);
     int q = q0;   // thread 0's current queue
     for (;;) {
@@ -256,7 +306,7 @@ __global__ __launch_bounds__(kBBlock, kPersistentBlocksPerCU) void k_pairs_b(con
         const int i = I.first + (int)threadIdx.x;
         if ((int)threadIdx.x < I.count && i - S.off < num)
             pairs_one<MODE>(i, s, S, deleted, pos_s, start, end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
-                            force_out, xsph_out, sub);
+                            force_out, xsph_out, sub, stage_b[threadIdx.x >> 6]);
     }
 }
 
