@@ -42,9 +42,11 @@ __global__ __launch_bounds__(kABlock) void k_adj_b2(int num, const float4* __res
                                                     rtps_sph_params sp, Grid g, float4* __restrict__ gx2_s,
                                                     float4* __restrict__ ge_s, float* __restrict__ grho_s,
                                                     float2* __restrict__ partial) {
+    __shared__ float4 stage[kABlock / 64][kAdjStageSlice];   // one slice per wave
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float gmu = 0.f, gK = 0.f;
-    if (i < num) adj_b2_body(i, pos_s, ve_s, ga_s, gX_s, rows, sp, g, gx2_s, ge_s, grho_s, gmu, gK);
+    if (i < num)
+        adj_b2_body(i, pos_s, ve_s, ga_s, gX_s, rows, sp, g, gx2_s, ge_s, grho_s, gmu, gK, stage[threadIdx.x >> 6]);
     adj_block_partial<kABlock>(gmu, gK, partial, blockIdx.x);
 }
 
@@ -56,9 +58,10 @@ __global__ __launch_bounds__(kABlock) void k_adj_b3(int num, const uint32_t* __r
                                                     const float4* __restrict__ ge_s, AdjRows rows,
                                                     rtps_sph_params sp, Grid g, float4* __restrict__ lam_x_out,
                                                     float4* __restrict__ lam_e_out) {
+    __shared__ float4 stage[kABlock / 64][kAdjStageSlice];   // one slice per wave
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= num) return;
-    adj_b3_body(i, idx, pos_s, grho_s, lxp_s, gx2_s, ge_s, rows, sp, g, lam_x_out, lam_e_out);
+    adj_b3_body(i, idx, pos_s, grho_s, lxp_s, gx2_s, ge_s, rows, sp, g, lam_x_out, lam_e_out, stage[threadIdx.x >> 6]);
 }
 
 // Walls + leapfrog of a recorded step.  Flags particles that left the grid

@@ -41,6 +41,30 @@ struct AdjRows {
     uint32_t prefix;
 };
 
+// Candidates per LDS copy of a row in B2 (4 records each) and B3 (a position
+// and a float each): 6 KB and 5 KB per wave.
+constexpr int kAdjStageCap = 96;
+constexpr int kAdjB3StageCap = 256;
+// LDS slice of the calling wave (float4 units): B2 needs 4 x kAdjStageCap,
+// B3 kAdjB3StageCap + kAdjB3StageCap / 4.
+constexpr int kAdjStageSlice = 4 * kAdjStageCap > kAdjB3StageCap + kAdjB3StageCap / 4
+                                   ? 4 * kAdjStageCap : kAdjB3StageCap + kAdjB3StageCap / 4;
+
+// The wave's live lanes and this lane's rank among them; the union [A, B) of
+// the live lanes' row windows (A >= B: no lane has candidates).
+struct WaveLanes {
+    uint32_t n, rank;
+};
+__device__ __forceinline__ WaveLanes wave_lanes() {
+    const uint64_t live = __ballot(true);
+    return WaveLanes{(uint32_t)__popcll(live),
+                     (uint32_t)__popcll(live & ((1ull << (threadIdx.x & 63)) - 1ull))};
+}
+__device__ __forceinline__ void wave_range(uint32_t a, uint32_t b, uint32_t& A, uint32_t& B) {
+    A = __ockl_wfred_min_u32(a < b ? a : 0xFFFFFFFFu);
+    B = __ockl_wfred_max_u32(a < b ? b : 0u);
+}
+
 // B1 for sorted particle i.  lam_x/v/e: adjoints of step n+1 (original
 // order).  Writes ga/gX (sorted), lam_v of step n (original order) and the
 // direct part of lam_x (sorted).
@@ -116,7 +140,8 @@ __device__ __forceinline__ void adj_b2_body(int i, const float4* __restrict__ po
                                             const float4* __restrict__ ga_s, const float4* __restrict__ gX_s,
                                             const AdjRows& rows, const rtps_sph_params& sp, const Grid& g,
                                             float4* __restrict__ gx2_s, float4* __restrict__ ge_s,
-                                            float* __restrict__ grho_s, float& gmu, float& gK) {
+                                            float* __restrict__ grho_s, float& gmu, float& gK,
+                                            float4* __restrict__ stage) {
     const float h = sp.smoothing_distance, h2 = h * h, eps = sp.EPSILON, eps2 = eps * eps;
     const float m = sp.mass, rho0 = sp.rest_density, K = sp.K, mu = sp.viscosity;
     const float cp = -0.5f * sp.wspiky_d_coef, cv = sp.wvisc_dd_coef;
@@ -184,41 +209,67 @@ __device__ __forceinline__ void adj_b2_body(int i, const float4* __restrict__ po
     };
     const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
     const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, rows.keys, rows.rx, rows.ry, rows.rz);
+    // candidate j of a row: P[j - off], V / GA / GX[j - off]
+    auto walk = [&](const float4* __restrict__ P, const float4* __restrict__ V, const float4* __restrict__ GA,
+                    const float4* __restrict__ GX, uint32_t a, uint32_t b, uint32_t off) {
+        uint32_t j = a;
+        for (; j + kGroup <= b; j += kGroup) {   // positions, then the accepted records, loaded together
+            float4 p[kGroup];
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) p[k] = P[j + k - off];
+            float3 r[kGroup];
+            float q2[kGroup];
+            bool ok[kGroup];
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) {
+                r[k] = make_float3(pi.x - p[k].x, pi.y - p[k].y, pi.z - p[k].z);
+                q2[k] = dot3(r[k], r[k]);
+                ok[k] = q2[k] <= h2 && j + k != (uint32_t)i;
+            }
+            float4 v[kGroup], ga[kGroup], gX[kGroup];
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k)
+                if (ok[k]) {
+                    v[k] = V[j + k - off];
+                    ga[k] = GA[j + k - off];
+                    gX[k] = GX[j + k - off];
+                }
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k)
+                if (ok[k]) pair(r[k], q2[k], v[k], ga[k], gX[k]);
+        }
+        for (; j < b; ++j) {
+            const float4 pj = P[j - off];
+            const float3 r = make_float3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+            const float q2 = dot3(r, r);
+            if (q2 <= h2 && j != (uint32_t)i) pair(r, q2, V[j - off], GA[j - off], GX[j - off]);
+        }
+    };
+    // rows of at most kAdjStageCap wave-wide candidates are copied once per
+    // wave into `stage` (full-line loads) and read from there
+    const WaveLanes wl = wave_lanes();
+    float4* __restrict__ sp_ = stage;
+    float4* __restrict__ sv = stage + kAdjStageCap;
+    float4* __restrict__ sga = stage + 2 * kAdjStageCap;
+    float4* __restrict__ sgx = stage + 3 * kAdjStageCap;
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy) {
-            uint32_t a, b;
+            uint32_t a, b, A, B;
             row_window(wc, c, dy, dz, h2, g, rows.start, rows.end, rows.keys, a, b, rows.prefix);
-            uint32_t j = a;
-            for (; j + kGroup <= b; j += kGroup) {   // positions, then the accepted records, loaded together
-                float4 p[kGroup];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k) p[k] = pos_s[j + k];
-                float3 r[kGroup];
-                float q2[kGroup];
-                bool ok[kGroup];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k) {
-                    r[k] = make_float3(pi.x - p[k].x, pi.y - p[k].y, pi.z - p[k].z);
-                    q2[k] = dot3(r[k], r[k]);
-                    ok[k] = q2[k] <= h2 && j + k != (uint32_t)i;
+            wave_range(a, b, A, B);
+            if (B <= A) continue;
+            if (B - A <= (uint32_t)kAdjStageCap) {
+                for (uint32_t t = wl.rank; t < B - A; t += wl.n) {
+                    sp_[t] = pos_s[A + t];
+                    sv[t] = ve_s[A + t];
+                    sga[t] = ga_s[A + t];
+                    sgx[t] = gX_s[A + t];
                 }
-                float4 v[kGroup], ga[kGroup], gX[kGroup];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k)
-                    if (ok[k]) {
-                        v[k] = ve_s[j + k];
-                        ga[k] = ga_s[j + k];
-                        gX[k] = gX_s[j + k];
-                    }
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k)
-                    if (ok[k]) pair(r[k], q2[k], v[k], ga[k], gX[k]);
-            }
-            for (; j < b; ++j) {
-                const float4 pj = pos_s[j];
-                const float3 r = make_float3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-                const float q2 = dot3(r, r);
-                if (q2 <= h2 && j != (uint32_t)i) pair(r, q2, ve_s[j], ga_s[j], gX_s[j]);
+                __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
+                walk(sp_, sv, sga, sgx, a, b, A);
+                __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
+            } else {
+                walk(pos_s, ve_s, ga_s, gX_s, a, b, 0u);
             }
         }
     gx2_s[i] = make_float4(gx.x, gx.y, gx.z, 0.f);
@@ -232,7 +283,8 @@ __device__ __forceinline__ void adj_b3_body(int i, const uint32_t* __restrict__ 
                                             const float* __restrict__ grho_s, const float4* __restrict__ lxp_s,
                                             const float4* __restrict__ gx2_s, const float4* __restrict__ ge_s,
                                             const AdjRows& rows, const rtps_sph_params& sp, const Grid& g,
-                                            float4* __restrict__ lam_x_out, float4* __restrict__ lam_e_out) {
+                                            float4* __restrict__ lam_x_out, float4* __restrict__ lam_e_out,
+                                            float4* __restrict__ stage) {
     const float h = sp.smoothing_distance, h2 = h * h;
     const float crho = sp.wpoly6_coef * sp.mass;
     const float4 pi = pos_s[i];
@@ -240,33 +292,55 @@ __device__ __forceinline__ void adj_b3_body(int i, const uint32_t* __restrict__ 
     float3 gx = make_float3(0, 0, 0);
     const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
     const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, rows.keys, rows.rx, rows.ry, rows.rz);
+    auto term = [&](float4 pj, float grj, bool ok) {
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        const float q2 = rx * rx + ry * ry + rz * rz;
+        if (!(ok && q2 <= h2)) return;
+        const float t = h2 - q2;
+        const float k = crho * (-6.f) * t * t * (gri + grj);
+        gx.x += k * rx;
+        gx.y += k * ry;
+        gx.z += k * rz;
+    };
+    // candidate j of a row: P[j - off], GR[j - off]
+    auto walk = [&](const float4* __restrict__ P, const float* __restrict__ GR, uint32_t a, uint32_t b,
+                    uint32_t off) {
+        uint32_t j = a;
+        for (; j + kGroup <= b; j += kGroup) {   // loads of a group issued together
+            float4 p[kGroup];
+            float gr[kGroup];
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) {
+                p[k] = P[j + k - off];
+                gr[k] = GR[j + k - off];
+            }
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) term(p[k], gr[k], j + k != (uint32_t)i);
+        }
+        for (; j < b; ++j) term(P[j - off], GR[j - off], j != (uint32_t)i);
+    };
+    // rows of at most kAdjB3StageCap wave-wide candidates are copied once per
+    // wave into `stage` (full-line loads) and read from there
+    const WaveLanes wl = wave_lanes();
+    float4* __restrict__ sp_ = stage;
+    float* __restrict__ sgr = (float*)(stage + kAdjB3StageCap);
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy) {
-            uint32_t a, b;
+            uint32_t a, b, A, B;
             row_window(wc, c, dy, dz, h2, g, rows.start, rows.end, rows.keys, a, b, rows.prefix);
-            auto term = [&](float4 pj, float grj, bool ok) {
-                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                const float q2 = rx * rx + ry * ry + rz * rz;
-                if (!(ok && q2 <= h2)) return;
-                const float t = h2 - q2;
-                const float k = crho * (-6.f) * t * t * (gri + grj);
-                gx.x += k * rx;
-                gx.y += k * ry;
-                gx.z += k * rz;
-            };
-            uint32_t j = a;
-            for (; j + kGroup <= b; j += kGroup) {   // loads of a group issued together
-                float4 p[kGroup];
-                float gr[kGroup];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k) {
-                    p[k] = pos_s[j + k];
-                    gr[k] = grho_s[j + k];
+            wave_range(a, b, A, B);
+            if (B <= A) continue;
+            if (B - A <= (uint32_t)kAdjB3StageCap) {
+                for (uint32_t t = wl.rank; t < B - A; t += wl.n) {
+                    sp_[t] = pos_s[A + t];
+                    sgr[t] = grho_s[A + t];
                 }
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k) term(p[k], gr[k], j + k != (uint32_t)i);
+                __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
+                walk(sp_, sgr, a, b, A);
+                __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
+            } else {
+                walk(pos_s, grho_s, a, b, 0u);
             }
-            for (; j < b; ++j) term(pos_s[j], grho_s[j], j != (uint32_t)i);
         }
     const float4 lp = lxp_s[i], g2 = gx2_s[i];
     const uint32_t o = idx[i];

@@ -85,12 +85,13 @@ __global__ __launch_bounds__(kBABlock) void k_badj_b2(const BatchItem* __restric
                                                       const uint32_t* __restrict__ keys, float4* __restrict__ gx2_s,
                                                       float4* __restrict__ ge_s, float* __restrict__ grho_s,
                                                       float2* __restrict__ partial) {
+    __shared__ float4 stage[kBABlock / 64][kAdjStageSlice];   // one slice per wave
     const ItemCtx c = item_ctx(items);
     const SceneDev& S = scenes[c.s];
     float gmu = 0.f, gK = 0.f;
     if (c.valid)
         adj_b2_body(c.i, pos_s, ve_s, ga_s, gX_s, scene_rows(S, c.s, start, end, keys), S.sp, S.gs, gx2_s, ge_s,
-                    grho_s, gmu, gK);
+                    grho_s, gmu, gK, stage[threadIdx.x >> 6]);
     adj_block_partial<kBABlock>(gmu, gK, partial, blockIdx.x);
 }
 
@@ -107,11 +108,12 @@ __global__ __launch_bounds__(kBABlock) void k_badj_b3(const BatchItem* __restric
                                                       const uint32_t* __restrict__ keys,
                                                       float4* __restrict__ lam_x_out,
                                                       float4* __restrict__ lam_e_out) {
+    __shared__ float4 stage[kBABlock / 64][kAdjStageSlice];   // one slice per wave
     const ItemCtx c = item_ctx(items);
     if (!c.valid) return;
     const SceneDev& S = scenes[c.s];
     adj_b3_body(c.i, idx, pos_s, grho_s, lxp_s, gx2_s, ge_s, scene_rows(S, c.s, start, end, keys), S.sp, S.gs,
-                lam_x_out, lam_e_out);
+                lam_x_out, lam_e_out, stage[threadIdx.x >> 6]);
 }
 
 // Block s adds partial[item_first[s], item_first[s + 1]) to acc[2 s], acc[2 s + 1].
