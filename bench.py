@@ -383,6 +383,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-single-steps", type=int, default=2)
     ap.add_argument("--profile", action="store_true", help="timed loop only (for rocprofv3 runs)")
+    ap.add_argument("--launch", default="", metavar="BD,BF,LOOP",
+                    help="fixed neighbour launches instead of autotuning: density and force threads per block and "
+                         "the force loop (profiling runs: counters perturb the autotuner's timings)")
     ap.add_argument("--force-variant", type=int, default=4,
                     help="process-default force loop before autotuning: 4 = direct with LDS-staged rows, "
                          "0 = direct, 1 = compacted neighbour lists")
@@ -420,7 +423,15 @@ def main():
     s.setOption(N.OPT_GRAPH, 1)
     s.setOption(N.OPT_FORCE_VARIANT, args.force_variant)
     s.setOption(N.OPT_TIMERS, 0 if args.no_stage_timers else 1)
-    tune = s.autotune(measure=True)          # the first warmup step tunes the neighbour launches
+    if args.launch:
+        bd, bf, loop = (int(v) for v in args.launch.split(","))
+        s.setOption(N.OPT_FORCE_VARIANT, loop)
+        s.setOption(N.OPT_BLOCK_DENSITY, bd)
+        s.setOption(N.OPT_BLOCK_FORCE, bf)
+        tune = {"block_density": bd, "block_force": bf, "force_variant": loop}
+        s.step(1)
+    else:
+        tune = s.autotune(measure=True)      # the first warmup step tunes the neighbour launches
     s.step(max(0, args.warmup - 1))
     s.synchronize()
     print(f"# warmup {args.warmup} steps done", file=sys.stderr, flush=True)

@@ -25,7 +25,7 @@ FIELD = {
     "color": 13, "color_s": 14, "mass": 15, "mass_s": 16,
 }
 (OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS, OPT_FORCE_VARIANT, OPT_WINDOW, OPT_TILE_REMAP, OPT_AUTOTUNE,
- OPT_SORT_TILE, OPT_COLOR, OPT_DENSITY_STAGE) = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10)
+ OPT_SORT_TILE, OPT_COLOR, OPT_DENSITY_STAGE, OPT_BLOCK_DENSITY, OPT_BLOCK_FORCE) = range(13)
 NUM_STAGES = 6
 STAGE_NAMES = ("hash", "sort", "cellindices", "permute", "density", "force")
 MAX_GRAVITY_SOURCES = 16

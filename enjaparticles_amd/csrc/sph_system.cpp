@@ -521,9 +521,10 @@ int autotune_step(rtps_sph* s, bool measure) {
     };
     const int blocks[] = {64, 128, 256};
     // a later candidate (larger block, later loop) must be faster by this
-    // factor: the first state is no guide to close calls (64-thread blocks keep
-    // the neighbour rows of consecutive tiles in fewer L2s as the state evolves)
-    constexpr float kMargin = 0.98f;
+    // factor: the first state is no guide to close calls (on the 1M dam the
+    // block sizes time within 2% of each other at the first step, and 64-thread
+    // blocks are 2-6% faster from a few hundred steps on)
+    constexpr float kMargin = 0.95f;
     // RTPS_TUNE_VERBOSE=1 prints every timed configuration to stderr
     const char* verbose_env = std::getenv("RTPS_TUNE_VERBOSE");
     const bool verbose = verbose_env && verbose_env[0] == '1';
@@ -1168,6 +1169,14 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
             if (value != 0 && value != 4 && value != 8 && value != 16)
                 return set_error(RTPS_ERR_ARG, "sort tile must be 0 (by size), 4, 8 or 16");
             g_sort_items = value;
+            break;
+        case RTPS_OPT_BLOCK_DENSITY:
+        case RTPS_OPT_BLOCK_FORCE:
+            if (value != 64 && value != 128 && value != 256)
+                return set_error(RTPS_ERR_ARG, "threads per block must be 64, 128 or 256");
+            if (option == RTPS_OPT_BLOCK_DENSITY) s->block_density = value; else s->block_force = value;
+            s->force_variant = -1;
+            s->tuned = true;
             break;
         case RTPS_OPT_DENSITY_STAGE:
             if (value < 0 || value > 1) return set_error(RTPS_ERR_ARG, "density stage option must be 0 or 1");

@@ -269,6 +269,11 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        once per wave into LDS (default), 0 = every lane reads
  *                        its candidates from global memory; process-wide, the
  *                        same densities either way
+ *   RTPS_OPT_BLOCK_DENSITY, RTPS_OPT_BLOCK_FORCE  threads per block (64, 128 or
+ *                        256) of the density / force launches of this handle;
+ *                        setting either fixes the launch configuration (the
+ *                        handle is then not autotuned; the force loop is the
+ *                        process default of RTPS_OPT_FORCE_VARIANT)
  *   RTPS_OPT_COLOR       1 = permute colours every step and renumber them with
  *                        the survivors of a deletion (permute.cl:41,
  *                        leapfrog.cl:107); 0 = colours stay where pushed.
@@ -289,6 +294,8 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
 #define RTPS_OPT_SORT_TILE 8
 #define RTPS_OPT_COLOR 9
 #define RTPS_OPT_DENSITY_STAGE 10
+#define RTPS_OPT_BLOCK_DENSITY 11
+#define RTPS_OPT_BLOCK_FORCE 12
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))
@@ -296,7 +303,7 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value);
  * 128 or 256) and the force-loop variant (0, 4 and 1, or 0 to 4 with
  * RTPS_OPT_AUTOTUNE = 2; variants 2 and 3 with the density kernel recording
  * rows) are timed on the live state and the fastest density + force pair kept
- * (a candidate tried later must be 2% faster).  The two kernels are idempotent on one sorted state, so the
+ * (a candidate tried later must be 5% faster).  The two kernels are idempotent on one sorted state, so the
  * step's result does not depend on the choice.  Choices are cached per
  * process by device and size class (and in the file named by the
  * RTPS_TUNE_CACHE environment variable, when set); measure = 0 takes a cached

@@ -255,6 +255,27 @@ def test_force_loop_variants_agree(cuda, scene):
     np.testing.assert_array_equal(out[4][1], out[0][1])
 
 
+def test_fixed_launch_options(cuda):
+    """RTPS_OPT_BLOCK_DENSITY / RTPS_OPT_BLOCK_FORCE fix the neighbour launches
+    (no autotuning): any block size gives the autotuned step's bits; other
+    values are rejected."""
+    sc = scenes.dam_custom(24, 24, 12, seed=85)
+    a = _system(sc)
+    a.autotune(measure=True)
+    a.step(3)
+    for bd, bf in ((64, 256), (128, 64)):
+        b = _system(sc)
+        b.setOption(11, bd)
+        b.setOption(12, bf)
+        b.step(4)
+        np.testing.assert_array_equal(b.read("pos"), a.read("pos"))
+        np.testing.assert_array_equal(b.read("vel"), a.read("vel"))
+        with pytest.raises(RuntimeError):
+            b.setOption(12, 96)
+        b.close()
+    a.close()
+
+
 def test_sort_tile_sizes_give_the_same_step(cuda):
     """Radix-sort tiles of 1024, 2048 and 4096 keys (RTPS_OPT_SORT_TILE) sort
     identically (stable), so the steps are bit-identical."""
