@@ -327,10 +327,11 @@ def slab_leg(steps: int = 100, warmup: int = 20, slabs: int = 8):
             "value": own * steps / dt, "unit": "particle-updates/s (own particles)", "stages_ms": stages}
 
 
-def compare4m_leg(steps: int = 100):
+def compare4m_leg(steps: int = 100, launch: str = ""):
     """Plain and batched forward at 4,194,304 particles: one 256x128x128 dam
     break, and 64 scenes of 64x64x16, stepped `steps` steps each (hipGraph),
-    with per-stage times.  HBM and L2 counters come from profiling this mode."""
+    with per-stage times.  HBM and L2 counters come from profiling this mode.
+    launch: "BD,BF,LOOP" fixes the plain system's neighbour launches."""
     from enjaparticles_amd import scenes
     from enjaparticles_amd.rtps import RTPS, BatchedSPH
     import enjaparticles_amd._native as N
@@ -338,7 +339,14 @@ def compare4m_leg(steps: int = 100):
     sc = scenes.dam4m()
     s = RTPS.generateSystemInstance(sc.settings)
     s.pushParticles(sc.pos, sc.vel)
-    s.autotune(measure=True)
+    if launch:
+        bd, bf, loop = (int(v) for v in launch.split(","))
+        s.setOption(N.OPT_FORCE_VARIANT, loop)
+        s.setOption(N.OPT_BLOCK_DENSITY, bd)
+        s.setOption(N.OPT_BLOCK_FORCE, bf)
+        s.step(1)
+    else:
+        s.autotune(measure=True)
     s.step(9)
     s.synchronize()
     t0 = time.perf_counter()
@@ -411,7 +419,7 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     if args.compare_4m > 0:
-        print(json.dumps({"compare_4m": compare4m_leg(args.compare_4m)}))
+        print(json.dumps({"compare_4m": compare4m_leg(args.compare_4m, args.launch)}))
         return
     from enjaparticles_amd import scenes
     from enjaparticles_amd.rtps import RTPS

@@ -315,9 +315,12 @@ constexpr int kNbrCap = 32;
 // VGPRs (from 85, with a few spilled bytes), which measured 3-8% faster than
 // five waves on the 1M dam break; eight waves spill too much.
 constexpr int kForceWavesPerSimd = 6;
+// The staged loop (kLoopStaged) at five waves: 96 VGPRs instead of 80 and no
+// spills in the row loop, 2-11% faster than six on the 1M dam.
+constexpr int kStagedWavesPerSimd = 5;
 
 template <bool SCALED, bool PACKED, bool FUSED, int LOOP>
-__global__ __launch_bounds__(kBlock, kForceWavesPerSimd) void k_force(int num_host, const int* __restrict__ d_num,
+__global__ __launch_bounds__(kBlock, LOOP == kLoopStaged ? kStagedWavesPerSimd : kForceWavesPerSimd) void k_force(int num_host, const int* __restrict__ d_num,
                                                   const float4* __restrict__ pos_s,
                                                   const float* __restrict__ dens_in,
                                                   const float4* __restrict__ ve_in,
