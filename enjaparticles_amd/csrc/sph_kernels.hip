@@ -129,8 +129,9 @@ constexpr int kGroupDensity = 4;
 constexpr int kStageCap = 256;
 int g_density_stage = 1;
 
-// Nine waves per SIMD (at most 56 VGPRs, the occupancy of the search-based
-// windows; the table windows would otherwise take 57 and leave eight).
+// At most 56 VGPRs asked for (nine waves per SIMD): the row-by-row walk fits
+// (55; 57 and eight waves without the cap); the staged walk takes 63 anyway
+// (eight waves, whose LDS slices take 128 of the 160 KB).
 template <bool SCALED, bool PACKED, int REC, bool STAGE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k_density(int num_host, const int* __restrict__ d_num,
                                                     const float4* __restrict__ pos_s,

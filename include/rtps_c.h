@@ -260,7 +260,7 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        sorted keys, 0 = whole rows
  *   RTPS_OPT_AUTOTUNE    1 = the first rtps_sph_step tunes the neighbour launches
  *                        over the force loops 0, 4 and 1 (default; see
- *                        rtps_sph_autotune), 2 = over all four loops (2 and 3
+ *                        rtps_sph_autotune), 2 = over all five loops (2 and 3
  *                        win while the neighbourhoods are sparse, lose once
  *                        they fill), 0 = fixed defaults
  *   RTPS_OPT_SORT_TILE   radix-sort tiles of 256 x value keys, process-wide:
