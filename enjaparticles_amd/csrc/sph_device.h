@@ -362,9 +362,10 @@ struct ForceAcc {
 // One accepted pair (rlen <= h, j != i) of force.cl:62-139.  Self pairs add
 // exactly zero in the reference (iej = 0 multiplies a finite value), so they
 // are skipped.  r = p_i - p_j (scaled), r2 = |r|^2.
-__device__ __forceinline__ void force_pair(ForceAcc& acc, float rx, float ry, float rz, float r2, float h, float h2,
-                                           float eps, float K, float rho0, float Pi, float di, float4 vi, float4 vj,
-                                           float dj) {
+// idj = 1 / dj, the hardware reciprocal (per pair here, or kept with a staged row).
+__device__ __forceinline__ void force_pair_r(ForceAcc& acc, float rx, float ry, float rz, float r2, float h, float h2,
+                                             float eps, float K, float rho0, float Pi, float di, float4 vi, float4 vj,
+                                             float dj, float idj) {
     // force.cl:73, rlen = max(|r|, eps), and 1/rlen from one hardware
     // reciprocal square root (~1 ulp; the reference builds OpenCL's length()
     // without correctly rounded sqrt): r2 clamped to eps^2 gives the same max.
@@ -375,7 +376,6 @@ __device__ __forceinline__ void force_pair(ForceAcc& acc, float rx, float ry, fl
     float rlen = r2c * rinv;
     float hr = h - rlen;
     float dW = hr * hr * rinv;                      // Wspiky_dr, cl_sph_kernels.h:81-97
-    float idj = fast_rcp(dj);
     float Pj = K * (dj - rho0);                     // force.cl:84-85
     float kern = dW * (Pi + Pj) * idj;              // :97
     float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
@@ -391,6 +391,12 @@ __device__ __forceinline__ void force_pair(ForceAcc& acc, float rx, float ry, fl
     acc.fx += rx * kern;                            // :136
     acc.fy += ry * kern;
     acc.fz += rz * kern;
+}
+
+__device__ __forceinline__ void force_pair(ForceAcc& acc, float rx, float ry, float rz, float r2, float h, float h2,
+                                           float eps, float K, float rho0, float Pi, float di, float4 vi, float4 vj,
+                                           float dj) {
+    force_pair_r(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, dj, fast_rcp(dj));
 }
 
 // force.cl:181-185 epilogue: force = m/rho_i * (-1/2 wspiky_d F + mu wvisc_dd V),

@@ -521,8 +521,9 @@ __global__ __launch_bounds__(kBlock, LOOP == kLoopStaged ? kStagedWavesPerSimd :
         const uint32_t nlive = (uint32_t)__popcll(live);
         const uint32_t rank = (uint32_t)__popcll(live & ((1ull << (threadIdx.x & 63)) - 1ull));
         // candidates [a, b) of one row, candidate j at P[j - off] / V[j - off]
+        // inv_w: P[.].w holds 1 / d_j (the staged copies), else one reciprocal per pair
         auto walk = [&](const float4* __restrict__ P, const float4* __restrict__ V, uint32_t a, uint32_t b,
-                        uint32_t off) {
+                        uint32_t off, bool inv_w) {
             uint32_t j = a;
             for (; j + kGroup <= b; j += kGroup) {
                 const float4* __restrict__ pg = P + (j - off);
@@ -545,18 +546,23 @@ __global__ __launch_bounds__(kBlock, LOOP == kLoopStaged ? kStagedWavesPerSimd :
                     if (ok[k]) v[k] = vg[k];
 #pragma unroll
                 for (int k = 0; k < kGroup; ++k)
-                    if (ok[k])
-                        force_pair(acc, rx[k], ry[k], rz[k], r2[k], h, h2, eps, K, rho0, Pi, di, vi, v[k],
-                                   PACKED ? v[k].w : dens_in[j + k]);
+                    if (ok[k]) {
+                        const float dj = PACKED ? v[k].w : dens_in[j + k];
+                        force_pair_r(acc, rx[k], ry[k], rz[k], r2[k], h, h2, eps, K, rho0, Pi, di, vi, v[k], dj,
+                                     inv_w ? q[k].w : fast_rcp(dj));
+                    }
             }
             for (; j < b; ++j) {
-                float4 pj = P[j - off];
+                const float4 pq = P[j - off];
+                float4 pj = pq;
                 if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
                 const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                 const float r2 = rx * rx + ry * ry + rz * rz;
                 if (r2 <= h2 && j != (uint32_t)i) {
                     const float4 vj = V[j - off];
-                    force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, PACKED ? vj.w : dens_in[j]);
+                    const float dj = PACKED ? vj.w : dens_in[j];
+                    force_pair_r(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, dj,
+                                 inv_w ? pq.w : fast_rcp(dj));
                 }
             }
         };
@@ -568,15 +574,18 @@ __global__ __launch_bounds__(kBlock, LOOP == kLoopStaged ? kStagedWavesPerSimd :
                 const uint32_t B = __ockl_wfred_max_u32(a < b ? b : 0u);
                 if (B <= A) continue;
                 if (B - A <= (uint32_t)kForceStageCap) {
+                    // the copy keeps 1 / d_j in the position's unused w: one
+                    // reciprocal per copied candidate instead of one per pair
                     for (uint32_t t = rank; t < B - A; t += nlive) {
-                        my_p[t] = pos_s[A + t];
-                        my_v[t] = ve_in[A + t];
+                        const float4 p = pos_s[A + t], e = ve_in[A + t];
+                        my_p[t] = make_float4(p.x, p.y, p.z, fast_rcp(PACKED ? e.w : dens_in[A + t]));
+                        my_v[t] = e;
                     }
                     __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
-                    walk(my_p, my_v, a, b, A);
+                    walk(my_p, my_v, a, b, A, true);
                     __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
                 } else {
-                    walk(pos_s, ve_in, a, b, 0u);
+                    walk(pos_s, ve_in, a, b, 0u, false);
                 }
             }
         }
