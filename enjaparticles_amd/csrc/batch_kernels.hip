@@ -184,9 +184,10 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
         const float Pi = K * (di - rho0);
         ForceAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         float4* __restrict__ stage_v = stage + kBStageCap;
-        // candidate j at P[j - off] / V[j - off]
+        // candidate j at P[j - off] / V[j - off]; inv_w: P[.].w holds 1 / d_j
+        // (the staged copies), else one reciprocal per pair
         auto walk = [&](const float4* __restrict__ P, const float4* __restrict__ V, uint32_t a, uint32_t b,
-                        uint32_t off) {
+                        uint32_t off, bool inv_w) {
             uint32_t j = a;
             for (; j + kGroup <= b; j += kGroup) {   // positions, then accepted velocities, loaded together
                 const float4* __restrict__ pg = P + (j - off);
@@ -209,7 +210,9 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                     if (ok[k]) v[k] = vg[k];
 #pragma unroll
                 for (int k = 0; k < kGroup; ++k)
-                    if (ok[k]) force_pair(acc, rx[k], ry[k], rz[k], r2[k], h, h2, eps, K, rho0, Pi, di, vi, v[k], v[k].w);
+                    if (ok[k])
+                        force_pair_r(acc, rx[k], ry[k], rz[k], r2[k], h, h2, eps, K, rho0, Pi, di, vi, v[k], v[k].w,
+                                     inv_w ? p[k].w : fast_rcp(v[k].w));
             }
             for (; j < b; ++j) {
                 float4 pj = P[j - off];
@@ -217,7 +220,8 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                 float r2 = rx * rx + ry * ry + rz * rz;
                 if (r2 <= h2 && j != (uint32_t)i) {
                     const float4 vj = V[j - off];
-                    force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, vj.w);
+                    force_pair_r(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, vj.w,
+                                 inv_w ? pj.w : fast_rcp(vj.w));
                 }
             }
         };
@@ -228,15 +232,17 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                 wave_range(a, b, A, B);
                 if (B <= A) continue;
                 if (B - A <= (uint32_t)kBStageCap) {
+                    // the copy keeps 1 / d_j in the position's unused w
                     for (uint32_t t = rank; t < B - A; t += nlive) {
-                        stage[t] = pos_s[A + t];
-                        stage_v[t] = ve[A + t];
+                        const float4 q = pos_s[A + t], e = ve[A + t];
+                        stage[t] = make_float4(q.x, q.y, q.z, fast_rcp(e.w));
+                        stage_v[t] = e;
                     }
                     __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
-                    walk(stage, stage_v, a, b, A);
+                    walk(stage, stage_v, a, b, A, true);
                     __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
                 } else {
-                    walk(pos_s, ve, a, b, 0u);
+                    walk(pos_s, ve, a, b, 0u, false);
                 }
             }
         float4 F, X;
