@@ -5,7 +5,7 @@
 // the bits are split evenly over ceil(key_bits / 8) passes.
 //
 // Per pass (R = 2^bits buckets, tiles of 256 x ITEMS keys, ITEMS = 4, 8 or
-// 16; g_sort_items picks it, by default 16 from 2^20 keys up and 8 below; the
+// 16; g_sort_items picks it, by default 16 from 2^19 keys up and 8 below; the
 // tile kernels run 1024 threads of ITEMS / 4 keys each, 16 waves per tile for
 // latency hiding):
 //   k_radix_hist    per-tile digit histogram, digit-major in hist[d * ntiles + t];
@@ -37,7 +37,9 @@ constexpr int kTileThreads = 1024;  // threads of the histogram and scatter kern
 constexpr int kTileWaves = kTileThreads / 64;
 
 // 256 x g_sort_items keys per sort tile (4, 8 or 16; 0 = by size: 4096-key
-// tiles measured 77 us for 1M keys against 84 us with 2048 and 115 us with 1024)
+// tiles measured 77 us for 1M keys against 84 us with 2048 and 115 us with
+// 1024, 65 against 67 us at 655,360 keys; 2048-key tiles are best at 393,216
+// and tie with 1024 at 196,608)
 int g_sort_items = 0;
 constexpr int kMaxRadix = 256;
 constexpr int kMaxPasses = 4;
@@ -278,7 +280,7 @@ hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int np
     if (npasses > kMaxPasses) return hipErrorInvalidValue;
     SortTemp t = carve(temp, n);
     const int items = g_sort_items == 4 || g_sort_items == 8 || g_sort_items == 16 ? g_sort_items
-                      : (n >= (1 << 20) ? 16 : 8);
+                      : (n >= (1 << 19) ? 16 : 8);
     const int ntiles = ntiles_for(n, items);
     uint32_t *ki = keys, *vi = vals, *ko = t.kalt, *vo = t.valt;
     for (int p = 0; p < npasses; ++p) {

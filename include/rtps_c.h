@@ -264,7 +264,7 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        win while the neighbourhoods are sparse, lose once
  *                        they fill), 0 = fixed defaults
  *   RTPS_OPT_SORT_TILE   radix-sort tiles of 256 x value keys, process-wide:
- *                        4, 8 or 16, or 0 (default) = 16 from 2^20 keys up, else 8
+ *                        4, 8 or 16, or 0 (default) = 16 from 2^19 keys up, else 8
  *   RTPS_OPT_DENSITY_STAGE  1 = the density kernel copies each row's candidates
  *                        once per wave into LDS (default), 0 = every lane reads
  *                        its candidates from global memory; process-wide, the
