@@ -305,11 +305,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
 // rather than, row by row, the largest per-row count (the row-by-row walk of
 // kLoopMasked keeps only ~40% of the lanes busy on a dam-break state).
 //
-// kLoopStaged: the direct loop, with each row's candidates (positions and
-// veleval|density) copied once per wave into LDS as for the density kernel's
-// STAGE (at most kForceStageCap candidates; wider unions read global memory).
+// kLoopStaged (the default): the direct loop, with each row's candidates
+// (positions, their w replaced by 1 / density, and veleval|density) copied
+// once per wave into LDS as for the density kernel's STAGE (at most
+// kForceStageCap candidates; wider unions read global memory).  Same pairs,
+// same order, same reciprocals: the direct loop's bits.
 constexpr int kLoopDirect = 0, kLoopCompact = 1, kLoopMasked = 2, kLoopList = 3, kLoopStaged = 4;
-constexpr int kForceStageCap = 192;   // 6 KB per wave: 24 waves per CU fit the 160 KB of LDS
+constexpr int kForceStageCap = 192;   // 6 KB per wave: the 20 waves per CU of kStagedWavesPerSimd take 120 KB
 constexpr int kNbrCap = 32;
 
 // At least six waves per SIMD: the compiler keeps the force kernel within 80
