@@ -145,8 +145,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
     if (MODE == kPairsDensity) {
         float sum = 0.0f;
         auto term = [&](float4 pj) {
-            float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
-            poly6_add(sum, rx * rx + ry * ry + rz * rz, h2);
+            poly6_add_r(sum, pi.x - pj.x * scale, pi.y - pj.y * scale, pi.z - pj.z * scale, h2);
         };
         // candidate j at P[j - off]
         auto walk = [&](const float4* __restrict__ P, uint32_t a, uint32_t b, uint32_t off) {

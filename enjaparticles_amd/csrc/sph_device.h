@@ -228,7 +228,6 @@ __device__ __forceinline__ void row_window_table(const WindowCtx& w, int3 c, int
     const uint32_t e0 = end[h0], e1 = end[h1], e2 = end[h2c];
     const uint32_t t_lo = entry(r_lo), t_hi = entry(r_hi + 1);
     const bool o0 = h0 < nbc && s0 != kEmpty, o1 = h1 < nbc && s1 != kEmpty, o2 = h2c < nbc && s2 != kEmpty;
-    const uint32_t lo = min(o0 ? s0 : kEmpty, min(o1 ? s1 : kEmpty, o2 ? s2 : kEmpty));
     const uint32_t hi = max(o0 ? e0 : 0u, max(o1 ? e1 : 0u, o2 ? e2 : 0u));
     // first index with sub-key >= r: the table entry when r's cell is
     // occupied, else the start of the next occupied cell, else the row end
@@ -346,9 +345,12 @@ __device__ __forceinline__ int remap_tile(int remap, int b, int nb) {
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // density.cl:44-78 poly6 term of one candidate: sum += (h^2 - r^2)^3 for
-// r <= h, else nothing; t = max(h^2 - r^2, 0) and one fused multiply-add.
-__device__ __forceinline__ void poly6_add(float& sum, float r2, float h2) {
-    const float t = fmaxf(h2 - r2, 0.0f);
+// r <= h, else nothing; t = max(h^2 - r^2, 0), with h^2 - |r|^2 formed by three
+// fused multiply-adds (one instruction less than |r|^2 first; the rounding stays
+// within the FP32 tolerance of the reference's (h^2 - |r|^2)^3).  The density
+// kernels (plain and batched) use this form.
+__device__ __forceinline__ void poly6_add_r(float& sum, float rx, float ry, float rz, float h2) {
+    const float t = fmaxf(fmaf(-rz, rz, fmaf(-ry, ry, fmaf(-rx, rx, h2))), 0.0f);
     sum = fmaf(t * t, t, sum);
 }
 

@@ -155,7 +155,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         return rx * rx + ry * ry + rz * rz;
     };
-    auto add_term = [&](float r2) { poly6_add(sum, r2, h2); };
+    auto accum = [&](float4 pj) {   // the poly6 term of one candidate
+        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+        poly6_add_r(sum, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, h2);
+    };
     const size_t st = (size_t)masks.stride;
     constexpr bool MASKS = REC == kRecBits, LIST = REC == kRecList;
     // list form: codes packed four to a slot, flushed as one 8-byte store
@@ -198,9 +201,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
 #pragma unroll
                         for (int k = 0; k < kGroupDensity; ++k) p[k] = pg[k];
 #pragma unroll
-                        for (int k = 0; k < kGroupDensity; ++k) add_term(sep2(p[k]));
+                        for (int k = 0; k < kGroupDensity; ++k) accum(p[k]);
                     }
-                    for (; j < b; ++j) add_term(sep2(my[j - A]));
+                    for (; j < b; ++j) accum(my[j - A]);
                     __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
                 } else {
                     uint32_t j = a;
@@ -210,9 +213,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
 #pragma unroll
                         for (int k = 0; k < kGroupDensity; ++k) p[k] = pg[k];
 #pragma unroll
-                        for (int k = 0; k < kGroupDensity; ++k) add_term(sep2(p[k]));
+                        for (int k = 0; k < kGroupDensity; ++k) accum(p[k]);
                     }
-                    for (; j < b; ++j) add_term(sep2(pos_s[j]));
+                    for (; j < b; ++j) accum(pos_s[j]);
                 }
             }
         }
@@ -246,8 +249,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
                 uint32_t nib = 0;
 #pragma unroll
                 for (int k = 0; k < kGroupDensity; ++k) {
-                    const float r2 = sep2(p[k]);
-                    add_term(r2);
+                    accum(p[k]);
+                    const float r2 = REC != kRecNone ? sep2(p[k]) : 0.0f;
                     const bool acc_k = r2 <= h2 && j + k != (uint32_t)i;
                     if (MASKS) nib |= (uint32_t)acc_k << k;
                     if (LIST && acc_k) append(rtag | (j + k - a));
@@ -258,8 +261,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
                 }
             }
             for (; j < b; ++j, ++q) {
-                const float r2 = sep2(pos_s[j]);
-                add_term(r2);
+                accum(pos_s[j]);
+                const float r2 = REC != kRecNone ? sep2(pos_s[j]) : 0.0f;
                 const bool acc_k = r2 <= h2 && j != (uint32_t)i;
                 if (MASKS) {
                     word |= (uint32_t)acc_k << (q & 31u);
