@@ -31,9 +31,9 @@ PEAK_VALU_FP32_TFLOPS = 78.6   # non-packed FP32 VALU (packed FP32 is disabled i
 PEAK_PACKED_FP32_TFLOPS = 157.3  # published FP32 peak, packed (two lanes per VALU op)
 # HBM bytes per launch of the fused force kernel on this workload (loop 4,
 # 64-thread blocks), from two rocprofv3 --pmc runs over the first 320 steps:
-# FETCH_SIZE 228549 KiB x 2.0 + WRITE_SIZE 78316 KiB x 1.0, the factors read off
+# FETCH_SIZE 227966 KiB x 2.0 + WRITE_SIZE 78307 KiB x 1.0, the factors read off
 # a 256 MiB device copy in the same runs (profiles/dam1m_counters.txt).
-FORCE_HBM_BYTES_PER_LAUNCH = (228549 * 2.0 + 78316 * 1.0) * 1024
+FORCE_HBM_BYTES_PER_LAUNCH = (227966 * 2.0 + 78307 * 1.0) * 1024
 
 
 def _cpu_model() -> str:
