@@ -520,3 +520,36 @@ def test_window_table_is_used_only_when_written(cuda):
     s.setOption(5, 2)
     assert s.countPairs() == k1
     s.close()
+
+
+def test_full_size_dam_invariants(cuda):
+    """The 1,048,576-particle dam break (the benchmark workload) after a few
+    steps, through properties that do not depend on size: the sorted keys
+    ascend and idx is a permutation; every occupied cell's [start, end)
+    holds exactly its keys; the stored SPH forces and XSPH corrections sum to
+    zero over all particles (equal masses: every pair term has a mirror term
+    of opposite sign, m / (rho_i rho_j) being symmetric), to FP32 rounding."""
+    sc = scenes.dam1m()
+    s = _system(sc)
+    s.setOption(2, 1)           # keep force / xsph
+    s.step(3)
+    n = s.getNum()
+    assert n == sc.n
+    keys = s.read("keys").astype(np.int64)
+    idx = s.read("idx")
+    assert np.all(np.diff(keys) >= 0)
+    assert np.array_equal(np.sort(idx), np.arange(n, dtype=np.uint32))
+    start, end = s.read("cell_start"), s.read("cell_end")
+    occ = np.unique(keys)
+    occ = occ[occ < len(start) - 1]
+    first = np.searchsorted(keys, occ, side="left")
+    last = np.searchsorted(keys, occ, side="right")
+    np.testing.assert_array_equal(start[occ], first)
+    np.testing.assert_array_equal(end[occ], last)
+    F = s.read("force")[:, :3].astype(np.float64)
+    X = s.read("xsph")[:, :3].astype(np.float64)
+    print(f"|sum F| / sum |F| = {np.abs(F.sum(0)).max() / np.abs(F).sum():.2e}, "
+          f"|sum X| / sum |X| = {np.abs(X.sum(0)).max() / np.abs(X).sum():.2e}")
+    assert np.abs(F.sum(0)).max() <= 1e-6 * np.abs(F).sum()     # measured ~2e-10
+    assert np.abs(X.sum(0)).max() <= 1e-6 * np.abs(X).sum()     # measured ~3e-11
+    s.close()
