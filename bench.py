@@ -411,7 +411,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU design is described in DESIGN.md and is not built", file=sys.stderr)
+              "the multi-GPU path (z-slab sharding with halo exchange) is a design only and is not built",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
