@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-updates/sec at 1M SPH (plain vs batched vs fwd+adjoint); HBM % peak"
 PEAK_HBM_GBS = 8000.0          # MI355X HBM3E
-PEAK_VALU_FP32_TFLOPS = 78.6   # non-packed FP32 VALU (packed FP32 is disabled in this build)
+PEAK_VALU_FP32_TFLOPS = 78.6   # non-packed FP32 VALU (the kernels are compiled without packed FP32)
 PEAK_PACKED_FP32_TFLOPS = 157.3  # published FP32 peak, packed (two lanes per VALU op)
 # HBM bytes per launch of the fused force kernel on this workload (loop 4,
 # 64-thread blocks), from two rocprofv3 --pmc runs over the first 320 steps:
@@ -410,7 +410,7 @@ def main():
                     help="only run the plain vs batched comparison at 4,194,304 particles for STEPS steps")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU path (z-slab sharding with halo exchange) is a design only and is not built",
               file=sys.stderr)
         sys.exit(2)
