@@ -58,22 +58,30 @@ def cpu_baseline(scene, steps: int = 2, single_steps: int = 2):
     flags = O.FLAG_COLLISION | O.FLAG_BITONIC
     st.step(scene.dt, flags)   # untimed first touch
     threads = O.num_threads()
+    per_step = []   # per-stage wall-clock seconds of every timed step (steady_clock inside the oracle)
     t0 = time.perf_counter()
     for _ in range(steps):
         st.step(scene.dt, flags)
+        per_step.append(O.last_stage_times())
     dt = time.perf_counter() - t0
+    names = list(per_step[0]) if per_step else []
+    stage_med = {k: round(1e3 * float(np.median([p[k] for p in per_step])), 3) for k in names}
     single = None
     if single_steps > 0:
         O.set_threads(1)
         try:
+            single_stages = []
             t1 = time.perf_counter()
             for _ in range(single_steps):
                 st.step(scene.dt, flags)
+                single_stages.append(O.last_stage_times())
             d1 = time.perf_counter() - t1
         finally:
             O.set_threads(threads)
         single = {"value": scene.n * single_steps / d1, "ms_per_step": 1e3 * d1 / single_steps,
-                  "steps": single_steps}
+                  "steps": single_steps,
+                  "stages_ms_median": {k: round(1e3 * float(np.median([p[k] for p in single_stages])), 3)
+                                       for k in names}}
     return {
         "value": scene.n * steps / dt,
         "unit": "particle-updates/s",
@@ -83,6 +91,7 @@ def cpu_baseline(scene, steps: int = 2, single_steps: int = 2):
                   f"of EnjaParticles' OpenCL kernels (hash, bitonic sort, cell indices, permute, density, "
                   f"force, walls, leapfrog), {threads} OpenMP threads",
         "ms_per_step": 1e3 * dt / steps,
+        "stages_ms_median": stage_med,
         "cpu_model": _cpu_model(),
         "host_cpus": os.cpu_count(),
         "single_thread": single,

@@ -99,6 +99,7 @@ def _declare(L):
         "oracle_derive_params": (i, [P, P, i, P, P, P]),
         "oracle_num_threads": (i, []),
         "oracle_set_threads": (None, [i]),
+        "oracle_last_stage_times": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -121,6 +122,16 @@ def _sfx(dtype):
 
 def num_threads() -> int:
     return lib().oracle_num_threads()
+
+
+STAGE_NAMES = ("hash", "sort", "cellindices", "permute", "density", "force", "gravity+walls", "leapfrog")
+
+
+def last_stage_times() -> dict:
+    """Wall-clock seconds of each stage of the last State.step (steady_clock)."""
+    out = np.zeros(8, np.float64)
+    lib().oracle_last_stage_times(out.ctypes.data)
+    return dict(zip(STAGE_NAMES, out.tolist()))
 
 
 def set_threads(n: int) -> None:

@@ -38,6 +38,7 @@
 // is used only to give the CPU baseline the reference's O(N log^2 N) cost.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -502,6 +503,18 @@ extern "C" void oracle_count_pairs(int num, const float* pos_s, const u32* start
 }
 
 // ---------------------------------------------------------------------------
+// Wall-clock seconds of each stage of the last step (steady_clock): hash,
+// sort, cellindices, permute, density, force, gravity + walls, leapfrog (the
+// deletion path's second hash and sort count into theirs).
+static double g_stage_s[8];
+using OClock = std::chrono::steady_clock;
+static double secs_since(OClock::time_point& t) {
+    const OClock::time_point now = OClock::now();
+    const double d = std::chrono::duration<double>(now - t).count();
+    t = now;
+    return d;
+}
+
 // One full step (SPH::updateGPU + SPH::integrate, SPH.cpp:149-300, 389-441).
 // flags bit0: wall collision on (the shipped code has the call commented out
 //             at SPH.cpp:297; the dam-break workloads enable it);
@@ -515,10 +528,16 @@ static int step_impl(int* num_io, T* pos_u, T* vel_u, T* veleval_u, const OSPHPa
                      const float* alpha, u32* keys, u32* idx, u32* start, u32* end, T* pos_s, T* vel_s,
                      T* veleval_s, T* density, T* force, T* xsph) {
     int num = *num_io;
+    for (double& v : g_stage_s) v = 0.0;
+    OClock::time_point t = OClock::now();
     hash_impl<T>(num, pos_u, gp, keys, idx);
+    g_stage_s[0] += secs_since(t);
     if (flags & 2) oracle_sort_bitonic(num, keys, idx); else oracle_sort_stable(num, keys, idx);
+    g_stage_s[1] += secs_since(t);
     int nc = oracle_cellindices(num, keys, gp->nb_cells, start, end);
+    g_stage_s[2] += secs_since(t);
     permute_impl<T>(num, idx, pos_u, vel_u, veleval_u, pos_s, vel_s, veleval_s);
+    g_stage_s[3] += secs_since(t);
     int deleted = 0;
     if (nc <= num && nc >= 0) {
         deleted = num - nc;
@@ -527,16 +546,24 @@ static int step_impl(int* num_io, T* pos_u, T* vel_u, T* veleval_u, const OSPHPa
         std::memcpy(pos_u, pos_s, sizeof(T) * 4 * (size_t)num);
         std::memcpy(vel_u, vel_s, sizeof(T) * 4 * (size_t)num);
         std::memcpy(veleval_u, veleval_s, sizeof(T) * 4 * (size_t)num);
+        (void)secs_since(t);
         hash_impl<T>(num, pos_u, gp, keys, idx);
+        g_stage_s[0] += secs_since(t);
         if (flags & 2) oracle_sort_bitonic(num, keys, idx); else oracle_sort_stable(num, keys, idx);
+        g_stage_s[1] += secs_since(t);
     }
     OSPHParams p = *sp;
     p.num = num;
+    (void)secs_since(t);
     density_impl<T>(num, pos_s, start, end, &p, gps, density);
+    g_stage_s[4] += secs_since(t);
     force_impl<T>(num, pos_s, density, veleval_s, start, end, &p, gps, force, xsph);
+    g_stage_s[5] += secs_since(t);
     if (nsrc > 0) gravity_impl<T>(num, nsrc, src, smass, alpha, pos_s, force, (T)p.simulation_scale);
     if (flags & 1) collision_impl<T>(num, pos_s, vel_s, force, &p, gps);
+    g_stage_s[6] += secs_since(t);
     leapfrog_impl<T>(num, pos_s, vel_s, force, xsph, idx, &p, dt, pos_u, vel_u, veleval_u);
+    g_stage_s[7] += secs_since(t);
     *num_io = num;
     return deleted;
 }
@@ -553,6 +580,11 @@ extern "C" int oracle_step_f64(int* num, double* pos_u, double* vel_u, double* v
                     double* pos_s, double* vel_s, double* veleval_s, double* density, double* force, double* xsph) {
     return step_impl<double>(num, pos_u, vel_u, veleval_u, sp, gp, gps, dt, flags, nsrc, src, smass, alpha, keys,
                              idx, start, end, pos_s, vel_s, veleval_s, density, force, xsph);
+}
+
+// Per-stage seconds of the last oracle step (8 values, see g_stage_s).
+extern "C" void oracle_last_stage_times(double* out) {
+    for (int k = 0; k < 8; ++k) out[k] = g_stage_s[k];
 }
 
 extern "C" int oracle_num_threads(void) {
