@@ -60,6 +60,35 @@ def test_ten_steps_match_fp32_oracle(cuda):
     s.close()
 
 
+def test_reference_derived_parameters_match_fp32_oracle(cuda):
+    """Settings without the workload's overrides: mass, rest distance, h and
+    the simulation scale are derived as SPHSettings.cpp:45-75 does (scale
+    0.31, not 1), so every neighbour kernel takes its scaled-position path;
+    a lattice at the derived rest spacing (in world units) steps like the FP32
+    oracle for 5 steps."""
+    st = scenes.dam_settings(1.0, 16384)
+    for k in ("mass", "rest_distance", "smoothing_distance", "simulation_scale"):
+        del st[k]
+    sp, gp, gs = O.derive_params(st)
+    assert abs(sp.simulation_scale - 1.0) > 0.5
+    pos = scenes.lattice(24, 24, 12, seed=86, spacing=sp.rest_distance / sp.simulation_scale)
+    sc = scenes.Scene("derived", st, pos, np.zeros_like(pos), 5, 0.003)
+    s = _system(sc)
+    ref = _oracle(sc)
+    for k in range(5):
+        s.step(1)
+        ref.step(sc.dt, O.FLAG_COLLISION)
+        dens = np.empty(sc.n, np.float32)
+        dens[s.read("idx")] = s.read("density")
+        rd = np.empty(sc.n, np.float32)
+        rd[ref.idx[: ref.num]] = ref.density[: ref.num]
+        np.testing.assert_allclose(s.read("pos")[:, :3], ref.pos_u[:, :3], rtol=1e-4, atol=0)
+        assert _rel_max(s.read("vel")[:, :3], ref.vel_u[:, :3]) <= 1e-4, k
+        np.testing.assert_allclose(dens, rd, rtol=1e-4)
+    assert s.getNum() == ref.num == sc.n
+    s.close()
+
+
 def test_hundred_steps_vs_fp64_oracle(cuda):
     """FP32 HIP vs FP64 oracle after 100 steps.  Pair sums are chaotic, so FP32
     rounding grows into visible position differences; the bar is the FP32
