@@ -39,6 +39,32 @@ def test_each_scene_equals_plain_system(cuda):
     b.close()
 
 
+def test_derived_parameter_scenes_equal_plain_systems(cuda):
+    """Scenes whose mass, h and simulation scale are derived as in
+    SPHSettings.cpp:45-75 (scale != 1, a different one per scene): the batched
+    kernels' scaled positions give what plain systems give."""
+    scs = []
+    for k, (max_num, nx) in enumerate([(4096, 12), (16384, 16), (65536, 20)]):
+        st = scenes.dam_settings(1.0, max_num)
+        for key in ("mass", "rest_distance", "smoothing_distance", "simulation_scale"):
+            del st[key]
+        sp, _, _ = O.derive_params(st)
+        assert abs(sp.simulation_scale - 1.0) > 0.3
+        pos = scenes.lattice(nx, nx, nx // 2, seed=90 + k, spacing=sp.rest_distance / sp.simulation_scale)
+        scs.append(scenes.Scene(f"derived{k}", st, pos, np.zeros_like(pos), 4, 0.003))
+    b = _batch(scs)
+    b.step(4)
+    for s, sc in enumerate(scs):
+        p = SPH(sc.settings)
+        p.pushParticles(sc.pos, sc.vel)
+        p.step(4)
+        np.testing.assert_array_equal(b.read(s, "idx"), p.read("idx"))
+        np.testing.assert_allclose(b.read(s, "pos")[:, :3], p.read("pos")[:, :3], rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(b.read(s, "density"), p.read("density"), rtol=1e-6)
+        p.close()
+    b.close()
+
+
 def test_scene_matches_oracle(cuda):
     scs = scenes.batched(3, 2000, 4000, seed=5)
     b = _batch(scs)
