@@ -10,6 +10,7 @@ normals within 1e-4), and closed inside the lattice.
 import numpy as np
 import pytest
 
+import oracle as O
 import surface_oracle as SO
 from enjaparticles_amd import scenes
 from enjaparticles_amd.rtps import RTPS
@@ -29,8 +30,24 @@ def _box(g):
     return (g.bnd_min.x, g.bnd_min.y, g.bnd_min.z), (g.bnd_max.x, g.bnd_max.y, g.bnd_max.z)
 
 
-def test_color_field_matches_brute_force(cuda):
-    sc, s = _state()
+def _derived_state(steps=10, seed=23):
+    """A lattice with the reference's derived parameters (simulation scale
+    0.31, not 1): the colour field's scaled-position path."""
+    st = scenes.dam_settings(1.0, 16384)
+    for k in ("mass", "rest_distance", "smoothing_distance", "simulation_scale"):
+        del st[k]
+    sp, _, _ = O.derive_params(st)
+    pos = scenes.lattice(16, 16, 8, seed=seed, spacing=sp.rest_distance / sp.simulation_scale)
+    sc = scenes.Scene("derived", st, pos, np.zeros_like(pos), steps, 0.003)
+    s = RTPS.generateSystemInstance(sc.settings)
+    s.pushParticles(sc.pos, sc.vel)
+    s.step(steps)
+    return sc, s
+
+
+@pytest.mark.parametrize("derived", [False, True])
+def test_color_field_matches_brute_force(cuda, derived):
+    sc, s = _derived_state() if derived else _state()
     res = 24
     f = s.colorField(res)
     sp, gp, gs = s.params()
