@@ -60,6 +60,47 @@ def test_ten_steps_match_fp32_oracle(cuda):
     s.close()
 
 
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127])
+def test_tiny_systems_match_fp32_oracle(cuda, n):
+    """A single isolated particle (free fall, self-term density), a pair, and
+    counts around one 64-lane wave (last wave partly empty or one lane
+    alone): 10 steps within the same 1e-4 bars as the 16k dam."""
+    full = scenes.dam16k(seed=1009)
+    sc = scenes.Scene("tiny", full.settings, full.pos[:n].copy(), np.zeros((n, 4), np.float32), 10, 0.003)
+    s = _system(sc)
+    st = _oracle(sc)
+    for k in range(10):
+        s.step(1)
+        st.step(sc.dt, O.FLAG_COLLISION)
+        assert s.getNum() == st.num == n
+        dens = np.empty(n, np.float32)
+        dens[s.read("idx")] = s.read("density")
+        ref = np.empty(n, np.float32)
+        ref[st.idx[: st.num]] = st.density[: st.num]
+        np.testing.assert_allclose(s.read("pos")[:, :3], st.pos_u[:, :3], rtol=1e-4, atol=0)
+        assert _rel_max(s.read("vel")[:, :3], st.vel_u[:, :3]) <= 1e-4, k
+        np.testing.assert_allclose(dens, ref, rtol=1e-4)
+    if n == 1:   # alone: the density is the self term m * poly6(0) = m * 315 / (64 pi h^3)
+        sp, _, _ = O.derive_params(sc.settings)
+        h = np.float64(sp.smoothing_distance)
+        assert dens[0] == pytest.approx(sp.mass * 315.0 / (64.0 * np.pi * h ** 3), rel=1e-5)
+    s.close()
+
+
+def test_empty_system_steps(cuda):
+    """No particles pushed: steps, reads and a later push all work."""
+    sc = scenes.dam16k(seed=1011)
+    s = RTPS.generateSystemInstance(sc.settings)
+    s.step(3)
+    assert s.getNum() == 0
+    assert s.read("pos").shape[0] == 0
+    s.pushParticles(sc.pos[:100], sc.vel[:100])
+    s.step(2)
+    assert s.getNum() == 100
+    assert np.isfinite(s.read("pos")).all()
+    s.close()
+
+
 def test_reference_derived_parameters_match_fp32_oracle(cuda):
     """Settings without the workload's overrides: mass, rest distance, h and
     the simulation scale are derived as SPHSettings.cpp:45-75 does (scale
