@@ -39,6 +39,28 @@ def test_each_scene_equals_plain_system(cuda):
     b.close()
 
 
+def test_ragged_tiny_and_empty_scenes_equal_plain_systems(cuda):
+    """Scenes of 1, 65, 0 and 2 particles beside a full-size one: every
+    scene, the empty one included, steps as a plain system does."""
+    scs = scenes.batched(5, 600, 6000, seed=79)
+    for s, n in [(0, 1), (1, 65), (2, 0), (3, 2)]:
+        sc = scs[s]
+        scs[s] = scenes.Scene(sc.name, sc.settings, sc.pos[:n].copy(), sc.vel[:n].copy(), sc.steps, sc.dt)
+    b = _batch(scs)
+    b.step(5)
+    for s, sc in enumerate(scs):
+        p = SPH(sc.settings)
+        p.pushParticles(sc.pos, sc.vel)
+        p.step(5)
+        assert b.getNum(s) == p.getNum() == sc.n
+        np.testing.assert_array_equal(b.read(s, "keys"), (np.uint32(s) << np.uint32(24)) | p.read("keys"))
+        np.testing.assert_array_equal(b.read(s, "idx"), p.read("idx"))
+        np.testing.assert_allclose(b.read(s, "pos")[:, :3], p.read("pos")[:, :3], rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(b.read(s, "density"), p.read("density"), rtol=1e-6)
+        p.close()
+    b.close()
+
+
 def test_derived_parameter_scenes_equal_plain_systems(cuda):
     """Scenes whose mass, h and simulation scale are derived as in
     SPHSettings.cpp:45-75 (scale != 1, a different one per scene): the batched
