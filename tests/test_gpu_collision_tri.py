@@ -32,10 +32,14 @@ def _terrain(n_side, z0, amp, seed):
     return np.array(tris, np.float32)
 
 
-def test_stage_matches_restatement(cuda):
+@pytest.mark.parametrize("ntri", [648, 1, 2, 256, 257])
+def test_stage_matches_restatement(cuda, ntri):
+    """648 triangles fill three 256-triangle LDS chunks, the last one partly;
+    256 and 257 sit on a chunk boundary; 1 and 2 leave most threads of the
+    chunk without a triangle."""
     import torch
     rng = np.random.default_rng(11)
-    tris = _terrain(18, 0.5, 0.01, 3)          # 648 triangles: three LDS chunks
+    tris = _terrain(18, 0.5, 0.01, 3)[:ntri]
     n = 20000
     pos = np.zeros((n, 4), np.float32)
     pos[:, :2] = rng.uniform(0.0, 1.0, (n, 2))
@@ -57,7 +61,7 @@ def test_stage_matches_restatement(cuda):
     d_got, d_want = got - force, want - force
     hit = np.abs(d_want[:, :3]).max(1) > 0
     print(f"{hit.sum()} of {n} particles hit, max triangle force {np.abs(d_want).max():.3e}")
-    assert 500 < hit.sum() < n
+    assert (500 if ntri > 250 else 0) < hit.sum() < n
     assert np.array_equal(np.abs(d_got[:, :3]).max(1) > 0, hit)
     scale = np.abs(d_want).max()
     np.testing.assert_allclose(d_got[:, :3], d_want[:, :3], rtol=0, atol=1e-4 * scale)
