@@ -89,6 +89,21 @@ def test_surface_matches_restatement(cuda, iso):
     s.close()
 
 
+def test_surface_above_the_field_is_empty(cuda):
+    """An iso level above the field's maximum: no cube is cut, the
+    restatement and the GPU both give no triangles."""
+    sc, s = _state(steps=5, seed=9)
+    res = 20
+    f = s.colorField(res)
+    iso = float(f.max()) * 1.5 + 1.0
+    v, n = s.surface(res, iso)
+    _, gp, _ = s.params()
+    rv, _, _ = SO.marching_cubes(f, iso, SO.mc_case_table(), *_box(gp))
+    assert v.shape == n.shape == (0, 3, 3)
+    assert len(rv) == 0
+    s.close()
+
+
 def test_post_process_keeps_the_settings_surface(cuda):
     sc, s = _state(steps=10, seed=5)
     s.postProcess()
