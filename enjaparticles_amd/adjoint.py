@@ -29,6 +29,9 @@ class AdjointSPH:
                                         checkpoint_every, max_steps, C.byref(h)))
         self._h = h
         self.num = num_particles
+        # forward passes run so far: the handle keeps the checkpoints of the
+        # last one only, which is what a backward replays
+        self.generation = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -54,6 +57,7 @@ class AdjointSPH:
         N.check(N.lib().rtps_adj_forward(self._h, pos0.data_ptr(), vel0.data_ptr(), int(nsteps), float(viscosity),
                                          float(gas_constant), int(collision), pos_t.data_ptr(), vel_t.data_ptr(),
                                          st))
+        self.generation += 1
         return pos_t, vel_t
 
     def backward(self, grad_pos_t, grad_vel_t=None):
@@ -71,18 +75,32 @@ class AdjointSPH:
         return dv0, dx0, dmu.value, dK.value
 
 
+def _replay_if_stale(ctx, visc, gas, pos0, vel0, scalars):
+    """The handle holds the checkpoints of its latest forward only.  When
+    another forward ran on it after this rollout's (a second rollout in the
+    same graph, an evaluation pass), run this rollout's forward again from its
+    saved inputs so the backward replays the right trajectory."""
+    if ctx.sim.generation != ctx.generation:
+        ctx.sim.forward(pos0.detach().contiguous(), vel0.detach().contiguous(), ctx.nsteps, scalars(visc),
+                        scalars(gas), ctx.collision)
+        ctx.generation = ctx.sim.generation
+
+
 class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vel0, viscosity, gas_constant, sim, pos0, nsteps, collision):
         pos_t, _ = sim.forward(pos0.detach().contiguous(), vel0.detach().contiguous(), nsteps,
                                float(viscosity), float(gas_constant), collision)
         ctx.sim = sim
-        ctx.save_for_backward(viscosity, gas_constant)
+        ctx.generation = sim.generation
+        ctx.nsteps, ctx.collision = nsteps, collision
+        ctx.save_for_backward(viscosity, gas_constant, pos0, vel0)
         return pos_t
 
     @staticmethod
     def backward(ctx, grad_pos_t):
-        visc, gas = ctx.saved_tensors
+        visc, gas, pos0, vel0 = ctx.saved_tensors
+        _replay_if_stale(ctx, visc, gas, pos0, vel0, float)
         g = grad_pos_t.detach().contiguous().clone()
         g[:, 3] = 0.0
         dv0, _dx0, dmu, dK = ctx.sim.backward(g)
@@ -125,6 +143,7 @@ class BatchedAdjointSPH:
         self.num_scenes = ns.value
         self.num = tot.value
         self.offsets = list(offs)
+        self.generation = 0   # forward passes so far (the handle keeps the latest one's checkpoints)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -157,6 +176,7 @@ class BatchedAdjointSPH:
                                           self._scalars(viscosity, "viscosity"),
                                           self._scalars(gas_constant, "gas_constant"), int(collision),
                                           pos_t.data_ptr(), vel_t.data_ptr(), st))
+        self.generation += 1
         return pos_t, vel_t
 
     def backward(self, grad_pos_t, grad_vel_t=None):
@@ -181,12 +201,15 @@ class _BatchedRollout(torch.autograd.Function):
         pos_t, _ = sim.forward(pos0.detach().contiguous(), vel0.detach().contiguous(), nsteps,
                                viscosity.detach().cpu(), gas_constant.detach().cpu(), collision)
         ctx.sim = sim
-        ctx.save_for_backward(viscosity, gas_constant)
+        ctx.generation = sim.generation
+        ctx.nsteps, ctx.collision = nsteps, collision
+        ctx.save_for_backward(viscosity, gas_constant, pos0, vel0)
         return pos_t
 
     @staticmethod
     def backward(ctx, grad_pos_t):
-        visc, gas = ctx.saved_tensors
+        visc, gas, pos0, vel0 = ctx.saved_tensors
+        _replay_if_stale(ctx, visc, gas, pos0, vel0, lambda t: t.detach().cpu())
         g = grad_pos_t.detach().contiguous().clone()
         g[:, 3] = 0.0
         dv0, _dx0, dmu, dK = ctx.sim.backward(g)

@@ -130,12 +130,18 @@ int rtps_adj_create(const char* const* keys, const char* const* vals, int n, int
                 return false;
             }())
             return set_error(RTPS_ERR_ARG, "the adjoint does not support point gravity sources");
+        if (integrator_of(a->settings) != 0)
+            return set_error(RTPS_ERR_ARG, "the adjoint differentiates the leapfrog step only (integrator " +
+                                               a->settings.GetSettingAs<std::string>("integrator") + ")");
         a->dt = a->settings.GetSettingAs<float>("time_step");
         a->g = make_grid(a->gp);
         a->gs = make_grid(a->gps);
         a->nb_cells = a->gp.nb_cells;
+        if (a->nb_cells <= 0 || a->nb_cells > (1 << 28) - 2)
+            return set_error(RTPS_ERR_ARG, "grid has no cells or more than 2^28 - 2 cells (" +
+                                               std::to_string(a->nb_cells) + ")");
         a->key_bits = sort_key_bits(a->nb_cells) + kSubBits;
-        a->pad_key = (1u << a->key_bits) - 1u;
+        a->pad_key = a->key_bits >= 32 ? 0xFFFFFFFFu : (1u << a->key_bits) - 1u;
         a->N = max_particles;
         a->ck = checkpoint_every;
         const size_t N = (size_t)max_particles, C = (size_t)a->nb_cells + 1;

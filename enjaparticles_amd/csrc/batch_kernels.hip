@@ -252,6 +252,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
             return;
         }
         const float4 v = vel_s[i];
+        if (S.gsrc.n) gravity_sources(F, p_unscaled, S.gsrc, sp.simulation_scale);   // SPH.cpp:289-296
         if (S.collision) collision_wall(F, p_unscaled, v, sp, S.bmin, S.bmax);
         float4 po, vo, eo;
         leapfrog(p_unscaled, v, F, X, sp, S.dt, po, vo, eo);

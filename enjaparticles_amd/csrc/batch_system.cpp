@@ -150,6 +150,10 @@ int build_batch_layout(int num_scenes, const int* n_settings, const char* const*
             return set_error(RTPS_ERR_UNKNOWN_SYSTEM, "scene " + std::to_string(s) + ": system is not sph");
         const int cap = st.GetSettingAs<int>("max_num_particles");
         if (cap <= 0) return set_error(RTPS_ERR_ARG, "scene " + std::to_string(s) + ": max_num_particles <= 0");
+        if (integrator_of(st) != 0)
+            return set_error(RTPS_ERR_ARG, "scene " + std::to_string(s) + ": integrator " +
+                                               st.GetSettingAs<std::string>("integrator") +
+                                               " (batched scenes integrate with leapfrog)");
         SceneDev& d = L.scenes[s];
         std::memset(&d, 0, sizeof(d));
         rtps_grid_params gp, gps;
@@ -168,6 +172,7 @@ int build_batch_layout(int num_scenes, const int* n_settings, const char* const*
         d.cap = cap;
         d.cell_base = (int)cells;
         d.collision = 1;
+        gravity_sources_of(st, d.gsrc);
         T += cap;
         cells += gp.nb_cells + 1;
         max_cells = std::max(max_cells, gp.nb_cells);
@@ -178,6 +183,7 @@ int build_batch_layout(int num_scenes, const int* n_settings, const char* const*
     // sort plan: the low local bits (cell << 4 | xq; pads are all ones) then the scene bits
     const int low = sort_key_bits(max_cells) + kSubBits;
     int np = sort_plan(low, L.shifts, L.widths);
+    if (np < 0 || np >= 4) return set_error(RTPS_ERR_ARG, "grid too large for segmented keys");
     const int sb = bits_for((unsigned)num_scenes);
     if (sb > 0) {
         L.shifts[np] = (int)kSceneShift;

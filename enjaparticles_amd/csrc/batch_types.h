@@ -26,6 +26,7 @@ struct SceneDev {
     int cap;                     // slots of the scene (its max_num_particles)
     int cell_base;               // first entry of the scene's cell table
     int collision;               // wall collision on
+    GravitySources gsrc;         // point sources of gravity.cl (n = 0: none)
 };
 
 struct BatchItem {

@@ -14,10 +14,13 @@
 // settings never contain them before calculate() runs, so reference-style
 // settings derive identically).
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "rtps_internal.h"
+#include "sph_device.h"
 
 namespace rtps {
 
@@ -140,6 +143,36 @@ int sort_key_bits(int nb_cells) {
     int bits = 0;
     while ((1u << bits) <= v && bits < 32) ++bits;
     return bits;
+}
+
+// Point sources of gravity.cl (settings gravity_sources / gravity_mass /
+// gravity_alphas, System.cpp:261-265), at most kMaxGravitySources.  With
+// every alpha * mass zero the sources add exactly zero (gravity.cl:25-34), so
+// n is 0 and the kernels skip them.
+void gravity_sources_of(const RTPSSettings& st, GravitySources& g) {
+    std::vector<rtps_float4> src = st.GetSettingAs<std::vector<rtps_float4>>("gravity_sources");
+    std::vector<float> mass = st.GetSettingAs<std::vector<float>>("gravity_mass");
+    std::vector<float> alpha = st.GetSettingAs<std::vector<float>>("gravity_alphas");
+    std::memset(&g, 0, sizeof(g));
+    const int n = (int)std::min(src.size(), (size_t)kMaxGravitySources);
+    bool any = false;
+    for (int i = 0; i < n; ++i) {
+        g.pos[i] = make_float4(src[i].x, src[i].y, src[i].z, src[i].w);
+        g.mass[i] = i < (int)mass.size() ? mass[i] : 0.f;
+        g.alpha[i] = i < (int)alpha.size() ? alpha[i] : 0.f;
+        any = any || g.alpha[i] * g.mass[i] != 0.f;
+    }
+    g.n = any ? n : 0;
+}
+
+// Settings "integrator" (SPH.cpp:89-95, 392-409): 0 = "leapfrog" (also when
+// the key is absent, as in no shipped settings file), 1 = "euler", -1 = any
+// other value (the reference then creates neither integrator and integrate()
+// does nothing; the library rejects it).
+int integrator_of(const RTPSSettings& st) {
+    if (!st.Exists("integrator")) return 0;
+    const std::string v = st.GetSettingAs<std::string>("integrator");
+    return v == "leapfrog" ? 0 : (v == "euler" ? 1 : -1);
 }
 
 }  // namespace rtps

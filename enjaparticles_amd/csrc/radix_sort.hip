@@ -305,10 +305,12 @@ hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int np
     return hipSuccess;
 }
 
-// Passes over the low key_bits bits, split evenly into digits of <= 8 bits.
+// Passes over the low key_bits bits, split evenly into digits of <= 8 bits;
+// shifts/widths hold kMaxPasses entries.  -1 for more than 32 bits.
 int sort_plan(int key_bits, int* shifts, int* widths) {
     const int passes = sort_passes(key_bits);
     if (passes == 0) return 0;
+    if (passes > kMaxPasses || key_bits > 32) return -1;
     const int width = (key_bits + passes - 1) / passes;
     int shift = 0;
     for (int p = 0; p < passes; ++p) {
@@ -323,6 +325,7 @@ hipError_t sort_pairs_pingpong(int n, int key_bits, uint32_t* keys, uint32_t* va
                                uint32_t** out_keys, uint32_t** out_vals) {
     int shifts[kMaxPasses], widths[kMaxPasses];
     const int np = sort_plan(key_bits, shifts, widths);
+    if (np < 0) return hipErrorInvalidValue;
     return sort_pairs_passes(n, shifts, widths, np, keys, vals, temp, st, out_keys, out_vals);
 }
 

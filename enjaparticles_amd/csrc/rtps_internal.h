@@ -28,6 +28,8 @@ void derive_params(RTPSSettings& s, rtps_sph_params& sp, rtps_grid_params& gp, r
 // SPH::updateParams only (re-read of the settings after a change).
 void read_sph_params(const RTPSSettings& s, rtps_sph_params& sp);
 int sort_key_bits(int nb_cells);
+void gravity_sources_of(const RTPSSettings& st, GravitySources& g);
+int integrator_of(const RTPSSettings& st);
 
 // ---- kernels (sph_kernels.hip) ----
 extern int g_density_stage;   // 1 = density rows staged through LDS (default), 0 = read from global memory
@@ -65,6 +67,12 @@ hipError_t launch_leapfrog(int num, const int* d_num, const uint32_t* start, int
                            const float4* vel_s, const float4* force, const float4* xsph, const uint32_t* idx,
                            const rtps_sph_params& sp, float dt, float4* pos_u, float4* vel_u, float4* veleval_u,
                            hipStream_t st, const float4* color_s = nullptr, float4* color_u = nullptr);
+// euler.cl (integrator "euler"); ve_s: the sorted veleval|density records,
+// copied to veleval_u on a deletion step
+hipError_t launch_euler(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
+                        const float4* vel_s, const float4* force, const float4* ve_s, const uint32_t* idx,
+                        const rtps_sph_params& sp, float dt, float4* pos_u, float4* vel_u, float4* veleval_u,
+                        hipStream_t st, const float4* color_s = nullptr, float4* color_u = nullptr);
 hipError_t launch_count_pairs(int num, const int* d_num, const float4* pos_s, const uint32_t* start,
                               const uint32_t* end, const rtps_sph_params& sp, const Grid& g, const Win& win,
                               unsigned long long* out, hipStream_t st);

@@ -492,4 +492,26 @@ __device__ __forceinline__ void leapfrog(float4 p, float4 v, float4 f, float4 xs
     vel_out = vn;
 }
 
+// euler.cl:33-78 for one particle (integrator "euler"): gravity is the
+// kernel's hard-coded -9.8 on z and the clamp its hard-coded 600 on |f.xyz|
+// (the settings' gravity and velocity_limit are not read), the clamp scales
+// all four components, f.w is kept, there is no XSPH term and veleval is not
+// produced.
+__device__ __forceinline__ void euler(float4 p, float4 v, float4 f, const rtps_sph_params& sp, float dt,
+                                      float4& pos_out, float4& vel_out) {
+#pragma clang fp contract(off)
+    const float s = sp.simulation_scale;
+    p.x *= s; p.y *= s; p.z *= s;
+    f.z += -9.8f;
+    float speed = __builtin_sqrtf(f.x * f.x + f.y * f.y + f.z * f.z);
+    if (speed > 600.0f) {
+        float k = 600.0f / speed;
+        f.x *= k; f.y *= k; f.z *= k; f.w *= k;
+    }
+    v.x += dt * f.x; v.y += dt * f.y; v.z += dt * f.z; v.w += dt * f.w;
+    p.x += dt * v.x; p.y += dt * v.y; p.z += dt * v.z;
+    pos_out = make_float4(p.x / s, p.y / s, p.z / s, 1.0f);
+    vel_out = v;
+}
+
 }  // namespace rtps

@@ -725,6 +725,37 @@ __global__ __launch_bounds__(kBlock) void k_leapfrog(int num_host, const int* __
     if (deleted && color_u) color_u[i] = color_s[i];   // leapfrog.cl:107
 }
 
+// a11 alternative: euler.cl:33-78 (integrator "euler").  veleval is not
+// written (euler.cl leaves it alone); on a deletion step the survivors'
+// sorted records become the unsorted ones (call_prep, SPH.cpp:445-457), so
+// veleval and the colours are copied from the sorted arrays there.  The
+// sorted veleval array holds veleval.xyz | density; its copy gets w = 0 (no
+// kernel reads veleval.w).
+__global__ __launch_bounds__(kBlock) void k_euler(int num_host, const int* __restrict__ d_num,
+                                                  const uint32_t* __restrict__ start, int nb_cells,
+                                                  const float4* __restrict__ pos_s, const float4* __restrict__ vel_s,
+                                                  const float4* __restrict__ force, const float4* __restrict__ ve_s,
+                                                  const uint32_t* __restrict__ idx, rtps_sph_params sp, float dt,
+                                                  float4* __restrict__ pos_u, float4* __restrict__ vel_u,
+                                                  float4* __restrict__ veleval_u,
+                                                  const float4* __restrict__ color_s, float4* __restrict__ color_u) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int num = d_num ? *d_num : num_host;
+    bool deleted = false;
+    if (d_num) deleted = effective_num(num, start, nb_cells, num);
+    if (i >= num) return;
+    float4 po, vo;
+    euler(pos_s[i], vel_s[i], force[i], sp, dt, po, vo);
+    const uint32_t o = deleted ? (uint32_t)i : idx[i];
+    pos_u[o] = po;
+    vel_u[o] = vo;
+    if (deleted) {
+        const float4 e = ve_s[i];
+        veleval_u[i] = make_float4(e.x, e.y, e.z, 0.0f);
+        if (color_u) color_u[i] = color_s[i];
+    }
+}
+
 // Pair totals of the neighbour loop on the current sorted state: out[0] =
 // candidates of the reference's 27-cell loop (K), out[1] = accepted pairs
 // (M, self included), out[2] = candidates the windowed rows actually test.
@@ -956,6 +987,17 @@ hipError_t launch_leapfrog(int num, const int* d_num, const uint32_t* start, int
     if (num <= 0) return hipSuccess;
     k_leapfrog<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, start, nb_cells, pos_s, vel_s, force, xsph, idx, sp, dt,
                                                     pos_u, vel_u, veleval_u, color_s, color_u);
+    return hipGetLastError();
+}
+
+hipError_t launch_euler(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
+                        const float4* vel_s, const float4* force, const float4* ve_s, const uint32_t* idx,
+                        const rtps_sph_params& sp, float dt, float4* pos_u, float4* vel_u, float4* veleval_u,
+                        hipStream_t st, const float4* color_s, float4* color_u) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (num <= 0) return hipSuccess;
+    k_euler<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, start, nb_cells, pos_s, vel_s, force, ve_s, idx, sp, dt,
+                                                 pos_u, vel_u, veleval_u, color_s, color_u);
     return hipGetLastError();
 }
 

@@ -39,6 +39,7 @@ struct rtps_sph {
     rtps_grid_params gp{}, gps{};
     Grid g{}, gs{};
     float dt = 0.f;
+    int integrator = 0;   // settings "integrator": 0 = leapfrog, 1 = euler (SPH.cpp:89-95, 392-409)
     int max_num = 0, num_host = 0, nb_cells = 0, key_bits = 0;
     uint32_t pad_key = 0;
     hipStream_t stream = nullptr;
@@ -154,22 +155,7 @@ hipError_t dalloc(T** p, size_t count) {
         if (e_ != hipSuccess) return hip_error(e_, #expr);     \
     } while (0)
 
-void load_gravity_sources(rtps_sph* s) {
-    std::vector<rtps_float4> src = s->settings.GetSettingAs<std::vector<rtps_float4>>("gravity_sources");
-    std::vector<float> mass = s->settings.GetSettingAs<std::vector<float>>("gravity_mass");
-    std::vector<float> alpha = s->settings.GetSettingAs<std::vector<float>>("gravity_alphas");
-    std::memset(&s->gsrc, 0, sizeof(s->gsrc));
-    int n = (int)std::min(src.size(), (size_t)kMaxGravitySources);
-    bool any = false;
-    for (int i = 0; i < n; ++i) {
-        s->gsrc.pos[i] = make_float4(src[i].x, src[i].y, src[i].z, src[i].w);
-        s->gsrc.mass[i] = i < (int)mass.size() ? mass[i] : 0.f;
-        s->gsrc.alpha[i] = i < (int)alpha.size() ? alpha[i] : 0.f;
-        any = any || s->gsrc.alpha[i] * s->gsrc.mass[i] != 0.f;
-    }
-    // With every alpha * mass zero the sources add exactly zero (gravity.cl:25-34).
-    s->gsrc.n = any ? n : 0;
-}
+void load_gravity_sources(rtps_sph* s) { gravity_sources_of(s->settings, s->gsrc); }
 
 // Triangles in the current simulation scale and boundary distance, on the device.
 hipError_t upload_triangles(rtps_sph* s) {
@@ -202,6 +188,7 @@ hipError_t upload_triangles(rtps_sph* s) {
 
 void apply_settings(rtps_sph* s) {
     read_sph_params(s->settings, s->sp);
+    s->integrator = std::max(0, integrator_of(s->settings));
     s->sp.num = s->num_host;
     s->dt = s->settings.GetSettingAs<float>("time_step");
     load_gravity_sources(s);
@@ -259,7 +246,18 @@ int launch_front(rtps_sph* s, StageTimer& t) {
 // (SPH::collision, SPH.cpp:303-336); leapfrog as separate kernels.
 int launch_back(rtps_sph* s) {
     const bool fo = s->opt_force_out != 0;
-    if (s->ntri > 0 && s->opt_collision) {
+    if (s->integrator == 1) {   // euler.cl after force, gravity sources, walls and triangles
+        RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp,
+                                  s->gs, win(s, true), s->force_s, s->xsph_s, s->stream, masks_of(s)));
+        RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
+                                       s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
+        if (s->ntri > 0 && s->opt_collision)
+            RTPS_TRY_HIP(launch_collision_tri(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s,
+                                              s->force_s, s->tri_dev, s->tri_reach, s->ntri, s->sp, s->stream));
+        RTPS_TRY_HIP(launch_euler(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
+                                  s->ve_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u, s->stream,
+                                  col_s(s), col_u(s)));
+    } else if (s->ntri > 0 && s->opt_collision) {
         RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp,
                                   s->gs, win(s, true), s->force_s, s->xsph_s, s->stream, masks_of(s)));
         RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
@@ -475,7 +473,7 @@ std::string tune_key(const rtps_sph* s) {
     while (n2 < (unsigned long)s->num_host) n2 <<= 1;
     return arch + "/cu" + std::to_string(cus) + "/n" + std::to_string(n2) + "/c" + std::to_string(s->nb_cells) +
            "/w" + std::to_string(s->opt_window) + "/r" + std::to_string(s->opt_remap) + "/s" +
-           std::to_string(s->sp.simulation_scale != 1.0f ? 1 : 0) + "/a" + std::to_string(s->opt_autotune == 2 ? 2 : 1);
+           std::to_string(s->sp.simulation_scale != 1.0f ? 1 : 0) + "/a" + std::to_string(s->opt_autotune == 2 ? 2 : 1) + "/i" + std::to_string(s->integrator);
 }
 
 void apply_tune(rtps_sph* s, const Tune& t) {
@@ -559,6 +557,11 @@ int autotune_step(rtps_sph* s, bool measure) {
             w.block = bs;
             w.variant = var;
             const float ms = time_it([&] {
+                // the euler step stores forces and leaves veleval alone: time the unfused force kernel
+                if (s->integrator == 1)
+                    return launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end,
+                                        s->sp, s->gs, w, s->force_s, s->xsph_s, s->stream,
+                                        recs[var == 2 || var == 3 ? var - 1 : 0]);
                 return launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, w,
                                           fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
                                           s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u,
@@ -601,9 +604,14 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
                              "No system of type: " + s->settings.GetSettingAs<std::string>("system"));
         s->max_num = s->settings.GetSettingAs<int>("max_num_particles");
         if (s->max_num <= 0) return set_error(RTPS_ERR_ARG, "max_num_particles must be positive");
+        if (integrator_of(s->settings) < 0)
+            return set_error(RTPS_ERR_ARG, "unknown integrator: " + s->settings.GetSettingAs<std::string>("integrator") +
+                                               " (leapfrog or euler)");
         derive_params(s->settings, s->sp, s->gp, s->gps);
-        if (s->gp.nb_cells <= 0 || s->gp.nb_cells >= (1 << 30))
-            return set_error(RTPS_ERR_ARG, "grid has no cells or too many cells");
+        // keys are (cell << 4) | x sixteenth with cell <= nb_cells: 32 bits hold nb_cells <= 2^28 - 2
+        if (s->gp.nb_cells <= 0 || s->gp.nb_cells > (1 << 28) - 2)
+            return set_error(RTPS_ERR_ARG, "grid has no cells or more than 2^28 - 2 cells (" +
+                                               std::to_string(s->gp.nb_cells) + ")");
         s->g = make_grid(s->gp);
         s->gs = make_grid(s->gps);
         s->nb_cells = s->gp.nb_cells;
@@ -825,7 +833,11 @@ int rtps_sph_update(rtps_sph* s) {
 int rtps_sph_integrate(rtps_sph* s) {
     if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_integrate: null handle");
     if (!s->update_pending) return set_error(RTPS_ERR_STATE, "integrate() needs a preceding update()");
-    RTPS_TRY_HIP(launch_leapfrog(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
+    if (s->integrator == 1)   // SPH::integrate (SPH.cpp:392-409)
+        RTPS_TRY_HIP(launch_euler(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
+                                  s->ve_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u, s->stream,
+                                  col_s(s), col_u(s)));
+    else RTPS_TRY_HIP(launch_leapfrog(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                  s->xsph_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u, s->stream,
                                  col_s(s), col_u(s)));
     RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
@@ -1124,6 +1136,8 @@ int rtps_sph_device_ptr(rtps_sph* s, int field, void** dev) {
 
 int rtps_sph_set_setting(rtps_sph* s, const char* key, const char* val) {
     if (!s || !key || !val) return set_error(RTPS_ERR_ARG, "rtps_sph_set_setting: null argument");
+    if (std::strcmp(key, "integrator") == 0 && std::strcmp(val, "leapfrog") != 0 && std::strcmp(val, "euler") != 0)
+        return set_error(RTPS_ERR_ARG, std::string("unknown integrator: ") + val + " (leapfrog or euler)");
     s->settings.SetSetting<std::string>(key, val);
     return RTPS_OK;
 }

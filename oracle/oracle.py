@@ -93,9 +93,13 @@ def _declare(L):
         "oracle_collision_wall_f64": (None, [i, P, P, P, P, P]),
         "oracle_leapfrog_f32": (None, [i, P, P, P, P, P, P, f, P, P, P]),
         "oracle_leapfrog_f64": (None, [i, P, P, P, P, P, P, d, P, P, P]),
+        "oracle_euler_f32": (None, [i, P, P, P, P, P, f, P, P]),
+        "oracle_euler_f64": (None, [i, P, P, P, P, P, d, P, P]),
         "oracle_count_pairs": (None, [i, P, P, P, P, P, P, P]),
         "oracle_step_f32": (i, [P, P, P, P, P, P, P, f, i, i, P, P, P] + [P] * 10),
         "oracle_step_f64": (i, [P, P, P, P, P, P, P, d, i, i, P, P, P] + [P] * 10),
+        "oracle_step_tl": (i, [P, P, P, P, P, P, P, d, i, i, P, P, P, P, P] + [P] * 10),
+        "oracle_tl_directions": (i, []),
         "oracle_derive_params": (i, [P, P, i, P, P, P]),
         "oracle_num_threads": (i, []),
         "oracle_set_threads": (None, [i]),
@@ -231,6 +235,14 @@ def leapfrog(pos_s, vel_s, force_s, xsph_s, idx, sphp, dt, pos_u, vel_u, veleval
     return pu, vu, eu
 
 
+def euler(pos_s, vel_s, force_s, idx, sphp, dt, pos_u, vel_u):
+    """euler.cl:33-78; returns updated copies of (pos_u, vel_u)."""
+    pu, vu = pos_u.copy(), vel_u.copy()
+    getattr(lib(), "oracle_euler_" + _sfx(pos_s.dtype))(
+        pos_s.shape[0], _p(pos_s), _p(vel_s), _p(force_s), _p(idx), _p(sphp), dt, _p(pu), _p(vu))
+    return pu, vu
+
+
 def count_pairs(pos_s, start, end, sphp, gps):
     K, M = C.c_longlong(), C.c_longlong()
     lib().oracle_count_pairs(pos_s.shape[0], _p(pos_s), _p(start), _p(end), _p(sphp), _p(gps),
@@ -240,6 +252,7 @@ def count_pairs(pos_s, start, end, sphp, gps):
 
 FLAG_COLLISION = 1
 FLAG_BITONIC = 2
+FLAG_EULER = 4      # integrator "euler" (euler.cl) instead of leapfrog
 
 
 class State:
@@ -285,4 +298,54 @@ class State:
                _p(self.force), _p(self.xsph))
         self.num = num.value
         self.deleted += d
+        return d
+
+
+class TLState:
+    """FP64 state carrying kTL tangent directions (forward-mode derivatives of
+    the oracle step, branches fixed as the values decide them).
+
+    pos/vel/veleval hold (n, 4, 1 + kTL) doubles: [..., 0] the value (the FP64
+    oracle's), [..., 1 + k] the derivative along direction k.  Seed the
+    directions with ``dvel0`` ((kTL, n, 4) initial-velocity tangents) and
+    ``dmu`` / ``dK`` (kTL viscosity / gas-constant tangents, applied at every
+    step).  After the steps, ``pos_u[:, :, 1 + k]`` is J_k, the derivative of
+    the final positions along direction k.
+    """
+
+    def __init__(self, pos, vel, sphp, gp, gps, dvel0=None, dmu=None, dK=None):
+        L = lib()
+        self.k = L.oracle_tl_directions()
+        n = pos.shape[0]
+        self.num = n
+        w = 1 + self.k
+        self.pos_u = np.zeros((n, 4, w))
+        self.vel_u = np.zeros((n, 4, w))
+        self.veleval_u = np.zeros((n, 4, w))
+        self.pos_u[:, :, 0] = pos
+        self.vel_u[:, :, 0] = vel
+        if dvel0 is not None:
+            for k in range(self.k):
+                self.vel_u[:, :, 1 + k] = dvel0[k]
+        self.dmu = np.zeros(self.k) if dmu is None else np.asarray(dmu, np.float64)
+        self.dK = np.zeros(self.k) if dK is None else np.asarray(dK, np.float64)
+        self.sphp, self.gp, self.gps = sphp, gp, gps
+        self.keys = np.zeros(n, np.uint32)
+        self.idx = np.zeros(n, np.uint32)
+        self.start = np.full(gp.nb_cells + 1, 666, np.uint32)
+        self.end = np.full(gp.nb_cells + 1, 666, np.uint32)
+        self.pos_s, self.vel_s, self.veleval_s, self.force, self.xsph = (np.zeros((n, 4, w)) for _ in range(5))
+        self.density = np.zeros((n, w))
+        self.src = np.zeros((1, 4), np.float32)
+        self.smass = np.zeros(1, np.float32)
+        self.alpha = np.zeros(1, np.float32)
+
+    def step(self, dt, flags=FLAG_COLLISION):
+        num = C.c_int(self.num)
+        d = lib().oracle_step_tl(C.byref(num), _p(self.pos_u), _p(self.vel_u), _p(self.veleval_u), _p(self.sphp),
+                                 _p(self.gp), _p(self.gps), dt, flags, 0, _p(self.src), _p(self.smass),
+                                 _p(self.alpha), _p(self.dmu), _p(self.dK), _p(self.keys), _p(self.idx),
+                                 _p(self.start), _p(self.end), _p(self.pos_s), _p(self.vel_s), _p(self.veleval_s),
+                                 _p(self.density), _p(self.force), _p(self.xsph))
+        self.num = num.value
         return d

@@ -25,6 +25,7 @@
 //   collision      rtpslib/system/sph/cl_src/collision_wall.cl:31-119,
 //                  rtpslib/system/sph/cl_src/cl_sph_collision.h:31-73
 //   leapfrog       rtpslib/system/sph/cl_src/leapfrog.cl:30-121
+//   euler          rtpslib/system/sph/cl_src/euler.cl:33-78 (integrator = euler)
 //   step order     rtpslib/system/SPH.cpp:149-300, 389-441
 //   parameters     rtpslib/system/SPHSettings.cpp:30-201, rtpslib/RTPSSettings.h:65-85,
 //                  rtpslib/domain/Domain.cpp:38-81, rtpslib/system/System.cpp:294-329
@@ -95,6 +96,68 @@ template <typename T> static inline int sat_f2i(T v) {
 }
 
 template <typename T> struct V4 { T x, y, z, w; };
+
+// ---------------------------------------------------------------------------
+// Tangent-linear arithmetic: a value with kTL directional derivatives
+// (forward-mode differentiation).  Running the FP64 step on TL numbers gives
+// J . u exactly for kTL input directions u at once, where J is the Jacobian
+// of the step at the current state with every branch (neighbour within h,
+// wall contact, friction sign, velocity clamp) fixed as the values decide it:
+// the derivative an adjoint of the same step computes, free of the
+// truncation and branch-switching error of finite differences.  Comparisons
+// and casts act on the value.  Layout: value then the kTL derivatives (as
+// double[1 + kTL]).
+constexpr int kTL = 3;
+struct TL {
+    double v, d[kTL];
+    TL() = default;
+    constexpr TL(double x) : v(x), d{} {}
+    explicit operator double() const { return v; }
+    explicit operator int() const { return (int)v; }
+    TL& operator+=(const TL& o) { v += o.v; for (int k = 0; k < kTL; ++k) d[k] += o.d[k]; return *this; }
+    TL& operator-=(const TL& o) { v -= o.v; for (int k = 0; k < kTL; ++k) d[k] -= o.d[k]; return *this; }
+    TL& operator*=(const TL& o) { for (int k = 0; k < kTL; ++k) d[k] = d[k] * o.v + v * o.d[k]; v *= o.v; return *this; }
+    TL& operator/=(const TL& o) {
+        const double q = v / o.v;
+        for (int k = 0; k < kTL; ++k) d[k] = (d[k] - q * o.d[k]) / o.v;
+        v = q;
+        return *this;
+    }
+};
+static inline TL operator+(TL a, const TL& b) { return a += b; }
+static inline TL operator-(TL a, const TL& b) { return a -= b; }
+static inline TL operator*(TL a, const TL& b) { return a *= b; }
+static inline TL operator/(TL a, const TL& b) { return a /= b; }
+static inline TL operator-(const TL& a) { TL r = a; r.v = -a.v; for (int k = 0; k < kTL; ++k) r.d[k] = -a.d[k]; return r; }
+static inline bool operator<(const TL& a, const TL& b) { return a.v < b.v; }
+static inline bool operator>(const TL& a, const TL& b) { return a.v > b.v; }
+static inline bool operator<=(const TL& a, const TL& b) { return a.v <= b.v; }
+static inline bool operator>=(const TL& a, const TL& b) { return a.v >= b.v; }
+static inline bool operator==(const TL& a, const TL& b) { return a.v == b.v; }
+namespace std {
+static inline TL sqrt(const TL& a) {
+    TL r;
+    r.v = std::sqrt(a.v);
+    for (int k = 0; k < kTL; ++k) r.d[k] = r.v > 0 ? a.d[k] / (2 * r.v) : 0.0;
+    return r;
+}
+}  // namespace std
+
+// Tangents of the scalar parameters (viscosity, gas constant) of the next
+// oracle_step_tl call, one per direction; zero for the other steps.
+static double g_tl_dmu[kTL], g_tl_dK[kTL];
+template <typename T> static inline T param_viscosity(float v) { return (T)v; }
+template <typename T> static inline T param_gas_constant(float v) { return (T)v; }
+template <> inline TL param_viscosity<TL>(float v) {
+    TL r((double)v);
+    for (int k = 0; k < kTL; ++k) r.d[k] = g_tl_dmu[k];
+    return r;
+}
+template <> inline TL param_gas_constant<TL>(float v) {
+    TL r((double)v);
+    for (int k = 0; k < kTL; ++k) r.d[k] = g_tl_dK[k];
+    return r;
+}
 
 template <typename T> static inline V4<T> ld(const T* a, long i) {
     return V4<T>{a[4 * i], a[4 * i + 1], a[4 * i + 2], a[4 * i + 3]};
@@ -296,7 +359,7 @@ static void force_impl(int num, const T* pos_s, const T* density, const T* velev
     const OSPHParams p = *sp;
     const OGridParams gs = *gps;
     const T scale = p.simulation_scale, h = p.smoothing_distance, eps = p.EPSILON;
-    const T K = p.K, rho0 = p.rest_density;
+    const T K = param_gas_constant<T>(p.K), rho0 = p.rest_density;
 #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < num; ++i) {
         V4<T> pi = ld(pos_s, i);
@@ -339,7 +402,7 @@ static void force_impl(int num, const T* pos_s, const T* density, const T* velev
         });
         T a = (T)p.mass * ((T)1 / di);
         T cp = (T)-0.5 * (T)p.wspiky_d_coef;
-        T cv = (T)p.viscosity * (T)p.wvisc_dd_coef;
+        T cv = param_viscosity<T>(p.viscosity) * (T)p.wvisc_dd_coef;
         V4<T> F{a * (cp * fx + cv * vx), a * (cp * fy + cv * vy), a * (cp * fz + cv * vz), a * (cp * (T)0 + cv * vw)};
         T cx = (T)2 * (T)p.mass * (T)p.wpoly6_coef;
         V4<T> X{cx * xx, cx * xy, cx * xz, (T)0};
@@ -480,6 +543,46 @@ extern "C" void oracle_leapfrog_f64(int num, const double* pos_s, const double* 
 }
 
 // ---------------------------------------------------------------------------
+// a11 (alternative): euler.cl:33-78, selected by settings "integrator" =
+// "euler" (SPH.cpp:89-95, 392-409).  Its quirks are restated as they are:
+// gravity is a hard-coded -9.8 on z (the settings' gravity is ignored), the
+// clamp is a hard-coded 600 on |f.xyz| (velocity_limit is ignored) and scales
+// all four components, f.w is not cleared, there is no XSPH term and
+// veleval is not written (it keeps what it held).  Scatters to idx[i].
+template <typename T>
+static void euler_impl(int num, const T* pos_s, const T* vel_s, const T* force, const u32* idx, const OSPHParams* sp,
+                       T dt, T* pos_u, T* vel_u) {
+    const T scale = sp->simulation_scale;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < num; ++i) {
+        V4<T> x = ld(pos_s, i);
+        x.x *= scale; x.y *= scale; x.z *= scale; x.w *= scale;
+        V4<T> v = ld(vel_s, i);
+        V4<T> f = ld(force, i);
+        f.z += (T)-9.8f;
+        T speed = len3(f.x, f.y, f.z);
+        if (speed > (T)600.0f) {
+            T s = (T)600.0f / speed;
+            f.x *= s; f.y *= s; f.z *= s; f.w *= s;
+        }
+        v.x += dt * f.x; v.y += dt * f.y; v.z += dt * f.z; v.w += dt * f.w;
+        x.x += dt * v.x; x.y += dt * v.y; x.z += dt * v.z;
+        x.x /= scale; x.y /= scale; x.z /= scale;
+        long o = idx[i];
+        st(vel_u, o, v);
+        st(pos_u, o, V4<T>{x.x, x.y, x.z, (T)1});
+    }
+}
+extern "C" void oracle_euler_f32(int num, const float* pos_s, const float* vel_s, const float* force, const u32* idx,
+                                 const OSPHParams* sp, float dt, float* pos_u, float* vel_u) {
+    euler_impl<float>(num, pos_s, vel_s, force, idx, sp, dt, pos_u, vel_u);
+}
+extern "C" void oracle_euler_f64(int num, const double* pos_s, const double* vel_s, const double* force,
+                                 const u32* idx, const OSPHParams* sp, double dt, double* pos_u, double* vel_u) {
+    euler_impl<double>(num, pos_s, vel_s, force, idx, sp, dt, pos_u, vel_u);
+}
+
+// ---------------------------------------------------------------------------
 // Candidate (K) and accepted (M) pair counts of the neighbour loop, summed
 // over all particles, on the actual state.  Used for the FLOP model.
 extern "C" void oracle_count_pairs(int num, const float* pos_s, const u32* start, const u32* end, const OSPHParams* sp,
@@ -518,7 +621,9 @@ static double secs_since(OClock::time_point& t) {
 // One full step (SPH::updateGPU + SPH::integrate, SPH.cpp:149-300, 389-441).
 // flags bit0: wall collision on (the shipped code has the call commented out
 //             at SPH.cpp:297; the dam-break workloads enable it);
-//       bit1: use the bitonic network instead of the stable sort.
+//       bit1: use the bitonic network instead of the stable sort;
+//       bit2: integrate with euler.cl instead of leapfrog.cl (SPH::integrate,
+//             SPH.cpp:392-409).
 // State arrays are (max_num x 4) in unsorted (original index) order.  The
 // scratch arrays are filled with the step's sorted intermediates.  *num may
 // shrink through the out-of-grid deletion path (SPH.cpp:220-250).
@@ -562,7 +667,8 @@ static int step_impl(int* num_io, T* pos_u, T* vel_u, T* veleval_u, const OSPHPa
     if (nsrc > 0) gravity_impl<T>(num, nsrc, src, smass, alpha, pos_s, force, (T)p.simulation_scale);
     if (flags & 1) collision_impl<T>(num, pos_s, vel_s, force, &p, gps);
     g_stage_s[6] += secs_since(t);
-    leapfrog_impl<T>(num, pos_s, vel_s, force, xsph, idx, &p, dt, pos_u, vel_u, veleval_u);
+    if (flags & 4) euler_impl<T>(num, pos_s, vel_s, force, idx, &p, dt, pos_u, vel_u);
+    else leapfrog_impl<T>(num, pos_s, vel_s, force, xsph, idx, &p, dt, pos_u, vel_u, veleval_u);
     g_stage_s[7] += secs_since(t);
     *num_io = num;
     return deleted;
@@ -581,6 +687,29 @@ extern "C" int oracle_step_f64(int* num, double* pos_u, double* vel_u, double* v
     return step_impl<double>(num, pos_u, vel_u, veleval_u, sp, gp, gps, dt, flags, nsrc, src, smass, alpha, keys,
                              idx, start, end, pos_s, vel_s, veleval_s, density, force, xsph);
 }
+
+// Tangent-linear FP64 step (see TL above): state arrays hold (1 + kTL)
+// doubles per component (value, then the derivative along each of the kTL
+// input directions), i.e. double[max_num][4][1 + kTL]; dmu / dK: the
+// viscosity's and gas constant's tangent per direction.  The values follow
+// oracle_step_f64 exactly.
+extern "C" int oracle_step_tl(int* num, double* pos_u, double* vel_u, double* veleval_u, const OSPHParams* sp,
+                              const OGridParams* gp, const OGridParams* gps, double dt, int flags, int nsrc,
+                              const float* src, const float* smass, const float* alpha, const double* dmu,
+                              const double* dK, u32* keys, u32* idx, u32* start, u32* end, double* pos_s,
+                              double* vel_s, double* veleval_s, double* density, double* force, double* xsph) {
+    static_assert(sizeof(TL) == (1 + kTL) * sizeof(double), "TL is value + kTL derivatives");
+    for (int k = 0; k < kTL; ++k) {
+        g_tl_dmu[k] = dmu ? dmu[k] : 0.0;
+        g_tl_dK[k] = dK ? dK[k] : 0.0;
+    }
+    const int d = step_impl<TL>(num, (TL*)pos_u, (TL*)vel_u, (TL*)veleval_u, sp, gp, gps, TL(dt), flags, nsrc, src,
+                                smass, alpha, keys, idx, start, end, (TL*)pos_s, (TL*)vel_s, (TL*)veleval_s,
+                                (TL*)density, (TL*)force, (TL*)xsph);
+    for (int k = 0; k < kTL; ++k) g_tl_dmu[k] = g_tl_dK[k] = 0.0;
+    return d;
+}
+extern "C" int oracle_tl_directions(void) { return kTL; }
 
 // Per-stage seconds of the last oracle step (8 values, see g_stage_s).
 extern "C" void oracle_last_stage_times(double* out) {

@@ -61,7 +61,12 @@ def test_derive_params_reference_defaults():
     sp = a[0]
     # mass = 0.0256 / (int)log2(262144) = 0.0256 / 18, rounded to 6 digits
     assert sp.mass == np.float32(float(f"{np.float32(0.0256 / 18):.6g}"))
-    assert sp.smoothing_distance == np.float32(float(f"{2 * np.float32(sp.rest_distance):.6g}")) or True
+    # rest_distance = .87 (m / rho0)^(1/3) and h = 2 rest_distance from the
+    # unrounded local floats (SPHSettings.cpp:53,58), each stored with 6 digits
+    m = np.float32(0.0256 / 18)
+    rd = np.float32(0.87 * np.float64(np.power(np.float32(m / np.float32(1000)), np.float32(1 / 3))))
+    assert sp.rest_distance == np.float32(float(f"{rd:.6g}"))
+    assert sp.smoothing_distance == np.float32(float(f"{np.float32(2) * rd:.6g}"))
 
 
 def test_unknown_system_is_an_error():
@@ -80,3 +85,30 @@ def test_argument_errors_need_no_gpu():
     assert L.rtps_gravity(10, 99, None, None, None, None, None, 1.0, None) == N.RTPS_ERR_ARG
     assert L.rtps_sort_key_bits(27000) == 15
     assert L.rtps_sort_key_bits(614125) == 20
+
+
+def test_unknown_integrator_is_an_error():
+    """integrator is "leapfrog" or "euler" (SPH.cpp:89-95); anything else is
+    refused before any device memory is allocated."""
+    import ctypes as C
+    st = scenes.dam_settings(1.0, 64)
+    st["integrator"] = "rk4"
+    ka, va, n = N.settings_arrays(st)
+    h = C.c_void_p()
+    rc = N.lib().rtps_sph_create(C.cast(ka, C.c_void_p), C.cast(va, C.c_void_p), n, C.byref(h))
+    assert rc == N.RTPS_ERR_ARG and not h.value
+    assert b"rk4" in N.lib().rtps_last_error()
+
+
+def test_oversized_grid_is_an_error():
+    """Keys are (cell << 4) | x sixteenth in 32 bits: grids of 2^28 - 1 cells or
+    more are refused (here 754^3 cells: a 30-unit box at h = 0.04)."""
+    import ctypes as C
+    st = scenes.dam_settings(30.0, 64)
+    ka, va, n = N.settings_arrays(st)
+    h = C.c_void_p()
+    rc = N.lib().rtps_sph_create(C.cast(ka, C.c_void_p), C.cast(va, C.c_void_p), n, C.byref(h))
+    assert rc == N.RTPS_ERR_ARG and not h.value
+    assert b"2^28" in N.lib().rtps_last_error()
+    rc = N.lib().rtps_adj_create(C.cast(ka, C.c_void_p), C.cast(va, C.c_void_p), n, 64, 10, 20, C.byref(h))
+    assert rc == N.RTPS_ERR_ARG and not h.value

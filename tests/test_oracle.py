@@ -300,6 +300,43 @@ def test_free_fall_and_clamp():
     assert abs(vu[0, 2] - 600 * dt) <= 1e-9
 
 
+def test_euler_free_fall_ignores_settings_gravity_and_limit():
+    """euler.cl:33-78: gravity is a hard-coded -9.8 on z whatever the settings
+    say, the clamp a hard-coded 600 (velocity_limit is not read), no XSPH, and
+    veleval is left as it was.  Closed forms of semi-implicit Euler free fall:
+    v_n = -9.8 n dt, p_n = p_0 - 9.8 dt^2 n (n + 1) / 2."""
+    st = scenes.dam_settings(1.0, 16)
+    st["gravity"] = "2 0 -3 0"
+    st["velocity_limit"] = "0.001"
+    st["integrator"] = "euler"
+    sp, gp, gs = O.derive_params(st)
+    dt = float(np.float32(0.003))
+    pos = np.array([[0.5, 0.5, 0.8, 1.0]], np.float64)
+    ve0 = np.array([[0.1, 0.2, 0.3, 0.0]], np.float64)
+    s = O.State(pos, np.zeros_like(pos), sp, gp, gs, dtype=np.float64, veleval=ve0)
+    g = float(np.float32(-9.8))
+    for n in range(1, 11):
+        s.step(dt, O.FLAG_EULER)
+        assert s.vel_u[0, 0] == 0.0 and s.pos_u[0, 0] == 0.5
+        assert abs(s.vel_u[0, 2] - g * n * dt) <= 1e-12
+        assert abs(s.pos_u[0, 2] - (0.8 + g * dt * dt * n * (n + 1) / 2)) <= 1e-12
+        np.testing.assert_array_equal(s.veleval_u, ve0)
+    # the same particle under leapfrog follows the settings' gravity instead
+    s2 = O.State(pos, np.zeros_like(pos), sp, gp, gs, dtype=np.float64)
+    s2.step(dt, 0)
+    assert s2.vel_u[0, 0] != 0.0
+    # clamp at 600 on |f.xyz| (after the -9.8), scaling f.w too; none at 100
+    z = np.zeros_like(pos)
+    f = np.array([[0.0, 0.0, 1e4, 5.0]], np.float64)
+    pu, vu = O.euler(pos, z, f, np.zeros(1, np.uint32), sp, dt, z, z)
+    k = 600.0 / (1e4 + g)
+    assert abs(vu[0, 2] - 600.0 * dt) <= 1e-9 and abs(vu[0, 3] - 5.0 * k * dt) <= 1e-12
+    f = np.array([[0.0, 100.0, 0.0, 0.0]], np.float64)
+    pu, vu = O.euler(pos, z, f, np.zeros(1, np.uint32), sp, dt, z, z)
+    assert abs(vu[0, 1] - 100.0 * dt) <= 1e-12 and abs(vu[0, 2] - g * dt) <= 1e-12
+    assert pu[0, 3] == 1.0
+
+
 def test_deletion_path():
     sc = scenes.dam_custom(8, 8, 8, seed=2)
     sp, gp, gs = O.derive_params(sc.settings)
@@ -328,3 +365,57 @@ def test_fp32_tracks_fp64():
         b.step(sc.dt, O.FLAG_COLLISION)
     np.testing.assert_allclose(a.pos_u[:, :3], b.pos_u[:, :3], rtol=1e-5)
     assert np.abs(a.density - b.density).max() <= 1e-4 * np.abs(b.density).max()
+
+
+def _tl_vs_fd(pos, vel, st, steps, origin_note):
+    """Forward-mode tangents of the FP64 step (TLState) against FP64 central
+    differences along the same directions: one random initial-velocity
+    direction, the viscosity and the gas constant (their FP32 parameter
+    values perturbed by +-1e-4 relative, divided by the FP32 difference)."""
+    sp, gp, gs = O.derive_params(st)
+    n = len(pos)
+    rng = np.random.default_rng(3)
+    u = np.zeros((3, n, 4))
+    u[0, :, :3] = rng.normal(size=(n, 3))
+    dt = float(np.float32(st["time_step"]))
+    tl = O.TLState(pos.astype(np.float64), vel.astype(np.float64), sp, gp, gs, dvel0=u, dmu=[0, 1, 0],
+                   dK=[0, 0, 1])
+    for _ in range(steps):
+        tl.step(dt, O.FLAG_COLLISION)
+
+    def run(v, mu, K):
+        sp2, _, _ = O.derive_params(st)
+        sp2.viscosity, sp2.K = mu, K
+        s = O.State(pos.astype(np.float64), v, sp2, gp, gs, dtype=np.float64)
+        for _ in range(steps):
+            s.step(dt, O.FLAG_COLLISION)
+        return s.pos_u.copy()
+
+    v = vel.astype(np.float64)
+    # the tangent state's values are the FP64 oracle's, bit for bit
+    np.testing.assert_array_equal(tl.pos_u[:, :, 0], run(v, sp.viscosity, sp.K))
+    d = 1e-6
+    fd = (run(v + d * u[0], sp.viscosity, sp.K) - run(v - d * u[0], sp.viscosity, sp.K)) / (2 * d)
+    assert np.abs(fd - tl.pos_u[:, :, 1]).max() <= 1e-6 * np.abs(fd).max(), origin_note
+    for k, name in ((2, "viscosity"), (3, "K")):
+        base = sp.viscosity if name == "viscosity" else sp.K
+        hi, lo = float(np.float32(base * (1 + 1e-4))), float(np.float32(base * (1 - 1e-4)))
+        a = run(v, hi, sp.K) if name == "viscosity" else run(v, sp.viscosity, hi)
+        b = run(v, lo, sp.K) if name == "viscosity" else run(v, sp.viscosity, lo)
+        fd = (a - b) / (hi - lo)
+        assert np.abs(fd - tl.pos_u[:, :, k]).max() <= 1e-5 * np.abs(fd).max(), (origin_note, name)
+
+
+def test_tangent_linear_step_matches_central_differences():
+    """Pins the forward-mode oracle (the reference for the adjoint's
+    dot-product tests) where the step is smooth along the directions: a block
+    falling freely (10 steps) and a block resting on the floor and two walls
+    (wall penalty, damping and friction active; 10 steps)."""
+    pos = scenes.lattice(8, 8, 4, 21, origin=(0.34, 0.34, 0.45))
+    vel = np.zeros_like(pos)
+    vel[:, :3] = scenes.normal(22, 3 * len(pos), 0.05).reshape(-1, 3)
+    _tl_vs_fd(pos, vel, scenes.dam_settings(1.0, len(pos), dt=0.002), 10, "free block")
+    pos = scenes.lattice(8, 8, 5, 23, origin=(0.0, 0.0, 0.0))
+    vel = np.zeros_like(pos)
+    vel[:, :3] = scenes.normal(24, 3 * len(pos), 0.05).reshape(-1, 3)
+    _tl_vs_fd(pos, vel, scenes.dam_settings(0.5, len(pos), dt=0.002), 10, "walls")
