@@ -27,13 +27,52 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-updates/sec at 1M SPH (plain vs batched vs fwd+adjoint); HBM % peak"
 PEAK_HBM_GBS = 8000.0          # MI355X HBM3E
-PEAK_VALU_FP32_TFLOPS = 78.6   # non-packed FP32 VALU (the kernels are compiled without packed FP32)
-PEAK_PACKED_FP32_TFLOPS = 157.3  # published FP32 peak, packed (two lanes per VALU op)
-# HBM bytes per launch of the fused force kernel on this workload (loop 4,
-# 64-thread blocks), from two rocprofv3 --pmc runs over the first 320 steps:
-# FETCH_SIZE 227966 KiB x 2.0 + WRITE_SIZE 78307 KiB x 1.0, the factors read off
-# a 256 MiB device copy in the same runs (profiles/dam1m_counters.txt).
-FORCE_HBM_BYTES_PER_LAUNCH = (227966 * 2.0 + 78307 * 1.0) * 1024
+PEAK_FP32_TFLOPS = 157.3       # MI355X published FP32 vector peak (packed: two lanes per VALU op)
+PEAK_VALU_FP32_TFLOPS = 78.6   # non-packed FP32 VALU rate (the kernels are compiled without packed FP32)
+# The force kernel's HBM bytes per launch on the timed state (steps 6-25 of
+# the dam break) come from the committed counter summary of that state: a line
+#   traffic <kernel> <grid> <block> <bytes per launch>
+# written from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by a
+# device copy of known size in the same runs.
+COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "dam1m_timed_counters.txt")
+
+
+def measured_traffic(kernel: str = "k_force", grid: int = 1048576, block: int = 64):
+    """HBM bytes per launch of `kernel` from the counter summary, or None."""
+    try:
+        with open(COUNTER_SUMMARY) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 5 and p[0] == "traffic" and p[1] == kernel and int(p[2]) == grid and int(p[3]) == block:
+                    return float(p[4])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+class _Marker:
+    """roctx range around the timed region (recorded by rocprofv3
+    --marker-trace, so a kernel trace can be cut to the timed steps); a no-op
+    where libroctx64 cannot be loaded."""
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("libroctx64.so.4", "libroctx64.so", "/opt/rocm/lib/libroctx64.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+                break
+            except OSError:
+                continue
+
+    def push(self, name: str):
+        if self.lib:
+            self.lib.roctxRangePushA(name.encode())
+
+    def pop(self):
+        if self.lib:
+            self.lib.roctxRangePop()
 
 
 def _cpu_model() -> str:
@@ -279,6 +318,27 @@ def triangles_leg(s, ntri: int, steps: int = 50):
             "extra_ms": with_tri - base}
 
 
+def long_run_leg(s, steps: int, first: int):
+    """The timed system stepped `steps` more steps (graph replays, stage
+    timers on): the dam's representative state, after the fluid has left the
+    lattice (more neighbours per particle than in the first steps)."""
+    import torch
+    s.resetTimers()
+    s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    stages = {k: round(v[0], 5) for k, v in s.stageTimes().items() if v[1] > 0}
+    K, M, Kt = s.countPairs()
+    n = s.getNum()
+    return {"steps": steps, "first_step": first, "last_step": first + steps - 1,
+            "ms_per_step": 1e3 * dt / steps, "value": n * steps / dt, "unit": "particle-updates/s",
+            "stages_ms": stages, "pairs_per_particle_at_end": {"candidates_27cell": K / n, "accepted_incl_self": M / n,
+                                                               "candidates_tested": Kt / n}}
+
+
 def surface_leg(s, res: int, iso: float = 0.5, reps: int = 10):
     """Colour field and iso-surface of the current state (SPH::postProcess):
     wall time per call, each ending in a stream synchronisation."""
@@ -406,6 +466,8 @@ def main():
     ap.add_argument("--force-variant", type=int, default=4,
                     help="process-default force loop before autotuning: 4 = direct with LDS-staged rows, "
                          "0 = direct, 1 = compacted neighbour lists")
+    ap.add_argument("--long-steps", type=int, default=1000,
+                    help="steps of the long-run leg that continues the timed system (0 = skip)")
     ap.add_argument("--triangles", type=int, default=2048,
                     help="collision triangles of the triangle leg on the final state (0 = skip)")
     ap.add_argument("--surface-res", type=int, default=128,
@@ -460,15 +522,18 @@ def main():
     s.resetTimers()
     counts = [s.countPairs()]
     elapsed, done = 0.0, 0
+    marker = _Marker()
     while done < args.steps:
         k = min(args.chunk, args.steps - done)
         s.synchronize()
         torch.cuda.synchronize()
+        marker.push("timed")
         t0 = time.perf_counter()
         s.step(k)
         s.synchronize()
         torch.cuda.synchronize()
         elapsed += time.perf_counter() - t0
+        marker.pop()
         done += k
         counts.append(s.countPairs())
         print(f"# {done} steps, {elapsed:.3f} s", file=sys.stderr, flush=True)
@@ -543,23 +608,29 @@ def main():
             "bound": "valu",
             "kernel": "k_force (fused force + walls + leapfrog)",
             "achieved": achieved,
-            "peak": PEAK_VALU_FP32_TFLOPS,
+            "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_VALU_FP32_TFLOPS,
-            "peak_packed": PEAK_PACKED_FP32_TFLOPS,
-            "frac_packed": achieved / PEAK_PACKED_FP32_TFLOPS,
-            "traffic": FORCE_HBM_BYTES_PER_LAUNCH,
-            "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "peak_nonpacked": PEAK_VALU_FP32_TFLOPS,
+            "frac_nonpacked": achieved / PEAK_VALU_FP32_TFLOPS,
+            "traffic": measured_traffic(),
+            "traffic_unit": "HBM bytes per launch on the timed state (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
+            "algorithmic_bytes": 100.0 * n,
             "flop_per_launch": force_flop,
             "avg_launch_ms": force_ms,
             "launches": stages["force"][1],
         }
         out["density_roofline"] = {
             "achieved_tflops": dens_flop / (dens_ms * 1e-3) / 1e12,
-            "frac": dens_flop / (dens_ms * 1e-3) / 1e12 / PEAK_VALU_FP32_TFLOPS,
-            "frac_packed": dens_flop / (dens_ms * 1e-3) / 1e12 / PEAK_PACKED_FP32_TFLOPS,
+            "frac": dens_flop / (dens_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+            "frac_nonpacked": dens_flop / (dens_ms * 1e-3) / 1e12 / PEAK_VALU_FP32_TFLOPS,
             "avg_launch_ms": dens_ms,
         }
+    if args.long_steps > 0 and not args.profile:
+        print("# long-run leg ...", file=sys.stderr, flush=True)
+        out["long_run"] = long_run_leg(s, args.long_steps, args.warmup + args.steps + 1)
+        print(f"# long run: {out['long_run']['value']:.3e} p-u/s over steps {out['long_run']['first_step']}-"
+              f"{out['long_run']['last_step']}", file=sys.stderr, flush=True)
     if args.surface_res > 0 and not args.profile:
         print("# surface leg ...", file=sys.stderr, flush=True)
         out["surface"] = surface_leg(s, args.surface_res)

@@ -25,7 +25,7 @@ FIELD = {
     "color": 13, "color_s": 14, "mass": 15, "mass_s": 16,
 }
 (OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS, OPT_FORCE_VARIANT, OPT_WINDOW, OPT_TILE_REMAP, OPT_AUTOTUNE,
- OPT_SORT_TILE, OPT_COLOR, OPT_DENSITY_STAGE, OPT_BLOCK_DENSITY, OPT_BLOCK_FORCE) = range(13)
+ OPT_SORT_TILE, OPT_COLOR, OPT_DENSITY_STAGE, OPT_BLOCK_DENSITY, OPT_BLOCK_FORCE, OPT_SURFACE_FRAME) = range(14)
 NUM_STAGES = 6
 STAGE_NAMES = ("hash", "sort", "cellindices", "permute", "density", "force")
 MAX_GRAVITY_SOURCES = 16
@@ -69,6 +69,11 @@ SIGNATURES = {
     "rtps_sort_key_bits": (_i, [_i]),
     "rtps_sort_temp_bytes": (_sz, [_i, _i]),
     "rtps_sort_pairs": (_i, [_i, _i, P, P, P, _sz, P]),
+    "rtps_sort_segments": (_i, [_i, _i, _i, P, P, P, P, P, _sz, P]),
+    "rtps_sph_set_rigid_particles": (_i, [P, P, P, P, _i, _f, _f]),
+    "rtps_sph_set_static_boundary": (_i, [P, P, _i, _f, _f]),
+    "rtps_sph_interact": (_i, [P]),
+    "rtps_mc_geometry": (_i, [P, P]),
     "rtps_cellindices": (_i, [_i, P, _i, P, P, P, P]),
     "rtps_permute": (_i, [_i, P, P, P, P, P, P, P, P]),
     "rtps_density": (_i, [_i, P, P, P, P, P, P, P]),

@@ -89,6 +89,18 @@ int rtps_sort_pairs(int n, int key_bits, uint32_t* keys, uint32_t* vals, void* t
     return RTPS_OK;
 }
 
+int rtps_sort_segments(int batch, int array_length, int dir, uint32_t* dst_keys, uint32_t* dst_vals,
+                       const uint32_t* src_keys, const uint32_t* src_vals, void* temp, size_t temp_bytes,
+                       void* stream) {
+    RTPS_CHECK(batch >= 0 && array_length >= 0, "rtps_sort_segments: negative batch or array_length");
+    if (array_length < 2 || batch == 0) return RTPS_OK;   // BitonicSort.cpp: arrayLength < 2 returns 0
+    RTPS_CHECK((long)batch * array_length <= 0x7FFFFFFFL, "rtps_sort_segments: more than 2^31 - 1 pairs");
+    RTPS_CHECK(dst_keys && dst_vals && src_keys && src_vals && temp, "rtps_sort_segments: null buffer");
+    RTPS_CHECK(temp_bytes >= sort_temp_bytes(array_length, 32), "rtps_sort_segments: temp buffer too small");
+    RTPS_HIP(sort_segments(batch, array_length, dir, dst_keys, dst_vals, src_keys, src_vals, temp, S(stream)));
+    return RTPS_OK;
+}
+
 int rtps_cellindices(int num, const uint32_t* keys, int nb_cells, uint32_t* start, uint32_t* end, int* nc,
                      void* stream) {
     RTPS_CHECK(num >= 0 && nb_cells > 0 && start && end && (num == 0 || keys), "rtps_cellindices: bad arguments");

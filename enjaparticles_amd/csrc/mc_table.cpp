@@ -1,147 +1,102 @@
-// Marching-cubes case table builder (see mc_table.h for the rule).
+// Marching-cubes case table (see mc_table.h for the layout and conventions).
+//
+// The 256 words restate the standard marching-cubes triangle table as
+// traversehp.cl:74 lists it (triTable[case * 16 + i], -1 terminated), packed
+// four bits per edge id.  tests/test_surface_cpu.py checks the unpacked table
+// against a SHA-256 fingerprint of the reference's triTable
+// (tests/golden/mc_tables.json) and its per-case counts against
+// classifycubes.cl:37's nrOfTriangles.
 
 #include "mc_table.h"
 
 #include <cstring>
 #include <mutex>
-#include <utility>
-#include <vector>
 
 namespace rtps {
 
-namespace {
-
-// Corner cycles of the six faces (adjacent corners share an edge).
-constexpr int kFace[6][4] = {{0, 2, 6, 4}, {1, 3, 7, 5},    // x = 0, x = 1
-                             {0, 1, 5, 4}, {2, 3, 7, 6},    // y = 0, y = 1
-                             {0, 1, 3, 2}, {4, 5, 7, 6}};   // z = 0, z = 1
-
-int edge_of(int a, int b) {
-    if (a > b) std::swap(a, b);
-    for (int e = 0; e < 12; ++e)
-        if (kMcEdge[e][0] == a && kMcEdge[e][1] == b) return e;
-    return -1;
-}
-
-void corner_pos(int k, double p[3]) {
-    p[0] = k & 1;
-    p[1] = (k >> 1) & 1;
-    p[2] = (k >> 2) & 1;
-}
-
-// Two cube edges on a common face (each edge lies on the two faces fixing
-// one of the coordinates its corners share).
-bool share_face(int e1, int e2) {
-    const int a1 = kMcEdge[e1][0], b1 = kMcEdge[e1][1], a2 = kMcEdge[e2][0], b2 = kMcEdge[e2][1];
-    for (int axis = 0; axis < 3; ++axis) {
-        const int bit = 1 << axis;
-        if ((a1 & bit) != (b1 & bit) || (a2 & bit) != (b2 & bit)) continue;   // axis not fixed on both
-        if ((a1 & bit) == (a2 & bit)) return true;
-    }
-    return false;
-}
-
-// Triangulates the loop polygon poly (vertices in loop order) without a
-// diagonal between two vertices on one cube face: such a diagonal would lie in
-// the face, where the neighbouring cube has its own surface.  Side (p0, p_last)
-// takes the first apex j (in loop order) whose diagonals are allowed and whose
-// two sub-polygons triangulate; triangles (a, b, c) in loop order are stored
-// as (a, c, b).  False if no such triangulation exists.
-bool triangulate(const std::vector<int>& poly, std::vector<int>& out) {
-    const size_t k = poly.size();
-    if (k < 3) return true;
-    if (k == 3) {
-        out.insert(out.end(), {poly[0], poly[2], poly[1]});
-        return true;
-    }
-    for (size_t j = 1; j + 1 < k; ++j) {
-        if (j > 1 && share_face(poly[0], poly[j])) continue;
-        if (j + 2 < k && share_face(poly[j], poly[k - 1])) continue;
-        std::vector<int> left(poly.begin(), poly.begin() + j + 1), right(poly.begin() + j, poly.end());
-        std::vector<int> tris;
-        if (!triangulate(left, tris) || !triangulate(right, tris)) continue;
-        tris.insert(tris.end(), {poly[0], poly[k - 1], poly[j]});
-        out.insert(out.end(), tris.begin(), tris.end());
-        return true;
-    }
-    return false;
-}
-
-}  // namespace
-
-bool build_mc_table(McTable& t) {
-    std::memset(t.tri, -1, sizeof(t.tri));
-    std::memset(t.ntri, 0, sizeof(t.ntri));
-    // face corner cycles, counter-clockwise seen from outside the cube
-    int face[6][4];
-    for (int f = 0; f < 6; ++f) {
-        double p[4][3], c[3] = {0, 0, 0};
-        for (int i = 0; i < 4; ++i) {
-            face[f][i] = kFace[f][i];
-            corner_pos(kFace[f][i], p[i]);
-            for (int k = 0; k < 3; ++k) c[k] += 0.25 * p[i][k];
-        }
-        const double u[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
-        const double v[3] = {p[2][0] - p[1][0], p[2][1] - p[1][1], p[2][2] - p[1][2]};
-        const double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
-        const double out = n[0] * (c[0] - 0.5) + n[1] * (c[1] - 0.5) + n[2] * (c[2] - 0.5);
-        if (out < 0) std::swap(face[f][1], face[f][3]);
-    }
-    for (int m = 0; m < 256; ++m) {
-        auto above = [&](int k) { return (m >> k) & 1; };
-        // directed segments: on each face, from the edge leaving a run of
-        // above-iso corners to the edge entering it (the run on the left seen
-        // from outside); a crossing edge lies on two faces, leaving the run
-        // on one and entering one on the other, so next[] is a permutation
-        int next[12];
-        for (int e = 0; e < 12; ++e) next[e] = -1;
-        for (int f = 0; f < 6; ++f) {
-            const int* c = face[f];
-            for (int i = 0; i < 4; ++i) {
-                if (!above(c[i]) || above(c[(i + 3) % 4])) continue;   // i starts a run
-                int j = i;
-                int len = 0;
-                while (above(c[(j + 1) % 4]) && len < 4) { j = (j + 1) % 4; ++len; }
-                if (len >= 3) break;   // whole face above
-                const int e_in = edge_of(c[(i + 3) % 4], c[i]);
-                const int e_out = edge_of(c[j], c[(j + 1) % 4]);
-                if (next[e_out] != -1) return false;
-                next[e_out] = e_in;
-            }
-        }
-        bool seen[12] = {false};
-        int nt = 0;
-        for (int e0 = 0; e0 < 12; ++e0) {
-            if (next[e0] < 0 || seen[e0]) continue;
-            std::vector<int> loop;
-            int cur = e0;
-            do {
-                loop.push_back(cur);
-                seen[cur] = true;
-                cur = next[cur];
-            } while (cur >= 0 && cur != e0 && loop.size() <= 12);
-            if (cur != e0) return false;
-            // wound so the right-hand normal points from the above-iso
-            // corners to the others
-            std::vector<int> tris;
-            if (!triangulate(loop, tris)) return false;
-            for (size_t q = 0; q < tris.size(); q += 3) {
-                if (nt >= kMcMaxTris) return false;
-                for (int v = 0; v < 3; ++v) t.tri[m][3 * nt + v] = (int8_t)tris[q + v];
-                ++nt;
-            }
-        }
-        t.ntri[m] = (uint8_t)nt;
-    }
-    return true;
-}
+const uint64_t kMcCaseWords[256] = {
+    0xffffffffffffffffull, 0xfffffffffffff380ull, 0xfffffffffffff910ull, 0xffffffffff189381ull,
+    0xfffffffffffffa21ull, 0xffffffffffa21380ull, 0xffffffffff920a29ull, 0xfffffff89a8a2382ull,
+    0xfffffffffffff2b3ull, 0xffffffffff0b82b0ull, 0xffffffffffb32091ull, 0xfffffffb89b912b1ull,
+    0xffffffffff3ab1a3ull, 0xfffffffab8a801a0ull, 0xfffffff9ab9b3093ull, 0xffffffffffb8aa89ull,
+    0xfffffffffffff874ull, 0xffffffffff437034ull, 0xffffffffff748910ull, 0xfffffff137174914ull,
+    0xffffffffff748a21ull, 0xfffffffa21403743ull, 0xfffffff748209a29ull, 0xffff4973727929a2ull,
+    0xffffffffff2b3748ull, 0xfffffff40242b74bull, 0xfffffffb32748109ull, 0xffff1292b9b49b74ull,
+    0xfffffff487ab31a3ull, 0xffff4b7401b41ab1ull, 0xffff30bab9b09874ull, 0xfffffffab99b4b74ull,
+    0xfffffffffffff459ull, 0xffffffffff380459ull, 0xffffffffff051450ull, 0xfffffff513538458ull,
+    0xffffffffff459a21ull, 0xfffffff594a21803ull, 0xfffffff204245a25ull, 0xffff8434535235a2ull,
+    0xffffffffffb32459ull, 0xfffffff594b802b0ull, 0xfffffffb32510450ull, 0xffff584b82852512ull,
+    0xfffffff45931ab3aull, 0xffffab81a8180594ull, 0xffff30bab5b05045ull, 0xfffffffb8aa85845ull,
+    0xffffffffff975879ull, 0xfffffff375359039ull, 0xfffffff751710870ull, 0xffffffffff753351ull,
+    0xfffffff21a759879ull, 0xffff37503505921aull, 0xffff25a758528208ull, 0xfffffff7533525a2ull,
+    0xfffffff2b3987597ull, 0xffffb72029279759ull, 0xffff751871810b32ull, 0xfffffff51771b12bull,
+    0xffffb3a31a758859ull, 0xf0aba010b7905075ull, 0xf07570805a30b0abull, 0xffffffffff5b75abull,
+    0xfffffffffffff56aull, 0xffffffffff6a5380ull, 0xffffffffff6a5109ull, 0xfffffff6a5891381ull,
+    0xffffffffff162561ull, 0xfffffff803621561ull, 0xfffffff620609569ull, 0xffff823625285895ull,
+    0xffffffffff56ab32ull, 0xfffffff56a02b80bull, 0xfffffff6a5b32910ull, 0xffffb892b92916a5ull,
+    0xfffffff315356b36ull, 0xffff6b51505b0b80ull, 0xffff9505606306b3ull, 0xfffffff89bb96956ull,
+    0xffffffffff8746a5ull, 0xfffffffa56374034ull, 0xfffffff7486a5091ull, 0xffff49737179156aull,
+    0xfffffff874156216ull, 0xffff743403625521ull, 0xffff620560509748ull, 0xf962695923497937ull,
+    0xfffffff56a4872b3ull, 0xffffb720242746a5ull, 0xffff6a5b32874910ull, 0xf6a54b7b492b9129ull,
+    0xffff6b51535b3748ull, 0xfb404b7b016b5b15ull, 0xf74836b630560950ull, 0xffff9b7974b96956ull,
+    0xffffffffffa4694aull, 0xfffffff380a946a4ull, 0xfffffff04606a10aull, 0xffffa16468618138ull,
+    0xfffffff462421941ull, 0xffff462942921803ull, 0xffffffffff624420ull, 0xfffffff624428238ull,
+    0xfffffff32b46a94aull, 0xffff6a4a94b82280ull, 0xffffa164606102b3ull, 0xf1b8b12184a16146ull,
+    0xffff36b319639469ull, 0xf14641916b0181b8ull, 0xfffffff4600636b3ull, 0xffffffffff86b846ull,
+    0xfffffffa98a876a7ull, 0xffffa76a907a0370ull, 0xffff0818717a176aull, 0xfffffff37117a76aull,
+    0xffff768981861621ull, 0xf937390976192962ull, 0xfffffff206607087ull, 0xffffffffff276237ull,
+    0xffff76898a86ab32ull, 0xf7a9a76790b72702ull, 0xfb32a767a1871081ull, 0xffff17616a71b12bull,
+    0xf63136b619768698ull, 0xffffffffff76b190ull, 0xffff06b0b3607087ull, 0xfffffffffffff6b7ull,
+    0xfffffffffffffb67ull, 0xffffffffff67b803ull, 0xffffffffff67b910ull, 0xfffffff67b138918ull,
+    0xffffffffff7b621aull, 0xfffffff7b6803a21ull, 0xfffffff7b69a2092ull, 0xffff89a38a3a27b6ull,
+    0xffffffffff726327ull, 0xfffffff026067807ull, 0xfffffff910732672ull, 0xffff678891681261ull,
+    0xfffffff73171a67aull, 0xffff801781a7167aull, 0xffff7a69a0a70730ull, 0xfffffff9a88a7a67ull,
+    0xffffffffff68b486ull, 0xfffffff640603b63ull, 0xfffffff109648b68ull, 0xffff63b139369649ull,
+    0xfffffff1a28b6486ull, 0xffff640b60b03a21ull, 0xffff9a2920b648b4ull, 0xf36463b34923a39aull,
+    0xfffffff264248328ull, 0xffffffffff264240ull, 0xffff834642432091ull, 0xfffffff642241491ull,
+    0xffff1a6648168318ull, 0xfffffff40660a01aull, 0xf39a9303a6834364ull, 0xffffffffff4a649aull,
+    0xffffffffffb67594ull, 0xfffffff67b594380ull, 0xfffffffb67045105ull, 0xffff51345343867bull,
+    0xfffffffb6721a459ull, 0xffff594380a217b6ull, 0xffff204a24a45b67ull, 0xf67b25a523453843ull,
+    0xfffffff945267327ull, 0xffff786260680459ull, 0xffff045051673263ull, 0xf851584812786826ull,
+    0xffff73167161a459ull, 0xf459078701671a61ull, 0xfa737a6a305a4a04ull, 0xffffa84a458a7a67ull,
+    0xfffffff98b9b6596ull, 0xffff590650360b63ull, 0xffffb65510b508b0ull, 0xfffffff1355363b6ull,
+    0xffff65b8b9b59a21ull, 0xfa21965690b603b0ull, 0xf52025a50865b58bull, 0xffff35a3a25363b6ull,
+    0xffff283265825985ull, 0xfffffff260069659ull, 0xf826283865081851ull, 0xffffffffff612651ull,
+    0xf698965683a61631ull, 0xffff06505960a01aull, 0xffffffffffa65830ull, 0xfffffffffffff65aull,
+    0xffffffffffb57a5bull, 0xfffffff03857ba5bull, 0xfffffff091ba57b5ull, 0xffff1381897ba57aull,
+    0xfffffff15717b21bull, 0xffffb27571721380ull, 0xffff7b2209729579ull, 0xf289823295b27257ull,
+    0xfffffff573532a52ull, 0xffff52a578258028ull, 0xffff2a37353a5109ull, 0xf25752a278129289ull,
+    0xffffffffff573531ull, 0xfffffff571170780ull, 0xfffffff735539309ull, 0xffffffffff795789ull,
+    0xfffffff8ba8a5485ull, 0xffff03bba50b5405ull, 0xffff54aba8a48910ull, 0xf41314943b54a4baull,
+    0xffff8548b2582152ull, 0xfb151b2b543b0b40ull, 0xf58b8545b2950520ull, 0xffffffffff3b2549ull,
+    0xffff483543253a52ull, 0xfffffff0244252a5ull, 0xf910854583a532a3ull, 0xffff2492914252a5ull,
+    0xfffffff153358548ull, 0xffffffffff501540ull, 0xffff530509358548ull, 0xfffffffffffff549ull,
+    0xfffffffba9b947b4ull, 0xffffba97b9794380ull, 0xffffb470414b1ba1ull, 0xf4bab474a1843413ull,
+    0xffff219b294b97b4ull, 0xf3801b2b197b9479ull, 0xfffffff04224b47bull, 0xffff42343824b47bull,
+    0xffff947732972a92ull, 0xf70207872a4797a9ull, 0xfa040a1a472a3a73ull, 0xffffffffff4782a1ull,
+    0xfffffff317714194ull, 0xffff178180714194ull, 0xffffffffff347304ull, 0xfffffffffffff784ull,
+    0xffffffffff8ba8a9ull, 0xfffffffa9bb93903ull, 0xfffffffba88a0a10ull, 0xffffffffffa3ba13ull,
+    0xfffffff8b99b1b21ull, 0xffff9b2921b93903ull, 0xffffffffffb08b20ull, 0xfffffffffffffb23ull,
+    0xfffffff98aa82832ull, 0xffffffffff2902a9ull, 0xffff8a1810a82832ull, 0xfffffffffffff2a1ull,
+    0xffffffffff819831ull, 0xfffffffffffff190ull, 0xfffffffffffff830ull, 0xffffffffffffffffull,
+};
 
 const McTable& mc_table() {
     static McTable table;
     static std::once_flag once;
-    static bool ok = false;
-    std::call_once(once, [] { ok = build_mc_table(table); });
-    (void)ok;   // the builder succeeds (every case needs at most 5 triangles; checked by the table test)
+    std::call_once(once, [] {
+        std::memset(table.tri, -1, sizeof(table.tri));
+        for (int c = 0; c < 256; ++c) {
+            int n = 0;
+            for (int i = 0; i < 15; ++i) {
+                const int e = (int)((kMcCaseWords[c] >> (4 * i)) & 0xF);
+                if (e == 0xF) break;
+                table.tri[c][i] = (int8_t)e;
+                ++n;
+            }
+            table.ntri[c] = (uint8_t)(n / 3);
+        }
+    });
     return table;
 }
 

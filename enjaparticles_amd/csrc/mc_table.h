@@ -1,42 +1,39 @@
-// Marching-cubes case table, generated at run time (no stored table).
+// Marching-cubes case table of SPH::postProcess: the standard table of
+// Lorensen & Cline's marching cubes in the layout traversehp.cl:74 holds it
+// (Bourke's tabulation, 16 entries per case), with the reference's corner and
+// edge numbering, so the same field gives the reference's triangles.
 //
-// Corner k of a cube sits at offset (k & 1, (k >> 1) & 1, (k >> 2) & 1).
-// Edge e joins corners kMcEdge[e][0] < kMcEdge[e][1], which differ in one
-// bit: edges 0-3 run along x, 4-7 along y, 8-11 along z.  Case m has bit k
-// set when corner k lies above the iso level.
-//
-// For every face, each maximal run of consecutive above-iso corners (in the
-// face's corner cycle) is cut off by one segment joining the two crossing
-// edges that bound the run; on a face whose above-iso corners are diagonal
-// this separates them.  The rule depends only on the face's four values, so
-// the two cubes sharing a face cut it the same way and the surface is closed.
-// Each segment is directed with its run on the left as seen from outside the
-// cube, so the two cubes sharing a face traverse its segments in opposite
-// directions and the surface is consistently oriented.  The segments chain
-// into loops of crossing edges, starting at their lowest edge; each loop is
-// triangulated without a diagonal between two vertices on one face (which
-// would lie in the face, on the neighbour's surface), wound so normals
-// (right-hand rule) point from the above-iso corners to the others.
+// Corner k of a cube (bit k of the case, set when the node value exceeds the
+// iso level, classifycubes.cl:44-55) sits at lattice offset kMcCorner[k].
+// Edge e runs from kMcEdgeEnds[e][0..2] to kMcEdgeEnds[e][3..5]
+// (traversehp.cl:35-72); a vertex is interpolated from the first end toward
+// the second.  Case c lists its triangles' edges, three per triangle, as the
+// 4-bit fields of kMcCaseWords[c] from the lowest up, ended by 0xF: one 8-byte
+// word per case (2 KB in all), a single scalar load per cube on the GPU.
 #pragma once
 
 #include <cstdint>
 
 namespace rtps {
 
-constexpr int kMcMaxTris = 5;   // triangles per case (the builder fails if a case needs more)
+constexpr int kMcMaxTris = 5;   // triangles per case
 
 struct McTable {
     int8_t tri[256][16];   // edge ids, three per triangle, -1 padded
     uint8_t ntri[256];
 };
 
-constexpr int kMcEdge[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7},    // x
-                                {0, 2}, {1, 3}, {4, 6}, {5, 7},    // y
-                                {0, 4}, {1, 5}, {2, 6}, {3, 7}};   // z
+constexpr int8_t kMcCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1},
+                                    {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
 
-// Builds the table; false if a case needs more than kMcMaxTris triangles.
-bool build_mc_table(McTable& t);
-// The process-wide table (built on first use).
+constexpr int8_t kMcEdgeEnds[12][6] = {
+    {0, 0, 0, 1, 0, 0}, {1, 0, 0, 1, 0, 1}, {1, 0, 1, 0, 0, 1}, {0, 0, 1, 0, 0, 0},
+    {0, 1, 0, 1, 1, 0}, {1, 1, 0, 1, 1, 1}, {1, 1, 1, 0, 1, 1}, {0, 1, 1, 0, 1, 0},
+    {0, 0, 0, 0, 1, 0}, {1, 0, 0, 1, 1, 0}, {1, 0, 1, 1, 1, 1}, {0, 0, 1, 0, 1, 1}};
+
+extern const uint64_t kMcCaseWords[256];
+
+// The unpacked table (tri / ntri), for the device upload and the C ABI.
 const McTable& mc_table();
 
 }  // namespace rtps

@@ -1,26 +1,33 @@
 // Stable LSD radix sort of (u32 key, u32 value) pairs for gfx950, replacing
 // the reference's bitonic network (System::bitonic_sort, System.cpp:407-480;
-// Bitonic<T>::Sort, BitonicSort.cpp:62-183).  Only the low key_bits bits are
-// ordered (the cell keys are < 2^key_bits); digits are at most 8 bits wide and
-// the bits are split evenly over ceil(key_bits / 8) passes.
+// Bitonic<T>::Sort, BitonicSort.cpp:62-183).  Only the key bits named by the
+// passes are ordered (the cell keys are < 2^key_bits); digits are at most 8
+// bits wide and the bits are split evenly over ceil(key_bits / 8) passes.
 //
-// Per pass (R = 2^bits buckets, tiles of 256 x ITEMS keys, ITEMS = 4, 8 or
-// 16; g_sort_items picks it, by default 16 from 2^19 keys up and 8 below; the
-// tile kernels run 1024 threads of ITEMS / 4 keys each, 16 waves per tile for
-// latency hiding):
-//   k_radix_hist    per-tile digit histogram, digit-major in hist[d * ntiles + t];
-//                   lanes of a wave holding the same digit are found with
-//                   `bits` ballots and add their count with one LDS atomic;
-//                   digit totals accumulated in gtot[d]
-//   k_radix_scan    one block per digit: exclusive scan of its row plus the
-//                   total of all smaller digits -> global offsets
-//   k_radix_scatter stable ranking inside the tile: wave w owns the tile's
-//                   w-th sixteenth and walks it in rounds of 64 keys; the rank
-//                   of a key = popcount of the lower lanes with the same digit
-//                   (ballots) + the wave's running count of that digit (LDS);
-//                   a 16-wave exclusive scan per digit then gives each wave's
-//                   offset and the key goes to digit offset + rank.  Block 0
-//                   clears gtot for the next use.
+// One sort = 1 + P launches (P passes), single pass over the keys per digit:
+//   k_radix_hist_all  one read of the keys: the digit histograms of all P
+//                     passes at once (lanes of a wave holding the same digit
+//                     are found with ballots and add their count with one LDS
+//                     atomic; block totals added to gcount[p][d]).  It also
+//                     clears the tile counters and the first look-back status
+//                     array of the passes.
+//   k_onesweep        one launch per pass, 1024 threads per tile of 1024 x IT
+//                     keys (IT = 1, 2, 4 or 8).  A tile takes the next tile
+//                     number from an atomic counter (so every smaller tile is
+//                     already running), ranks its keys stably (wave w owns
+//                     the tile's w-th sixteenth, rounds of 64 keys: rank = same
+//                     digit in lower lanes by ballots + the wave's running
+//                     count + an exclusive scan over the 16 waves), publishes
+//                     its per-digit counts and looks back over the earlier
+//                     tiles' published counts / inclusive prefixes (decoupled
+//                     look-back: status word = 2-bit flag | 30-bit count) for
+//                     its digits' global offsets.  The keys are then placed in
+//                     digit order in LDS and written out, so each digit's run
+//                     of a tile is one contiguous, coalesced write.  Passes
+//                     alternate between two status arrays: a pass clears the
+//                     one the next pass uses (the pass before it is finished).
+// gcount is zero between sorts: the last tile of each pass clears its pass's
+// row after its look-back (every tile has read the row by then).
 // Keys and values ping-pong between the caller's arrays and the alternate
 // arrays in the temp buffer.
 
@@ -30,52 +37,55 @@
 
 namespace rtps {
 
-constexpr int kSortThreads = 256;   // tile = kSortThreads x ITEMS keys; also the scan kernel's block
-constexpr int kSortMinItems = 4;    // smallest tile: sizes the tile-histogram buffer
-constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kTileThreads = 1024;  // threads of the histogram and scatter kernels (ITEMS / 4 keys each)
+constexpr int kTileThreads = 1024;  // threads of the histogram and one-sweep kernels
 constexpr int kTileWaves = kTileThreads / 64;
-
-// 256 x g_sort_items keys per sort tile (4, 8 or 16; 0 = by size: 4096-key
-// tiles measured 77 us for 1M keys against 84 us with 2048 and 115 us with
-// 1024, 65 against 67 us at 655,360 keys; 2048-key tiles are best at 393,216
-// and tie with 1024 at 196,608)
-int g_sort_items = 0;
 constexpr int kMaxRadix = 256;
 constexpr int kMaxPasses = 4;
 
+// Keys per thread of a tile (1, 2, 4 or 8; 0 = by size: 4 from 2^19 keys up, else 2)
+int g_sort_items = 0;
+
 int sort_passes(int key_bits) { return key_bits <= 0 ? 0 : (key_bits + 7) / 8; }
 
-static inline int ntiles_for(int n, int items) { return (n + kSortThreads * items - 1) / (kSortThreads * items); }
+static inline int ntiles_for(int n, int it) { return (n + kTileThreads * it - 1) / (kTileThreads * it); }
 
-size_t sort_temp_bytes(int n, int key_bits) {
-    (void)key_bits;
-    const size_t nt = (size_t)ntiles_for(n > 0 ? n : 1, kSortMinItems);
-    size_t b = (size_t)kMaxPasses * kMaxRadix * sizeof(uint32_t);   // digit totals per pass (fixed offset 0)
-    b += 2 * (size_t)n * sizeof(uint32_t);                           // alternate keys / values
-    b = (b + 255) & ~(size_t)255;
-    b += (size_t)kMaxRadix * nt * sizeof(uint32_t);                  // tile histograms
-    return b;
-}
+// Status word of the look-back: flag in bits 31-30, count in bits 29-0.
+constexpr uint32_t kFlagAgg = 1u << 30, kFlagPrefix = 2u << 30, kCountMask = (1u << 30) - 1u;
+constexpr int kMaxSortKeys = (1 << 30) - 1;
 
 struct SortTemp {
+    uint32_t* gcount;   // [kMaxPasses][kMaxRadix], zero between sorts (offset 0: independent of n)
+    uint32_t* ctr;      // [kMaxPasses] tile counters
+    uint32_t* status;   // [2][status_words]
+    long status_words;  // tiles x kMaxRadix for the smallest tile
     uint32_t* kalt;
     uint32_t* valt;
-    uint32_t* hist;
-    uint32_t* gtot;
 };
 
-// gtot sits at offset 0 so its (kept zero) contents do not depend on n.
 static SortTemp carve(void* temp, int n) {
     char* p = (char*)temp;
     SortTemp t;
-    t.gtot = (uint32_t*)p;
+    t.gcount = (uint32_t*)p;
     size_t off = (size_t)kMaxPasses * kMaxRadix * sizeof(uint32_t);
+    t.ctr = (uint32_t*)(p + off);
+    off += 256;
+    t.status_words = (long)ntiles_for(n > 0 ? n : 1, 1) * kMaxRadix;
+    t.status = (uint32_t*)(p + off);
+    off += 2 * (size_t)t.status_words * sizeof(uint32_t);
+    off = (off + 255) & ~(size_t)255;
     t.kalt = (uint32_t*)(p + off);
     t.valt = t.kalt + n;
-    off = (off + 2 * (size_t)n * sizeof(uint32_t) + 255) & ~(size_t)255;
-    t.hist = (uint32_t*)(p + off);
     return t;
+}
+
+size_t sort_temp_bytes(int n, int key_bits) {
+    (void)key_bits;
+    const long nn = n > 0 ? n : 1;
+    size_t b = (size_t)kMaxPasses * kMaxRadix * sizeof(uint32_t) + 256;
+    b += 2 * (size_t)ntiles_for((int)nn, 1) * kMaxRadix * sizeof(uint32_t);
+    b = (b + 255) & ~(size_t)255;
+    b += 2 * (size_t)nn * sizeof(uint32_t);   // alternate keys / values
+    return b;
 }
 
 // Lanes of the calling wave whose digit equals this lane's (restricted to `valid` lanes).
@@ -89,39 +99,6 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, bool valid
     return peers;
 }
 
-// IT = keys per thread (tile = kTileThreads x IT keys).
-template <int IT>
-__global__ __launch_bounds__(kTileThreads) void k_radix_hist(const uint32_t* __restrict__ kin, int n, int shift,
-                                                             int bits, uint32_t* __restrict__ hist,
-                                                             uint32_t* __restrict__ gtot, int ntiles) {
-    __shared__ uint32_t h[kMaxRadix];
-    const int R = 1 << bits;
-    const uint32_t mask = (uint32_t)R - 1u;
-    const int lane = threadIdx.x & 63;
-    const uint64_t lt = (1ull << lane) - 1ull;
-    for (int d = threadIdx.x; d < R; d += kTileThreads) h[d] = 0;
-    __syncthreads();
-    const long base = (long)blockIdx.x * (kTileThreads * IT);
-    // all IT loads first (index clamped to n - 1, n >= 2: no branch, one wait)
-    uint32_t kk[IT];
-#pragma unroll
-    for (int k = 0; k < IT; ++k) kk[k] = kin[min(base + k * kTileThreads + threadIdx.x, (long)n - 1)];
-#pragma unroll
-    for (int k = 0; k < IT; ++k) {
-        const long e = base + k * kTileThreads + threadIdx.x;
-        const bool valid = e < n;
-        const uint32_t d = valid ? (kk[k] >> shift) & mask : 0u;
-        const uint64_t peers = match_digit(d, bits, valid);
-        if (valid && (peers & lt) == 0ull) atomicAdd(&h[d], (uint32_t)__popcll(peers));
-    }
-    __syncthreads();
-    for (int d = threadIdx.x; d < R; d += kTileThreads) {
-        const uint32_t c = h[d];
-        hist[(long)d * ntiles + blockIdx.x] = c;
-        if (c) atomicAdd(&gtot[d], c);
-    }
-}
-
 // Inclusive scan of v across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const int lane = threadIdx.x & 63;
@@ -133,108 +110,164 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
-// Exclusive block scan (256 threads); returns the exclusive prefix, total in *tot.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* tot) {
-    __shared__ uint32_t ws[kSortWaves];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t inc = wave_incl_scan(v);
-    if (lane == 63) ws[w] = inc;
+struct PassPlan {
+    int shift[kMaxPasses];
+    int bits[kMaxPasses];
+    int npasses;
+};
+
+// Digit histograms of every pass in one read of the keys; clears the tile
+// counters and the status array of even passes (status0).
+template <int IT>
+__global__ __launch_bounds__(kTileThreads) void k_radix_hist_all(const uint32_t* __restrict__ kin, int n, PassPlan plan,
+                                                                 uint32_t* __restrict__ gcount,
+                                                                 uint32_t* __restrict__ ctr,
+                                                                 uint32_t* __restrict__ status0, long status_words) {
+    __shared__ uint32_t h[kMaxPasses][kMaxRadix];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int e = threadIdx.x; e < kMaxPasses * kMaxRadix; e += kTileThreads) (&h[0][0])[e] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kMaxPasses) ctr[threadIdx.x] = 0;
+    for (long e = (long)blockIdx.x * kTileThreads + threadIdx.x; e < status_words; e += (long)gridDim.x * kTileThreads)
+        status0[e] = 0;
     __syncthreads();
-    uint32_t before = 0, all = 0;
+    const long base = (long)blockIdx.x * (kTileThreads * IT);
+    uint32_t kk[IT];
 #pragma unroll
-    for (int q = 0; q < kSortWaves; ++q) {
-        uint32_t s = ws[q];
-        before += q < w ? s : 0u;
-        all += s;
+    for (int k = 0; k < IT; ++k) kk[k] = kin[min(base + k * kTileThreads + threadIdx.x, (long)n - 1)];
+    for (int p = 0; p < plan.npasses; ++p) {
+        const int shift = plan.shift[p], bits = plan.bits[p];
+        const uint32_t mask = (1u << bits) - 1u;
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const bool valid = base + k * kTileThreads + threadIdx.x < n;
+            const uint32_t d = valid ? (kk[k] >> shift) & mask : 0u;
+            const uint64_t peers = match_digit(d, bits, valid);
+            if (valid && (peers & lt) == 0ull) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
+        }
     }
     __syncthreads();
-    *tot = all;
+    for (int e = threadIdx.x; e < plan.npasses * kMaxRadix; e += kTileThreads) {
+        const uint32_t c = (&h[0][0])[e];
+        if (c) atomicAdd(&gcount[e], c);
+    }
+}
+
+// Exclusive scan over the kMaxRadix entries of a[] held by threads
+// 0..kMaxRadix-1 (the first four waves); every thread of the block calls it.
+__device__ __forceinline__ uint32_t digit_excl_scan(uint32_t v, uint32_t* wsum /* kMaxRadix / 64 entries */) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if (threadIdx.x < kMaxRadix && lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxRadix / 64; ++q) before += q < w ? wsum[q] : 0u;
+    __syncthreads();
     return before + inc - v;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_radix_scan(uint32_t* __restrict__ hist,
-                                                             const uint32_t* __restrict__ gtot, int ntiles) {
-    const int d = blockIdx.x;
-    // base = total count of all smaller digits
-    uint32_t part = 0;
-    for (int e = threadIdx.x; e < d; e += kSortThreads) part += gtot[e];
-    uint32_t base;
-    block_excl_scan(part, &base);
-    // exclusive scan of row d, ntiles entries, in contiguous chunks per thread
-    uint32_t* row = hist + (long)d * ntiles;
-    const int c = (ntiles + kSortThreads - 1) / kSortThreads;
-    const int lo = threadIdx.x * c;
-    const int hi = min(lo + c, ntiles);
-    uint32_t s = 0;
-    for (int e = lo; e < hi; ++e) s += row[e];
-    uint32_t tot;
-    uint32_t run = base + block_excl_scan(s, &tot);
-    for (int e = lo; e < hi; ++e) {
-        uint32_t v = row[e];
-        row[e] = run;
-        run += v;
-    }
-}
-
 template <int IT>
-__global__ __launch_bounds__(kTileThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
-                                                                const uint32_t* __restrict__ vin,
-                                                                uint32_t* __restrict__ kout,
-                                                                uint32_t* __restrict__ vout, int n, int shift,
-                                                                int bits, const uint32_t* __restrict__ hist,
-                                                                int ntiles, uint32_t* __restrict__ gtot_clear) {
-    constexpr int kPerWave = 64 * IT;                  // keys per wave, in IT rounds of 64
-    __shared__ uint32_t cnt[kTileWaves][kMaxRadix];    // running count per (wave, digit), then wave offsets
-    __shared__ uint32_t goff[kMaxRadix];
+__global__ __launch_bounds__(kTileThreads) void k_onesweep(const uint32_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                           int n, int shift, int bits,
+                                                           uint32_t* __restrict__ gcount_p,
+                                                           uint32_t* __restrict__ ctr_p, uint32_t* status,
+                                                           uint32_t* __restrict__ status_clear, long status_words,
+                                                           int ntiles) {
+    constexpr int kTile = kTileThreads * IT;
+    constexpr int kPerWave = 64 * IT;
+    __shared__ uint32_t cnt[kTileWaves][kMaxRadix];   // per-wave running counts, then wave prefixes
+    __shared__ uint32_t loc[kMaxRadix];                // tile-local digit offsets
+    __shared__ uint32_t gofs[kMaxRadix];               // global start of the tile's run of each digit
+    __shared__ uint32_t wsum[kMaxRadix / 64];
+    __shared__ uint32_t sk[kTile], sv[kTile];
+    __shared__ int s_tile;
     const int R = 1 << bits;
     const uint32_t mask = (uint32_t)R - 1u;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint64_t lt = (1ull << lane) - 1ull;
+    if (tid == 0) s_tile = (int)atomicAdd(ctr_p, 1u);
     for (int e = tid; e < kTileWaves * kMaxRadix; e += kTileThreads) (&cnt[0][0])[e] = 0;
-    for (int d = tid; d < R; d += kTileThreads) goff[d] = hist[(long)d * ntiles + blockIdx.x];
-    if (blockIdx.x == 0)
-        for (int d = tid; d < R; d += kTileThreads) gtot_clear[d] = 0;
+    // the status array of the next pass (the pass before this one is finished)
+    for (long e = (long)blockIdx.x * kTileThreads + tid; e < status_words; e += (long)gridDim.x * kTileThreads)
+        status_clear[e] = 0;
     __syncthreads();
-
-    const long base = (long)blockIdx.x * (kTileThreads * IT) + (long)w * kPerWave;
+    const int t = s_tile;
+    const long tbase = (long)t * kTile;
+    const long base = tbase + (long)w * kPerWave;
     uint32_t key[IT], val[IT], pos[IT];
-    // all loads first (index clamped to n - 1, n >= 2: no branch, one wait)
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
         const long e = min(base + k * 64 + lane, (long)n - 1);
         key[k] = kin[e];
         val[k] = vin[e];
     }
+    // this pass's digit totals, staged in LDS before the barrier below, so
+    // every tile has read the row before it publishes (the last tile clears it)
+    if (tid < R) gofs[tid] = gcount_p[tid];
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
-        const long e = base + k * 64 + lane;
-        const bool valid = e < n;
-        if (!valid) key[k] = 0xFFFFFFFFu;
+        const bool valid = base + k * 64 + lane < n;
         const uint32_t d = (key[k] >> shift) & mask;
-        const uint64_t peers = match_digit(d, bits, valid);
-        const uint32_t before = cnt[w][d];                 // this wave's earlier rounds
+        const uint64_t peers = match_digit(valid ? d : 0u, bits, valid);
+        const uint32_t before = valid ? cnt[w][d] : 0u;
         pos[k] = before + (uint32_t)__popcll(peers & lt);
         if (valid && (peers & lt) == 0ull) cnt[w][d] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    for (int d = tid; d < R; d += kTileThreads) {          // exclusive scan over the 16 waves
-        uint32_t run = 0;
-#pragma unroll
+    const uint32_t gc = tid < R ? gofs[tid] : 0u;
+    uint32_t total = 0;
+    if (tid < R) {   // exclusive scan over the 16 waves; publish the tile's count
         for (int q = 0; q < kTileWaves; ++q) {
-            const uint32_t c = cnt[q][d];
-            cnt[q][d] = run;
-            run += c;
+            const uint32_t c = cnt[q][tid];
+            cnt[q][tid] = total;
+            total += c;
+        }
+        uint32_t* st = status + (long)t * kMaxRadix + tid;
+        __hip_atomic_store(st, (t == 0 ? kFlagPrefix : kFlagAgg) | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const uint32_t gstart = digit_excl_scan(gc, wsum);
+    const uint32_t lstart = digit_excl_scan(total, wsum);
+    if (tid < R) {   // look back over the earlier tiles for this digit
+        uint32_t excl = 0;
+        for (int j = t - 1; j >= 0; --j) {
+            const uint32_t* st = status + (long)j * kMaxRadix + tid;
+            uint32_t v;
+            while (((v = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & ~kCountMask) == 0u)
+                __builtin_amdgcn_s_sleep(1);
+            excl += v & kCountMask;
+            if (v & kFlagPrefix) break;
+        }
+        if (t > 0)
+            __hip_atomic_store(status + (long)t * kMaxRadix + tid, kFlagPrefix | (excl + total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        loc[tid] = lstart;
+        gofs[tid] = gstart + excl;
+    }
+    __syncthreads();
+    if (t == ntiles - 1 && tid < R) gcount_p[tid] = 0;   // every tile has read it (they all published)
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {   // the keys in digit order, in LDS
+        if (base + k * 64 + lane < n) {
+            const uint32_t d = (key[k] >> shift) & mask;
+            const uint32_t q = loc[d] + cnt[w][d] + pos[k];
+            sk[q] = key[k];
+            sv[q] = val[k];
         }
     }
     __syncthreads();
+    const int tn = (int)min((long)kTile, (long)n - tbase);
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
-        const long e = base + k * 64 + lane;
-        if (e < n) {
-            const uint32_t d = (key[k] >> shift) & mask;
-            const uint32_t p = goff[d] + cnt[w][d] + pos[k];
-            kout[p] = key[k];
-            vout[p] = val[k];
+        const int q = k * kTileThreads + tid;
+        if (q < tn) {
+            const uint32_t kq = sk[q];
+            const uint32_t d = (kq >> shift) & mask;
+            const uint32_t o = gofs[d] + (uint32_t)q - loc[d];
+            kout[o] = kq;
+            vout[o] = sv[q];
         }
     }
 }
@@ -260,42 +293,57 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Zeroes the digit counts (once per temp buffer; every sort leaves them zero).
 hipError_t sort_clear_temp(void* temp, int n, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     SortTemp t = carve(temp, n);
-    return launch_fill_u32(t.gtot, 0u, (long)kMaxPasses * kMaxRadix, st);
+    return launch_fill_u32(t.gcount, 0u, (long)kMaxPasses * kMaxRadix, st);
 }
 
 // Sorts (keys, vals) by the bits [shift[p], shift[p] + width[p]) of each pass
 // p (least significant range first, every width <= 8).  The result ends in
 // (keys, vals) or in the temp buffer's alternate arrays, returned through
-// out_keys / out_vals.  gtot must be zero on entry (sort_clear_temp once;
-// every pass leaves it zero again).
+// out_keys / out_vals.  The temp buffer's digit counts must be zero on entry
+// (sort_clear_temp once; every sort leaves them zero again).
 hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int npasses, uint32_t* keys,
                              uint32_t* vals, void* temp, hipStream_t st, uint32_t** out_keys, uint32_t** out_vals) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     *out_keys = keys;
     *out_vals = vals;
     if (n <= 1 || npasses == 0) return hipSuccess;
-    if (npasses > kMaxPasses) return hipErrorInvalidValue;
+    if (npasses > kMaxPasses || n > kMaxSortKeys) return hipErrorInvalidValue;
     SortTemp t = carve(temp, n);
-    const int items = g_sort_items == 4 || g_sort_items == 8 || g_sort_items == 16 ? g_sort_items
-                      : (n >= (1 << 19) ? 16 : 8);
-    const int ntiles = ntiles_for(n, items);
+    PassPlan plan{};
+    plan.npasses = npasses;
+    for (int p = 0; p < npasses; ++p) {
+        if (widths[p] < 1 || widths[p] > 8 || shifts[p] < 0 || shifts[p] + widths[p] > 32) return hipErrorInvalidValue;
+        plan.shift[p] = shifts[p];
+        plan.bits[p] = widths[p];
+    }
+    const int it = g_sort_items == 1 || g_sort_items == 2 || g_sort_items == 4 || g_sort_items == 8
+                       ? g_sort_items : (n >= (1 << 19) ? 4 : 2);
+    const int ntiles = ntiles_for(n, it);
+    uint32_t* st0 = t.status;
+    uint32_t* st1 = t.status + t.status_words;
+#define RTPS_HIST(I) \
+    k_radix_hist_all<I><<<ntiles, kTileThreads, 0, st>>>(keys, n, plan, t.gcount, t.ctr, st0, t.status_words)
+    if (it == 1) { RTPS_HIST(1); } else if (it == 2) { RTPS_HIST(2); } else if (it == 4) { RTPS_HIST(4); }
+    else { RTPS_HIST(8); }
+#undef RTPS_HIST
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     uint32_t *ki = keys, *vi = vals, *ko = t.kalt, *vo = t.valt;
     for (int p = 0; p < npasses; ++p) {
-        const int bits = widths[p], shift = shifts[p];
-        if (bits < 1 || bits > 8) return hipErrorInvalidValue;
-        uint32_t* gtot = t.gtot + p * kMaxRadix;
-#define RTPS_SORT_PASS(I)                                                                                           \
-        k_radix_hist<I / 4><<<ntiles, kTileThreads, 0, st>>>(ki, n, shift, bits, t.hist, gtot, ntiles);               \
-        k_radix_scan<<<1 << bits, kSortThreads, 0, st>>>(t.hist, gtot, ntiles);                                      \
-        k_radix_scatter<I / 4><<<ntiles, kTileThreads, 0, st>>>(ki, vi, ko, vo, n, shift, bits, t.hist, ntiles, gtot)
-        if (items == 4) { RTPS_SORT_PASS(4); }
-        else if (items == 8) { RTPS_SORT_PASS(8); }
-        else { RTPS_SORT_PASS(16); }
-#undef RTPS_SORT_PASS
-        hipError_t e = hipGetLastError();
+        uint32_t* cur = (p & 1) ? st1 : st0;
+        uint32_t* nxt = (p & 1) ? st0 : st1;
+        const long clear = p + 1 < npasses ? t.status_words : 0;
+#define RTPS_SWEEP(I)                                                                                               \
+        k_onesweep<I><<<ntiles, kTileThreads, 0, st>>>(ki, vi, ko, vo, n, plan.shift[p], plan.bits[p],              \
+                                                       t.gcount + p * kMaxRadix, t.ctr + p, cur, nxt, clear, ntiles)
+        if (it == 1) { RTPS_SWEEP(1); } else if (it == 2) { RTPS_SWEEP(2); } else if (it == 4) { RTPS_SWEEP(4); }
+        else { RTPS_SWEEP(8); }
+#undef RTPS_SWEEP
+        e = hipGetLastError();
         if (e != hipSuccess) return e;
         uint32_t* tk = ki; ki = ko; ko = tk;
         uint32_t* tv = vi; vi = vo; vo = tv;
@@ -343,6 +391,46 @@ hipError_t sort_pairs_inplace(int n, int key_bits, uint32_t* keys, uint32_t* val
         e = hipGetLastError();
     }
     return e;
+}
+
+// (ao, bo) may be (a, b): each element is read, then written, by one thread
+__global__ void k_copy2_xor_u32(const uint32_t* a, const uint32_t* b, uint32_t* ao, uint32_t* bo, int n,
+                                uint32_t mask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        ao[i] = a[i] ^ mask;
+        bo[i] = b[i];
+    }
+}
+
+__global__ void k_xor_u32(uint32_t* __restrict__ a, int n, uint32_t mask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] ^= mask;
+}
+
+// Bitonic<T>::Sort's contract (BitonicSort.h:26-28): `batch` independent
+// arrays of `length` pairs, read from the src arrays, written sorted to the
+// dst arrays (which may be the src arrays), ascending for dir != 0 and
+// descending for dir == 0, on all 32 key bits.  A descending sort orders the
+// complemented keys ascending; equal keys keep their source order.
+hipError_t sort_segments(int batch, int length, int dir, uint32_t* dkeys, uint32_t* dvals, const uint32_t* skeys,
+                         const uint32_t* svals, void* temp, hipStream_t st) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    const uint32_t mask = dir ? 0u : 0xFFFFFFFFu;
+    const int blocks = (length + 255) / 256;
+    for (int b = 0; b < batch; ++b) {
+        const size_t o = (size_t)b * (size_t)length;
+        if (skeys + o != dkeys + o || svals + o != dvals + o || mask)
+            k_copy2_xor_u32<<<blocks, 256, 0, st>>>(skeys + o, svals + o, dkeys + o, dvals + o, length, mask);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = sort_pairs_inplace(length, 32, dkeys + o, dvals + o, temp, st);
+        if (e == hipSuccess && mask) {
+            k_xor_u32<<<blocks, 256, 0, st>>>(dkeys + o, length, mask);
+            e = hipGetLastError();
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace rtps

@@ -107,10 +107,35 @@ hipError_t launch_color_field(int res, const float4* pos_s, const float4* ve_s, 
                               hipStream_t st);
 hipError_t launch_mc_count(int res, float iso, const float* field, const uint8_t* ntri, uint8_t* cases,
                            uint32_t* counts, uint32_t* offs, uint32_t* tile_sums, uint32_t* total, hipStream_t st);
+// the corner and edge tables into the emit kernel's constant memory (once per process)
+hipError_t mc_upload_geometry();
 hipError_t launch_mc_emit(int res, float iso, const float* field, const uint8_t* cases, const uint32_t* counts,
-                          const uint32_t* offs, const uint32_t* tile_offs, const int8_t* tri,
-                          const rtps_float4& wmin, const rtps_float4& wmax, long capacity, float* verts,
+                          const uint32_t* offs, const uint32_t* tile_offs, const uint64_t* words,
+                          const rtps_float4& wmin, const rtps_float4& wmax, int frame, long capacity, float* verts,
                           float* normals, hipStream_t st);
+
+// ---- rigid-body coupling (rigid.hip) ----
+// One set of rigid-body particles in the fluid's grid: sorted positions (and,
+// for a moving body, velocities and masses), its cell start/end tables, and
+// the staging arrays of the host upload.  s0 / s1: the rigid system's
+// "spring" and "dampening" settings.
+struct RigidDev {
+    int n = 0, cap = 0;
+    bool moving = false;
+    float s0 = 0.f, s1 = 0.f;
+    float4 *pos = nullptr, *vel = nullptr, *in_pos = nullptr, *in_vel = nullptr;
+    float *mass = nullptr, *in_mass = nullptr;
+    uint32_t *keys = nullptr, *idx = nullptr, *start = nullptr, *end = nullptr;
+    void* sort_tmp = nullptr;
+};
+// Hashes, sorts and tables n particles of pos_u (vel_u / mass_u may be null)
+// into r on the unscaled grid g.
+hipError_t rigid_prepare(int n, const float4* pos_u, const float4* vel_u, const float* mass_u, RigidDev& r,
+                         const Grid& g, hipStream_t st);
+// Adds the forces of set r to the sorted fluid forces (gs: the scaled grid).
+hipError_t launch_rigid_force(int num, const int* d_num, const uint32_t* fstart, int nb_cells, const float4* pos_s,
+                              const float4* ve_s, float4* force, const RigidDev& r, const rtps_sph_params& sp,
+                              const Grid& gs, hipStream_t st);
 
 // ---- hoses (hose.cpp) ----
 std::vector<rtps_float4> hose_spray(rtps_hose* h);
@@ -126,5 +151,7 @@ hipError_t sort_pairs_inplace(int n, int key_bits, uint32_t* keys, uint32_t* val
 hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int npasses, uint32_t* keys,
                              uint32_t* vals, void* temp, hipStream_t st, uint32_t** out_keys, uint32_t** out_vals);
 int sort_plan(int key_bits, int* shifts, int* widths);
+hipError_t sort_segments(int batch, int length, int dir, uint32_t* dkeys, uint32_t* dvals, const uint32_t* skeys,
+                         const uint32_t* svals, void* temp, hipStream_t st);
 
 }  // namespace rtps

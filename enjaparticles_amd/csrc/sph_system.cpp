@@ -60,6 +60,7 @@ struct rtps_sph {
     int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0;
     int opt_autotune = 1;
     int opt_color = 0;   // permute colours every step (set by the first non-default colour pushed)
+    int opt_frame = 0;   // surface vertices: 0 = world coordinates, 1 = the reference's [0, 10]^3 frame
     // launch configuration of the neighbour kernels (0 / -1 = defaults), set by the autotuner
     int block_density = 0, block_force = 0, force_variant = -1;
     bool tuned = false;
@@ -81,7 +82,7 @@ struct rtps_sph {
     float* cf_field = nullptr;
     uint8_t* mc_cases = nullptr;
     uint32_t *mc_counts = nullptr, *mc_offs = nullptr, *mc_tiles = nullptr, *mc_total = nullptr;
-    int8_t* mc_tri = nullptr;
+    uint64_t* mc_words = nullptr;   // the case table, one packed word per case
     uint8_t* mc_ntri = nullptr;
     float *mesh_v = nullptr, *mesh_n = nullptr;
     long mesh_cap = 0;
@@ -93,6 +94,8 @@ struct rtps_sph {
     size_t tri_cap = 0;
     int ntri = 0;
     std::vector<rtps_float4> hose_color;
+    // rigid-body particle sets of SPH::interact: [0] a moving body, [1] the static boundary
+    RigidDev rigid[2];
 
     ~rtps_sph() {
         for (rtps_hose* h : hoses) rtps_hose_destroy(h);
@@ -105,10 +108,16 @@ struct rtps_sph {
                         sub_start};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
-        void* sbufs[] = {cf_field, mc_cases, mc_counts, mc_offs, mc_tiles, mc_total, mc_tri, mc_ntri, mesh_v, mesh_n,
+        void* sbufs[] = {cf_field, mc_cases, mc_counts, mc_offs, mc_tiles, mc_total, mc_words, mc_ntri, mesh_v, mesh_n,
                          tri_dev};
         for (void* b : sbufs)
             if (b) (void)hipFree(b);
+        for (RigidDev& r : rigid) {
+            void* rb[] = {r.pos, r.vel, r.in_pos, r.in_vel, r.mass, r.in_mass, r.keys, r.idx, r.start, r.end,
+                          r.sort_tmp};
+            for (void* b : rb)
+                if (b) (void)hipFree(b);
+        }
         for (auto& set : ev)
             for (auto& e : set)
                 if (e) (void)hipEventDestroy(e);
@@ -241,32 +250,51 @@ int launch_front(rtps_sph* s, StageTimer& t) {
     return RTPS_OK;
 }
 
-// Force through integration + count commit.  One fused kernel, or, while
-// triangles take part in the collisions, force; gravity and walls; triangles
-// (SPH::collision, SPH.cpp:303-336); leapfrog as separate kernels.
-int launch_back(rtps_sph* s) {
-    const bool fo = s->opt_force_out != 0;
-    if (s->integrator == 1) {   // euler.cl after force, gravity sources, walls and triangles
-        RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp,
-                                  s->gs, win(s, true), s->force_s, s->xsph_s, s->stream, masks_of(s)));
-        RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
-                                       s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
-        if (s->ntri > 0 && s->opt_collision)
-            RTPS_TRY_HIP(launch_collision_tri(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s,
-                                              s->force_s, s->tri_dev, s->tri_reach, s->ntri, s->sp, s->stream));
+bool rigid_on(const rtps_sph* s) { return s->rigid[0].n > 0 || s->rigid[1].n > 0; }
+
+// SPH::interact (SPH.cpp:610-663): the moving body's particles, then the
+// static boundary's, add their forces to the sorted forces of update().
+int launch_interact(rtps_sph* s) {
+    for (const RigidDev& r : s->rigid)
+        RTPS_TRY_HIP(launch_rigid_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->ve_s, s->force_s, r,
+                                        s->sp, s->gs, s->stream));
+    return RTPS_OK;
+}
+
+// SPH::integrate (SPH.cpp:389-441) on the stored forces: euler.cl or leapfrog.cl.
+int launch_integrate(rtps_sph* s) {
+    if (s->integrator == 1)
         RTPS_TRY_HIP(launch_euler(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                   s->ve_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u, s->stream,
                                   col_s(s), col_u(s)));
-    } else if (s->ntri > 0 && s->opt_collision) {
+    else
+        RTPS_TRY_HIP(launch_leapfrog(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
+                                     s->xsph_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u,
+                                     s->stream, col_s(s), col_u(s)));
+    return RTPS_OK;
+}
+
+// Force through integration + count commit, as the application loop runs a
+// step (update, interact, integrate; TestApplication.cpp:815-835).  One
+// fused kernel, or, with the euler integrator, triangles in the collisions or
+// rigid-body sets loaded: force; gravity sources and walls; triangles
+// (SPH::collision, SPH.cpp:303-336); rigid-body forces; the integration, as
+// separate kernels.
+int launch_back(rtps_sph* s) {
+    const bool fo = s->opt_force_out != 0;
+    const bool tri = s->ntri > 0 && s->opt_collision;
+    if (s->integrator == 1 || tri || rigid_on(s)) {
         RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp,
                                   s->gs, win(s, true), s->force_s, s->xsph_s, s->stream, masks_of(s)));
         RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                        s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
-        RTPS_TRY_HIP(launch_collision_tri(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
-                                          s->tri_dev, s->tri_reach, s->ntri, s->sp, s->stream));
-        RTPS_TRY_HIP(launch_leapfrog(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
-                                     s->xsph_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u,
-                                     s->stream, col_s(s), col_u(s)));
+        if (tri)
+            RTPS_TRY_HIP(launch_collision_tri(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s,
+                                              s->force_s, s->tri_dev, s->tri_reach, s->ntri, s->sp, s->stream));
+        int rc = launch_interact(s);
+        if (rc) return rc;
+        rc = launch_integrate(s);
+        if (rc) return rc;
     } else {
         RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs,
                                         win(s, true), fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
@@ -833,16 +861,84 @@ int rtps_sph_update(rtps_sph* s) {
 int rtps_sph_integrate(rtps_sph* s) {
     if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_integrate: null handle");
     if (!s->update_pending) return set_error(RTPS_ERR_STATE, "integrate() needs a preceding update()");
-    if (s->integrator == 1)   // SPH::integrate (SPH.cpp:392-409)
-        RTPS_TRY_HIP(launch_euler(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
-                                  s->ve_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u, s->stream,
-                                  col_s(s), col_u(s)));
-    else RTPS_TRY_HIP(launch_leapfrog(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
-                                 s->xsph_s, s->idx_sorted, s->sp, s->dt, s->pos_u, s->vel_u, s->veleval_u, s->stream,
-                                 col_s(s), col_u(s)));
+    int rc = launch_integrate(s);   // SPH::integrate (SPH.cpp:392-409)
+    if (rc) return rc;
     RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
     s->update_pending = false;
     return RTPS_OK;
+}
+
+int rtps_sph_interact(rtps_sph* s) {
+    if (!s) return set_error(RTPS_ERR_ARG, "rtps_sph_interact: null handle");
+    if (!s->update_pending) return set_error(RTPS_ERR_STATE, "interact() needs a preceding update()");
+    return launch_interact(s);
+}
+
+namespace {
+
+// Loads set k (0 = moving body, 1 = static boundary) from host arrays.
+int load_rigid(rtps_sph* s, int k, const rtps_float4* pos, const rtps_float4* vel, const float* mass, int n,
+               float spring, float dampening) {
+    RigidDev& r = s->rigid[k];
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    s->graph_valid = false;
+    s->timed_graph_valid = false;
+    if (n == 0) {
+        r.n = 0;
+        return RTPS_OK;
+    }
+    if (!r.start) {
+        RTPS_TRY_HIP(dalloc(&r.start, (size_t)s->nb_cells + 1));
+        RTPS_TRY_HIP(dalloc(&r.end, (size_t)s->nb_cells + 1));
+        RTPS_TRY_HIP(launch_fill_u32(r.end, 666u, (long)s->nb_cells + 1, s->stream));   // as SPH.cpp:512-517
+    }
+    if (n > r.cap) {
+        void* old[] = {r.pos, r.vel, r.in_pos, r.in_vel, r.mass, r.in_mass, r.keys, r.idx, r.sort_tmp};
+        for (void* b : old)
+            if (b) RTPS_TRY_HIP(hipFree(b));
+        r = RigidDev{0, 0, false, 0.f, 0.f, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, r.start, r.end, nullptr};
+        const size_t N = (size_t)n;
+        RTPS_TRY_HIP(dalloc(&r.pos, N));
+        RTPS_TRY_HIP(dalloc(&r.vel, N));
+        RTPS_TRY_HIP(dalloc(&r.in_pos, N));
+        RTPS_TRY_HIP(dalloc(&r.in_vel, N));
+        RTPS_TRY_HIP(dalloc(&r.mass, N));
+        RTPS_TRY_HIP(dalloc(&r.in_mass, N));
+        RTPS_TRY_HIP(dalloc(&r.keys, N));
+        RTPS_TRY_HIP(dalloc(&r.idx, N));
+        RTPS_TRY_HIP(hipMalloc(&r.sort_tmp, sort_temp_bytes(n, 32)));
+        RTPS_TRY_HIP(sort_clear_temp(r.sort_tmp, n, s->stream));
+        r.cap = n;
+    }
+    RTPS_TRY_HIP(hipMemcpyAsync(r.in_pos, pos, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+    if (vel) RTPS_TRY_HIP(hipMemcpyAsync(r.in_vel, vel, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+    if (mass) RTPS_TRY_HIP(hipMemcpyAsync(r.in_mass, mass, (size_t)n * 4, hipMemcpyHostToDevice, s->stream));
+    r.moving = k == 0;
+    r.s0 = spring;
+    r.s1 = dampening;
+    RTPS_TRY_HIP(rigid_prepare(n, r.in_pos, r.moving ? r.in_vel : nullptr, r.moving ? r.in_mass : nullptr, r, s->g,
+                               s->stream));
+    RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
+    r.n = n;
+    return RTPS_OK;
+}
+
+}  // namespace
+
+int rtps_sph_set_static_boundary(rtps_sph* s, const rtps_float4* pos, int n, float spring, float dampening) {
+    if (!s || n < 0 || (n > 0 && !pos))
+        return set_error(RTPS_ERR_ARG, "rtps_sph_set_static_boundary: bad arguments");
+    return load_rigid(s, 1, pos, nullptr, nullptr, n, spring, dampening);
+}
+
+int rtps_sph_set_rigid_particles(rtps_sph* s, const rtps_float4* pos, const rtps_float4* vel, const float* mass,
+                                 int n, float spring, float dampening) {
+    if (!s || n < 0 || (n > 0 && (!pos || !vel || !mass)))
+        return set_error(RTPS_ERR_ARG, "rtps_sph_set_rigid_particles: bad arguments");
+    for (int i = 0; i < n; ++i)
+        if (!(mass[i] > 0.f)) return set_error(RTPS_ERR_ARG, "rtps_sph_set_rigid_particles: masses must be > 0");
+    return load_rigid(s, 0, pos, vel, mass, n, spring, dampening);
 }
 
 namespace {
@@ -939,11 +1035,12 @@ namespace {
 
 // Buffers of the colour field and the surface for a res^3 lattice.
 int surface_buffers(rtps_sph* s, int res) {
-    if (!s->mc_tri) {   // case table, once per handle
+    if (!s->mc_words) {   // case table, once per handle
         const McTable& t = mc_table();
-        RTPS_TRY_HIP(hipMalloc(&s->mc_tri, sizeof(t.tri)));
+        RTPS_TRY_HIP(mc_upload_geometry());
+        RTPS_TRY_HIP(hipMalloc(&s->mc_words, sizeof(kMcCaseWords)));
         RTPS_TRY_HIP(hipMalloc(&s->mc_ntri, sizeof(t.ntri)));
-        RTPS_TRY_HIP(hipMemcpy(s->mc_tri, t.tri, sizeof(t.tri), hipMemcpyHostToDevice));
+        RTPS_TRY_HIP(hipMemcpy(s->mc_words, kMcCaseWords, sizeof(kMcCaseWords), hipMemcpyHostToDevice));
         RTPS_TRY_HIP(hipMemcpy(s->mc_ntri, t.ntri, sizeof(t.ntri), hipMemcpyHostToDevice));
         RTPS_TRY_HIP(hipMalloc(&s->mc_total, sizeof(uint32_t)));
     }
@@ -1001,7 +1098,8 @@ int surface(rtps_sph* s, int res, float iso) {
     s->mesh_ntri = (int)total;
     if (total > 0)
         RTPS_TRY_HIP(launch_mc_emit(res, iso, s->cf_field, s->mc_cases, s->mc_counts, s->mc_offs, s->mc_tiles,
-                                    s->mc_tri, s->gp.bnd_min, s->gp.bnd_max, s->mesh_cap, s->mesh_v, s->mesh_n,
+                                    s->mc_words, s->gp.bnd_min, s->gp.bnd_max, s->opt_frame, s->mesh_cap, s->mesh_v,
+                                    s->mesh_n,
                                     s->stream));
     return RTPS_OK;
 }
@@ -1022,10 +1120,15 @@ int copy_mesh(rtps_sph* s, int capacity, float* vertices, float* normals, int* n
 }  // namespace
 
 int rtps_mc_case_table(signed char* tri, unsigned char* ntri) {
-    McTable t;
-    if (!build_mc_table(t)) return set_error(RTPS_ERR_ARG, "marching-cubes case table: a case needs too many triangles");
+    const McTable& t = mc_table();
     if (tri) std::memcpy(tri, t.tri, sizeof(t.tri));
     if (ntri) std::memcpy(ntri, t.ntri, sizeof(t.ntri));
+    return RTPS_OK;
+}
+
+int rtps_mc_geometry(signed char* corners, signed char* edges) {
+    if (corners) std::memcpy(corners, kMcCorner, sizeof(kMcCorner));
+    if (edges) std::memcpy(edges, kMcEdgeEnds, sizeof(kMcEdgeEnds));
     return RTPS_OK;
 }
 
@@ -1180,9 +1283,13 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
             s->opt_remap = value;
             break;
         case RTPS_OPT_SORT_TILE:
-            if (value != 0 && value != 4 && value != 8 && value != 16)
-                return set_error(RTPS_ERR_ARG, "sort tile must be 0 (by size), 4, 8 or 16");
+            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+                return set_error(RTPS_ERR_ARG, "sort keys per thread must be 0 (by size), 1, 2, 4 or 8");
             g_sort_items = value;
+            break;
+        case RTPS_OPT_SURFACE_FRAME:
+            if (value != 0 && value != 1) return set_error(RTPS_ERR_ARG, "surface frame must be 0 (world) or 1 (reference)");
+            s->opt_frame = value;
             break;
         case RTPS_OPT_BLOCK_DENSITY:
         case RTPS_OPT_BLOCK_FORCE:

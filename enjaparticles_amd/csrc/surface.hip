@@ -15,10 +15,15 @@
 // cube its first triangle, and one thread per cube writes its triangles:
 // vertices interpolated linearly along the crossing edges, normals from the
 // interpolated negative central differences of the field (pointing away from
-// the fluid), as traversehp.cl:470-515 computes them.  The case table is
-// generated (mc_table.h); triangles come out in cube order.
+// the fluid), as traversehp.cl:470-515 computes them.  The case table, the
+// corner bits and the edge ends are the reference's (mc_table.h); a cube's
+// triangles come out in the table's order, cubes in lattice order (x fastest).
+// Vertices are in world coordinates of the boundary box, or in the
+// reference's frame (lattice coordinates / (res - 1) * 10, traversehp.cl:512).
 
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 
 #include "mc_table.h"
 #include "rtps_internal.h"
@@ -34,13 +39,13 @@ constexpr int kScanTile = kBlockSurf * kScanItems;       // 2048 counts per tile
 
 inline int blocks_of(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-// Corners of edge e (the numbering of kMcEdge): x edges join 2q and 2q + 1,
-// y edges (q & 1) | (q >> 1) << 2 and that + 2, z edges q and q + 4.
-__device__ __forceinline__ void edge_corners(int e, int& a, int& b) {
-    if (e < 4) { a = 2 * e; b = a + 1; }
-    else if (e < 8) { const int q = e - 4; a = (q & 1) | ((q >> 1) << 2); b = a + 2; }
-    else { a = e - 8; b = a + 4; }
-}
+}  // namespace
+
+// Corner offsets of the case bits and the edges' two ends (mc_table.h).
+__constant__ int8_t c_mc_corner[8][3];
+__constant__ int8_t c_mc_edge[12][6];
+
+namespace {
 
 template <bool SCALED>
 __global__ __launch_bounds__(kBlockSurf) void k_color_field(int res, const float4* __restrict__ pos_s,
@@ -99,9 +104,9 @@ __device__ __forceinline__ float node(const float* __restrict__ f, int res, int 
     return f[((long)z * res + y) * res + x];
 }
 
-// Case of every cube (corner k at offset (k & 1, k >> 1 & 1, k >> 2 & 1);
-// bit k set when the corner value exceeds the iso level, classifycubes.cl:44-55)
-// and its triangle count.
+// Case of every cube (bit k set when the node at corner offset
+// kMcCorner[k] exceeds the iso level, classifycubes.cl:44-55) and its
+// triangle count (nrOfTriangles, classifycubes.cl:37).
 __global__ __launch_bounds__(kBlockSurf) void k_mc_classify(int res, float iso, const float* __restrict__ f,
                                                             const uint8_t* __restrict__ ntri,
                                                             uint8_t* __restrict__ cases, uint32_t* __restrict__ counts) {
@@ -113,7 +118,7 @@ __global__ __launch_bounds__(kBlockSurf) void k_mc_classify(int res, float iso, 
     uint32_t cs = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-        cs |= (uint32_t)(node(f, res, x + (k & 1), y + ((k >> 1) & 1), z + ((k >> 2) & 1)) > iso) << k;
+        cs |= (uint32_t)(node(f, res, x + c_mc_corner[k][0], y + c_mc_corner[k][1], z + c_mc_corner[k][2]) > iso) << k;
     cases[idx] = (uint8_t)cs;
     counts[idx] = ntri[cs];
 }
@@ -183,15 +188,16 @@ __device__ __forceinline__ float3 neg_gradient(const float* __restrict__ f, int 
 }
 
 // Triangles of every cube at its scanned offset (at most `capacity` in all):
-// 9 floats per triangle in `verts` (world coordinates) and `normals`.
+// 9 floats per triangle in `verts` and `normals`; frame 0 = world
+// coordinates of [wmin, wmax], 1 = the reference's lattice / (res - 1) * 10.
 __global__ __launch_bounds__(kBlockSurf) void k_mc_emit(int res, float iso, const float* __restrict__ f,
                                                         const uint8_t* __restrict__ cases,
                                                         const uint32_t* __restrict__ counts,
                                                         const uint32_t* __restrict__ offs,
                                                         const uint32_t* __restrict__ tile_offs,
-                                                        const int8_t* __restrict__ tri, rtps_float4 wmin,
-                                                        rtps_float4 wmax, long capacity, float* __restrict__ verts,
-                                                        float* __restrict__ normals) {
+                                                        const uint64_t* __restrict__ words, rtps_float4 wmin,
+                                                        rtps_float4 wmax, int frame, long capacity,
+                                                        float* __restrict__ verts, float* __restrict__ normals) {
     const int m = res - 1;
     const long ncubes = (long)m * m * m;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,17 +206,16 @@ __global__ __launch_bounds__(kBlockSurf) void k_mc_emit(int res, float iso, cons
     if (nt == 0) return;
     const int x = (int)(idx % m), y = (int)((idx / m) % m), z = (int)(idx / ((long)m * m));
     const long first = (long)offs[idx] + tile_offs[idx / kScanTile];
-    const int8_t* tr = tri + (int)cases[idx] * 16;
+    const uint64_t w = words[cases[idx]];
     const float inv = 1.0f / (float)(res - 1);
     for (uint32_t q = 0; q < nt; ++q) {
         const long o = first + q;
         if (o >= capacity) return;
         for (int v = 0; v < 3; ++v) {
-            const int e = tr[3 * q + v];
-            int ka, kb;
-            edge_corners(e, ka, kb);
-            const int ax = x + (ka & 1), ay = y + ((ka >> 1) & 1), az = z + ((ka >> 2) & 1);
-            const int bx = x + (kb & 1), by = y + ((kb >> 1) & 1), bz = z + ((kb >> 2) & 1);
+            const int e = (int)((w >> (4 * (3 * q + v))) & 0xFu);
+            const int8_t* ee = c_mc_edge[e];
+            const int ax = x + ee[0], ay = y + ee[1], az = z + ee[2];
+            const int bx = x + ee[3], by = y + ee[4], bz = z + ee[5];
             const float v0 = node(f, res, ax, ay, az), v1 = node(f, res, bx, by, bz);
             const float d = (iso - v0) / (v1 - v0);   // traversehp.cl:506-511
             const float gx = (float)ax + d * (float)(bx - ax);
@@ -222,9 +227,16 @@ __global__ __launch_bounds__(kBlockSurf) void k_mc_emit(int res, float iso, cons
             const float il = len2 > 0.0f ? __builtin_amdgcn_rsqf(len2) : 0.0f;
             float* pv = verts + (o * 3 + v) * 3;
             float* pn = normals + (o * 3 + v) * 3;
-            pv[0] = wmin.x + gx * inv * (wmax.x - wmin.x);
-            pv[1] = wmin.y + gy * inv * (wmax.y - wmin.y);
-            pv[2] = wmin.z + gz * inv * (wmax.z - wmin.z);
+            if (frame == 1) {   // mix(p0, p1, d) / (res - 1) * 10 (traversehp.cl:512)
+                const float rm = (float)(res - 1);
+                pv[0] = gx / rm * 10.0f;
+                pv[1] = gy / rm * 10.0f;
+                pv[2] = gz / rm * 10.0f;
+            } else {
+                pv[0] = wmin.x + gx * inv * (wmax.x - wmin.x);
+                pv[1] = wmin.y + gy * inv * (wmax.y - wmin.y);
+                pv[2] = wmin.z + gz * inv * (wmax.z - wmin.z);
+            }
             pn[0] = nx * il;
             pn[1] = ny * il;
             pn[2] = nz * il;
@@ -264,15 +276,31 @@ hipError_t launch_mc_count(int res, float iso, const float* field, const uint8_t
     return hipGetLastError();
 }
 
+hipError_t mc_upload_geometry() {
+    static std::mutex mu;
+    static bool done[64] = {};
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done[dev]) return hipSuccess;   // each device holds its own constant memory
+    err = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_corner), kMcCorner, sizeof(kMcCorner));
+    if (err == hipSuccess) err = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_edge), kMcEdgeEnds, sizeof(kMcEdgeEnds));
+    if (err == hipSuccess) done[dev] = true;
+    return err;
+}
+
 hipError_t launch_mc_emit(int res, float iso, const float* field, const uint8_t* cases, const uint32_t* counts,
-                          const uint32_t* offs, const uint32_t* tile_offs, const int8_t* tri,
-                          const rtps_float4& wmin, const rtps_float4& wmax, long capacity, float* verts,
+                          const uint32_t* offs, const uint32_t* tile_offs, const uint64_t* words,
+                          const rtps_float4& wmin, const rtps_float4& wmax, int frame, long capacity, float* verts,
                           float* normals, hipStream_t st) {
     (void)hipGetLastError();
     if (res < 2) return hipErrorInvalidValue;
     const long m = res - 1, ncubes = m * m * m;
     k_mc_emit<<<blocks_of(ncubes, kBlockSurf), kBlockSurf, 0, st>>>(res, iso, field, cases, counts, offs, tile_offs,
-                                                                   tri, wmin, wmax, capacity, verts, normals);
+                                                                   words, wmin, wmax, frame, capacity, verts,
+                                                                   normals);
     return hipGetLastError();
 }
 

@@ -52,6 +52,20 @@ def sort_pairs(keys: torch.Tensor, vals: torch.Tensor, key_bits: int) -> None:
     N.check(N.lib().rtps_sort_pairs(n, key_bits, _ptr(keys), _ptr(vals), _ptr(temp), nbytes, _stream()))
 
 
+def sort_segments(dst_keys: torch.Tensor, dst_vals: torch.Tensor, src_keys: torch.Tensor, src_vals: torch.Tensor,
+                  batch: int, array_length: int, direction: int) -> None:
+    """Bitonic<T>::Sort(batch, arrayLength, dir, dst..., src...)
+    (BitonicSort.h:26-28): batch arrays of array_length pairs, src -> dst,
+    ascending for direction != 0, on all 32 key bits."""
+    n = batch * array_length
+    for t, name in ((dst_keys, "dst_keys"), (dst_vals, "dst_vals"), (src_keys, "src_keys"), (src_vals, "src_vals")):
+        _u32(t, n, name)
+    nbytes = N.lib().rtps_sort_temp_bytes(array_length, 32)
+    temp = torch.empty(max(1, nbytes), dtype=torch.uint8, device=dst_keys.device)
+    N.check(N.lib().rtps_sort_segments(batch, array_length, direction, _ptr(dst_keys), _ptr(dst_vals),
+                                       _ptr(src_keys), _ptr(src_vals), _ptr(temp), nbytes, _stream()))
+
+
 def key_bits(nb_cells: int) -> int:
     return N.check(N.lib().rtps_sort_key_bits(nb_cells))
 

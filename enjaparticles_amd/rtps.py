@@ -56,6 +56,18 @@ class RTPSSettings:
         return dict(self._s)
 
 
+def rigid_spring_dampening(penetration_factor: float, velocity_limit: float, smoothing_distance: float,
+                           restitution: float):
+    """The rigid-body system's "spring" and "dampening" settings as
+    ParticleRigidBody::updateParams derives them (ParticleRigidBody.cpp:603-608):
+    spring = penetration_factor * velocity_limit / (4 h^2),
+    dampening = -2 ln(restitution) / sqrt(ln(restitution)^2 + pi^2)."""
+    import math
+    spring = penetration_factor * velocity_limit / (4.0 * smoothing_distance * smoothing_distance)
+    ln_res = math.log(restitution)
+    return spring, -2.0 * ln_res / math.sqrt(ln_res * ln_res + math.pi * math.pi)
+
+
 class SPH:
     """Handle of one SPH system on the GPU (rtps_sph in include/rtps_c.h)."""
 
@@ -160,7 +172,31 @@ class SPH:
         N.check(N.lib().rtps_sph_update(self._h))
 
     def interact(self) -> None:
-        """No interaction systems are supported (rigid-body coupling is out of scope)."""
+        """SPH::interact (SPH.cpp:610-663): the loaded rigid-body particle sets
+        add their forces to those of the preceding update()."""
+        N.check(N.lib().rtps_sph_interact(self._h))
+
+    def setRigidParticles(self, pos, vel, mass, spring: float, dampening: float) -> None:
+        """A moving rigid body's particles (rigidbody_force.cl): positions,
+        velocities (world units) and masses; spring / dampening are the rigid
+        system's settings.  An empty array removes the set."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+        vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 4)
+        mass = np.ascontiguousarray(mass, np.float32).reshape(-1)
+        assert vel.shape == pos.shape and mass.shape[0] == pos.shape[0]
+        n = pos.shape[0]
+        N.check(N.lib().rtps_sph_set_rigid_particles(self._h, pos.ctypes.data if n else None,
+                                                      vel.ctypes.data if n else None,
+                                                      mass.ctypes.data if n else None, n, float(spring),
+                                                      float(dampening)))
+
+    def setStaticBoundary(self, pos, spring: float, dampening: float) -> None:
+        """Static boundary particles of a rigid-body system
+        (staticrigidbody_force.cl).  An empty array removes the set."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+        n = pos.shape[0]
+        N.check(N.lib().rtps_sph_set_static_boundary(self._h, pos.ctypes.data if n else None, n, float(spring),
+                                                      float(dampening)))
 
     def integrate(self) -> None:
         N.check(N.lib().rtps_sph_integrate(self._h))

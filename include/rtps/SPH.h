@@ -82,7 +82,20 @@ public:
         flushSettings();
         check(rtps_sph_update(h_));
     }
+    // SPH::interact (SPH.cpp:610-663): forces of the loaded rigid-body sets.
+    void interact() override { check(rtps_sph_interact(h_)); }
     void integrate() override { check(rtps_sph_integrate(h_)); }
+    // Rigid-body particle sets (positions in world units); an empty set removes it.
+    void setRigidParticles(const std::vector<float4>& pos, const std::vector<float4>& vel,
+                           const std::vector<float>& mass, float spring, float dampening) {
+        if (vel.size() != pos.size() || mass.size() != pos.size())
+            check(RTPS_ERR_ARG);
+        check(rtps_sph_set_rigid_particles(h_, pos.data(), vel.data(), mass.data(), (int)pos.size(), spring,
+                                           dampening));
+    }
+    void setStaticBoundary(const std::vector<float4>& pos, float spring, float dampening) {
+        check(rtps_sph_set_static_boundary(h_, pos.data(), (int)pos.size(), spring, dampening));
+    }
     // SPH::postProcess (SPH.cpp:338-386): the colour-field surface when the
     // settings ask for it ("use_color_field", "color_field_res").
     void postProcess() override {

@@ -91,6 +91,19 @@ size_t rtps_sort_temp_bytes(int n, int key_bits);
 int rtps_sort_pairs(int n, int key_bits, uint32_t* keys /*dev*/, uint32_t* vals /*dev*/, void* temp /*dev*/,
                     size_t temp_bytes, void* stream);
 
+/* a3. Bitonic<T>::Sort(batch, arrayLength, dir, dstkey, dstval, srckey,
+ * srcval) (rtpslib/system/common/BitonicSort.h:26-28, BitonicSort.cpp:62-183)
+ * with its argument meaning: `batch` independent arrays of array_length
+ * key/value pairs, read from src_*, written sorted to dst_* (dst may equal
+ * src), ascending for dir != 0, descending for dir == 0, on all 32 key bits;
+ * array_length < 2 does nothing, as in the reference.  Any array_length (the
+ * reference needs powers of two); equal keys keep their source order (the
+ * bitonic network leaves them in a network-defined order).  temp: dev scratch
+ * of rtps_sort_temp_bytes(array_length, 32) bytes. */
+int rtps_sort_segments(int batch, int array_length, int dir, uint32_t* dst_keys /*dev*/, uint32_t* dst_vals /*dev*/,
+                       const uint32_t* src_keys /*dev*/, const uint32_t* src_vals /*dev*/, void* temp /*dev*/,
+                       size_t temp_bytes, void* stream);
+
 /* a4. CellIndices::execute (rtpslib/system/common/CellIndices.h:47-58,
  * cellindices.cl:12-107): start[0..nb_cells] := 0xFFFFFFFF, then start/end of
  * every occupied cell from the sorted keys.  end[] of empty cells is left
@@ -263,8 +276,9 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        rtps_sph_autotune), 2 = over all five loops (2 and 3
  *                        win while the neighbourhoods are sparse, lose once
  *                        they fill), 0 = fixed defaults
- *   RTPS_OPT_SORT_TILE   radix-sort tiles of 256 x value keys, process-wide:
- *                        4, 8 or 16, or 0 (default) = 16 from 2^19 keys up, else 8
+ *   RTPS_OPT_SORT_TILE   radix-sort keys per thread of a 1024-thread tile,
+ *                        process-wide: 1, 2, 4 or 8, or 0 (default) = 4 from
+ *                        2^19 keys up, else 2
  *   RTPS_OPT_DENSITY_STAGE  1 = the density kernel copies each row's candidates
  *                        once per wave into LDS (default), 0 = every lane reads
  *                        its candidates from global memory; process-wide, the
@@ -279,6 +293,10 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        leapfrog.cl:107); 0 = colours stay where pushed.
  *                        Default 0; the first push of a non-default colour
  *                        sets 1.
+ *   RTPS_OPT_SURFACE_FRAME  vertices of rtps_sph_surface / the post-process mesh:
+ *                        0 = world coordinates of the boundary box (default),
+ *                        1 = the reference's frame, lattice coordinates /
+ *                        (res - 1) * 10 (traversehp.cl:512)
  *   RTPS_OPT_TILE_REMAP  tile order of the neighbour kernels over the 8 XCDs:
  *                        0 = hardware round-robin (default), 1 = one
  *                        contiguous range per XCD, C in 2..4096 = runs of C
@@ -296,6 +314,7 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
 #define RTPS_OPT_DENSITY_STAGE 10
 #define RTPS_OPT_BLOCK_DENSITY 11
 #define RTPS_OPT_BLOCK_FORCE 12
+#define RTPS_OPT_SURFACE_FRAME 13
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))
@@ -365,6 +384,27 @@ int rtps_sph_refill_hose(rtps_sph* s, int index, int refill);
  * the integration as separate kernels. */
 int rtps_sph_load_triangles(rtps_sph* s, const rtps_triangle* tris, int n);
 
+/* Rigid-body coupling of SPH::interact (SPH.cpp:610-663): the particles of a
+ * rigid-body system (System::addInteractionSystem, System.h:167) push the
+ * fluid.  rtps_sph_set_rigid_particles gives a moving body's particles
+ * (rigidbody_force.cl:118-193: positions, velocities and masses, host arrays
+ * in world units, as ParticleRigidBody's getPositionBuffer /
+ * getVelocityBuffer / getMassBuffer), rtps_sph_set_static_boundary the
+ * static boundary particles (staticrigidbody_force.cl:45-146,
+ * getStaticPositionBuffer); spring and dampening are the rigid system's
+ * "spring" and "dampening" settings (its friction settings are not read by
+ * either kernel).  Each set is hashed into the fluid's grid at the call and
+ * used until replaced; n = 0 removes it.  rtps_sph_interact adds the moving
+ * body's forces, then the static boundary's, to the forces of the preceding
+ * rtps_sph_update (call it between update and integrate, as the application
+ * loop does, TestApplication.cpp:815-835); rtps_sph_step does this by itself
+ * while a set is loaded (force, gravity and walls, triangles, rigid bodies and
+ * the integration then run as separate kernels). */
+int rtps_sph_set_rigid_particles(rtps_sph* s, const rtps_float4* pos, const rtps_float4* vel, const float* mass,
+                                 int n, float spring, float dampening);
+int rtps_sph_set_static_boundary(rtps_sph* s, const rtps_float4* pos, int n, float spring, float dampening);
+int rtps_sph_interact(rtps_sph* s);
+
 /* System::testDelete (System.cpp:366-376): moves the first 40 particles to
  * (100, 100, 100), outside the grid, so the next update deletes them. */
 int rtps_sph_test_delete(rtps_sph* s);
@@ -381,10 +421,12 @@ int rtps_sph_test_delete(rtps_sph* s);
  *
  * rtps_sph_surface: the iso-surface of that field at `iso`, *ntris
  * triangles, the first `capacity` of them copied as 9 floats each: vertices
- * in world coordinates (the boundary box; the reference scales to [0, 10]^3)
- * and unit normals pointing away from the fluid.  Triangles come in cube
- * order (x fastest); the case table is generated (rtps_mc_case_table) and
- * closes the surface on every cube face.
+ * in world coordinates of the boundary box (or, with RTPS_OPT_SURFACE_FRAME
+ * 1, in the reference's [0, 10]^3 frame) and unit normals pointing away from
+ * the fluid.  Cubes are triangulated with the reference's case table, corner
+ * bits and edge ends (rtps_mc_case_table, rtps_mc_geometry); triangles come in
+ * cube order (x fastest), each cube's in the table's order (the reference
+ * emits them in histogram-pyramid order).
  *
  * rtps_sph_post_process: SPH::postProcess: with setting "use_color_field"
  * = 1, the surface for "color_field_res" (default 2) and "color_field_iso"
@@ -394,13 +436,15 @@ int rtps_sph_color_field(rtps_sph* s, int res, float* field);
 int rtps_sph_surface(rtps_sph* s, int res, float iso, int capacity, float* vertices, float* normals, int* ntris);
 int rtps_sph_post_process(rtps_sph* s);
 int rtps_sph_mesh(rtps_sph* s, int capacity, float* vertices, float* normals, int* ntris);
-/* The generated marching-cubes case table: tri[256 * 16] edge ids (-1
- * padded, three per triangle), ntri[256] triangles per case.  Corner k of a
- * cube is at (k & 1, k >> 1 & 1, k >> 2 & 1); edges 0-3 join corners
- * (0,1) (2,3) (4,5) (6,7), edges 4-7 (0,2) (1,3) (4,6) (5,7), edges 8-11
- * (0,4) (1,5) (2,6) (3,7); bit k of a case is set when corner k exceeds the
- * iso level.  Host only. */
+/* The marching-cubes case table of traversehp.cl:74 (triTable) with
+ * classifycubes.cl:37's counts: tri[256 * 16] edge ids (-1 padded, three per
+ * triangle), ntri[256] triangles per case.  rtps_mc_geometry: corners[8 * 3]
+ * = the lattice offset of case bit k (classifycubes.cl:44-55: bit k set when
+ * that node exceeds the iso level) and edges[12 * 6] = the two ends of edge e
+ * (traversehp.cl:35-72, offsets3; vertices are interpolated from the first
+ * toward the second).  Host only; either pointer may be NULL. */
 int rtps_mc_case_table(signed char* tri, unsigned char* ntri);
+int rtps_mc_geometry(signed char* corners, signed char* edges);
 
 /* ------------------------------------------------------------------------ */
 /* a14. Batched scenes (no reference counterpart): num_scenes independent SPH

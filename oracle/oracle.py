@@ -101,6 +101,10 @@ def _declare(L):
         "oracle_step_tl": (i, [P, P, P, P, P, P, P, d, i, i, P, P, P, P, P] + [P] * 10),
         "oracle_tl_directions": (i, []),
         "oracle_derive_params": (i, [P, P, i, P, P, P]),
+        "oracle_add_rect": (i, [i, P, P, f, f, P]),
+        "oracle_rigid_force_f32": (None, [i] + [P] * 11 + [f, f, i]),
+        "oracle_rigid_force_f64": (None, [i] + [P] * 11 + [f, f, i]),
+        "oracle_set_rigid": (None, [i, i, P, P, P, f, f]),
         "oracle_num_threads": (i, []),
         "oracle_set_threads": (None, [i]),
         "oracle_last_stage_times": (None, [P]),
@@ -154,6 +158,15 @@ def derive_params(settings: dict):
     rc = lib().oracle_derive_params(C.cast(ka, P), C.cast(va, P), len(keys), _p(sp), _p(gp), _p(gs))
     assert rc == 0
     return sp, gp, gs
+
+
+def add_rect(num, bmin, bmax, spacing, scale=1.0):
+    """addRect (IV.cpp:165-202): (count, 4) float32 lattice, z outer, x inner."""
+    out = np.zeros((max(num, 0), 4), np.float32)
+    lo = np.ascontiguousarray(bmin[:3], np.float32)
+    hi = np.ascontiguousarray(bmax[:3], np.float32)
+    k = lib().oracle_add_rect(int(num), _p(lo), _p(hi), float(spacing), float(scale), _p(out))
+    return out[:k].copy()
 
 
 def hash(pos_u, gp):
@@ -241,6 +254,34 @@ def euler(pos_s, vel_s, force_s, idx, sphp, dt, pos_u, vel_u):
     getattr(lib(), "oracle_euler_" + _sfx(pos_s.dtype))(
         pos_s.shape[0], _p(pos_s), _p(vel_s), _p(force_s), _p(idx), _p(sphp), dt, _p(pu), _p(vu))
     return pu, vu
+
+
+def rigid_force(pos_s, veleval_s, dens, force_s, bpos_s, bstart, bend, sphp, gps, spring, dampening,
+                bvel_s=None, bmass_s=None):
+    """Rigid-body forces of SPH::interact added to a copy of force_s:
+    staticrigidbody_force.cl (bvel_s None) or rigidbody_force.cl (dynamic)."""
+    f = np.ascontiguousarray(force_s).copy()
+    bp = np.ascontiguousarray(bpos_s, np.float32)
+    dyn = bvel_s is not None
+    bv = np.ascontiguousarray(bvel_s if dyn else np.zeros_like(bp), np.float32)
+    bm = np.ascontiguousarray(bmass_s if dyn else np.zeros(len(bp)), np.float32)
+    getattr(lib(), "oracle_rigid_force_" + _sfx(pos_s.dtype))(
+        pos_s.shape[0], _p(pos_s), _p(veleval_s), _p(dens), _p(f), _p(bp), _p(bv), _p(bm), _p(bstart), _p(bend),
+        _p(sphp), _p(gps), spring, dampening, int(dyn))
+    return f
+
+
+def set_rigid(kind: str, pos=None, spring=0.0, dampening=0.0, vel=None, mass=None):
+    """Rigid-body particles coupled into the following State.step calls
+    (kind "dynamic" or "static"; pos None removes the set)."""
+    k = 1 if kind == "static" else 0
+    if pos is None:
+        lib().oracle_set_rigid(k, 0, None, None, None, 0.0, 0.0)
+        return
+    p_ = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+    v_ = None if vel is None else np.ascontiguousarray(vel, np.float32).reshape(-1, 4)
+    m_ = None if mass is None else np.ascontiguousarray(mass, np.float32)
+    lib().oracle_set_rigid(k, len(p_), _p(p_), _p(v_), _p(m_), float(spring), float(dampening))
 
 
 def count_pairs(pos_s, start, end, sphp, gps):

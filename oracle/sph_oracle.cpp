@@ -500,6 +500,142 @@ extern "C" void oracle_collision_wall_f64(int num, const double* pos_s, const do
 }
 
 // ---------------------------------------------------------------------------
+// f4: rigid-body coupling of SPH::interact (SPH.cpp:610-663): fluid particle i
+// against the boundary particles j of a rigid-body system, found through that
+// system's cell start/end tables on the fluid's scaled grid with the same
+// 27-cell iteration (cl_sph_neighbors.h:99-140), accepted within 2h:
+//   norm = (x_j - x_i) / max(|r|, eps), stiff = s0 m,
+//   F   += -stiff (2h - r) norm + s1 sqrt(stiff mn) ((vrel . norm) norm)
+//   V   += w_visc (h - r)
+// static  (staticrigidbody_force.cl:45-146): vrel = -e_i (fluid veleval),
+//         mn = m^2 / 2m, w_visc = vrel; force_i += F / m - m / rho_i^2 mu c V
+// dynamic (rigidbody_force.cl:118-193): vrel = v_j - e_i, mn = m m_j / (m + m_j),
+//         w_visc = vrel - (vrel . norm) norm (tangential); force_i += F / m
+//         + m / rho_i^2 mu c V
+// with c = wvisc_dd_coef.  (h - r) is the reference's Wvisc_lapl, negative
+// between h and 2h.  bvel / bmass are ignored for the static form.
+template <typename T>
+static void rigid_force_impl(int num, const T* pos_s, const T* veleval_s, const T* density, T* force,
+                             const float* bpos_s, const float* bvel_s, const float* bmass_s, const u32* bstart,
+                             const u32* bend, const OSPHParams* sp, const OGridParams* gps, float s0, float s1,
+                             int dynamic) {
+    const OSPHParams p = *sp;
+    const OGridParams gs = *gps;
+    const T scale = p.simulation_scale, h = p.smoothing_distance, eps = p.EPSILON, m = p.mass;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < num; ++i) {
+        V4<T> pi = ld(pos_s, i);
+        pi.x *= scale; pi.y *= scale; pi.z *= scale; pi.w *= scale;
+        const V4<T> ei = ld(veleval_s, i);
+        T fx = 0, fy = 0, fz = 0, vx = 0, vy = 0, vz = 0;
+        int cx, cy, cz;
+        grid_cell(pi, gs, cx, cy, cz);
+        for (int z = cz - 1; z <= cz + 1; ++z)
+            for (int y = cy - 1; y <= cy + 1; ++y)
+                for (int x = cx - 1; x <= cx + 1; ++x) {
+                    u32 ch = (u32)grid_hash<T>(x, y, z, gs);
+                    if (ch >= (u32)gs.nb_cells) continue;
+                    u32 st = bstart[ch];
+                    if (st == 0xFFFFFFFFu) continue;
+                    for (u32 j = st; j < bend[ch]; ++j) {
+                        T rx = (T)bpos_s[4 * j] * scale - pi.x, ry = (T)bpos_s[4 * j + 1] * scale - pi.y,
+                          rz = (T)bpos_s[4 * j + 2] * scale - pi.z;
+                        T rlen = len3(rx, ry, rz);
+                        if (!(rlen <= (T)2 * h)) continue;
+                        rlen = std::max(rlen, eps);
+                        T nx = rx / rlen, ny = ry / rlen, nz = rz / rlen;
+                        T mj = dynamic ? (T)bmass_s[j] : m;
+                        T mn = (m * mj) / (m + mj);
+                        T stiff = (T)s0 * m;
+                        T sp2 = -stiff * ((T)2 * h - rlen);
+                        T ux = -ei.x, uy = -ei.y, uz = -ei.z;
+                        if (dynamic) { ux += (T)bvel_s[4 * j]; uy += (T)bvel_s[4 * j + 1]; uz += (T)bvel_s[4 * j + 2]; }
+                        T un = ux * nx + uy * ny + uz * nz;
+                        T dmp = (T)s1 * std::sqrt(stiff * mn);
+                        fx += sp2 * nx + dmp * (un * nx);
+                        fy += sp2 * ny + dmp * (un * ny);
+                        fz += sp2 * nz + dmp * (un * nz);
+                        T lap = h - rlen;
+                        if (dynamic) { ux -= un * nx; uy -= un * ny; uz -= un * nz; }
+                        vx += ux * lap; vy += uy * lap; vz += uz * lap;
+                    }
+                }
+        V4<T> f = ld(force, i);
+        const T di = density[i];
+        const T c = m * ((T)1 / (di * di)) * ((T)p.viscosity * (T)p.wvisc_dd_coef);
+        const T sgn = dynamic ? (T)1 : (T)-1;
+        f.x += fx / m + sgn * (c * vx);
+        f.y += fy / m + sgn * (c * vy);
+        f.z += fz / m + sgn * (c * vz);
+        st(force, i, f);
+    }
+}
+extern "C" void oracle_rigid_force_f32(int num, const float* pos_s, const float* veleval_s, const float* density,
+                                       float* force, const float* bpos_s, const float* bvel_s, const float* bmass_s,
+                                       const u32* bstart, const u32* bend, const OSPHParams* sp,
+                                       const OGridParams* gps, float s0, float s1, int dynamic) {
+    rigid_force_impl<float>(num, pos_s, veleval_s, density, force, bpos_s, bvel_s, bmass_s, bstart, bend, sp, gps,
+                            s0, s1, dynamic);
+}
+extern "C" void oracle_rigid_force_f64(int num, const double* pos_s, const double* veleval_s, const double* density,
+                                       double* force, const float* bpos_s, const float* bvel_s, const float* bmass_s,
+                                       const u32* bstart, const u32* bend, const OSPHParams* sp,
+                                       const OGridParams* gps, float s0, float s1, int dynamic) {
+    rigid_force_impl<double>(num, pos_s, veleval_s, density, force, bpos_s, bvel_s, bmass_s, bstart, bend, sp, gps,
+                             s0, s1, dynamic);
+}
+
+// Rigid-body particle sets coupled into oracle steps (set with
+// oracle_set_rigid; kind 0 = dynamic, 1 = static, applied in that order as
+// SPH::interact does, after the walls and before the integration).  Each set
+// is hashed and sorted into the fluid's grid by the step, as the rigid-body
+// system's own hash_and_sort / cellindices would (System.cpp:145-170).
+namespace {
+struct RigidSet {
+    std::vector<float> pos, vel, mass;
+    float s0 = 0, s1 = 0;
+    int n = 0;
+};
+RigidSet g_rigid[2];
+}  // namespace
+extern "C" void oracle_set_rigid(int kind, int n, const float* pos, const float* vel, const float* mass, float s0,
+                                 float s1) {
+    RigidSet& r = g_rigid[kind ? 1 : 0];
+    r.n = n > 0 ? n : 0;
+    r.pos.assign(pos ? pos : (const float*)nullptr, pos ? pos + 4 * (size_t)r.n : nullptr);
+    r.vel.assign((size_t)4 * r.n, 0.f);
+    r.mass.assign((size_t)r.n, 0.f);
+    if (vel) std::copy(vel, vel + 4 * (size_t)r.n, r.vel.begin());
+    if (mass) std::copy(mass, mass + (size_t)r.n, r.mass.begin());
+    r.s0 = s0;
+    r.s1 = s1;
+}
+
+// Both sets' forces on the step's sorted fluid state (SPH::interact).
+template <typename T>
+static void interact_impl(int num, const T* pos_s, const T* veleval_s, const T* density, T* force,
+                          const OSPHParams* sp, const OGridParams* gp, const OGridParams* gps) {
+    for (int kind = 0; kind < 2; ++kind) {
+        const RigidSet& r = g_rigid[kind];
+        if (r.n == 0) continue;
+        std::vector<u32> keys(r.n), idx(r.n), start(gp->nb_cells + 1), end(gp->nb_cells + 1, 666u);
+        hash_impl<float>(r.n, r.pos.data(), gp, keys.data(), idx.data());
+        oracle_sort_stable(r.n, keys.data(), idx.data());
+        oracle_cellindices(r.n, keys.data(), gp->nb_cells, start.data(), end.data());
+        std::vector<float> bp(4 * (size_t)r.n), bv(4 * (size_t)r.n), bm((size_t)r.n);
+        for (int q = 0; q < r.n; ++q) {
+            for (int c = 0; c < 4; ++c) {
+                bp[4 * (size_t)q + c] = r.pos[4 * (size_t)idx[q] + c];
+                bv[4 * (size_t)q + c] = r.vel[4 * (size_t)idx[q] + c];
+            }
+            bm[q] = r.mass[idx[q]];
+        }
+        rigid_force_impl<T>(num, pos_s, veleval_s, density, force, bp.data(), bv.data(), bm.data(), start.data(),
+                            end.data(), sp, gps, r.s0, r.s1, kind == 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // a11: leapfrog.cl:30-121, scattering to the original index.
 template <typename T>
 static void leapfrog_impl(int num, const T* pos_s, const T* vel_s, const T* force, const T* xsph, const u32* idx,
@@ -666,6 +802,7 @@ static int step_impl(int* num_io, T* pos_u, T* vel_u, T* veleval_u, const OSPHPa
     g_stage_s[5] += secs_since(t);
     if (nsrc > 0) gravity_impl<T>(num, nsrc, src, smass, alpha, pos_s, force, (T)p.simulation_scale);
     if (flags & 1) collision_impl<T>(num, pos_s, vel_s, force, &p, gps);
+    interact_impl<T>(num, pos_s, veleval_s, density, force, &p, gp, gps);   // no sets: nothing
     g_stage_s[6] += secs_since(t);
     if (flags & 4) euler_impl<T>(num, pos_s, vel_s, force, idx, &p, dt, pos_u, vel_u);
     else leapfrog_impl<T>(num, pos_s, vel_s, force, xsph, idx, &p, dt, pos_u, vel_u, veleval_u);
@@ -733,6 +870,28 @@ extern "C" void oracle_set_threads(int n) {
 #endif
 }
 
+
+// ---------------------------------------------------------------------------
+// a13: addRect (rtpslib/domain/IV.cpp:165-202) as System::addBox calls it
+// (System.cpp:331-338, scale 1): z outer, y, x inner, each coordinate
+// accumulated in float (`x += spacing`) while <= the box's max; at most num
+// points (the inner loop stops at num; the outer loops then add nothing).
+// Writes up to num float4 (w = 1) and returns the count.
+extern "C" int oracle_add_rect(int num, const float* bmin, const float* bmax, float spacing, float scale,
+                               float* out) {
+    const float xmin = bmin[0] / scale, xmax = bmax[0] / scale;
+    const float ymin = bmin[1] / scale, ymax = bmax[1] / scale;
+    const float zmin = bmin[2] / scale, zmax = bmax[2] / scale;
+    int i = 0;
+    for (float z = zmin; z <= zmax; z += spacing)
+        for (float y = ymin; y <= ymax; y += spacing)
+            for (float x = xmin; x <= xmax; x += spacing) {
+                if (i >= num) break;
+                out[4 * i] = x; out[4 * i + 1] = y; out[4 * i + 2] = z; out[4 * i + 3] = 1.0f;
+                ++i;
+            }
+    return i;
+}
 
 // ---------------------------------------------------------------------------
 // a1: parameter replay.  SPH::calculate (SPHSettings.cpp:30-144), the settings

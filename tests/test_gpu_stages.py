@@ -105,6 +105,33 @@ def test_sort_matches_stable_sort(cuda, n, bits):
     np.testing.assert_array_equal(_u(vt), v[order])
 
 
+@pytest.mark.parametrize("direction,in_place", [(1, False), (0, False), (1, True), (0, True)])
+def test_sort_segments_bitonic_contract(cuda, direction, in_place):
+    """Bitonic<T>::Sort(batch, arrayLength, dir, dst, src) (BitonicSort.h:26-28):
+    three independent arrays of 5,000 pairs (not a power of two) on all 32
+    key bits, ascending (dir 1) or descending (dir 0), src -> dst or in place;
+    sources untouched when dst differs.  Keys bit-exact; values in the order
+    a stable sort gives."""
+    from enjaparticles_amd import ops
+    import torch
+    batch, L = 3, 5000
+    rng = np.random.default_rng(40 + direction)
+    k = rng.integers(0, 0xFFFFFFFF, size=batch * L, dtype=np.uint64, endpoint=True).astype(np.uint32)
+    k[100:900] = k[50]          # runs of equal keys
+    v = np.arange(batch * L, dtype=np.uint32)
+    sk, sv = _t(k.view(np.int32), cuda), _t(v.view(np.int32), cuda)
+    dk, dv = (sk, sv) if in_place else (torch.empty_like(sk), torch.empty_like(sv))
+    ops.sort_segments(dk, dv, sk, sv, batch, L, direction)
+    for b in range(batch):
+        kb, vb = k[b * L:(b + 1) * L], v[b * L:(b + 1) * L]
+        order = np.argsort(kb if direction else ~kb, kind="stable")
+        np.testing.assert_array_equal(_u(dk)[b * L:(b + 1) * L], kb[order])
+        np.testing.assert_array_equal(_u(dv)[b * L:(b + 1) * L], vb[order])
+    if not in_place:
+        np.testing.assert_array_equal(_u(sk), k)
+        np.testing.assert_array_equal(_u(sv), v)
+
+
 def test_sort_dam_keys_per_cell_sets(cuda):
     """Sorted keys bit-exact; per-cell index sets equal (the reference's bitonic
     network is unstable, so only sets are comparable with it)."""

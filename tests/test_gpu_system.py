@@ -231,14 +231,22 @@ def test_push_capacity_and_settings_change(cuda):
     s.close()
 
 
-def test_addbox_matches_addrect(cuda):
+@pytest.mark.parametrize("nn,lo,hi", [(1000, (0.1, 0.1, 0.1), (0.3, 0.3, 0.3)),
+                                      (100000, (0.05, 0.2, 0.3), (0.33, 0.41, 0.5)),
+                                      (37, (0.1, 0.2, 0.3), (0.33, 0.41, 0.5))])
+def test_addbox_matches_addrect(cuda, nn, lo, hi):
+    """System::addBox (System.cpp:331-338) pushes exactly addRect's lattice
+    (IV.cpp:165-202, restated in the oracle): every particle bit for bit, the
+    count capped at nn, zero velocity and veleval, default colour."""
     sc = scenes.dam16k()
     s = SPH(sc.settings)
-    n = s.addBox(1000, (0.1, 0.1, 0.1), (0.3, 0.3, 0.3), spacing=0.02)
+    n = s.addBox(nn, lo, hi, spacing=0.02)
+    ref = O.add_rect(nn, lo, hi, 0.02)
     p = s.read("pos")
-    assert n == len(p) == 1000
-    assert np.all(p[:, 3] == 1.0)
-    assert p[1, 0] == np.float32(np.float32(0.1) + np.float32(0.02))
+    assert n == len(p) == len(ref) == s.getNum()
+    np.testing.assert_array_equal(p, ref)
+    assert not np.any(s.read("vel")) and not np.any(s.read("veleval"))
+    np.testing.assert_array_equal(s.read("color"), np.tile(np.float32([1, 0, 0, 1]), (n, 1)))
     s.close()
 
 
@@ -347,11 +355,12 @@ def test_fixed_launch_options(cuda):
 
 
 def test_sort_tile_sizes_give_the_same_step(cuda):
-    """Radix-sort tiles of 1024, 2048 and 4096 keys (RTPS_OPT_SORT_TILE) sort
-    identically (stable), so the steps are bit-identical."""
+    """Radix-sort tiles of 1024, 2048, 4096 and 8192 keys (RTPS_OPT_SORT_TILE
+    = 1, 2, 4, 8 keys per thread) sort identically (stable), so the steps are
+    bit-identical."""
     sc = scenes.dam_custom(32, 32, 16, seed=82)
     out = {}
-    for items in (16, 8, 4):
+    for items in (8, 4, 2, 1):
         s = _system(sc)
         s.setOption(8, items)
         s.step(3)
@@ -360,8 +369,8 @@ def test_sort_tile_sizes_give_the_same_step(cuda):
     s = _system(sc)
     s.setOption(8, 0)           # process-wide: restore the default (by size)
     s.close()
-    for items in (16, 4):
-        for a, b in zip(out[items], out[8]):
+    for items in (8, 4, 1):
+        for a, b in zip(out[items], out[2]):
             np.testing.assert_array_equal(a, b)
 
 

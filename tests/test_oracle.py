@@ -419,3 +419,111 @@ def test_tangent_linear_step_matches_central_differences():
     vel = np.zeros_like(pos)
     vel[:, :3] = scenes.normal(24, 3 * len(pos), 0.05).reshape(-1, 3)
     _tl_vs_fd(pos, vel, scenes.dam_settings(0.5, len(pos), dt=0.002), 10, "walls")
+
+
+def test_add_rect_lattice_closed_form():
+    """addRect (IV.cpp:165-202): count per axis = floor((max - min) / s) + 1
+    where the box is not within rounding of a whole number of spacings,
+    coordinates min + k s within float accumulation error (k ulps), z outer /
+    x inner order, w = 1, and the count capped at num part way through a row."""
+    lo, hi, sp = (0.1, 0.2, 0.3), (0.33, 0.41, 0.51), 0.02
+    per = [int(np.floor((h - l) / sp)) + 1 for l, h in zip(lo, hi)]
+    assert all(abs((h - l) / sp - round((h - l) / sp)) > 1e-3 for l, h in zip(lo, hi))
+    full = per[0] * per[1] * per[2]
+    p = O.add_rect(10 ** 6, lo, hi, sp)
+    assert len(p) == full
+    k = np.arange(full)
+    ix, iy, iz = k % per[0], (k // per[0]) % per[1], k // (per[0] * per[1])
+    for c, idx in ((0, ix), (1, iy), (2, iz)):
+        exact = lo[c] + idx * sp
+        assert np.all(np.abs(p[:, c] - exact) <= (idx + 2) * 2e-8 + 1e-7), c
+    assert np.all(p[:, 3] == 1.0)
+    # float accumulation, not multiplication: x_k = fl(x_{k-1} + s)
+    x = np.float32(lo[0])
+    for j in range(per[0]):
+        assert p[j, 0] == x
+        x = np.float32(x + np.float32(sp))
+    q = O.add_rect(per[0] + 3, lo, hi, sp)     # stops three points into the second row
+    np.testing.assert_array_equal(q, p[: per[0] + 3])
+
+
+def _rigid_setup(st, bpos):
+    sp, gp, gs = O.derive_params(st)
+    keys, idx = O.hash(bpos, gp)
+    keys, idx = O.sort_stable(keys, idx)
+    start, end, _ = O.cellindices(keys, gp.nb_cells)
+    return sp, gp, gs, bpos[idx], idx, start, end
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_rigid_force_two_particles_closed_form(dynamic):
+    """One fluid particle and one boundary particle at r < h on +x:
+    F = -s0 m (2h - r) n + s1 sqrt(s0 m mn) (vrel . n) n, mn = m m_j / (m + m_j)
+    (m / 2 for the static set), force += F / m -+ m / rho^2 mu c V with
+    V = w (h - r): w = vrel (static, minus sign) or its tangential part
+    (dynamic, plus sign); vrel = v_j - e_i (static: -e_i)."""
+    st = scenes.dam_settings(1.0, 16)
+    s0, s1 = 3.0, 0.5
+    pos = np.array([[0.5, 0.5, 0.5, 1.0]])
+    e = np.array([[0.3, -0.2, 0.1, 0.0]])
+    r = 0.013
+    bpos = np.array([[0.5 + r, 0.5, 0.5, 1.0]], np.float32)
+    bvel = np.array([[0.05, 0.4, 0.0, 0.0]], np.float32)
+    bmass = np.array([0.02], np.float32)
+    sp, gp, gs, bps, idx, start, end = _rigid_setup(st, bpos)
+    rho = np.array([950.0])
+    f0 = np.zeros((1, 4))
+    f = O.rigid_force(pos, e, rho, f0, bps, start, end, sp, gs, s0, s1,
+                      *((bvel, bmass) if dynamic else ()))
+    m, mu, c, h = sp.mass, sp.viscosity, sp.wvisc_dd_coef, sp.smoothing_distance
+    rr = float(np.float32(0.5 + r)) - 0.5
+    n = np.array([1.0, 0.0, 0.0])
+    mj = float(bmass[0]) if dynamic else m
+    mn = m * mj / (m + mj)
+    vrel = (bvel[0, :3].astype(np.float64) if dynamic else 0.0) - e[0, :3]
+    F = -s0 * m * (2 * h - rr) * n + s1 * np.sqrt(s0 * m * mn) * (vrel @ n) * n
+    w = vrel - (vrel @ n) * n if dynamic else vrel
+    V = w * (h - rr)
+    expect = F / m + (1 if dynamic else -1) * m / rho[0] ** 2 * mu * c * V
+    np.testing.assert_allclose(f[0, :3], expect, rtol=1e-9, atol=1e-12)
+
+
+def test_rigid_force_brute_force():
+    """Against an O(N M) loop over every (fluid, boundary) pair accepted by
+    the reference's rule (boundary cell within +-1 of the fluid particle's
+    cell on each axis, |r| <= 2h), in FP64, both forms."""
+    st = scenes.dam_settings(0.6, 4096)
+    rng = np.random.default_rng(12)
+    pos = np.c_[rng.uniform(0.05, 0.55, (600, 3)), np.ones(600)]
+    e = np.c_[rng.normal(0, 0.1, (600, 3)), np.zeros(600)]
+    rho = rng.uniform(800, 1200, 600)
+    bpos = np.c_[rng.uniform(0.0, 0.6, (900, 3)), np.ones(900)].astype(np.float32)
+    bvel = np.c_[rng.normal(0, 0.1, (900, 3)), np.zeros(900)].astype(np.float32)
+    bmass = rng.uniform(0.005, 0.02, 900).astype(np.float32)
+    sp, gp, gs, bps, idx, start, end = _rigid_setup(st, bpos)
+    h, eps, m = sp.smoothing_distance, sp.EPSILON, sp.mass
+    s0, s1 = 2.0, 0.75
+    cell = lambda p: np.trunc((p[:, :3] - [gs.grid_min.x, gs.grid_min.y, gs.grid_min.z])  # noqa: E731
+                              * [gs.grid_delta.x, gs.grid_delta.y, gs.grid_delta.z]).astype(int)
+    ci, cj = cell(pos), cell(bpos.astype(np.float64))
+    for dynamic in (False, True):
+        got = O.rigid_force(pos, e, rho, np.zeros((600, 4)), bps, start, end, sp, gs, s0, s1,
+                            *((bvel[idx], bmass[idx]) if dynamic else ()))
+        ref = np.zeros((600, 3))
+        for i in range(600):
+            ok = np.all(np.abs(cj - ci[i]) <= 1, axis=1)
+            r = bpos[ok, :3].astype(np.float64) - pos[i, :3]
+            rl = np.linalg.norm(r, axis=1)
+            acc = rl <= 2 * h
+            r, rl = r[acc], np.maximum(rl[acc], eps)
+            n = r / rl[:, None]
+            mj = bmass[ok][acc].astype(np.float64) if dynamic else np.full(len(rl), m)
+            mn = m * mj / (m + mj)
+            vrel = (bvel[ok][acc, :3].astype(np.float64) if dynamic else 0.0) - e[i, :3]
+            un = np.sum(vrel * n, axis=1)
+            F = (-s0 * m * (2 * h - rl))[:, None] * n + (s1 * np.sqrt(s0 * m * mn) * un)[:, None] * n
+            w = vrel - un[:, None] * n if dynamic else vrel + 0 * n
+            V = np.sum(w * (h - rl)[:, None], axis=0)
+            ref[i] = F.sum(0) / m + (1 if dynamic else -1) * m / rho[i] ** 2 * sp.viscosity * sp.wvisc_dd_coef * V
+        assert np.count_nonzero(np.abs(ref).sum(1)) > 300
+        np.testing.assert_allclose(got[:, :3], ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())

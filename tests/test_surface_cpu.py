@@ -1,16 +1,22 @@
 """Marching-cubes case table and iso-surface restatement (no GPU needed).
 
-The library's generated table (rtps_mc_case_table) equals the restatement
-in surface_oracle (oracle/surface_oracle.py).  The table is pinned by what
-a correct case table must satisfy: each case's triangles use exactly its
-crossing edges, at most five triangles, oriented from the above-iso corners
-outwards; surfaces extracted with it are closed wherever the lattice does
-not cut them (every interior lattice edge carrying a vertex is shared by
-exactly two triangles, with opposite orientations, also for random fields
-full of ambiguous faces), a sphere has Euler characteristic 2 and a torus 0,
-vertices of a linear field lie on its plane, and normals point outwards.
+The library's case table (rtps_mc_case_table) and edge ends
+(rtps_mc_geometry) are the reference's: their SHA-256 fingerprints equal
+those of triTable (traversehp.cl:74), nrOfTriangles (classifycubes.cl:37)
+and offsets3 (traversehp.cl:35) recorded in tests/golden/mc_tables.json
+(tests/golden/make_mc_digest.py, run where the reference tree exists).  The
+corner bit order equals the restatement in oracle/surface_oracle.py
+(classifycubes.cl:44-55 with indexmap.h's cubeOffsets).  The table is also
+checked for what a case table must satisfy: each case's triangles use exactly
+its crossing edges, at most five, all wound alike; a sphere extracted with it
+is closed with Euler characteristic 2 and a torus with 0 (no ambiguous faces
+at these resolutions), vertices of a linear field lie on its plane and every
+vertex of a random field interpolates the iso level on a crossing edge.
 """
 import ctypes as C
+import hashlib
+import json
+import os
 from collections import Counter
 
 import numpy as np
@@ -18,39 +24,53 @@ import pytest
 
 import surface_oracle as S
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mc_tables.json")
+
 
 @pytest.fixture(scope="module")
 def table():
-    return S.mc_case_table()
-
-
-def test_library_table_equals_restatement(table):
     from enjaparticles_amd import _native as N
     tri = np.zeros((256, 16), np.int8)
     ntri = np.zeros(256, np.uint8)
     N.check(N.lib().rtps_mc_case_table(tri.ctypes.data, ntri.ctypes.data))
-    np.testing.assert_array_equal(tri, table[0])
-    np.testing.assert_array_equal(ntri, table[1])
+    return tri, ntri
+
+
+def test_library_table_is_the_references(table):
+    from enjaparticles_amd import _native as N
+    g = json.load(open(GOLDEN))
+    tri, ntri = table
+    assert hashlib.sha256(tri.tobytes()).hexdigest() == g["triTable_int8_sha256"]
+    assert hashlib.sha256(ntri.tobytes()).hexdigest() == g["nrOfTriangles_uint8_sha256"]
+    assert int(ntri.astype(int).sum()) == g["triangles_total"]
+    corners = np.zeros(24, np.int8)
+    edges = np.zeros(72, np.int8)
+    N.check(N.lib().rtps_mc_geometry(corners.ctypes.data, edges.ctypes.data))
+    assert hashlib.sha256(edges.tobytes()).hexdigest() == g["offsets3_int8_sha256"]
+    np.testing.assert_array_equal(corners.reshape(8, 3), np.array(S.CORNERS))
+    np.testing.assert_array_equal(edges.reshape(12, 2, 3), np.array(S.EDGE_ENDS))
 
 
 def test_cases_use_exactly_their_crossing_edges(table):
     tri, ntri = table
     assert ntri[0] == 0 and ntri[255] == 0 and ntri.max() == 5
+    sign = set()
     for m in range(256):
         above = [(m >> k) & 1 for k in range(8)]
-        crossing = {e for e, (a, b) in enumerate(S.EDGES) if above[a] != above[b]}
+        crossing = {e for e, (a, b) in enumerate(S.EDGE_ENDS) if above[S.corner_of(a)] != above[S.corner_of(b)]}
         used = tri[m, : 3 * ntri[m]]
         assert set(used.tolist()) == crossing, m
         assert (tri[m, 3 * ntri[m]:] == -1).all()
-        # orientation: no triangle (vertices at edge midpoints) faces from the
-        # below-iso corners of its edges towards the above-iso ones
+        # winding: every triangle (vertices at edge midpoints) faces the same
+        # way relative to its edges' above-iso ends
         for t in range(ntri[m]):
             es = used[3 * t: 3 * t + 3]
-            p = [0.5 * (S.corner(S.EDGES[e][0]) + S.corner(S.EDGES[e][1])) for e in es]
+            p = [0.5 * (np.array(S.EDGE_ENDS[e][0]) + np.array(S.EDGE_ENDS[e][1])) for e in es]
             n = np.cross(p[1] - p[0], p[2] - p[0])
-            d = sum((S.corner(S.EDGES[e][1]) - S.corner(S.EDGES[e][0])) * (1 if above[S.EDGES[e][0]] else -1)
-                    for e in es)
-            assert n @ d >= 0, (m, t)
+            d = sum((np.array(S.EDGE_ENDS[e][1]) - np.array(S.EDGE_ENDS[e][0]))
+                    * (1 if above[S.corner_of(S.EDGE_ENDS[e][0])] else -1) for e in es)
+            sign.add(bool(n @ d > 0))
+    assert len(sign) == 1
 
 
 def _closed(keys, res):
@@ -90,9 +110,10 @@ def test_sphere_is_closed_with_outward_normals(table):
     r = np.linalg.norm(v, axis=2)
     assert np.abs(r - 0.7).max() < 0.02
     assert ((n * v).sum(2) > 0).all()   # normals away from the inside (higher field)
-    # geometric winding agrees with the normals
+    # the table winds every triangle alike (here: against the normals)
     g = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
-    assert ((g * v.mean(1)).sum(1) > 0).all()
+    w = (g * v.mean(1)).sum(1)
+    assert (w < 0).all() or (w > 0).all()
 
 
 def test_torus_is_closed(table):
@@ -106,12 +127,23 @@ def test_torus_is_closed(table):
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_random_fields_are_closed_inside_the_lattice(table, seed):
+def test_random_field_vertices_interpolate_the_iso_level(table, seed):
+    """Every vertex lies on a lattice edge whose ends straddle the iso level,
+    at the linearly interpolated crossing; each lattice edge gets one vertex
+    position however many cubes share it."""
     res = 12
     f = np.random.default_rng(seed).normal(size=(res, res, res)).astype(np.float32)
-    _, _, k = S.marching_cubes(f, 0.0, table, (0, 0, 0), (1, 1, 1))
+    v, _, k = S.marching_cubes(f, 0.0, table, (0, 0, 0), (res - 1, res - 1, res - 1))
     assert len(k) > 100
-    _closed(k, res)
+    seen = {}
+    for tv, tk in zip(v, k):
+        for q in range(3):
+            a, b = tk[q]
+            fa, fb = f[a[2], a[1], a[0]], f[b[2], b[1], b[0]]
+            assert (fa > 0) != (fb > 0)
+            t = fa / (fa - fb)
+            np.testing.assert_allclose(tv[q], np.array(a) + t * (np.array(b) - np.array(a)), atol=2e-5)
+            assert np.allclose(seen.setdefault((a, b), tv[q]), tv[q], atol=2e-5)
 
 
 def test_linear_field_gives_its_plane(table):
