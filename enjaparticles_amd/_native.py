@@ -74,6 +74,7 @@ SIGNATURES = {
     "rtps_sph_set_static_boundary": (_i, [P, P, _i, _f, _f]),
     "rtps_sph_interact": (_i, [P]),
     "rtps_mc_geometry": (_i, [P, P]),
+    "rtps_batch_autotune": (_i, [P, _i, P, P, P, P]),
     "rtps_cellindices": (_i, [_i, P, _i, P, P, P, P]),
     "rtps_permute": (_i, [_i, P, P, P, P, P, P, P, P]),
     "rtps_density": (_i, [_i, P, P, P, P, P, P, P]),

@@ -265,11 +265,11 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
 
 // MODE: kPairsDensity, kPairsForceStep (force + walls + leapfrog, scattered to
 // the unsorted arrays) or kPairsForceStore (force and XSPH stored in sorted
-// order for the adjoint's recorded step).
-// (kBBlock = 256: one block per CU is one wave per SIMD, so the register budget
-// of kPersistentBlocksPerCU waves per SIMD keeps all the persistent blocks resident.)
-template <int MODE>
-__global__ __launch_bounds__(kBBlock, kPersistentBlocksPerCU) void k_pairs_b(const BatchItem* __restrict__ items, int n_items,
+// order for the adjoint's recorded step).  BLOCK threads per block = slots per
+// work item; compiled for WAVES waves per SIMD (the register budget that keeps
+// the grid's WAVES * 256 / BLOCK blocks per CU resident).
+template <int MODE, int BLOCK, int WAVES>
+__global__ __launch_bounds__(BLOCK, WAVES) void k_pairs_b(const BatchItem* __restrict__ items, int n_items,
                                                      int* __restrict__ queue, const SceneDev* __restrict__ scenes,
                                                      const int* __restrict__ d_num, const float4* __restrict__ pos_s,
                                                      const uint32_t* __restrict__ start,
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kBBlock, kPersistentBlocksPerCU) void k_pairs_b(con
                                                      float4* __restrict__ force_out, float4* __restrict__ xsph_out,
                                                      const uint32_t* __restrict__ sub) {
     __shared__ int sh_item;
-    __shared__ float4 stage_b[kBBlock / 64][2 * kBStageCap];   // one slice per wave (24 KB per block)
+    __shared__ float4 stage_b[BLOCK / 64][2 * kBStageCap];   // one 6 KB slice per wave
     const int q0 = (int)(blockIdx.x % kXcdQueues);
     int q = q0;   // thread 0's current queue
     for (;;) {
@@ -340,31 +340,32 @@ hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* sce
     return hipGetLastError();
 }
 
-hipError_t launch_pairs_b(int mode, int grid, const BatchItem* items, int n_items, int* queue,
+int pairs_blocks_per_cu(const PairsCfg& c) { return c.waves * 256 / c.block; }
+
+bool pairs_cfg_valid(const PairsCfg& c) {
+    return (c.block == 128 || c.block == 256) && c.waves >= 4 && c.waves <= 6;
+}
+
+hipError_t launch_pairs_b(int mode, const PairsCfg& cfg, int grid, const BatchItem* items, int n_items, int* queue,
                           const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
                           const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,
                           const uint32_t* idx, float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
                           float4* force_out, float4* xsph_out, const uint32_t* sub) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (mode == kPairsForceStore && (!force_out || !xsph_out)) return hipErrorInvalidValue;
+    if (!pairs_cfg_valid(cfg)) return hipErrorInvalidValue;
+#define RTPS_PAIRS(M, B, W)                                                                                             k_pairs_b<M, B, W><<<grid, B, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start, end, keys, ve,                                                 vel_s, idx, pos_u, vel_u, veleval_u, force_out, xsph_out, sub)
+#define RTPS_PAIRS_W(M, B)                                                                                              if (cfg.waves == 4) { RTPS_PAIRS(M, B, 4); } else if (cfg.waves == 5) { RTPS_PAIRS(M, B, 5); }                      else { RTPS_PAIRS(M, B, 6); }
+#define RTPS_PAIRS_BW(M)                                                                                                if (cfg.block == 256) { RTPS_PAIRS_W(M, 256) } else { RTPS_PAIRS_W(M, 128) }
     switch (mode) {
-        case kPairsDensity:
-            k_pairs_b<kPairsDensity><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start,
-                                                               end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
-                                                               force_out, xsph_out, sub);
-            break;
-        case kPairsForceStep:
-            k_pairs_b<kPairsForceStep><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start,
-                                                                 end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
-                                                                 force_out, xsph_out, sub);
-            break;
-        case kPairsForceStore:
-            k_pairs_b<kPairsForceStore><<<grid, kBBlock, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start,
-                                                                  end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
-                                                                  force_out, xsph_out, sub);
-            break;
+        case kPairsDensity: RTPS_PAIRS_BW(kPairsDensity) break;
+        case kPairsForceStep: RTPS_PAIRS_BW(kPairsForceStep) break;
+        case kPairsForceStore: RTPS_PAIRS_BW(kPairsForceStore) break;
         default: return hipErrorInvalidValue;
     }
+#undef RTPS_PAIRS_BW
+#undef RTPS_PAIRS_W
+#undef RTPS_PAIRS
     return hipGetLastError();
 }
 

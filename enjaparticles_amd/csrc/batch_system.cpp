@@ -7,8 +7,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -34,8 +40,14 @@ struct rtps_batch {
            *ve_s = nullptr;
     uint32_t *keys = nullptr, *idx = nullptr, *start = nullptr, *end = nullptr;
     uint32_t *keys_sorted = nullptr, *idx_sorted = nullptr;
+    // work items of 256 and of 128 slots (the two tile sizes of the pair kernels)
     BatchItem* items = nullptr;
-    int n_items = 0, persistent_grid = 0;
+    BatchItem* items128 = nullptr;
+    int n_items = 0, n_items128 = 0, cus = 256;
+    // launch configurations of the density and force pair kernels (the autotuner's choice)
+    PairsCfg cfg_density, cfg_force;
+    int opt_autotune = 1;
+    bool tuned = false;
     int* queues = nullptr;
     void* sort_tmp = nullptr;
     uint32_t* sub = nullptr;   // sub-cell starts of the windowed rows, 16 per cell
@@ -51,7 +63,7 @@ struct rtps_batch {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         void* bufs[] = {scenes, d_num, sid, pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s, keys, idx,
-                        start, end, items, queues, sort_tmp, sub};
+                        start, end, items, items128, queues, sort_tmp, sub};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         for (auto& e : ev)
@@ -78,7 +90,29 @@ void mark(rtps_batch* b, int i) {
     if (b->opt_timers) (void)hipEventRecord(b->ev[i], b->stream);
 }
 
-int launch_step(rtps_batch* b) {
+// Work items and persistent grid of a pair-kernel configuration.
+const BatchItem* items_of(const rtps_batch* b, const PairsCfg& c) { return c.block == 128 ? b->items128 : b->items; }
+int nitems_of(const rtps_batch* b, const PairsCfg& c) { return c.block == 128 ? b->n_items128 : b->n_items; }
+int grid_of(const rtps_batch* b, const PairsCfg& c) {
+    return std::max(1, std::min(nitems_of(b, c), b->cus * pairs_blocks_per_cu(c)));
+}
+
+int launch_density_b(rtps_batch* b, const PairsCfg& c) {
+    RTPS_TRY_HIP(launch_pairs_b(kPairsDensity, c, grid_of(b, c), items_of(b, c), nitems_of(b, c), b->queues,
+                                b->scenes, b->d_num, b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s,
+                                b->idx_sorted, b->pos_u, b->vel_u, b->veleval_u, b->stream, nullptr, nullptr, b->sub));
+    return RTPS_OK;
+}
+
+int launch_force_b(rtps_batch* b, const PairsCfg& c) {
+    RTPS_TRY_HIP(launch_pairs_b(kPairsForceStep, c, grid_of(b, c), items_of(b, c), nitems_of(b, c),
+                                b->queues + kXcdQueues, b->scenes, b->d_num, b->pos_s, b->start, b->end,
+                                b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted, b->pos_u, b->vel_u, b->veleval_u,
+                                b->stream, nullptr, nullptr, b->sub));
+    return RTPS_OK;
+}
+
+int launch_front_b(rtps_batch* b) {
     mark(b, B_HASH);
     RTPS_TRY_HIP(launch_hash_b(b->T, b->sid, b->scenes, b->d_num, b->pos_u, b->keys, b->idx, b->start,
                                b->total_cells, b->queues, b->stream));
@@ -90,16 +124,169 @@ int launch_step(rtps_batch* b) {
     mark(b, B_PERMUTE);
     RTPS_TRY_HIP(launch_permute(b->T, nullptr, b->idx_sorted, b->pos_u, b->vel_u, b->veleval_u, b->pos_s, b->vel_s,
                                 b->ve_s, b->stream));
+    return RTPS_OK;
+}
+
+int launch_step(rtps_batch* b) {
+    int rc = launch_front_b(b);
+    if (rc) return rc;
     mark(b, B_DENSITY);
-    RTPS_TRY_HIP(launch_pairs_b(kPairsDensity, b->persistent_grid, b->items, b->n_items, b->queues, b->scenes, b->d_num,
-                                b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted,
-                                b->pos_u, b->vel_u, b->veleval_u, b->stream, nullptr, nullptr, b->sub));
+    rc = launch_density_b(b, b->cfg_density);
+    if (rc) return rc;
     mark(b, B_FORCE);
-    RTPS_TRY_HIP(launch_pairs_b(kPairsForceStep, b->persistent_grid, b->items, b->n_items, b->queues + kXcdQueues, b->scenes, b->d_num,
-                                b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted,
-                                b->pos_u, b->vel_u, b->veleval_u, b->stream, nullptr, nullptr, b->sub));
+    rc = launch_force_b(b, b->cfg_force);
+    if (rc) return rc;
     RTPS_TRY_HIP(launch_commit_b(b->S, b->scenes, b->d_num, b->start, b->stream));
     mark(b, B_N);
+    return RTPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner of the pair kernels (no reference counterpart): threads per block
+// (= slots per work item, 128 or 256) and register budget (4, 5 or 6 waves per
+// SIMD) of the density and the force kernel, timed on the live sorted state
+// of the first step; each kernel keeps its fastest configuration (a
+// configuration tried later must be 3% faster).  Both kernels are idempotent
+// on one sorted state (density rewrites veleval_s.w from the positions; the
+// force step writes the unsorted arrays from the sorted ones), so tuning is
+// an ordinary step.  Choices are cached per process by device and layout size
+// class, and in the file named by RTPS_TUNE_CACHE when it is set.
+std::mutex g_btune_mu;
+std::map<std::string, std::pair<PairsCfg, PairsCfg>> g_btune;
+bool g_btune_file_loaded = false;
+
+std::string btune_key(const rtps_batch* b) {
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    std::string arch = "unknown";
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        arch = prop.gcnArchName;
+        for (char& ch : arch)
+            if (ch == ' ' || ch == ':') ch = '_';
+    }
+    unsigned long t2 = 1, s2 = 1;
+    while (t2 < (unsigned long)b->T) t2 <<= 1;
+    while (s2 < (unsigned long)b->S) s2 <<= 1;
+    return "batch/" + arch + "/cu" + std::to_string(b->cus) + "/t" + std::to_string(t2) + "/s" + std::to_string(s2);
+}
+
+void btune_load_locked() {
+    if (g_btune_file_loaded) return;
+    g_btune_file_loaded = true;
+    const char* f = std::getenv("RTPS_TUNE_CACHE");
+    if (!f || !*f) return;
+    std::ifstream in(f);
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream is(line);
+        std::string k;
+        PairsCfg d, fo;
+        if (is >> k >> d.block >> d.waves >> fo.block >> fo.waves && k.rfind("batch/", 0) == 0 &&
+            pairs_cfg_valid(d) && pairs_cfg_valid(fo))
+            g_btune[k] = {d, fo};
+    }
+}
+
+bool btune_get(const std::string& k, PairsCfg& d, PairsCfg& f) {
+    std::lock_guard<std::mutex> lk(g_btune_mu);
+    btune_load_locked();
+    auto it = g_btune.find(k);
+    if (it == g_btune.end()) return false;
+    d = it->second.first;
+    f = it->second.second;
+    return true;
+}
+
+void btune_put(const std::string& k, const PairsCfg& d, const PairsCfg& f) {
+    std::lock_guard<std::mutex> lk(g_btune_mu);
+    btune_load_locked();
+    g_btune[k] = {d, f};
+    const char* file = std::getenv("RTPS_TUNE_CACHE");
+    if (file && *file) {
+        std::ofstream out(file, std::ios::app);
+        out << k << ' ' << d.block << ' ' << d.waves << ' ' << f.block << ' ' << f.waves << '\n';
+    }
+}
+
+// One step with the pair-kernel configurations tuned on the way (measure =
+// false: take a cached choice when there is one).
+int autotune_step_b(rtps_batch* b, bool measure) {
+    const std::string key = btune_key(b);
+    PairsCfg d, f;
+    if (!measure && btune_get(key, d, f)) {
+        b->cfg_density = d;
+        b->cfg_force = f;
+        b->tuned = true;
+        b->graph_valid = false;
+        return launch_step(b);
+    }
+    const int timers = b->opt_timers;
+    b->opt_timers = 0;
+    int rc = launch_front_b(b);
+    b->opt_timers = timers;
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    RTPS_TRY_HIP(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    if (err != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_error(err, "hipEventCreate");
+    }
+    const PairsCfg cands[] = {{256, 5}, {256, 4}, {256, 6}, {128, 5}, {128, 4}, {128, 6}};
+    constexpr int kReps = 3;
+    constexpr float kMargin = 0.97f;
+    const char* verbose_env = std::getenv("RTPS_TUNE_VERBOSE");
+    const bool verbose = verbose_env && verbose_env[0] == '1';
+    // ms per launch of kReps launches after one untimed one, each after its
+    // queue counters are cleared; < 0 on error
+    auto time_it = [&](int which, const PairsCfg& c) -> float {
+        float total = 0.f;
+        for (int r = 0; r <= kReps; ++r) {
+            if (launch_fill_u32((uint32_t*)(b->queues + which * kXcdQueues), 0u, kXcdQueues, b->stream) != hipSuccess)
+                return -1.f;
+            if (hipEventRecord(e0, b->stream) != hipSuccess) return -1.f;
+            if ((which == 0 ? launch_density_b(b, c) : launch_force_b(b, c)) != RTPS_OK) return -1.f;
+            if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.f;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.f;
+            if (r > 0) total += ms;
+        }
+        return total / kReps;
+    };
+    PairsCfg best[2];
+    bool failed = false;
+    for (int which = 0; which < 2 && !failed; ++which) {
+        float best_ms = 1e30f;
+        for (const PairsCfg& c : cands) {
+            const float ms = time_it(which, c);
+            if (verbose)
+                std::fprintf(stderr, "rtps batch autotune: %s block %d waves %d: %.4f ms\n",
+                             which == 0 ? "density" : "force", c.block, c.waves, ms);
+            if (ms < 0.f) { failed = true; break; }
+            if (best_ms >= 1e30f || ms < kMargin * best_ms) {
+                best_ms = ms;
+                best[which] = c;
+            }
+        }
+        if (which == 0 && !failed) {   // the force kernel reads the densities of the chosen density launch
+            if (launch_fill_u32((uint32_t*)b->queues, 0u, kXcdQueues, b->stream) != hipSuccess ||
+                launch_density_b(b, best[0]) != RTPS_OK)
+                failed = true;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (failed) return hip_error(hipGetLastError(), "batch autotune launch");
+    b->cfg_density = best[0];
+    b->cfg_force = best[1];
+    b->tuned = true;
+    b->graph_valid = false;
+    btune_put(key, best[0], best[1]);
+    // finish the step: the force step with the chosen configuration, then the count commit
+    RTPS_TRY_HIP(launch_fill_u32((uint32_t*)(b->queues + kXcdQueues), 0u, kXcdQueues, b->stream));
+    rc = launch_force_b(b, b->cfg_force);
+    if (rc) return rc;
+    RTPS_TRY_HIP(launch_commit_b(b->S, b->scenes, b->d_num, b->start, b->stream));
     return RTPS_OK;
 }
 
@@ -193,11 +380,13 @@ int build_batch_layout(int num_scenes, const int* n_settings, const char* const*
     L.npasses = np;
     // work items: 256-slot tiles of every scene
     L.items.clear();
+    L.items128.clear();
     L.item_first.assign(num_scenes + 1, 0);
     for (int s = 0; s < num_scenes; ++s) {
         const SceneDev& d = L.scenes[s];
         L.item_first[s] = (int)L.items.size();
         for (int f = 0; f < d.cap; f += 256) L.items.push_back(BatchItem{s, d.off + f, std::min(256, d.cap - f), 0});
+        for (int f = 0; f < d.cap; f += 128) L.items128.push_back(BatchItem{s, d.off + f, std::min(128, d.cap - f), 0});
     }
     L.item_first[num_scenes] = (int)L.items.size();
     return RTPS_OK;
@@ -230,13 +419,13 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
         b->npasses = L.npasses;
         const std::vector<BatchItem>& items = L.items;
         b->n_items = (int)items.size();
-        int dev = 0, cus = 256;
+        b->n_items128 = (int)L.items128.size();
+        int dev = 0;
         if (hipGetDevice(&dev) == hipSuccess) {
             hipDeviceProp_t prop;
             if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-                cus = prop.multiProcessorCount;
+                b->cus = prop.multiProcessorCount;
         }
-        b->persistent_grid = std::max(1, std::min(b->n_items, cus * kPersistentBlocksPerCU));
 
         std::vector<uint16_t> sid((size_t)T);
         for (int s = 0; s < num_scenes; ++s)
@@ -260,6 +449,7 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
         if ((size_t)cells * kSubCells * 4 <= ((size_t)2 << 30))   // at most 2 GiB; else the kernels search
             RTPS_TRY_HIP(hipMalloc((void**)&b->sub, (size_t)cells * kSubCells * 4));
         RTPS_TRY_HIP(hipMalloc((void**)&b->items, sizeof(BatchItem) * items.size()));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->items128, sizeof(BatchItem) * L.items128.size()));
         RTPS_TRY_HIP(hipMalloc((void**)&b->queues, kQueueInts * sizeof(int)));
         RTPS_TRY_HIP(hipMalloc(&b->sort_tmp, sort_temp_bytes((int)T, 32)));
         RTPS_TRY_HIP(hipMemcpyAsync(b->scenes, b->scenes_h.data(), sizeof(SceneDev) * num_scenes,
@@ -268,6 +458,8 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
         RTPS_TRY_HIP(hipMemcpyAsync(b->sid, sid.data(), NT * sizeof(uint16_t), hipMemcpyHostToDevice, b->stream));
         RTPS_TRY_HIP(hipMemcpyAsync(b->items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice,
                                     b->stream));
+        RTPS_TRY_HIP(hipMemcpyAsync(b->items128, L.items128.data(), sizeof(BatchItem) * L.items128.size(),
+                                    hipMemcpyHostToDevice, b->stream));
         RTPS_TRY_HIP(hipMemsetAsync(b->queues, 0, kQueueInts * sizeof(int), b->stream));
         RTPS_TRY_HIP(launch_fill_u32(b->start, 666u, cells, b->stream));
         RTPS_TRY_HIP(launch_fill_u32(b->end, 666u, cells, b->stream));
@@ -327,6 +519,11 @@ int rtps_batch_push_particles(rtps_batch* b, int scene, const rtps_float4* pos, 
 
 int rtps_batch_step(rtps_batch* b, int nsteps) {
     if (!b || nsteps < 0) return set_error(RTPS_ERR_ARG, "rtps_batch_step: bad arguments");
+    if (nsteps > 0 && b->opt_autotune && !b->tuned) {   // the first step tunes the pair kernels
+        int rc = autotune_step_b(b, false);
+        if (rc) return rc;
+        --nsteps;
+    }
     if (b->opt_timers) {
         for (int k = 0; k < nsteps; ++k) {
             int rc = launch_step(b);
@@ -409,6 +606,16 @@ int rtps_batch_set_option(rtps_batch* b, int option, int value) {
     switch (option) {
         case RTPS_OPT_GRAPH: b->opt_graph = value ? 1 : 0; break;
         case RTPS_OPT_TIMERS: b->opt_timers = value ? 1 : 0; break;
+        case RTPS_OPT_AUTOTUNE: b->opt_autotune = value ? 1 : 0; break;
+        case RTPS_OPT_BLOCK_DENSITY:
+        case RTPS_OPT_BLOCK_FORCE: {   // value = block * 10 + waves, e.g. 2565 (256 threads, 5 waves per SIMD)
+            const PairsCfg c{value / 10, value % 10};
+            if (!pairs_cfg_valid(c))
+                return set_error(RTPS_ERR_ARG, "pair kernel launch: block*10 + waves with block 128 or 256, waves 4..6");
+            (option == RTPS_OPT_BLOCK_DENSITY ? b->cfg_density : b->cfg_force) = c;
+            b->tuned = true;   // fixed by the caller
+            break;
+        }
         case RTPS_OPT_COLLISION: {
             for (auto& d : b->scenes_h) d.collision = value ? 1 : 0;
             RTPS_TRY_HIP(hipMemcpyAsync(b->scenes, b->scenes_h.data(), sizeof(SceneDev) * b->S,
@@ -441,3 +648,15 @@ int rtps_batch_reset_timers(rtps_batch* b) {
 }
 
 }  // extern "C"
+
+extern "C" int rtps_batch_autotune(rtps_batch* b, int measure, int* block_density, int* waves_density,
+                                   int* block_force, int* waves_force) {
+    if (!b) return set_error(RTPS_ERR_ARG, "rtps_batch_autotune: null handle");
+    int rc = autotune_step_b(b, measure != 0);
+    if (rc) return rc;
+    if (block_density) *block_density = b->cfg_density.block;
+    if (waves_density) *waves_density = b->cfg_density.waves;
+    if (block_force) *block_force = b->cfg_force.block;
+    if (waves_force) *waves_force = b->cfg_force.waves;
+    return RTPS_OK;
+}

@@ -39,12 +39,18 @@ hipError_t launch_hash_b(int T, const uint16_t* sid, const SceneDev* scenes, con
 hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* scenes, uint32_t* start, uint32_t* end,
                                 hipStream_t st, uint32_t* sub = nullptr);
 constexpr int kPairsDensity = 0, kPairsForceStep = 1, kPairsForceStore = 2;
-// Blocks per CU of the persistent pair kernels (their grid is CUs x this):
-// 5 blocks of 4 waves, compiled for 5 waves per SIMD (at most 96 VGPRs; the
-// 256-scene step measured 5% faster than with 4; 6 with a 6-wave register
-// budget was slower, 8 much slower).
-constexpr int kPersistentBlocksPerCU = 5;
-hipError_t launch_pairs_b(int mode, int grid, const BatchItem* items, int n_items, int* queue,
+// Launch configuration of a persistent pair kernel: `block` threads per block
+// (= slots per work item: 128 or 256) and the register budget of `waves`
+// waves per SIMD (4, 5 or 6: at most 128, 96 or 80 VGPRs); the grid is
+// CUs x waves x 256 / block blocks, all resident.  The batched system's
+// autotuner picks one per kernel; the default is 256 threads at 5 waves.
+struct PairsCfg {
+    int block = 256;
+    int waves = 5;
+};
+int pairs_blocks_per_cu(const PairsCfg& c);
+bool pairs_cfg_valid(const PairsCfg& c);
+hipError_t launch_pairs_b(int mode, const PairsCfg& cfg, int grid, const BatchItem* items, int n_items, int* queue,
                           const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
                           const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,
                           const uint32_t* idx, float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,

@@ -532,6 +532,14 @@ class BatchedSPH:
             N.check(N.lib().rtps_batch_read(self._h, scene, N.FIELD[field], out.ctypes.data, count))
         return out
 
+    def autotune(self, measure: bool = True) -> dict:
+        """rtps_batch_autotune: one step that tunes the pair kernels' block size
+        and register budget; returns the configurations in use."""
+        v = [C.c_int() for _ in range(4)]
+        N.check(N.lib().rtps_batch_autotune(self._h, int(measure), *(C.byref(x) for x in v)))
+        return {"density": {"block": v[0].value, "waves": v[1].value},
+                "force": {"block": v[2].value, "waves": v[3].value}}
+
     def stageTimes(self) -> dict:
         ms = (C.c_float * N.NUM_STAGES)()
         cnt = (C.c_int * N.NUM_STAGES)()

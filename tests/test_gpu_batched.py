@@ -136,3 +136,32 @@ def test_capacity_and_arguments(cuda):
     b.close()
     with pytest.raises(RTPSError):
         BatchedSPH([{"system": "flock", "max_num_particles": "10"}])
+
+
+def test_pair_kernel_configurations_give_the_same_step(cuda):
+    """Every launch configuration the batched autotuner chooses from (block
+    of 128 or 256 slots, 4, 5 or 6 waves per SIMD, for density and force)
+    computes the same step, bit for bit; the autotuner's own step too."""
+    scs = scenes.batched(5, 600, 6000, seed=83)
+    ref = None
+    for block in (256, 128):
+        for waves in (5, 4, 6):
+            b = _batch(scs)
+            b.setOption(11, block * 10 + waves)   # RTPS_OPT_BLOCK_DENSITY
+            b.setOption(12, block * 10 + waves)   # RTPS_OPT_BLOCK_FORCE
+            b.step(4)
+            out = [(b.read(s, "pos"), b.read(s, "density")) for s in range(len(scs))]
+            b.close()
+            if ref is None:
+                ref = out
+                continue
+            for (p, d), (pr, dr) in zip(out, ref):
+                np.testing.assert_array_equal(p, pr, err_msg=f"block {block} waves {waves}")
+                np.testing.assert_array_equal(d, dr, err_msg=f"block {block} waves {waves}")
+    b = _batch(scs)
+    cfg = b.autotune(measure=True)
+    assert cfg["density"]["block"] in (128, 256) and cfg["force"]["waves"] in (4, 5, 6)
+    b.step(3)
+    for s in range(len(scs)):
+        np.testing.assert_array_equal(b.read(s, "pos"), ref[s][0])
+    b.close()
