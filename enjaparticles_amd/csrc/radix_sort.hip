@@ -37,24 +37,41 @@
 
 namespace rtps {
 
-constexpr int kTileThreads = 1024;  // threads of the histogram and one-sweep kernels
-constexpr int kTileWaves = kTileThreads / 64;
+constexpr int kHistThreads = 1024;  // threads of the histogram kernel
 constexpr int kMaxRadix = 256;
 constexpr int kMaxPasses = 4;
 
-// Keys per thread of a tile (1, 2, 4 or 8; 0 = by size: 4 from 2^19 keys up, else 2)
+// One-sweep tile, process-wide: threads * 100 + keys per thread (0 = by
+// size: 1024 x 4 from 2^19 keys up, else 1024 x 2)
 int g_sort_items = 0;
+
+bool sort_config_valid(int block, int it) {
+    switch (block) {
+        case 1024: return it == 1 || it == 2 || it == 4 || it == 8;
+        case 512: return it == 4 || it == 8;
+        case 256: return it == 4 || it == 8 || it == 16;
+        default: return false;
+    }
+}
 
 int sort_passes(int key_bits) { return key_bits <= 0 ? 0 : (key_bits + 7) / 8; }
 
-static inline int ntiles_for(int n, int it) { return (n + kTileThreads * it - 1) / (kTileThreads * it); }
+// tiles of `tile` keys
+static inline int ntiles_for(int n, int tile) { return (n + tile - 1) / tile; }
+constexpr int kMinTile = 1024;   // the smallest tile: sizes the status arrays
 
 // Status word of the look-back: flag in bits 31-30, count in bits 29-0.
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagPrefix = 2u << 30, kCountMask = (1u << 30) - 1u;
+constexpr int kLookback = 8;   // status words of earlier tiles loaded together in the look-back
+// Copies of each pass's digit totals: histogram block b adds into copy
+// b % kCountCopies, so a total receives ntiles / kCountCopies atomics instead
+// of ntiles (same-address atomics from every CU serialise); readers add the
+// copies up.
+constexpr int kCountCopies = 16;
 constexpr int kMaxSortKeys = (1 << 30) - 1;
 
 struct SortTemp {
-    uint32_t* gcount;   // [kMaxPasses][kMaxRadix], zero between sorts (offset 0: independent of n)
+    uint32_t* gcount;   // [kMaxPasses][kCountCopies][kMaxRadix], zero between sorts (offset 0: independent of n)
     uint32_t* ctr;      // [kMaxPasses] tile counters
     uint32_t* status;   // [2][status_words]
     long status_words;  // tiles x kMaxRadix for the smallest tile
@@ -66,10 +83,10 @@ static SortTemp carve(void* temp, int n) {
     char* p = (char*)temp;
     SortTemp t;
     t.gcount = (uint32_t*)p;
-    size_t off = (size_t)kMaxPasses * kMaxRadix * sizeof(uint32_t);
+    size_t off = (size_t)kMaxPasses * kCountCopies * kMaxRadix * sizeof(uint32_t);
     t.ctr = (uint32_t*)(p + off);
     off += 256;
-    t.status_words = (long)ntiles_for(n > 0 ? n : 1, 1) * kMaxRadix;
+    t.status_words = (long)ntiles_for(n > 0 ? n : 1, kMinTile) * kMaxRadix;
     t.status = (uint32_t*)(p + off);
     off += 2 * (size_t)t.status_words * sizeof(uint32_t);
     off = (off + 255) & ~(size_t)255;
@@ -81,8 +98,8 @@ static SortTemp carve(void* temp, int n) {
 size_t sort_temp_bytes(int n, int key_bits) {
     (void)key_bits;
     const long nn = n > 0 ? n : 1;
-    size_t b = (size_t)kMaxPasses * kMaxRadix * sizeof(uint32_t) + 256;
-    b += 2 * (size_t)ntiles_for((int)nn, 1) * kMaxRadix * sizeof(uint32_t);
+    size_t b = (size_t)kMaxPasses * kCountCopies * kMaxRadix * sizeof(uint32_t) + 256;
+    b += 2 * (size_t)ntiles_for((int)nn, kMinTile) * kMaxRadix * sizeof(uint32_t);
     b = (b + 255) & ~(size_t)255;
     b += 2 * (size_t)nn * sizeof(uint32_t);   // alternate keys / values
     return b;
@@ -119,37 +136,39 @@ struct PassPlan {
 // Digit histograms of every pass in one read of the keys; clears the tile
 // counters and the status array of even passes (status0).
 template <int IT>
-__global__ __launch_bounds__(kTileThreads) void k_radix_hist_all(const uint32_t* __restrict__ kin, int n, PassPlan plan,
+__global__ __launch_bounds__(kHistThreads) void k_radix_hist_all(const uint32_t* __restrict__ kin, int n, PassPlan plan,
                                                                  uint32_t* __restrict__ gcount,
                                                                  uint32_t* __restrict__ ctr,
                                                                  uint32_t* __restrict__ status0, long status_words) {
     __shared__ uint32_t h[kMaxPasses][kMaxRadix];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
-    for (int e = threadIdx.x; e < kMaxPasses * kMaxRadix; e += kTileThreads) (&h[0][0])[e] = 0;
+    for (int e = threadIdx.x; e < kMaxPasses * kMaxRadix; e += kHistThreads) (&h[0][0])[e] = 0;
     if (blockIdx.x == 0 && threadIdx.x < kMaxPasses) ctr[threadIdx.x] = 0;
-    for (long e = (long)blockIdx.x * kTileThreads + threadIdx.x; e < status_words; e += (long)gridDim.x * kTileThreads)
+    for (long e = (long)blockIdx.x * kHistThreads + threadIdx.x; e < status_words; e += (long)gridDim.x * kHistThreads)
         status0[e] = 0;
     __syncthreads();
-    const long base = (long)blockIdx.x * (kTileThreads * IT);
+    const long base = (long)blockIdx.x * (kHistThreads * IT);
     uint32_t kk[IT];
 #pragma unroll
-    for (int k = 0; k < IT; ++k) kk[k] = kin[min(base + k * kTileThreads + threadIdx.x, (long)n - 1)];
+    for (int k = 0; k < IT; ++k) kk[k] = kin[min(base + k * kHistThreads + threadIdx.x, (long)n - 1)];
     for (int p = 0; p < plan.npasses; ++p) {
         const int shift = plan.shift[p], bits = plan.bits[p];
         const uint32_t mask = (1u << bits) - 1u;
 #pragma unroll
         for (int k = 0; k < IT; ++k) {
-            const bool valid = base + k * kTileThreads + threadIdx.x < n;
+            const bool valid = base + k * kHistThreads + threadIdx.x < n;
             const uint32_t d = valid ? (kk[k] >> shift) & mask : 0u;
             const uint64_t peers = match_digit(d, bits, valid);
             if (valid && (peers & lt) == 0ull) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
         }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < plan.npasses * kMaxRadix; e += kTileThreads) {
+    const int copy = (int)(blockIdx.x % kCountCopies);
+    for (int e = threadIdx.x; e < plan.npasses * kMaxRadix; e += kHistThreads) {
         const uint32_t c = (&h[0][0])[e];
-        if (c) atomicAdd(&gcount[e], c);
+        const int p = e / kMaxRadix, d = e % kMaxRadix;
+        if (c) atomicAdd(&gcount[((size_t)p * kCountCopies + copy) * kMaxRadix + d], c);
     }
 }
 
@@ -167,8 +186,8 @@ __device__ __forceinline__ uint32_t digit_excl_scan(uint32_t v, uint32_t* wsum /
     return before + inc - v;
 }
 
-template <int IT>
-__global__ __launch_bounds__(kTileThreads) void k_onesweep(const uint32_t* __restrict__ kin,
+template <int BLOCK, int IT>
+__global__ __launch_bounds__(BLOCK) void k_onesweep(const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            int n, int shift, int bits,
@@ -176,7 +195,8 @@ __global__ __launch_bounds__(kTileThreads) void k_onesweep(const uint32_t* __res
                                                            uint32_t* __restrict__ ctr_p, uint32_t* status,
                                                            uint32_t* __restrict__ status_clear, long status_words,
                                                            int ntiles) {
-    constexpr int kTile = kTileThreads * IT;
+    constexpr int kTile = BLOCK * IT;
+    constexpr int kTileWaves = BLOCK / 64;
     constexpr int kPerWave = 64 * IT;
     __shared__ uint32_t cnt[kTileWaves][kMaxRadix];   // per-wave running counts, then wave prefixes
     __shared__ uint32_t loc[kMaxRadix];                // tile-local digit offsets
@@ -189,9 +209,9 @@ __global__ __launch_bounds__(kTileThreads) void k_onesweep(const uint32_t* __res
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint64_t lt = (1ull << lane) - 1ull;
     if (tid == 0) s_tile = (int)atomicAdd(ctr_p, 1u);
-    for (int e = tid; e < kTileWaves * kMaxRadix; e += kTileThreads) (&cnt[0][0])[e] = 0;
+    for (int e = tid; e < kTileWaves * kMaxRadix; e += BLOCK) (&cnt[0][0])[e] = 0;
     // the status array of the next pass (the pass before this one is finished)
-    for (long e = (long)blockIdx.x * kTileThreads + tid; e < status_words; e += (long)gridDim.x * kTileThreads)
+    for (long e = (long)blockIdx.x * BLOCK + tid; e < status_words; e += (long)gridDim.x * BLOCK)
         status_clear[e] = 0;
     __syncthreads();
     const int t = s_tile;
@@ -204,9 +224,15 @@ __global__ __launch_bounds__(kTileThreads) void k_onesweep(const uint32_t* __res
         key[k] = kin[e];
         val[k] = vin[e];
     }
-    // this pass's digit totals, staged in LDS before the barrier below, so
-    // every tile has read the row before it publishes (the last tile clears it)
-    if (tid < R) gofs[tid] = gcount_p[tid];
+    // this pass's digit totals (the sum of the copies), staged in LDS before
+    // the barrier below, so every tile has read them before it publishes
+    // (the last tile clears them)
+    if (tid < R) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int q = 0; q < kCountCopies; ++q) c += gcount_p[q * kMaxRadix + tid];
+        gofs[tid] = c;
+    }
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
         const bool valid = base + k * 64 + lane < n;
@@ -231,14 +257,33 @@ __global__ __launch_bounds__(kTileThreads) void k_onesweep(const uint32_t* __res
     const uint32_t gstart = digit_excl_scan(gc, wsum);
     const uint32_t lstart = digit_excl_scan(total, wsum);
     if (tid < R) {   // look back over the earlier tiles for this digit
+        // kLookback status words in flight at a time: when every tile reaches
+        // its look-back at once (one tile per CU), the prefix travels down the
+        // chain kLookback tiles per load latency instead of one
         uint32_t excl = 0;
-        for (int j = t - 1; j >= 0; --j) {
-            const uint32_t* st = status + (long)j * kMaxRadix + tid;
-            uint32_t v;
-            while (((v = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & ~kCountMask) == 0u)
-                __builtin_amdgcn_s_sleep(1);
-            excl += v & kCountMask;
-            if (v & kFlagPrefix) break;
+        int j = t - 1;
+        while (j >= 0) {
+            uint32_t v[kLookback];
+#pragma unroll
+            for (int k = 0; k < kLookback; ++k)
+                v[k] = j - k >= 0 ? __hip_atomic_load(status + (long)(j - k) * kMaxRadix + tid, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : kFlagPrefix;   // before tile 0 (tile 0 is always an inclusive prefix)
+            int used = kLookback;
+            bool done = false;
+#pragma unroll
+            for (int k = 0; k < kLookback; ++k) {
+                if (done || used < kLookback) continue;
+                if ((v[k] & ~kCountMask) == 0u) {   // tile j - k has not published yet: reload from it
+                    used = k;
+                    continue;
+                }
+                excl += v[k] & kCountMask;
+                if (v[k] & kFlagPrefix) done = true;
+            }
+            if (done) break;
+            if (used < kLookback) __builtin_amdgcn_s_sleep(1);
+            j -= used;
         }
         if (t > 0)
             __hip_atomic_store(status + (long)t * kMaxRadix + tid, kFlagPrefix | (excl + total), __ATOMIC_RELAXED,
@@ -247,7 +292,8 @@ __global__ __launch_bounds__(kTileThreads) void k_onesweep(const uint32_t* __res
         gofs[tid] = gstart + excl;
     }
     __syncthreads();
-    if (t == ntiles - 1 && tid < R) gcount_p[tid] = 0;   // every tile has read it (they all published)
+    if (t == ntiles - 1 && tid < R)   // every tile has read them (they all published)
+        for (int q = 0; q < kCountCopies; ++q) gcount_p[q * kMaxRadix + tid] = 0;
 #pragma unroll
     for (int k = 0; k < IT; ++k) {   // the keys in digit order, in LDS
         if (base + k * 64 + lane < n) {
@@ -261,7 +307,7 @@ __global__ __launch_bounds__(kTileThreads) void k_onesweep(const uint32_t* __res
     const int tn = (int)min((long)kTile, (long)n - tbase);
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
-        const int q = k * kTileThreads + tid;
+        const int q = k * BLOCK + tid;
         if (q < tn) {
             const uint32_t kq = sk[q];
             const uint32_t d = (kq >> shift) & mask;
@@ -297,7 +343,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st) {
 hipError_t sort_clear_temp(void* temp, int n, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     SortTemp t = carve(temp, n);
-    return launch_fill_u32(t.gcount, 0u, (long)kMaxPasses * kMaxRadix, st);
+    return launch_fill_u32(t.gcount, 0u, (long)kMaxPasses * kCountCopies * kMaxRadix, st);
 }
 
 // Sorts (keys, vals) by the bits [shift[p], shift[p] + width[p]) of each pass
@@ -320,16 +366,18 @@ hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int np
         plan.shift[p] = shifts[p];
         plan.bits[p] = widths[p];
     }
-    const int it = g_sort_items == 1 || g_sort_items == 2 || g_sort_items == 4 || g_sort_items == 8
-                       ? g_sort_items : (n >= (1 << 19) ? 4 : 2);
-    const int ntiles = ntiles_for(n, it);
+    // one-sweep tile: `block` threads x `it` keys (g_sort_items = block * 100 + it, or 0 = by size)
+    int block = 1024, it = n >= (1 << 19) ? 4 : 2;
+    if (g_sort_items > 0) {
+        block = g_sort_items / 100;
+        it = g_sort_items % 100;
+    }
+    if (!sort_config_valid(block, it)) return hipErrorInvalidValue;
+    const int ntiles = ntiles_for(n, block * it);
+    const int hist_blocks = ntiles_for(n, kHistThreads * 4);
     uint32_t* st0 = t.status;
     uint32_t* st1 = t.status + t.status_words;
-#define RTPS_HIST(I) \
-    k_radix_hist_all<I><<<ntiles, kTileThreads, 0, st>>>(keys, n, plan, t.gcount, t.ctr, st0, t.status_words)
-    if (it == 1) { RTPS_HIST(1); } else if (it == 2) { RTPS_HIST(2); } else if (it == 4) { RTPS_HIST(4); }
-    else { RTPS_HIST(8); }
-#undef RTPS_HIST
+    k_radix_hist_all<4><<<hist_blocks, kHistThreads, 0, st>>>(keys, n, plan, t.gcount, t.ctr, st0, t.status_words);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     uint32_t *ki = keys, *vi = vals, *ko = t.kalt, *vo = t.valt;
@@ -337,11 +385,18 @@ hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int np
         uint32_t* cur = (p & 1) ? st1 : st0;
         uint32_t* nxt = (p & 1) ? st0 : st1;
         const long clear = p + 1 < npasses ? t.status_words : 0;
-#define RTPS_SWEEP(I)                                                                                               \
-        k_onesweep<I><<<ntiles, kTileThreads, 0, st>>>(ki, vi, ko, vo, n, plan.shift[p], plan.bits[p],              \
-                                                       t.gcount + p * kMaxRadix, t.ctr + p, cur, nxt, clear, ntiles)
-        if (it == 1) { RTPS_SWEEP(1); } else if (it == 2) { RTPS_SWEEP(2); } else if (it == 4) { RTPS_SWEEP(4); }
-        else { RTPS_SWEEP(8); }
+#define RTPS_SWEEP(B, I)                                                                                            \
+        k_onesweep<B, I><<<ntiles, B, 0, st>>>(ki, vi, ko, vo, n, plan.shift[p], plan.bits[p],                      \
+                                               t.gcount + (size_t)p * kCountCopies * kMaxRadix, t.ctr + p, cur, nxt, \
+                                               clear, ntiles)
+        if (block == 1024) {
+            if (it == 1) { RTPS_SWEEP(1024, 1); } else if (it == 2) { RTPS_SWEEP(1024, 2); }
+            else if (it == 4) { RTPS_SWEEP(1024, 4); } else { RTPS_SWEEP(1024, 8); }
+        } else if (block == 512) {
+            if (it == 4) { RTPS_SWEEP(512, 4); } else { RTPS_SWEEP(512, 8); }
+        } else {
+            if (it == 4) { RTPS_SWEEP(256, 4); } else if (it == 8) { RTPS_SWEEP(256, 8); } else { RTPS_SWEEP(256, 16); }
+        }
 #undef RTPS_SWEEP
         e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -141,7 +141,8 @@ hipError_t launch_rigid_force(int num, const int* d_num, const uint32_t* fstart,
 std::vector<rtps_float4> hose_spray(rtps_hose* h);
 
 // ---- radix sort (radix_sort.hip) ----
-extern int g_sort_items;   // 256 x this keys per sort tile: 4, 8 or 16, 0 = by size (process-wide)
+extern int g_sort_items;   // one-sweep tile: threads * 100 + keys per thread, 0 = by size (process-wide)
+bool sort_config_valid(int block, int keys_per_thread);
 int sort_passes(int key_bits);
 size_t sort_temp_bytes(int n, int key_bits);
 hipError_t sort_clear_temp(void* temp, int n, hipStream_t st);

@@ -1283,8 +1283,9 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
             s->opt_remap = value;
             break;
         case RTPS_OPT_SORT_TILE:
-            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
-                return set_error(RTPS_ERR_ARG, "sort keys per thread must be 0 (by size), 1, 2, 4 or 8");
+            if (value != 0 && !sort_config_valid(value / 100, value % 100))
+                return set_error(RTPS_ERR_ARG, "sort tile must be 0 (by size) or threads * 100 + keys per thread: "
+                                               "102401, 102402, 102404, 102408, 51204, 51208, 25604, 25608, 25616");
             g_sort_items = value;
             break;
         case RTPS_OPT_SURFACE_FRAME:

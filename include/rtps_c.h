@@ -276,9 +276,9 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        rtps_sph_autotune), 2 = over all five loops (2 and 3
  *                        win while the neighbourhoods are sparse, lose once
  *                        they fill), 0 = fixed defaults
- *   RTPS_OPT_SORT_TILE   radix-sort keys per thread of a 1024-thread tile,
- *                        process-wide: 1, 2, 4 or 8, or 0 (default) = 4 from
- *                        2^19 keys up, else 2
+ *   RTPS_OPT_SORT_TILE   radix-sort tile, process-wide: threads * 100 + keys
+ *                        per thread (1024 x 1, 2, 4, 8; 512 x 4, 8; 256 x 4,
+ *                        8, 16, e.g. 102404), or 0 (default) = by size
  *   RTPS_OPT_DENSITY_STAGE  1 = the density kernel copies each row's candidates
  *                        once per wave into LDS (default), 0 = every lane reads
  *                        its candidates from global memory; process-wide, the

@@ -355,12 +355,11 @@ def test_fixed_launch_options(cuda):
 
 
 def test_sort_tile_sizes_give_the_same_step(cuda):
-    """Radix-sort tiles of 1024, 2048, 4096 and 8192 keys (RTPS_OPT_SORT_TILE
-    = 1, 2, 4, 8 keys per thread) sort identically (stable), so the steps are
-    bit-identical."""
+    """Every radix-sort tile (RTPS_OPT_SORT_TILE = threads * 100 + keys per
+    thread) sorts identically (stable), so the steps are bit-identical."""
     sc = scenes.dam_custom(32, 32, 16, seed=82)
     out = {}
-    for items in (8, 4, 2, 1):
+    for items in (102408, 102404, 102402, 102401, 51204, 51208, 25604, 25608, 25616):
         s = _system(sc)
         s.setOption(8, items)
         s.step(3)
@@ -369,8 +368,8 @@ def test_sort_tile_sizes_give_the_same_step(cuda):
     s = _system(sc)
     s.setOption(8, 0)           # process-wide: restore the default (by size)
     s.close()
-    for items in (8, 4, 1):
-        for a, b in zip(out[items], out[2]):
+    for items in out:
+        for a, b in zip(out[items], out[102402]):
             np.testing.assert_array_equal(a, b)
 
 
