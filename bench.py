@@ -53,12 +53,15 @@ def measured_traffic(kernel: str = "k_force", grid: int = 1048576, block: int = 
 class _Marker:
     """roctx range around the timed region (recorded by rocprofv3
     --marker-trace, so a kernel trace can be cut to the timed steps); a no-op
-    where libroctx64 cannot be loaded."""
+    where no roctx library can be loaded.  rocprofv3 records the ranges of
+    the rocprofiler-sdk roctx library; the older libroctx64 is a fallback for
+    tracers that hook that one."""
 
     def __init__(self):
         import ctypes
         self.lib = None
-        for name in ("libroctx64.so.4", "libroctx64.so", "/opt/rocm/lib/libroctx64.so"):
+        for name in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1",
+                     "libroctx64.so.4", "/opt/rocm/lib/libroctx64.so.4"):
             try:
                 self.lib = ctypes.CDLL(name)
                 self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
@@ -318,25 +321,34 @@ def triangles_leg(s, ntri: int, steps: int = 50):
             "extra_ms": with_tri - base}
 
 
-def long_run_leg(s, steps: int, first: int):
+def long_run_leg(s, steps: int, first: int, timer_steps: int = 100):
     """The timed system stepped `steps` more steps (graph replays, stage
-    timers on): the dam's representative state, after the fluid has left the
-    lattice (more neighbours per particle than in the first steps)."""
+    timers off): the dam's representative state, after the fluid has left the
+    lattice (more neighbours per particle than in the first steps).  The
+    per-stage times come from `timer_steps` further steps with the stage
+    timers on."""
     import torch
-    s.resetTimers()
+    import enjaparticles_amd._native as N
     s.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     s.step(steps)
     s.synchronize()
     dt = time.perf_counter() - t0
-    stages = {k: round(v[0], 5) for k, v in s.stageTimes().items() if v[1] > 0}
     K, M, Kt = s.countPairs()
     n = s.getNum()
-    return {"steps": steps, "first_step": first, "last_step": first + steps - 1,
+    s.setOption(N.OPT_TIMERS, 1)
+    s.resetTimers()
+    s.step(timer_steps)
+    s.synchronize()
+    stages = {k: round(v[0], 5) for k, v in s.stageTimes().items() if v[1] > 0}
+    s.setOption(N.OPT_TIMERS, 0)
+    last = first + steps - 1
+    return {"steps": steps, "first_step": first, "last_step": last,
             "ms_per_step": 1e3 * dt / steps, "value": n * steps / dt, "unit": "particle-updates/s",
-            "stages_ms": stages, "pairs_per_particle_at_end": {"candidates_27cell": K / n, "accepted_incl_self": M / n,
-                                                               "candidates_tested": Kt / n}}
+            "stages_ms": stages, "stages_steps": [last + 1, last + timer_steps],
+            "pairs_per_particle_at_end": {"candidates_27cell": K / n, "accepted_incl_self": M / n,
+                                          "candidates_tested": Kt / n}}
 
 
 def surface_leg(s, res: int, iso: float = 0.5, reps: int = 10):
