@@ -69,6 +69,7 @@ SIGNATURES = {
     "rtps_sort_key_bits": (_i, [_i]),
     "rtps_sort_temp_bytes": (_sz, [_i, _i]),
     "rtps_sort_pairs": (_i, [_i, _i, P, P, P, _sz, P]),
+    "rtps_sort_pairs_noalloc": (_i, [_i, _i, P, P, P]),
     "rtps_sort_segments": (_i, [_i, _i, _i, P, P, P, P, P, _sz, P]),
     "rtps_sph_set_rigid_particles": (_i, [P, P, P, P, _i, _f, _f]),
     "rtps_sph_set_static_boundary": (_i, [P, P, _i, _f, _f]),

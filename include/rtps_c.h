@@ -91,6 +91,16 @@ size_t rtps_sort_temp_bytes(int n, int key_bits);
 int rtps_sort_pairs(int n, int key_bits, uint32_t* keys /*dev*/, uint32_t* vals /*dev*/, void* temp /*dev*/,
                     size_t temp_bytes, void* stream);
 
+/* rtps_sort_pairs without the temp argument and without any allocation: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  The sort's scratch is a fixed 11 MiB device array inside the
+ * library (one per GPU, held until the process ends), so n is limited to
+ * 2^20 keys; larger n returns RTPS_ERR_ARG.  The array keeps the last call's
+ * contents and the sort clears what must start at zero; calls that overlap on
+ * one GPU (two streams or threads at once) share the array and race. */
+int rtps_sort_pairs_noalloc(int n, int key_bits, uint32_t* keys /*dev*/, uint32_t* vals /*dev*/, void* stream);
+
 /* a3. Bitonic<T>::Sort(batch, arrayLength, dir, dstkey, dstval, srckey,
  * srcval) (rtpslib/system/common/BitonicSort.h:26-28, BitonicSort.cpp:62-183)
  * with its argument meaning: `batch` independent arrays of array_length
