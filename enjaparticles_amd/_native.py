@@ -172,7 +172,9 @@ def lib():
             import torch  # noqa: F401
         except ImportError:
             pass
-        L = C.CDLL(LIB_PATH)
+        # RTLD_NOW: every symbol is bound at load, so a library with an
+        # unresolved reference fails here instead of at its first use.
+        L = C.CDLL(LIB_PATH, mode=os.RTLD_NOW)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
             fn.restype = res

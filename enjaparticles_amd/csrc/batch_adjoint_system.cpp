@@ -34,7 +34,7 @@ struct BRecord {   // one forward step's sorted state of the whole batch
 
 struct rtps_badj {
     BatchLayout L;
-    int S = 0, T = 0, C = 0, ck = 20, steps = 0, n_items = 0, persistent_grid = 1;
+    int S = 0, T = 0, C = 0, ck = 20, steps = 0, n_items = 0, persistent_grid = 1, force_grid = 1;
     std::vector<void*> allocs;
     SceneDev* scenes = nullptr;
     int *d_num = nullptr, *queues = nullptr, *item_first = nullptr, *left_grid = nullptr;
@@ -85,12 +85,12 @@ int forward_step(rtps_badj* a, int in, int out, BRecord& r, hipStream_t st) {
     RTPS_TRY_HIP(hipMemcpyAsync(r.idx, is, (size_t)T * 4, hipMemcpyDeviceToDevice, st));
     RTPS_TRY_HIP(launch_cellindices_b(T, r.keys, a->scenes, r.start, r.end, st));
     RTPS_TRY_HIP(launch_permute(T, nullptr, r.idx, a->pos[in], a->vel[in], a->ve[in], r.pos_s, r.vel_s, r.ve_s, st));
-    RTPS_TRY_HIP(launch_pairs_b(kPairsDensity, PairsCfg{}, a->persistent_grid, a->items, a->n_items, a->queues, a->scenes,
-                                a->d_num, r.pos_s, r.start, r.end, r.keys, r.ve_s, r.vel_s, r.idx, nullptr, nullptr,
-                                nullptr, st));
-    RTPS_TRY_HIP(launch_pairs_b(kPairsForceStore, PairsCfg{}, a->persistent_grid, a->items, a->n_items, a->queues + kXcdQueues,
-                                a->scenes, a->d_num, r.pos_s, r.start, r.end, r.keys, r.ve_s, r.vel_s, r.idx, nullptr,
-                                nullptr, nullptr, st, r.force_s, a->xsph));
+    RTPS_TRY_HIP(launch_pairs_b(kPairsDensity, pairs_default_cfg(kPairsDensity), a->persistent_grid, a->items,
+                                a->n_items, a->queues, a->scenes, a->d_num, r.pos_s, r.start, r.end, r.keys, r.ve_s,
+                                r.vel_s, r.idx, nullptr, nullptr, nullptr, st));
+    RTPS_TRY_HIP(launch_pairs_b(kPairsForceStore, pairs_default_cfg(kPairsForceStore), a->force_grid, a->items,
+                                a->n_items, a->queues + kXcdQueues, a->scenes, a->d_num, r.pos_s, r.start, r.end,
+                                r.keys, r.ve_s, r.vel_s, r.idx, nullptr, nullptr, nullptr, st, r.force_s, a->xsph));
     RTPS_TRY_HIP(launch_badj_integrate(a->n_items, a->items, a->scenes, r.pos_s, r.vel_s, r.force_s, a->xsph, r.idx,
                                        r.start, a->pos[out], a->vel[out], a->ve[out], a->left_grid, st));
     return RTPS_OK;
@@ -130,7 +130,9 @@ int rtps_badj_create(int num_scenes, const int* n_settings, const char* const* k
             if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
                 cus = prop.multiProcessorCount;
         }
-        a->persistent_grid = std::max(1, std::min(a->n_items, cus * pairs_blocks_per_cu(PairsCfg{})));
+        a->persistent_grid =
+            std::max(1, std::min(a->n_items, cus * pairs_blocks_per_cu(pairs_default_cfg(kPairsDensity))));
+        a->force_grid = std::max(1, std::min(a->n_items, cus * pairs_blocks_per_cu(pairs_default_cfg(kPairsForceStore))));
         const size_t N = (size_t)a->T, C = (size_t)a->C;
         RTPS_TRY_HIP(a->alloc(&a->scenes, (size_t)num_scenes));
         RTPS_TRY_HIP(a->alloc(&a->d_num, (size_t)num_scenes));

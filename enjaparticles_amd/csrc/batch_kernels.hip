@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "batch_types.h"
 #include "rtps_internal.h"
 #include "sph_device.h"
@@ -342,9 +344,12 @@ hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* sce
 
 int pairs_blocks_per_cu(const PairsCfg& c) { return c.waves * 256 / c.block; }
 
-bool pairs_cfg_valid(const PairsCfg& c) {
-    return (c.block == 128 || c.block == 256) && c.waves >= 4 && c.waves <= 6;
+bool pairs_cfg_valid(int mode, const PairsCfg& c) {
+    if (c.block != 128 && c.block != 256) return false;
+    return mode == kPairsDensity ? (c.waves >= 4 && c.waves <= 6) : (c.waves == 3 || c.waves == 4);
 }
+
+PairsCfg pairs_default_cfg(int mode) { return mode == kPairsDensity ? PairsCfg{256, 5} : PairsCfg{256, 4}; }
 
 hipError_t launch_pairs_b(int mode, const PairsCfg& cfg, int grid, const BatchItem* items, int n_items, int* queue,
                           const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
@@ -353,19 +358,34 @@ hipError_t launch_pairs_b(int mode, const PairsCfg& cfg, int grid, const BatchIt
                           float4* force_out, float4* xsph_out, const uint32_t* sub) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (mode == kPairsForceStore && (!force_out || !xsph_out)) return hipErrorInvalidValue;
-    if (!pairs_cfg_valid(cfg)) return hipErrorInvalidValue;
-#define RTPS_PAIRS(M, B, W)                                                                                             k_pairs_b<M, B, W><<<grid, B, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start, end, keys, ve,                                                 vel_s, idx, pos_u, vel_u, veleval_u, force_out, xsph_out, sub)
-#define RTPS_PAIRS_W(M, B)                                                                                              if (cfg.waves == 4) { RTPS_PAIRS(M, B, 4); } else if (cfg.waves == 5) { RTPS_PAIRS(M, B, 5); }                      else { RTPS_PAIRS(M, B, 6); }
-#define RTPS_PAIRS_BW(M)                                                                                                if (cfg.block == 256) { RTPS_PAIRS_W(M, 256) } else { RTPS_PAIRS_W(M, 128) }
+    if (!pairs_cfg_valid(mode, cfg)) return hipErrorInvalidValue;
+    // Instantiated shapes: density at 4, 5 and 6 waves, the force modes at 3
+    // and 4 (pairs_cfg_valid has admitted only these).
+    auto go = [&](auto m, auto b, auto w) {
+        k_pairs_b<decltype(m)::value, decltype(b)::value, decltype(w)::value><<<grid, decltype(b)::value, 0, st>>>(
+            items, n_items, queue, scenes, d_num, pos_s, start, end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
+            force_out, xsph_out, sub);
+    };
+    auto by_waves = [&](auto m, auto b) {
+        if constexpr (decltype(m)::value == kPairsDensity) {
+            if (cfg.waves == 4) go(m, b, std::integral_constant<int, 4>{});
+            else if (cfg.waves == 5) go(m, b, std::integral_constant<int, 5>{});
+            else go(m, b, std::integral_constant<int, 6>{});
+        } else {
+            if (cfg.waves == 3) go(m, b, std::integral_constant<int, 3>{});
+            else go(m, b, std::integral_constant<int, 4>{});
+        }
+    };
+    auto by_block = [&](auto m) {
+        if (cfg.block == 256) by_waves(m, std::integral_constant<int, 256>{});
+        else by_waves(m, std::integral_constant<int, 128>{});
+    };
     switch (mode) {
-        case kPairsDensity: RTPS_PAIRS_BW(kPairsDensity) break;
-        case kPairsForceStep: RTPS_PAIRS_BW(kPairsForceStep) break;
-        case kPairsForceStore: RTPS_PAIRS_BW(kPairsForceStore) break;
+        case kPairsDensity: by_block(std::integral_constant<int, kPairsDensity>{}); break;
+        case kPairsForceStep: by_block(std::integral_constant<int, kPairsForceStep>{}); break;
+        case kPairsForceStore: by_block(std::integral_constant<int, kPairsForceStore>{}); break;
         default: return hipErrorInvalidValue;
     }
-#undef RTPS_PAIRS_BW
-#undef RTPS_PAIRS_W
-#undef RTPS_PAIRS
     return hipGetLastError();
 }
 

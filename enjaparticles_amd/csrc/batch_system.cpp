@@ -45,7 +45,7 @@ struct rtps_batch {
     BatchItem* items128 = nullptr;
     int n_items = 0, n_items128 = 0, cus = 256;
     // launch configurations of the density and force pair kernels (the autotuner's choice)
-    PairsCfg cfg_density, cfg_force;
+    PairsCfg cfg_density = pairs_default_cfg(kPairsDensity), cfg_force = pairs_default_cfg(kPairsForceStep);
     int opt_autotune = 1;
     bool tuned = false;
     int* queues = nullptr;
@@ -182,7 +182,7 @@ void btune_load_locked() {
         std::string k;
         PairsCfg d, fo;
         if (is >> k >> d.block >> d.waves >> fo.block >> fo.waves && k.rfind("batch/", 0) == 0 &&
-            pairs_cfg_valid(d) && pairs_cfg_valid(fo))
+            pairs_cfg_valid(kPairsDensity, d) && pairs_cfg_valid(kPairsForceStep, fo))
             g_btune[k] = {d, fo};
     }
 }
@@ -232,7 +232,8 @@ int autotune_step_b(rtps_batch* b, bool measure) {
         (void)hipEventDestroy(e0);
         return hip_error(err, "hipEventCreate");
     }
-    const PairsCfg cands[] = {{256, 5}, {256, 4}, {256, 6}, {128, 5}, {128, 4}, {128, 6}};
+    const std::vector<PairsCfg> cands[2] = {{{256, 5}, {256, 4}, {256, 6}, {128, 5}, {128, 4}, {128, 6}},
+                                            {{256, 4}, {256, 3}, {128, 4}, {128, 3}}};
     constexpr int kReps = 3;
     constexpr float kMargin = 0.97f;
     const char* verbose_env = std::getenv("RTPS_TUNE_VERBOSE");
@@ -257,7 +258,7 @@ int autotune_step_b(rtps_batch* b, bool measure) {
     bool failed = false;
     for (int which = 0; which < 2 && !failed; ++which) {
         float best_ms = 1e30f;
-        for (const PairsCfg& c : cands) {
+        for (const PairsCfg& c : cands[which]) {
             const float ms = time_it(which, c);
             if (verbose)
                 std::fprintf(stderr, "rtps batch autotune: %s block %d waves %d: %.4f ms\n",
@@ -610,8 +611,9 @@ int rtps_batch_set_option(rtps_batch* b, int option, int value) {
         case RTPS_OPT_BLOCK_DENSITY:
         case RTPS_OPT_BLOCK_FORCE: {   // value = block * 10 + waves, e.g. 2565 (256 threads, 5 waves per SIMD)
             const PairsCfg c{value / 10, value % 10};
-            if (!pairs_cfg_valid(c))
-                return set_error(RTPS_ERR_ARG, "pair kernel launch: block*10 + waves with block 128 or 256, waves 4..6");
+            if (!pairs_cfg_valid(option == RTPS_OPT_BLOCK_DENSITY ? kPairsDensity : kPairsForceStep, c))
+                return set_error(RTPS_ERR_ARG, "pair kernel launch: block*10 + waves with block 128 or 256, waves 4..6 "
+                                               "(density) or 3..4 (force)");
             (option == RTPS_OPT_BLOCK_DENSITY ? b->cfg_density : b->cfg_force) = c;
             b->tuned = true;   // fixed by the caller
             break;

@@ -41,15 +41,19 @@ hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* sce
 constexpr int kPairsDensity = 0, kPairsForceStep = 1, kPairsForceStore = 2;
 // Launch configuration of a persistent pair kernel: `block` threads per block
 // (= slots per work item: 128 or 256) and the register budget of `waves`
-// waves per SIMD (4, 5 or 6: at most 128, 96 or 80 VGPRs); the grid is
-// CUs x waves x 256 / block blocks, all resident.  The batched system's
-// autotuner picks one per kernel; the default is 256 threads at 5 waves.
+// waves per SIMD (3, 4, 5 or 6: at most 168, 128, 96 or 80 VGPRs); the grid
+// is CUs x waves x 256 / block blocks.  The density kernel is built for 4, 5
+// and 6 waves; the force kernels for 3 and 4 only, since their live values
+// exceed the 96 registers of 5 waves and would spill to scratch memory.  The
+// batched system's autotuner picks one per kernel; the defaults are 256
+// threads at 5 waves (density) and at 4 waves (force).
 struct PairsCfg {
     int block = 256;
     int waves = 5;
 };
 int pairs_blocks_per_cu(const PairsCfg& c);
-bool pairs_cfg_valid(const PairsCfg& c);
+bool pairs_cfg_valid(int mode, const PairsCfg& c);
+PairsCfg pairs_default_cfg(int mode);
 hipError_t launch_pairs_b(int mode, const PairsCfg& cfg, int grid, const BatchItem* items, int n_items, int* queue,
                           const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
                           const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,

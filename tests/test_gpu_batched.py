@@ -140,15 +140,16 @@ def test_capacity_and_arguments(cuda):
 
 def test_pair_kernel_configurations_give_the_same_step(cuda):
     """Every launch configuration the batched autotuner chooses from (block
-    of 128 or 256 slots, 4, 5 or 6 waves per SIMD, for density and force)
-    computes the same step, bit for bit; the autotuner's own step too."""
+    of 128 or 256 slots; 4, 5 or 6 waves per SIMD for density, 3 or 4 for
+    force) computes the same step, bit for bit; the autotuner's own step
+    too.  Force at 5 waves (it would spill registers) is refused."""
     scs = scenes.batched(5, 600, 6000, seed=83)
     ref = None
     for block in (256, 128):
-        for waves in (5, 4, 6):
+        for wd, wf in ((5, 4), (4, 3), (6, 4), (5, 3)):
             b = _batch(scs)
-            b.setOption(11, block * 10 + waves)   # RTPS_OPT_BLOCK_DENSITY
-            b.setOption(12, block * 10 + waves)   # RTPS_OPT_BLOCK_FORCE
+            b.setOption(11, block * 10 + wd)   # RTPS_OPT_BLOCK_DENSITY
+            b.setOption(12, block * 10 + wf)   # RTPS_OPT_BLOCK_FORCE
             b.step(4)
             out = [(b.read(s, "pos"), b.read(s, "density")) for s in range(len(scs))]
             b.close()
@@ -156,11 +157,16 @@ def test_pair_kernel_configurations_give_the_same_step(cuda):
                 ref = out
                 continue
             for (p, d), (pr, dr) in zip(out, ref):
-                np.testing.assert_array_equal(p, pr, err_msg=f"block {block} waves {waves}")
-                np.testing.assert_array_equal(d, dr, err_msg=f"block {block} waves {waves}")
+                np.testing.assert_array_equal(p, pr, err_msg=f"block {block} waves {wd}/{wf}")
+                np.testing.assert_array_equal(d, dr, err_msg=f"block {block} waves {wd}/{wf}")
     b = _batch(scs)
+    with pytest.raises(RTPSError):
+        b.setOption(12, 2565)
+    with pytest.raises(RTPSError):
+        b.setOption(11, 2563)
     cfg = b.autotune(measure=True)
-    assert cfg["density"]["block"] in (128, 256) and cfg["force"]["waves"] in (4, 5, 6)
+    assert cfg["density"]["block"] in (128, 256) and cfg["density"]["waves"] in (4, 5, 6)
+    assert cfg["force"]["block"] in (128, 256) and cfg["force"]["waves"] in (3, 4)
     b.step(3)
     for s in range(len(scs)):
         np.testing.assert_array_equal(b.read(s, "pos"), ref[s][0])
