@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
 // once per wave into LDS as for the density kernel's STAGE (at most
 // kForceStageCap candidates; wider unions read global memory).  Same pairs,
 // same order, same reciprocals: the direct loop's bits.
-constexpr int kLoopDirect = 0, kLoopCompact = 1, kLoopMasked = 2, kLoopList = 3, kLoopStaged = 4;
+constexpr int kLoopDirect = 0, kLoopCompact = 1, kLoopMasked = 2, kLoopList = 3, kLoopStaged = 4, kLoopStagedBits = 5;
 constexpr int kForceStageCap = 192;   // 6 KB per wave: the 20 waves per CU of kStagedWavesPerSimd take 120 KB
 constexpr int kNbrCap = 32;
 
@@ -326,7 +326,7 @@ constexpr int kForceWavesPerSimd = 6;
 constexpr int kStagedWavesPerSimd = 5;
 
 template <bool SCALED, bool PACKED, bool FUSED, int LOOP>
-__global__ __launch_bounds__(kBlock, LOOP == kLoopStaged ? kStagedWavesPerSimd : kForceWavesPerSimd) void k_force(int num_host, const int* __restrict__ d_num,
+__global__ __launch_bounds__(kBlock, LOOP >= kLoopStaged ? kStagedWavesPerSimd : kForceWavesPerSimd) void k_force(int num_host, const int* __restrict__ d_num,
                                                   const float4* __restrict__ pos_s,
                                                   const float* __restrict__ dens_in,
                                                   const float4* __restrict__ ve_in,
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(kBlock, LOOP == kLoopStaged ? kStagedWavesPerSimd :
             }
         }
     }
-    if constexpr (LOOP == kLoopStaged) {
+    if constexpr (LOOP == kLoopStaged || LOOP == kLoopStagedBits) {
         const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
         const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
         extern __shared__ float4 force_rows[];   // 2 x kForceStageCap per wave: positions, then veleval|density
@@ -571,6 +571,46 @@ __global__ __launch_bounds__(kBlock, LOOP == kLoopStaged ? kStagedWavesPerSimd :
                 }
             }
         };
+        // Loop 5 on the staged copies: the lane's window is taken in chunks of
+        // up to 32 candidates; the tests of a chunk set one bit per accepted
+        // candidate, then the pair terms run once per set bit, lowest first.
+        // The wave's pair iterations are its lanes' largest accepted count
+        // instead of their longest window, and the pairs are summed in the
+        // same ascending order as walk (the same bits).
+        auto walk_bits = [&](const float4* __restrict__ P, const float4* __restrict__ V, uint32_t a, uint32_t b,
+                             uint32_t off) {
+            for (uint32_t j0 = a; j0 < b; j0 += 32u) {
+                const uint32_t n = min(b - j0, 32u);
+                const float4* __restrict__ pc = P + (j0 - off);
+                uint32_t m = 0u, k = 0u;
+                auto test = [&](float4 pj, uint32_t t) {
+                    if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                    const float r2 = rx * rx + ry * ry + rz * rz;
+                    m |= (r2 <= h2 && j0 + t != (uint32_t)i) ? (1u << t) : 0u;
+                };
+                for (; k + kGroup <= n; k += kGroup) {
+                    float4 q[kGroup];
+#pragma unroll
+                    for (int t = 0; t < kGroup; ++t) q[t] = pc[k + t];
+#pragma unroll
+                    for (int t = 0; t < kGroup; ++t) test(q[t], k + (uint32_t)t);
+                }
+                for (; k < n; ++k) test(pc[k], k);
+                while (m != 0u) {
+                    const uint32_t t = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1u;
+                    const float4 pq = pc[t];
+                    float4 pj = pq;
+                    if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                    const float r2 = rx * rx + ry * ry + rz * rz;
+                    const float4 vj = V[j0 + t - off];
+                    const float dj = PACKED ? vj.w : dens_in[j0 + t];
+                    force_pair_r(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, dj, pq.w);
+                }
+            }
+        };
         for (int dz = -1; dz <= 1; ++dz) {
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
@@ -587,7 +627,8 @@ __global__ __launch_bounds__(kBlock, LOOP == kLoopStaged ? kStagedWavesPerSimd :
                         my_v[t] = e;
                     }
                     __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
-                    walk(my_p, my_v, a, b, A, true);
+                    if constexpr (LOOP == kLoopStagedBits) walk_bits(my_p, my_v, a, b, A);
+                    else walk(my_p, my_v, a, b, A, true);
                     __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
                 } else {
                     walk(pos_s, ve_in, a, b, 0u, false);
@@ -831,7 +872,7 @@ int g_force_variant = kLoopStaged;
 static inline int nbr_block(const Win& w) { return w.block > 0 && w.block <= kBlock && w.block % 64 == 0 ? w.block : kBlock; }
 static inline int nbr_variant(const Win& w) {
     const int v = w.variant >= 0 ? w.variant : g_force_variant;
-    return v >= kLoopDirect && v <= kLoopStaged ? v : kLoopDirect;
+    return v >= kLoopDirect && v <= kLoopStagedBits ? v : kLoopDirect;
 }
 
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
@@ -917,7 +958,7 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
     const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
     const int loop = loop_of(win, m);
     const size_t lds = loop == kLoopCompact ? (size_t)kNbrCap * bs * sizeof(uint32_t)
-                       : (loop == kLoopStaged ? (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4) : 0);
+                       : (loop >= kLoopStaged ? (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4) : 0);
     GravitySources none{};
     rtps_float4 z{0, 0, 0, 0};
 #define RTPS_FORCE_L(S, P, L)                                                                                       \
@@ -927,6 +968,7 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
 #define RTPS_FORCE(S, P)                                                                                            \
     if (loop == kLoopList) RTPS_FORCE_L(S, P, kLoopList);                                                       \
     else if (loop == kLoopStaged) RTPS_FORCE_L(S, P, kLoopStaged);                                                  \
+    else if (loop == kLoopStagedBits) RTPS_FORCE_L(S, P, kLoopStagedBits);                                          \
     else if (loop == kLoopMasked) RTPS_FORCE_L(S, P, kLoopMasked);                                                  \
     else if (loop == kLoopCompact) RTPS_FORCE_L(S, P, kLoopCompact);                                                \
     else RTPS_FORCE_L(S, P, kLoopDirect)
@@ -951,7 +993,7 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
     const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
     const int loop = loop_of(win, m);
     const size_t lds = loop == kLoopCompact ? (size_t)kNbrCap * bs * sizeof(uint32_t)
-                       : (loop == kLoopStaged ? (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4) : 0);
+                       : (loop >= kLoopStaged ? (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4) : 0);
 #define RTPS_FFUSED_L(S, L)                                                                                         \
     k_force<S, true, true, L><<<nb, bs, lds, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
                                                  xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt,  \
@@ -959,6 +1001,7 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
 #define RTPS_FFUSED(S)                                                                                              \
     if (loop == kLoopList) RTPS_FFUSED_L(S, kLoopList);                                                         \
     else if (loop == kLoopStaged) RTPS_FFUSED_L(S, kLoopStaged);                                                    \
+    else if (loop == kLoopStagedBits) RTPS_FFUSED_L(S, kLoopStagedBits);                                            \
     else if (loop == kLoopMasked) RTPS_FFUSED_L(S, kLoopMasked);                                                    \
     else if (loop == kLoopCompact) RTPS_FFUSED_L(S, kLoopCompact);                                                  \
     else RTPS_FFUSED_L(S, kLoopDirect)

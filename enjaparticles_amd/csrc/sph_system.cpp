@@ -557,8 +557,8 @@ int autotune_step(rtps_sph* s, bool measure) {
     // density: plain, recording bits, recording lists (each left recorded for the force loops timed next);
     // the recorded forms (force loops 2 and 3) are candidates only with RTPS_OPT_AUTOTUNE = 2
     const bool full = s->opt_autotune == 2;
-    const int nmodes = full ? 3 : 1, nvars = full ? 5 : 3;
-    const int vars[] = {0, 4, 1, 2, 3};   // loops in the order they are tried
+    const int nmodes = full ? 3 : 1, nvars = full ? 6 : 4;
+    const int vars[] = {0, 4, 5, 1, 2, 3};   // loops in the order they are tried
     float best_d[3] = {1e30f, 1e30f, 1e30f};
     int bd[3] = {256, 256, 256};
     bool failed = false;
@@ -1271,7 +1271,7 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_FORCE_OUT: s->opt_force_out = value ? 1 : 0; break;
         case RTPS_OPT_TIMERS: s->opt_timers = value ? 1 : 0; break;
         case RTPS_OPT_FORCE_VARIANT:
-            if (value < 0 || value > 4) return set_error(RTPS_ERR_ARG, "force variant must be 0 to 4");
+            if (value < 0 || value > 5) return set_error(RTPS_ERR_ARG, "force variant must be 0 to 5");
             g_force_variant = value;
             break;
         case RTPS_OPT_WINDOW:

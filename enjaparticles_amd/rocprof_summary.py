@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import csv
+import json
 import re
 import sqlite3
 from collections import defaultdict
@@ -34,9 +35,19 @@ def short_name(name: str) -> str:
 
 
 def regions(db_path: str, name: str):
-    """(start, end) of every marker range called `name`."""
+    """(start, end) of every marker range called `name`: rocprofv3 records a
+    roctx range under the API's name with the message in its extdata."""
     db = sqlite3.connect(db_path)
-    return db.execute("select start, end from regions where name = ? order by start", (name,)).fetchall()
+    out = []
+    for start, end, rname, extdata in db.execute("select start, end, name, extdata from regions order by start"):
+        message = None
+        try:
+            message = json.loads(extdata).get("message") if extdata else None
+        except (ValueError, AttributeError):
+            pass
+        if rname == name or message == name:
+            out.append((start, end))
+    return out
 
 
 def summarize(db_path: str, top: int = 40, match: str | None = None, region: str | None = None):
