@@ -475,9 +475,9 @@ def main():
     ap.add_argument("--launch", default="", metavar="BD,BF,LOOP",
                     help="fixed neighbour launches instead of autotuning: density and force threads per block and "
                          "the force loop (profiling runs: counters perturb the autotuner's timings)")
-    ap.add_argument("--force-variant", type=int, default=4,
-                    help="process-default force loop before autotuning: 4 = direct with LDS-staged rows, "
-                         "0 = direct, 1 = compacted neighbour lists")
+    ap.add_argument("--force-variant", type=int, default=5,
+                    help="process-default force loop before autotuning: 5 = LDS-staged rows with accepted-candidate "
+                         "bits, 4 = LDS-staged rows, 0 = direct")
     ap.add_argument("--long-steps", type=int, default=1000,
                     help="steps of the long-run leg that continues the timed system (0 = skip)")
     ap.add_argument("--triangles", type=int, default=2048,

@@ -28,7 +28,7 @@ FIELD = {
  OPT_SORT_TILE, OPT_COLOR, OPT_DENSITY_STAGE, OPT_BLOCK_DENSITY, OPT_BLOCK_FORCE, OPT_SURFACE_FRAME) = range(14)
 # The library's process-wide default of OPT_FORCE_VARIANT (the force loop of
 # handles that are not autotuned).
-DEFAULT_FORCE_LOOP = 4
+DEFAULT_FORCE_LOOP = 5
 NUM_STAGES = 6
 STAGE_NAMES = ("hash", "sort", "cellindices", "permute", "density", "force")
 MAX_GRAVITY_SOURCES = 16

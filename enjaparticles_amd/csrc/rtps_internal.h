@@ -15,7 +15,6 @@ namespace rtps {
 struct Grid;
 struct GravitySources;
 struct Win;
-struct RowMasks;
 
 // ---- error reporting (capi.cpp) ----
 int set_error(int code, const std::string& msg);
@@ -33,7 +32,7 @@ int integrator_of(const RTPSSettings& st);
 
 // ---- kernels (sph_kernels.hip) ----
 extern int g_density_stage;   // 1 = density rows staged through LDS (default), 0 = read from global memory
-extern int g_force_variant;   // 0 = direct branch per candidate, 1 = compacted lists, 2 = recorded masks
+extern int g_force_variant;   // force loop: 0 = direct, 4 = rows staged in LDS, 5 = staged with accepted bits
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
                        uint32_t* idx, const Grid& g, uint32_t* start_clear, int n_clear, int sub_bits,
                        hipStream_t st);
@@ -48,18 +47,16 @@ hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const 
                           const float4* cu = nullptr, float4* cs = nullptr);
 hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
                           const rtps_sph_params& sp, const Grid& g, const Win& win, float* dens, float4* packed,
-                          hipStream_t st, const RowMasks* masks = nullptr);
+                          hipStream_t st);
 hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const float* dens, const float4* ve,
                         bool packed, const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp,
-                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st,
-                        const RowMasks* masks = nullptr);
+                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st);
 hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, const float4* ve_packed,
                               const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
                               const Win& win, float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
                               const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,
                               float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
-                              const RowMasks* masks = nullptr, const float4* color_s = nullptr,
-                              float4* color_u = nullptr);
+                              const float4* color_s = nullptr, float4* color_u = nullptr);
 hipError_t launch_post_force(int num, const int* d_num, const uint32_t* start, int nb_cells, const float4* pos_s,
                              const float4* vel_s, float4* force, const rtps_sph_params& sp,
                              const GravitySources& gsrc, int collision, const rtps_grid_params& gps, hipStream_t st);

@@ -122,30 +122,8 @@ struct Win {
     const uint32_t* sub = nullptr;
 };
 
-// Neighbour rows recorded by the density kernel for the force kernel: per
-// sorted particle i and row r (r = 3 (dz + 1) + (dy + 1)), rows[r * stride + i]
-// = (first candidate a, candidate count n) of the row's window, and
-//  * bit form: bit t of words[(r * kMaskWords + w) * stride + i] = candidate
-//    a + 32 w + t passed r <= h (self excluded).  Candidates past
-//    32 kMaskWords of a row have no bits; the force kernel tests those itself.
-//  * list form: the accepted candidates in row / ascending order as 16-bit
-//    codes (r << 12) | (j - a_r), four per 8-byte slot, slot s of particle i at
-//    codes[s * stride + i] (slot-major, so a wave's loads of slot s are
-//    contiguous); count[i] = their number, or kListCap + 1 when the list
-//    overflowed (the force kernel then runs the direct loop for i).
-constexpr int kMaskWords = 4;
 // Candidates per group in the neighbour loops (loads in flight per lane).
 constexpr int kGroup = 4;
-constexpr int kListCap = 192;          // codes per particle
-constexpr uint32_t kCodeOffBits = 12;  // offset inside the row window
-struct RowMasks {
-    uint2* rows;
-    uint32_t* words;
-    uint2* codes;
-    uint32_t* count;
-    int stride;
-    int form;   // 1 = bit form, 2 = list form
-};
 
 struct WindowCtx {
     float ux, uy, uz;        // continuous cell coordinates (p - min) * delta

@@ -116,29 +116,25 @@ __global__ __launch_bounds__(kBlock) void k_permute(int num_host, const int* __r
 // ---------------------------------------------------------------------------
 // a7: density.cl:86-126.  PACKED: the density is written to veleval_s[i].w
 // (the system layout, so the force loop loads one float4 per neighbour).
-// REC: also record every row's window and the accepted candidates for the
-// force kernel (RowMasks): kRecBits (bit form) or kRecList (list form).
-constexpr int kRecNone = 0, kRecBits = 1, kRecList = 2;
-// candidates per density group (a divisor of 32: the bit words fill in whole groups)
+// candidates per density group, their loads issued together
 constexpr int kGroupDensity = 4;
-// STAGE (REC == kRecNone): each wave copies the union of its lanes' windows
-// of a row (when at most kStageCap candidates) into its own LDS slice with
-// full-line loads, and the lanes read their candidates from there; wider
-// unions (waves straddling two rows of cells) read global memory as before.
-// Same candidates in the same order: the same sums.
+// STAGE: each wave copies the union of its lanes' windows of a row (when at
+// most kStageCap candidates) into its own LDS slice with full-line loads, and
+// the lanes read their candidates from there; wider unions (waves straddling
+// two rows of cells) read global memory as before.  Same candidates in the
+// same order: the same sums.
 constexpr int kStageCap = 256;
 int g_density_stage = 1;
 
 // At most 56 VGPRs asked for (nine waves per SIMD): the row-by-row walk fits
 // (55; 57 and eight waves without the cap); the staged walk takes 63 anyway
 // (eight waves, whose LDS slices take 128 of the 160 KB).
-template <bool SCALED, bool PACKED, int REC, bool STAGE>
+template <bool SCALED, bool PACKED, bool STAGE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k_density(int num_host, const int* __restrict__ d_num,
                                                     const float4* __restrict__ pos_s,
                                                     const uint32_t* __restrict__ start,
                                                     const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
-                                                    Win win, float* __restrict__ dens, float4* __restrict__ packed,
-                                                    RowMasks masks) {
+                                                    Win win, float* __restrict__ dens, float4* __restrict__ packed) {
     const int i = remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
     if (d_num) effective_num(num, start, g.nb_cells, num);
@@ -150,32 +146,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
     const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
     const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
     float sum = 0.0f;
-    auto sep2 = [&](float4 pj) {
-        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
-        float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        return rx * rx + ry * ry + rz * rz;
-    };
     auto accum = [&](float4 pj) {   // the poly6 term of one candidate
         if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
         poly6_add_r(sum, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, h2);
     };
-    const size_t st = (size_t)masks.stride;
-    constexpr bool MASKS = REC == kRecBits, LIST = REC == kRecList;
-    // list form: codes packed four to a slot, flushed as one 8-byte store
-    uint32_t lo = 0, hi = 0, cnt = 0;
-    bool over = false;
-    auto append = [&](uint32_t code) {
-        if (cnt >= (uint32_t)kListCap) {
-            over = true;
-            return;
+    // candidates [a, b) of one row at P[j - off], kGroupDensity loads at a time
+    auto walk = [&](const float4* __restrict__ P, uint32_t a, uint32_t b, uint32_t off) {
+        uint32_t j = a;
+        for (; j + kGroupDensity <= b; j += kGroupDensity) {
+            const float4* __restrict__ pg = P + (j - off);   // immediate offsets for the group's loads
+            float4 p[kGroupDensity];
+#pragma unroll
+            for (int k = 0; k < kGroupDensity; ++k) p[k] = pg[k];
+#pragma unroll
+            for (int k = 0; k < kGroupDensity; ++k) accum(p[k]);
         }
-        const uint32_t k = cnt & 3u;
-        if (k == 0u) lo = code;
-        else if (k == 1u) lo |= code << 16;
-        else if (k == 2u) hi = code;
-        else hi |= code << 16;
-        if (k == 3u) masks.codes[(size_t)(cnt >> 2) * st + i] = make_uint2(lo, hi);
-        ++cnt;
+        for (; j < b; ++j) accum(P[j - off]);
     };
     if constexpr (STAGE) {
         extern __shared__ float4 stage_rows[];   // kStageCap per wave
@@ -194,88 +180,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
                 if (B - A <= (uint32_t)kStageCap) {
                     for (uint32_t t = rank; t < B - A; t += nlive) my[t] = pos_s[A + t];
                     __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
-                    uint32_t j = a;
-                    for (; j + kGroupDensity <= b; j += kGroupDensity) {
-                        const float4* __restrict__ pg = my + (j - A);
-                        float4 p[kGroupDensity];
-#pragma unroll
-                        for (int k = 0; k < kGroupDensity; ++k) p[k] = pg[k];
-#pragma unroll
-                        for (int k = 0; k < kGroupDensity; ++k) accum(p[k]);
-                    }
-                    for (; j < b; ++j) accum(my[j - A]);
+                    walk(my, a, b, A);
                     __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
                 } else {
-                    uint32_t j = a;
-                    for (; j + kGroupDensity <= b; j += kGroupDensity) {
-                        const float4* __restrict__ pg = pos_s + j;
-                        float4 p[kGroupDensity];
-#pragma unroll
-                        for (int k = 0; k < kGroupDensity; ++k) p[k] = pg[k];
-#pragma unroll
-                        for (int k = 0; k < kGroupDensity; ++k) accum(p[k]);
-                    }
-                    for (; j < b; ++j) accum(pos_s[j]);
+                    walk(pos_s, a, b, 0u);
                 }
             }
         }
-        const float rho = sp.wpoly6_coef * sp.mass * sum;
-        if (PACKED) packed[i].w = rho; else dens[i] = rho;
-        return;
-    }
-    int r = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        for (int dy = -1; dy <= 1; ++dy, ++r) {
-            uint32_t a, b;
-            row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
-            // accepted-candidate bits of this row, flushed every 32 candidates
-            uint32_t word = 0, q = 0;
-            int wq = 0;
-            auto flush = [&]() {
-                if (wq < kMaskWords) masks.words[(size_t)(r * kMaskWords + wq) * st + i] = word;
-                word = 0;
-                ++wq;
-            };
-            if (REC != kRecNone) masks.rows[(size_t)r * st + i] = make_uint2(a, b - a);
-            if (LIST && b - a >= (1u << kCodeOffBits)) over = true;   // offsets would not fit a code
-            const uint32_t rtag = (uint32_t)r << kCodeOffBits;
-            // groups of kGroupDensity candidates: all loads issued before the first use
-            uint32_t j = a;
-            for (; j + kGroupDensity <= b; j += kGroupDensity, q += kGroupDensity) {
-                const float4* __restrict__ pg = pos_s + j;   // immediate offsets for the group's loads
-                float4 p[kGroupDensity];
-#pragma unroll
-                for (int k = 0; k < kGroupDensity; ++k) p[k] = pg[k];
-                uint32_t nib = 0;
-#pragma unroll
-                for (int k = 0; k < kGroupDensity; ++k) {
-                    accum(p[k]);
-                    const float r2 = REC != kRecNone ? sep2(p[k]) : 0.0f;
-                    const bool acc_k = r2 <= h2 && j + k != (uint32_t)i;
-                    if (MASKS) nib |= (uint32_t)acc_k << k;
-                    if (LIST && acc_k) append(rtag | (j + k - a));
-                }
-                if (MASKS) {
-                    word |= nib << (q & 31u);
-                    if ((q & 31u) == 32u - kGroupDensity) flush();
-                }
+    } else {
+        for (int dz = -1; dz <= 1; ++dz) {
+            for (int dy = -1; dy <= 1; ++dy) {
+                uint32_t a, b;
+                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
+                walk(pos_s, a, b, 0u);
             }
-            for (; j < b; ++j, ++q) {
-                accum(pos_s[j]);
-                const float r2 = REC != kRecNone ? sep2(pos_s[j]) : 0.0f;
-                const bool acc_k = r2 <= h2 && j != (uint32_t)i;
-                if (MASKS) {
-                    word |= (uint32_t)acc_k << (q & 31u);
-                    if ((q & 31u) == 31u) flush();
-                }
-                if (LIST && acc_k) append(rtag | (j - a));
-            }
-            if (MASKS && (q & 31u) != 0u) flush();
         }
-    }
-    if (LIST) {
-        if ((cnt & 3u) != 0u && !over) masks.codes[(size_t)(cnt >> 2) * st + i] = make_uint2(lo, hi);
-        masks.count[i] = over ? (uint32_t)kListCap + 1u : cnt;
     }
     const float rho = sp.wpoly6_coef * sp.mass * sum;
     if (PACKED) packed[i].w = rho; else dens[i] = rho;
@@ -288,34 +207,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
 // (collision_wall.cl) and the leapfrog step (leapfrog.cl), scattering to the
 // unsorted arrays; force/xsph are then stored only if force_out != nullptr.
 //
-// COMPACT: about a third of the windowed candidates pass the r <= h test, so
-// a branch on it keeps the whole wave in the pair math for almost every
-// candidate.  The compact variant tests kGroup candidates per lane per
-// iteration and appends the accepted j to a per-thread list in dynamic LDS
-// (kNbrCap entries, entry k of thread t at word k * blockDim + t:
-// conflict-free); when a lane's list could overflow, every lane drains its
-// list, kGroup pairs per iteration.  Pairs are visited in the same
-// ascending-j order as the direct loop.
-//
-// LOOP (the force-loop variant): kLoopDirect, kLoopCompact, or kLoopMasked:
-// walk the accepted-candidate bits the density kernel recorded for this step
-// (RowMasks), kGroup accepted neighbours per iteration with their loads
-// issued together; no row searches and no position tests.  Also ascending j.
-//
-// kLoopList: walk the list of accepted neighbours the density kernel recorded
-// (16-bit codes, RowMasks list form), four per iteration; each lane runs
-// through its own list, so a wave costs about its largest neighbour count
-// rather than, row by row, the largest per-row count (the row-by-row walk of
-// kLoopMasked keeps only ~40% of the lanes busy on a dam-break state).
-//
-// kLoopStaged (the default): the direct loop, with each row's candidates
-// (positions, their w replaced by 1 / density, and veleval|density) copied
-// once per wave into LDS as for the density kernel's STAGE (at most
-// kForceStageCap candidates; wider unions read global memory).  Same pairs,
-// same order, same reciprocals: the direct loop's bits.
-constexpr int kLoopDirect = 0, kLoopCompact = 1, kLoopMasked = 2, kLoopList = 3, kLoopStaged = 4, kLoopStagedBits = 5;
+// LOOP (the force-loop variant):
+// kLoopDirect: each lane walks its own window of every row in global memory,
+// kGroup candidates per iteration (position loads, then the veleval loads of
+// the accepted ones, issued together) with the pair terms under a branch on
+// r <= h.
+// kLoopStaged: the direct loop, with each row's candidates (positions, their
+// w replaced by 1 / density, and veleval|density) copied once per wave into
+// LDS as for the density kernel's STAGE (at most kForceStageCap candidates;
+// wider unions read global memory).  Same pairs, same order, same
+// reciprocals: the direct loop's bits.
+// kLoopStagedBits (the default): the staged loop with the window taken in
+// chunks of 32 candidates, each chunk first tested into a mask of accepted
+// candidates and the pair terms then run once per set bit.  About a third of
+// the windowed candidates pass r <= h, so a branch on the test keeps the wave
+// in the pair terms for nearly every candidate; with the bits the wave's
+// pair iterations are its lanes' largest accepted count.  Ascending j again:
+// the direct loop's bits.
+constexpr int kLoopDirect = 0, kLoopStaged = 4, kLoopStagedBits = 5;
 constexpr int kForceStageCap = 192;   // 6 KB per wave: the 20 waves per CU of kStagedWavesPerSimd take 120 KB
-constexpr int kNbrCap = 32;
 
 // At least six waves per SIMD: the compiler keeps the force kernel within 80
 // VGPRs (from 85, with a few spilled bytes), which measured 3-8% faster than
@@ -339,7 +249,7 @@ __global__ __launch_bounds__(kBlock, LOOP >= kLoopStaged ? kStagedWavesPerSimd :
                                                   GravitySources gsrc, int collision, rtps_float4 bmin,
                                                   rtps_float4 bmax, float dt, float4* __restrict__ pos_u,
                                                   float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
-                                                  RowMasks masks, const float4* __restrict__ color_s,
+                                                  const float4* __restrict__ color_s,
                                                   float4* __restrict__ color_u) {
     const int i = remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
     int num = d_num ? *d_num : num_host;
@@ -370,108 +280,7 @@ __global__ __launch_bounds__(kBlock, LOOP >= kLoopStaged ? kStagedWavesPerSimd :
         return rx * rx + ry * ry + rz * rz;
     };
 
-    bool direct = false;
-    if constexpr (LOOP == kLoopMasked) {
-        const size_t st = (size_t)masks.stride;
-        for (int r = 0; r < 9; ++r) {
-            const uint2 rw = masks.rows[(size_t)r * st + i];
-            const uint32_t a = rw.x, n = rw.y;
-            const uint32_t nw = min((n + 31u) >> 5, (uint32_t)kMaskWords);
-            uint32_t mw[kMaskWords];
-#pragma unroll
-            for (int w = 0; w < kMaskWords; ++w)
-                mw[w] = (uint32_t)w < nw ? masks.words[(size_t)(r * kMaskWords + w) * st + i] : 0u;
-            uint32_t m = mw[0], base = a;
-            int kk = 0;
-            // next accepted candidate of this row, ascending
-            auto next = [&](uint32_t& j) {
-                while (m == 0u && kk < kMaskWords - 1) {
-                    ++kk;
-                    m = kk == 1 ? mw[1] : (kk == 2 ? mw[2] : mw[3]);
-                    base += 32u;
-                }
-                if (m == 0u) return false;
-                j = base + (uint32_t)__builtin_ctz(m);
-                m &= m - 1u;
-                return true;
-            };
-            for (;;) {
-                uint32_t jj[kGroup];
-                bool v[kGroup];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k) v[k] = next(jj[k]);
-                if (__ballot(v[0]) == 0ull) break;
-                float4 q[kGroup], e[kGroup];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k)
-                    if (v[k]) q[k] = pos_s[jj[k]];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k)
-                    if (v[k]) e[k] = ve_in[jj[k]];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k)
-                    if (v[k]) {
-                        float4 pj = q[k];
-                        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
-                        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                        const float r2 = rx * rx + ry * ry + rz * rz;
-                        const float dj = PACKED ? e[k].w : dens_in[jj[k]];
-                        force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, e[k], dj);
-                    }
-            }
-            // candidates past the recorded bits (rows longer than 32 kMaskWords)
-            for (uint32_t j = a + 32u * kMaskWords; j < a + n; ++j) {
-                float rx, ry, rz;
-                float r2 = sep(j, rx, ry, rz);
-                if (r2 <= h2 && j != (uint32_t)i) pair(j, rx, ry, rz, r2);
-            }
-        }
-    } else if constexpr (LOOP == kLoopList) {
-        const size_t st = (size_t)masks.stride;
-        const uint32_t cnt = masks.count[i];
-        direct = cnt > (uint32_t)kListCap;   // overflowed list: the direct loop below
-        if (!direct) {
-            // row starts of this particle, indexed by the codes' row field
-            __shared__ uint32_t abase[9 * kBlock];
-            const int bs = (int)blockDim.x, t = (int)threadIdx.x;
-#pragma unroll
-            for (int r = 0; r < 9; ++r) abase[r * bs + t] = masks.rows[(size_t)r * st + i].x;
-            const uint32_t ng = (cnt + 3u) >> 2;
-            uint2 pk = ng > 0u ? masks.codes[i] : make_uint2(0u, 0u);
-            for (uint32_t gi = 0; gi < ng; ++gi) {
-                const uint2 cur = pk;
-                if (gi + 1u < ng) pk = masks.codes[(size_t)(gi + 1u) * st + i];   // next slot, one ahead
-                const uint32_t code[kGroup] = {cur.x & 0xFFFFu, cur.x >> 16, cur.y & 0xFFFFu, cur.y >> 16};
-                uint32_t jj[kGroup];
-                bool v[kGroup];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k) {
-                    v[k] = gi * 4u + (uint32_t)k < cnt;
-                    // entries past cnt in the last slot are stale: index row 0 for them (unused)
-                    const uint32_t row = v[k] ? min(code[k] >> kCodeOffBits, 8u) : 0u;
-                    jj[k] = abase[(int)row * bs + t] + (code[k] & ((1u << kCodeOffBits) - 1u));
-                }
-                float4 q[kGroup], e[kGroup];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k)
-                    if (v[k]) q[k] = pos_s[jj[k]];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k)
-                    if (v[k]) e[k] = ve_in[jj[k]];
-#pragma unroll
-                for (int k = 0; k < kGroup; ++k)
-                    if (v[k]) {
-                        float4 pj = q[k];
-                        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
-                        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                        const float r2 = rx * rx + ry * ry + rz * rz;
-                        const float dj = PACKED ? e[k].w : dens_in[jj[k]];
-                        force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, e[k], dj);
-                    }
-            }
-        }
-    }
-    if (LOOP == kLoopDirect || (LOOP == kLoopList && direct)) {
+    if constexpr (LOOP == kLoopDirect) {
         const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
         const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
         for (int dz = -1; dz <= 1; ++dz) {
@@ -636,73 +445,6 @@ __global__ __launch_bounds__(kBlock, LOOP >= kLoopStaged ? kStagedWavesPerSimd :
             }
         }
     }
-    if constexpr (LOOP == kLoopCompact) {
-        const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
-        const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
-        extern __shared__ uint32_t nbr_lds[];   // kNbrCap * blockDim.x words
-        const int bs = (int)blockDim.x;
-        uint32_t* mine = nbr_lds + threadIdx.x;
-        int cnt = 0;
-        // kGroup list entries per lane per iteration, their loads issued together
-        auto drain = [&]() {
-            for (int k = 0; __ballot(k < cnt) != 0ull; k += kGroup) {
-                uint32_t jj[kGroup];
-                bool v[kGroup];
-#pragma unroll
-                for (int q = 0; q < kGroup; ++q) {
-                    v[q] = k + q < cnt;
-                    jj[q] = v[q] ? mine[(k + q) * bs] : 0u;
-                }
-                float4 pq[kGroup], e[kGroup];
-#pragma unroll
-                for (int q = 0; q < kGroup; ++q)
-                    if (v[q]) pq[q] = pos_s[jj[q]];
-#pragma unroll
-                for (int q = 0; q < kGroup; ++q)
-                    if (v[q]) e[q] = ve_in[jj[q]];
-#pragma unroll
-                for (int q = 0; q < kGroup; ++q)
-                    if (v[q]) {
-                        float4 pj = pq[q];
-                        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
-                        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                        const float r2 = rx * rx + ry * ry + rz * rz;
-                        const float dj = PACKED ? e[q].w : dens_in[jj[q]];
-                        force_pair(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, e[q], dj);
-                    }
-            }
-            cnt = 0;
-        };
-        for (int dz = -1; dz <= 1; ++dz) {
-            for (int dy = -1; dy <= 1; ++dy) {
-                uint32_t a, b;
-                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
-                for (uint32_t j = a; __ballot(j < b) != 0ull; j += kGroup) {
-                    if (__ballot(cnt > kNbrCap - kGroup) != 0ull) drain();
-                    float4 pq[kGroup];
-                    bool in[kGroup];
-#pragma unroll
-                    for (int q = 0; q < kGroup; ++q) {
-                        in[q] = j + (uint32_t)q < b;
-                        if (in[q]) pq[q] = pos_s[j + q];
-                    }
-#pragma unroll
-                    for (int q = 0; q < kGroup; ++q) {
-                        float4 pj = pq[q];
-                        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
-                        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                        const float r2 = rx * rx + ry * ry + rz * rz;
-                        if (in[q] && r2 <= h2 && j + q != (uint32_t)i) {
-                            mine[cnt * bs] = j + (uint32_t)q;
-                            ++cnt;
-                        }
-                    }
-                }
-            }
-        }
-        drain();
-    }
-
     float4 F, X;
     force_epilogue(acc, di, sp, F, X);
     if (force_out) { force_out[i] = F; xsph_out[i] = X; }
@@ -862,17 +604,17 @@ __global__ void k_commit_num(int* __restrict__ d_num, const uint32_t* __restrict
 // ---------------------------------------------------------------------------
 // Launchers shared by the C ABI and the system.
 
-// Force-loop variant: 0 = one branch per candidate, 1 = compacted LDS lists,
-// 2 / 3 = recorded rows (bits / lists), 4 = 0 with the rows staged through LDS
-// (the default: 21% faster than 0 on the 1M dam).
-int g_force_variant = kLoopStaged;
+// Force-loop variant: 0 = one branch per candidate, 4 = 0 with the rows
+// staged through LDS, 5 = 4 with the accepted candidates of each 32-candidate
+// chunk marked in a bit mask first (the default).
+int g_force_variant = kLoopStagedBits;
 
 // Threads per block of a neighbour launch (a multiple of 64, at most kBlock)
 // and its force-loop variant.
 static inline int nbr_block(const Win& w) { return w.block > 0 && w.block <= kBlock && w.block % 64 == 0 ? w.block : kBlock; }
 static inline int nbr_variant(const Win& w) {
     const int v = w.variant >= 0 ? w.variant : g_force_variant;
-    return v >= kLoopDirect && v <= kLoopStagedBits ? v : kLoopDirect;
+    return v == kLoopStaged || v == kLoopStagedBits ? v : kLoopDirect;
 }
 
 hipError_t launch_hash(int num, const int* d_num, int n_pad, uint32_t pad_key, const float4* pos_u, uint32_t* keys,
@@ -914,63 +656,47 @@ hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const 
 
 hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
                           const rtps_sph_params& sp, const Grid& g, const Win& win, float* dens, float4* packed,
-                          hipStream_t st, const RowMasks* masks) {
+                          hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
     const int bs = nbr_block(win);
     const int nb = blocks_for(num, bs);
-    const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
-    const bool stage = g_density_stage != 0 && !m.rows;
+    const bool stage = g_density_stage != 0;
     const size_t lds = stage ? (size_t)(bs / 64) * kStageCap * sizeof(float4) : 0;
-#define RTPS_DENS(S, P, M, T)                                                                                       \
-    k_density<S, P, M, T><<<nb, bs, lds, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed, m)
-#define RTPS_DENS_R(R, T)                                                                                           \
-    if (packed) { if (scaled) RTPS_DENS(true, true, R, T); else RTPS_DENS(false, true, R, T); }                       \
-    else { if (scaled) RTPS_DENS(true, false, R, T); else RTPS_DENS(false, false, R, T); }
-    if (m.rows && m.form == kRecList) { RTPS_DENS_R(kRecList, false) }
-    else if (m.rows && m.form == kRecBits) { RTPS_DENS_R(kRecBits, false) }
-    else if (stage) { RTPS_DENS_R(kRecNone, true) }
-    else { RTPS_DENS_R(kRecNone, false) }
-#undef RTPS_DENS_R
+#define RTPS_DENS(S, P, T) k_density<S, P, T><<<nb, bs, lds, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed)
+#define RTPS_DENS_T(T)                                                                                              \
+    if (packed) { if (scaled) RTPS_DENS(true, true, T); else RTPS_DENS(false, true, T); }                             \
+    else { if (scaled) RTPS_DENS(true, false, T); else RTPS_DENS(false, false, T); }
+    if (stage) { RTPS_DENS_T(true) } else { RTPS_DENS_T(false) }
+#undef RTPS_DENS_T
 #undef RTPS_DENS
     return hipGetLastError();
 }
 
-// The masked loop needs the rows the density kernel of this step recorded;
-// without them it falls back to the direct loop.
-static inline int loop_of(const Win& w, const RowMasks& m) {
-    const int v = nbr_variant(w);
-    if (v == kLoopMasked && !(m.rows && m.form == kRecBits)) return kLoopDirect;
-    if (v == kLoopList && !(m.rows && m.form == kRecList)) return kLoopDirect;
-    return v;
+static inline size_t force_lds(int loop, int bs) {
+    return loop == kLoopDirect ? 0 : (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4);
 }
 
 hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const float* dens, const float4* ve,
                         bool packed, const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp,
-                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st,
-                        const RowMasks* masks) {
+                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
     const int bs = nbr_block(win);
     const int nb = blocks_for(num, bs);
-    const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
-    const int loop = loop_of(win, m);
-    const size_t lds = loop == kLoopCompact ? (size_t)kNbrCap * bs * sizeof(uint32_t)
-                       : (loop >= kLoopStaged ? (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4) : 0);
+    const int loop = nbr_variant(win);
+    const size_t lds = force_lds(loop, bs);
     GravitySources none{};
     rtps_float4 z{0, 0, 0, 0};
 #define RTPS_FORCE_L(S, P, L)                                                                                       \
     k_force<S, P, false, L><<<nb, bs, lds, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force, xsph,   \
-                                               nullptr, nullptr, none, 0, z, z, 0.f, nullptr, nullptr, nullptr, m,    \
+                                               nullptr, nullptr, none, 0, z, z, 0.f, nullptr, nullptr, nullptr,       \
                                                nullptr, nullptr)
 #define RTPS_FORCE(S, P)                                                                                            \
-    if (loop == kLoopList) RTPS_FORCE_L(S, P, kLoopList);                                                       \
-    else if (loop == kLoopStaged) RTPS_FORCE_L(S, P, kLoopStaged);                                                  \
+    if (loop == kLoopStaged) RTPS_FORCE_L(S, P, kLoopStaged);                                                       \
     else if (loop == kLoopStagedBits) RTPS_FORCE_L(S, P, kLoopStagedBits);                                          \
-    else if (loop == kLoopMasked) RTPS_FORCE_L(S, P, kLoopMasked);                                                  \
-    else if (loop == kLoopCompact) RTPS_FORCE_L(S, P, kLoopCompact);                                                \
     else RTPS_FORCE_L(S, P, kLoopDirect)
     if (packed) { if (scaled) { RTPS_FORCE(true, true); } else { RTPS_FORCE(false, true); } }
     else { if (scaled) { RTPS_FORCE(true, false); } else { RTPS_FORCE(false, false); } }
@@ -984,26 +710,21 @@ hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, co
                               const Win& win, float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
                               const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,
                               float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
-                              const RowMasks* masks, const float4* color_s, float4* color_u) {
+                              const float4* color_s, float4* color_u) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
     const int bs = nbr_block(win);
     const int nb = blocks_for(num, bs);
-    const RowMasks m = masks ? *masks : RowMasks{nullptr, nullptr, nullptr, nullptr, 0, 0};
-    const int loop = loop_of(win, m);
-    const size_t lds = loop == kLoopCompact ? (size_t)kNbrCap * bs * sizeof(uint32_t)
-                       : (loop >= kLoopStaged ? (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4) : 0);
+    const int loop = nbr_variant(win);
+    const size_t lds = force_lds(loop, bs);
 #define RTPS_FFUSED_L(S, L)                                                                                         \
     k_force<S, true, true, L><<<nb, bs, lds, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
                                                  xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt,  \
-                                                 pos_u, vel_u, veleval_u, m, color_s, color_u)
+                                                 pos_u, vel_u, veleval_u, color_s, color_u)
 #define RTPS_FFUSED(S)                                                                                              \
-    if (loop == kLoopList) RTPS_FFUSED_L(S, kLoopList);                                                         \
-    else if (loop == kLoopStaged) RTPS_FFUSED_L(S, kLoopStaged);                                                    \
+    if (loop == kLoopStaged) RTPS_FFUSED_L(S, kLoopStaged);                                                         \
     else if (loop == kLoopStagedBits) RTPS_FFUSED_L(S, kLoopStagedBits);                                            \
-    else if (loop == kLoopMasked) RTPS_FFUSED_L(S, kLoopMasked);                                                    \
-    else if (loop == kLoopCompact) RTPS_FFUSED_L(S, kLoopCompact);                                                  \
     else RTPS_FFUSED_L(S, kLoopDirect)
     if (scaled) { RTPS_FFUSED(true); } else { RTPS_FFUSED(false); }
 #undef RTPS_FFUSED

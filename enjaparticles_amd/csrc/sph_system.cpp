@@ -55,7 +55,6 @@ struct rtps_sph {
     int* d_num = nullptr;
     void* sort_tmp = nullptr;
     // neighbour rows recorded by density for the force loops 2 (bit form) and 3 (list form)
-    RowMasks rm{nullptr, nullptr, nullptr, nullptr, 0, 1}, rl{nullptr, nullptr, nullptr, nullptr, 0, 2};
     GravitySources gsrc{};
     int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0;
     int opt_autotune = 1;
@@ -103,9 +102,8 @@ struct rtps_sph {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& g : gexec_t)
             if (g) (void)hipGraphExecDestroy(g);
-        void* bufs[] = {pos_u, vel_u, veleval_u, pos_s,    vel_s,   ve_s,     force_s,  xsph_s,   color_u,  color_s,
-                        keys,  idx,   start,     end,      d_num,   sort_tmp, rm.rows,  rm.words, rm.codes, rm.count,
-                        sub_start};
+        void* bufs[] = {pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s,  force_s,  xsph_s,   color_u,
+                        color_s, keys, idx, start, end, d_num, sort_tmp, sub_start};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         void* sbufs[] = {cf_field, mc_cases, mc_counts, mc_offs, mc_tiles, mc_total, mc_words, mc_ntri, mesh_v, mesh_n,
@@ -126,13 +124,6 @@ struct rtps_sph {
 };
 
 namespace {
-
-// Force-loop variant in use, and the recorded rows when it is the masked loop.
-int loop_variant(const rtps_sph* s) { return s->force_variant >= 0 ? s->force_variant : g_force_variant; }
-const RowMasks* masks_of(const rtps_sph* s) {
-    const int v = loop_variant(s);
-    return v == 2 ? &s->rm : (v == 3 ? &s->rl : nullptr);
-}
 
 // Sub-cell start table of the windowed rows, when it is in use.
 // Written by cellindices while the window option asks for it; read by the
@@ -246,7 +237,7 @@ int launch_front(rtps_sph* s, StageTimer& t) {
                                 s->ve_s, s->stream, col_u(s), col_s(s)));
     t.mark(ST_DENSITY);
     RTPS_TRY_HIP(launch_density(n, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, win(s), nullptr, s->ve_s,
-                                s->stream, masks_of(s)));
+                                s->stream));
     return RTPS_OK;
 }
 
@@ -285,7 +276,7 @@ int launch_back(rtps_sph* s) {
     const bool tri = s->ntri > 0 && s->opt_collision;
     if (s->integrator == 1 || tri || rigid_on(s)) {
         RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp,
-                                  s->gs, win(s, true), s->force_s, s->xsph_s, s->stream, masks_of(s)));
+                                  s->gs, win(s, true), s->force_s, s->xsph_s, s->stream));
         RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                        s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
         if (tri)
@@ -299,7 +290,7 @@ int launch_back(rtps_sph* s) {
         RTPS_TRY_HIP(launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs,
                                         win(s, true), fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
                                         s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u,
-                                        s->veleval_u, s->stream, masks_of(s), col_s(s), col_u(s)));
+                                        s->veleval_u, s->stream, col_s(s), col_u(s)));
     }
     RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
     return RTPS_OK;
@@ -464,7 +455,9 @@ void tune_load_file_locked() {
     std::ifstream in(f);
     std::string k;
     Tune t;
-    while (in >> k >> t.bd >> t.bf >> t.var) g_tune_cache[k] = t;
+    auto block_ok = [](int b) { return b == 64 || b == 128 || b == 256; };
+    while (in >> k >> t.bd >> t.bf >> t.var)
+        if (block_ok(t.bd) && block_ok(t.bf) && (t.var == 0 || t.var == 4 || t.var == 5)) g_tune_cache[k] = t;
 }
 
 bool tune_get(const std::string& key, Tune& t) {
@@ -501,7 +494,7 @@ std::string tune_key(const rtps_sph* s) {
     while (n2 < (unsigned long)s->num_host) n2 <<= 1;
     return arch + "/cu" + std::to_string(cus) + "/n" + std::to_string(n2) + "/c" + std::to_string(s->nb_cells) +
            "/w" + std::to_string(s->opt_window) + "/r" + std::to_string(s->opt_remap) + "/s" +
-           std::to_string(s->sp.simulation_scale != 1.0f ? 1 : 0) + "/a" + std::to_string(s->opt_autotune == 2 ? 2 : 1) + "/i" + std::to_string(s->integrator);
+           std::to_string(s->sp.simulation_scale != 1.0f ? 1 : 0) + "/i" + std::to_string(s->integrator);
 }
 
 void apply_tune(rtps_sph* s, const Tune& t) {
@@ -554,33 +547,27 @@ int autotune_step(rtps_sph* s, bool measure) {
     // RTPS_TUNE_VERBOSE=1 prints every timed configuration to stderr
     const char* verbose_env = std::getenv("RTPS_TUNE_VERBOSE");
     const bool verbose = verbose_env && verbose_env[0] == '1';
-    // density: plain, recording bits, recording lists (each left recorded for the force loops timed next);
-    // the recorded forms (force loops 2 and 3) are candidates only with RTPS_OPT_AUTOTUNE = 2
-    const bool full = s->opt_autotune == 2;
-    const int nmodes = full ? 3 : 1, nvars = full ? 6 : 4;
-    const int vars[] = {0, 4, 5, 1, 2, 3};   // loops in the order they are tried
-    float best_d[3] = {1e30f, 1e30f, 1e30f};
-    int bd[3] = {256, 256, 256};
+    const int vars[] = {5, 4, 0};   // force loops in the order they are tried
+    float best_d = 1e30f;
+    int bd = 256;
     bool failed = false;
-    const RowMasks* recs[3] = {nullptr, &s->rm, &s->rl};
-    for (int mode = 0; mode < nmodes; ++mode)
-        for (int bs : blocks) {
-            Win w = win(s, false);
-            w.block = bs;
-            const float ms = time_it([&] {
-                return launch_density(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, w, nullptr,
-                                      s->ve_s, s->stream, recs[mode]);
-            });
-            if (verbose) std::fprintf(stderr, "rtps autotune: density record %d block %d: %.4f ms\n", mode, bs, ms);
-            if (ms < 0.f) failed = true;
-            else if (best_d[mode] >= 1e30f || ms < kMargin * best_d[mode]) { best_d[mode] = ms; bd[mode] = bs; }
-        }
+    for (int bs : blocks) {
+        Win w = win(s, false);
+        w.block = bs;
+        const float ms = time_it([&] {
+            return launch_density(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, w, nullptr,
+                                  s->ve_s, s->stream);
+        });
+        if (verbose) std::fprintf(stderr, "rtps autotune: density block %d: %.4f ms\n", bs, ms);
+        if (ms < 0.f) failed = true;
+        else if (best_d >= 1e30f || ms < kMargin * best_d) { best_d = ms; bd = bs; }
+    }
     const bool fo = s->opt_force_out != 0;
     float best_total = 1e30f;
     Tune best{256, 256, 0};
-    for (int vi = 0; vi < nvars && !failed; ++vi)
+    for (int var : vars)
         for (int bs : blocks) {
-            const int var = vars[vi];
+            if (failed) break;
             Win w = win(s, true);
             w.block = bs;
             w.variant = var;
@@ -588,19 +575,17 @@ int autotune_step(rtps_sph* s, bool measure) {
                 // the euler step stores forces and leaves veleval alone: time the unfused force kernel
                 if (s->integrator == 1)
                     return launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end,
-                                        s->sp, s->gs, w, s->force_s, s->xsph_s, s->stream,
-                                        recs[var == 2 || var == 3 ? var - 1 : 0]);
+                                        s->sp, s->gs, w, s->force_s, s->xsph_s, s->stream);
                 return launch_force_fused(s->num_host, s->d_num, s->pos_s, s->ve_s, s->start, s->end, s->sp, s->gs, w,
                                           fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
                                           s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u,
-                                          s->vel_u, s->veleval_u, s->stream, recs[var == 2 || var == 3 ? var - 1 : 0]);
+                                          s->vel_u, s->veleval_u, s->stream);
             });
-            const int mode = var == 2 || var == 3 ? var - 1 : 0;
             if (verbose) std::fprintf(stderr, "rtps autotune: force loop %d block %d: %.4f ms\n", var, bs, ms);
             if (ms < 0.f) failed = true;
-            else if (best_total >= 1e30f || ms + best_d[mode] < kMargin * best_total) {
-                best_total = ms + best_d[mode];
-                best = Tune{bd[mode], bs, var};
+            else if (best_total >= 1e30f || ms + best_d < kMargin * best_total) {
+                best_total = ms + best_d;
+                best = Tune{bd, bs, var};
             }
         }
     (void)hipEventDestroy(e0);
@@ -610,7 +595,7 @@ int autotune_step(rtps_sph* s, bool measure) {
     tune_put(key, best);
     // finish the step with the chosen configuration
     RTPS_TRY_HIP(launch_density(s->num_host, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, win(s), nullptr,
-                                s->ve_s, s->stream, masks_of(s)));
+                                s->ve_s, s->stream));
     return launch_back(s);
 }
 
@@ -666,13 +651,6 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
         RTPS_TRY_HIP(dalloc(&s->d_num, 1));
         if ((size_t)s->nb_cells * kSubCells * sizeof(uint32_t) <= ((size_t)2 << 30))   // at most 2 GiB
             RTPS_TRY_HIP(dalloc(&s->sub_start, (size_t)s->nb_cells * kSubCells));
-        RTPS_TRY_HIP(dalloc(&s->rm.rows, 9 * N));
-        RTPS_TRY_HIP(dalloc(&s->rm.words, 9 * (size_t)kMaskWords * N));
-        RTPS_TRY_HIP(dalloc(&s->rm.codes, (size_t)(kListCap / 4) * N));
-        RTPS_TRY_HIP(dalloc(&s->rm.count, N));
-        s->rm.stride = (int)N;
-        s->rl = s->rm;
-        s->rl.form = 2;
         RTPS_TRY_HIP(hipMalloc(&s->sort_tmp, sort_temp_bytes((int)N, s->key_bits)));
         float4* zf[] = {s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s, s->ve_s, s->force_s, s->xsph_s};
         for (float4* p : zf) RTPS_TRY_HIP(hipMemsetAsync(p, 0, N * sizeof(float4), s->stream));
@@ -848,7 +826,7 @@ int rtps_sph_update(rtps_sph* s) {
     int rc = launch_front(s, t);
     if (rc) return rc;
     RTPS_TRY_HIP(launch_force(s->num_host, s->d_num, s->pos_s, nullptr, s->ve_s, true, s->start, s->end, s->sp, s->gs,
-                              win(s, true), s->force_s, s->xsph_s, s->stream, masks_of(s)));
+                              win(s, true), s->force_s, s->xsph_s, s->stream));
     RTPS_TRY_HIP(launch_post_force(s->num_host, s->d_num, s->start, s->nb_cells, s->pos_s, s->vel_s, s->force_s,
                                    s->sp, s->gsrc, s->opt_collision, s->gps, s->stream));
     if (s->ntri > 0 && s->opt_collision)
@@ -1271,7 +1249,9 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_FORCE_OUT: s->opt_force_out = value ? 1 : 0; break;
         case RTPS_OPT_TIMERS: s->opt_timers = value ? 1 : 0; break;
         case RTPS_OPT_FORCE_VARIANT:
-            if (value < 0 || value > 5) return set_error(RTPS_ERR_ARG, "force variant must be 0 to 5");
+            if (value != 0 && value != 4 && value != 5)
+                return set_error(RTPS_ERR_ARG, "force loop must be 0 (direct), 4 (staged rows) or 5 (staged rows "
+                                               "with accepted bits)");
             g_force_variant = value;
             break;
         case RTPS_OPT_WINDOW:
@@ -1309,7 +1289,7 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
             s->opt_color = value;
             break;
         case RTPS_OPT_AUTOTUNE:
-            if (value < 0 || value > 2) return set_error(RTPS_ERR_ARG, "autotune must be 0, 1 or 2");
+            if (value < 0 || value > 1) return set_error(RTPS_ERR_ARG, "autotune must be 0 or 1");
             s->opt_autotune = value;
             if (!value) apply_tune(s, Tune{0, 0, -1});
             s->tuned = false;

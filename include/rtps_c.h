@@ -268,26 +268,21 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        step k-1 runs and rtps_sph_step returns after the
  *                        last step finished
  *   RTPS_OPT_FORCE_VARIANT  force-loop kernel, process-wide default for handles
- *                        that are not autotuned: 0 = one branch per candidate,
- *                        1 = compacted neighbour lists in LDS, 2 = the
- *                        density kernel records each row's window and accepted
- *                        candidates as bits and the force kernel walks those
- *                        row by row, 3 = the density kernel records a list of
- *                        16-bit neighbour codes and each lane walks its own list,
+ *                        that are not autotuned: 0 = each lane walks its
+ *                        windows in global memory, one branch per candidate;
  *                        4 = 0 with each row's candidates copied once per wave
- *                        into LDS (default), 5 = 4 with each chunk of 32
- *                        candidates tested first into a bit mask and the pair
- *                        terms run once per accepted candidate
+ *                        into LDS; 5 = 4 with each chunk of 32 candidates
+ *                        tested first into a bit mask and the pair terms run
+ *                        once per accepted candidate (default).  The three give
+ *                        the same bits.
  *   RTPS_OPT_WINDOW      1 = scan only the x window of each neighbour row, its
  *                        bounds looked up in a table of sub-cell starts that
  *                        cellindices writes (default; 64 bytes per grid cell),
  *                        2 = the same window found by binary search of the
  *                        sorted keys, 0 = whole rows
  *   RTPS_OPT_AUTOTUNE    1 = the first rtps_sph_step tunes the neighbour launches
- *                        over the force loops 0, 4, 5 and 1 (default; see
- *                        rtps_sph_autotune), 2 = over all six loops (2 and 3
- *                        win while the neighbourhoods are sparse, lose once
- *                        they fill), 0 = fixed defaults
+ *                        over the force loops 5, 4 and 0 (default; see
+ *                        rtps_sph_autotune), 0 = fixed defaults
  *   RTPS_OPT_SORT_TILE   radix-sort tile, process-wide: threads * 100 + keys
  *                        per thread (1024 x 1, 2, 4, 8; 512 x 4, 8; 256 x 4,
  *                        8, 16, e.g. 102404), or 0 (default) = by size
@@ -331,9 +326,8 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))
  * during which the threads per block of the density and force launches (64,
- * 128 or 256) and the force-loop variant (0, 4, 5 and 1, or 0 to 5 with
- * RTPS_OPT_AUTOTUNE = 2; variants 2 and 3 with the density kernel recording
- * rows) are timed on the live state and the fastest density + force pair kept
+ * 128 or 256) and the force loop (5, 4 and 0) are timed on the live state
+ * and the fastest density + force pair kept
  * (a candidate tried later must be 5% faster).  The two kernels are idempotent on one sorted state, so the
  * step's result does not depend on the choice.  Choices are cached per
  * process by device and size class (and in the file named by the
