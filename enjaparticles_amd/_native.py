@@ -25,7 +25,8 @@ FIELD = {
     "color": 13, "color_s": 14, "mass": 15, "mass_s": 16,
 }
 (OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS, OPT_FORCE_VARIANT, OPT_WINDOW, OPT_TILE_REMAP, OPT_AUTOTUNE,
- OPT_SORT_TILE, OPT_COLOR, OPT_DENSITY_STAGE, OPT_BLOCK_DENSITY, OPT_BLOCK_FORCE, OPT_SURFACE_FRAME) = range(14)
+ OPT_SORT_TILE, OPT_COLOR, OPT_DENSITY_STAGE, OPT_BLOCK_DENSITY, OPT_BLOCK_FORCE, OPT_SURFACE_FRAME,
+ OPT_TILE_QUEUES) = range(15)
 # The library's process-wide default of OPT_FORCE_VARIANT (the force loop of
 # handles that are not autotuned).
 DEFAULT_FORCE_LOOP = 5

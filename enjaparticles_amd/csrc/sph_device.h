@@ -120,6 +120,10 @@ struct Win {
     // sub-cell starts (cellindices with a table, kSubCells entries per cell):
     // the window bounds are looked up instead of searched; nullptr = search
     const uint32_t* sub = nullptr;
+    // per-XCD tile queues (claim_tile): kTileQueues counters and a count of
+    // finished blocks, all zero between launches; nullptr = one tile per block
+    int* tq = nullptr;
+    int ntiles = 0;   // tiles of the launch (set by the launcher)
 };
 
 // Candidates per group in the neighbour loops (loads in flight per lane).
@@ -318,6 +322,71 @@ __device__ __forceinline__ int xcd_chunk_tile(int b, int nb, int C) {
 // contiguous range per XCD, C >= 2 = chunks of C tiles.
 __device__ __forceinline__ int remap_tile(int remap, int b, int nb) {
     return remap == 0 ? b : (remap == 1 ? xcd_tile(b, nb) : xcd_chunk_tile(b, nb, remap));
+}
+
+// Per-XCD tile queues of the persistent neighbour launches.  The launch has
+// a resident grid; block b runs on XCD b % 8 and takes its tiles from queue
+// b % 8, which holds the contiguous range [q T / 8, (q + 1) T / 8) of the T
+// tiles, so each XCD's L2 serves one contiguous stretch of the sorted
+// particles and the neighbour rows its blocks share.  The n_q blocks of
+// queue q take the range's first n_q tiles by their rank b / 8 (no atomic);
+// the rest are handed out in order by an atomic counter per queue, each on
+// its own 256-byte line.  A block whose queue is drained moves on to the
+// next queue (drained queues stay drained), which balances XCDs with more
+// neighbours per particle against the others.
+constexpr int kTileQueues = 8;
+constexpr int kTileQueueStride = 64;                                   // ints between two counters
+constexpr int kTileQueueWords = kTileQueues * kTileQueueStride + 1;    // + the finished-block count
+
+__device__ __forceinline__ int tile_queue_lo(int ntiles, int q) { return (int)((long)ntiles * q / kTileQueues); }
+__device__ __forceinline__ int tile_queue_blocks(int q) {   // blocks whose home queue is q
+    return ((int)gridDim.x + kTileQueues - 1 - q) / kTileQueues;
+}
+
+// The block's first tile: the tile of its rank in its home queue's range, or
+// -1 when the range is shorter than its blocks (block-uniform, no atomics).
+__device__ __forceinline__ int first_tile(int ntiles) {
+    const int q = (int)(blockIdx.x % kTileQueues), k = (int)(blockIdx.x / kTileQueues);
+    const int lo = tile_queue_lo(ntiles, q), hi = tile_queue_lo(ntiles, q + 1);
+    return lo + k < hi ? lo + k : -1;
+}
+
+// The next tile of the calling block, or -1 once every queue is drained
+// (block-uniform).  q: the block's current queue, initially its home queue
+// blockIdx.x % 8 (only thread 0's copy is used).
+__device__ __forceinline__ int claim_tile(int* __restrict__ tq, int ntiles, int& q) {
+    __shared__ int sh_tile;
+    if (threadIdx.x == 0) {
+        const int q0 = q;
+        int t = -1;
+        for (;;) {
+            const int lo = tile_queue_lo(ntiles, q) + tile_queue_blocks(q);
+            const int hi = tile_queue_lo(ntiles, q + 1);
+            if (lo < hi) {
+                const int k = atomicAdd(&tq[q * kTileQueueStride], 1);
+                if (lo + k < hi) {
+                    t = lo + k;
+                    break;
+                }
+            }
+            q = (q + 1) % kTileQueues;
+            if (q == q0) break;   // all drained
+        }
+        sh_tile = t;
+    }
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(sh_tile);
+    __syncthreads();   // every wave has read sh_tile before thread 0 writes the next
+    return t;
+}
+
+// Called once by every block after its last claim: the last block to get
+// here clears the counters for the next launch (all other blocks have made
+// their last claim by then: thread 0 waits for each claim's result before
+// counting itself finished).
+__device__ __forceinline__ void finish_tiles(int* __restrict__ tq) {
+    if (threadIdx.x == 0 && atomicAdd(&tq[kTileQueues * kTileQueueStride], 1) == (int)gridDim.x - 1)
+        for (int k = 0; k <= kTileQueues; ++k) atomicExch(&tq[k * kTileQueueStride], 0);
 }
 
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }

@@ -11,6 +11,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "rtps_internal.h"
 #include "sph_device.h"
 
@@ -130,12 +134,12 @@ int g_density_stage = 1;
 // (55; 57 and eight waves without the cap); the staged walk takes 63 anyway
 // (eight waves, whose LDS slices take 128 of the 160 KB).
 template <bool SCALED, bool PACKED, bool STAGE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k_density(int num_host, const int* __restrict__ d_num,
-                                                    const float4* __restrict__ pos_s,
-                                                    const uint32_t* __restrict__ start,
-                                                    const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
-                                                    Win win, float* __restrict__ dens, float4* __restrict__ packed) {
-    const int i = remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void density_tile(int tile, int num_host, const int* __restrict__ d_num,
+                                             const float4* __restrict__ pos_s, const uint32_t* __restrict__ start,
+                                             const uint32_t* __restrict__ end, const rtps_sph_params& sp,
+                                             const Grid& g, const Win& win, float* __restrict__ dens,
+                                             float4* __restrict__ packed) {
+    const int i = tile * (int)blockDim.x + (int)threadIdx.x;
     int num = d_num ? *d_num : num_host;
     if (d_num) effective_num(num, start, g.nb_cells, num);
     if (i >= num) return;
@@ -200,6 +204,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
     if (PACKED) packed[i].w = rho; else dens[i] = rho;
 }
 
+// One tile of blockDim.x sorted particles per block, or (QUEUED) a resident
+// grid working through the per-XCD tile queues of win.tq.  The queued
+// kernels are separate instances: the loop around the body raises its
+// register demand, which must not cost the one-tile kernels.
+template <bool SCALED, bool PACKED, bool STAGE, bool QUEUED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k_density(int num_host, const int* __restrict__ d_num,
+                                                    const float4* __restrict__ pos_s,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
+                                                    Win win, float* __restrict__ dens, float4* __restrict__ packed) {
+    if constexpr (QUEUED) {
+        int q = (int)(blockIdx.x % kTileQueues);
+        for (int t = first_tile(win.ntiles);;) {
+            if (t < 0) t = claim_tile(win.tq, win.ntiles, q);
+            if (t < 0) break;
+            density_tile<SCALED, PACKED, STAGE>(t, num_host, d_num, pos_s, start, end, sp, g, win, dens, packed);
+            t = -1;
+        }
+        finish_tiles(win.tq);
+    } else {
+        density_tile<SCALED, PACKED, STAGE>(remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x), num_host, d_num,
+                                            pos_s, start, end, sp, g, win, dens, packed);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // a8 (+a9, a10, a11 when FUSED): force.cl:148-186.  PACKED: veleval and
 // density come from one float4 (xyz = veleval, w = density).  FUSED: the
@@ -234,24 +263,23 @@ constexpr int kForceWavesPerSimd = 6;
 // The staged loop (kLoopStaged) at five waves: 96 VGPRs instead of 80 and no
 // spills in the row loop, 2-11% faster than six on the 1M dam.
 constexpr int kStagedWavesPerSimd = 5;
+// The tile-queue instances at four (128 VGPRs): the loop keeps more values live.
+constexpr int kQueuedWavesPerSimd = 4;
 
 template <bool SCALED, bool PACKED, bool FUSED, int LOOP>
-__global__ __launch_bounds__(kBlock, LOOP >= kLoopStaged ? kStagedWavesPerSimd : kForceWavesPerSimd) void k_force(int num_host, const int* __restrict__ d_num,
-                                                  const float4* __restrict__ pos_s,
-                                                  const float* __restrict__ dens_in,
-                                                  const float4* __restrict__ ve_in,
-                                                  const uint32_t* __restrict__ start,
-                                                  const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
-                                                  Win win, float4* __restrict__ force_out,
-                                                  float4* __restrict__ xsph_out,
-                                                  // FUSED only:
-                                                  const float4* __restrict__ vel_s, const uint32_t* __restrict__ idx,
-                                                  GravitySources gsrc, int collision, rtps_float4 bmin,
-                                                  rtps_float4 bmax, float dt, float4* __restrict__ pos_u,
-                                                  float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
-                                                  const float4* __restrict__ color_s,
-                                                  float4* __restrict__ color_u) {
-    const int i = remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void force_tile(int tile, int num_host, const int* __restrict__ d_num,
+                                           const float4* __restrict__ pos_s, const float* __restrict__ dens_in,
+                                           const float4* __restrict__ ve_in, const uint32_t* __restrict__ start,
+                                           const uint32_t* __restrict__ end, const rtps_sph_params& sp, const Grid& g,
+                                           const Win& win, float4* __restrict__ force_out,
+                                           float4* __restrict__ xsph_out,
+                                           // FUSED only:
+                                           const float4* __restrict__ vel_s, const uint32_t* __restrict__ idx,
+                                           const GravitySources& gsrc, int collision, const rtps_float4& bmin,
+                                           const rtps_float4& bmax, float dt, float4* __restrict__ pos_u,
+                                           float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
+                                           const float4* __restrict__ color_s, float4* __restrict__ color_u) {
+    const int i = tile * (int)blockDim.x + (int)threadIdx.x;
     int num = d_num ? *d_num : num_host;
     bool deleted = false;
     if (d_num) deleted = effective_num(num, start, g.nb_cells, num);
@@ -464,6 +492,44 @@ __global__ __launch_bounds__(kBlock, LOOP >= kLoopStaged ? kStagedWavesPerSimd :
     }
 }
 
+// One tile of blockDim.x sorted particles per block, or (QUEUED) a resident
+// grid working through the per-XCD tile queues of win.tq (separate instances,
+// at four waves per SIMD: the loop around the body needs the registers).
+template <bool SCALED, bool PACKED, bool FUSED, int LOOP, bool QUEUED>
+__global__ __launch_bounds__(kBlock, QUEUED ? kQueuedWavesPerSimd : (LOOP >= kLoopStaged ? kStagedWavesPerSimd : kForceWavesPerSimd)) void k_force(int num_host, const int* __restrict__ d_num,
+                                                  const float4* __restrict__ pos_s,
+                                                  const float* __restrict__ dens_in,
+                                                  const float4* __restrict__ ve_in,
+                                                  const uint32_t* __restrict__ start,
+                                                  const uint32_t* __restrict__ end, rtps_sph_params sp, Grid g,
+                                                  Win win, float4* __restrict__ force_out,
+                                                  float4* __restrict__ xsph_out,
+                                                  // FUSED only:
+                                                  const float4* __restrict__ vel_s, const uint32_t* __restrict__ idx,
+                                                  GravitySources gsrc, int collision, rtps_float4 bmin,
+                                                  rtps_float4 bmax, float dt, float4* __restrict__ pos_u,
+                                                  float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
+                                                  const float4* __restrict__ color_s,
+                                                  float4* __restrict__ color_u) {
+    if constexpr (QUEUED) {
+        int q = (int)(blockIdx.x % kTileQueues);
+        for (int t = first_tile(win.ntiles);;) {
+            if (t < 0) t = claim_tile(win.tq, win.ntiles, q);
+            if (t < 0) break;
+            force_tile<SCALED, PACKED, FUSED, LOOP>(t, num_host, d_num, pos_s, dens_in, ve_in, start, end, sp, g, win,
+                                                    force_out, xsph_out, vel_s, idx, gsrc, collision, bmin, bmax, dt,
+                                                    pos_u, vel_u, veleval_u, color_s, color_u);
+            t = -1;
+        }
+        finish_tiles(win.tq);
+    } else {
+        force_tile<SCALED, PACKED, FUSED, LOOP>(remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x), num_host,
+                                                d_num, pos_s, dens_in, ve_in, start, end, sp, g, win, force_out,
+                                                xsph_out, vel_s, idx, gsrc, collision, bmin, bmax, dt, pos_u, vel_u,
+                                                veleval_u, color_s, color_u);
+    }
+}
+
 // a9 + a10 on stored forces (the unfused update() path): point sources then
 // walls, in the reference's order (SPH.cpp:289-297).
 __global__ __launch_bounds__(kBlock) void k_post_force(int num_host, const int* __restrict__ d_num,
@@ -654,20 +720,56 @@ hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const 
     return hipGetLastError();
 }
 
+// Blocks of a resident grid of `kernel` (every block on the device at once),
+// at most `ntiles`: the launch shape of the tile-queue neighbour kernels.
+// Looked up once per kernel, block size and LDS size (host queries only, so
+// it is safe inside a stream capture).
+static int resident_grid(const void* kernel, int bs, size_t lds, int ntiles) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, long>, int> cache;
+    const long key = (long)bs << 32 | (long)lds;
+    int total = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find({kernel, key});
+        if (it != cache.end()) total = it->second;
+    }
+    if (total == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bs, lds) != hipSuccess || per_cu <= 0)
+            per_cu = 1;
+        (void)hipGetLastError();
+        total = cus * per_cu;
+        std::lock_guard<std::mutex> lk(mu);
+        cache[{kernel, key}] = total;
+    }
+    return total < ntiles ? total : ntiles;
+}
+
 hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
-                          const rtps_sph_params& sp, const Grid& g, const Win& win, float* dens, float4* packed,
+                          const rtps_sph_params& sp, const Grid& g, const Win& win_in, float* dens, float4* packed,
                           hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
-    const int bs = nbr_block(win);
+    const int bs = nbr_block(win_in);
     const int nb = blocks_for(num, bs);
     const bool stage = g_density_stage != 0;
     const size_t lds = stage ? (size_t)(bs / 64) * kStageCap * sizeof(float4) : 0;
-#define RTPS_DENS(S, P, T) k_density<S, P, T><<<nb, bs, lds, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed)
+    Win win = win_in;
+    win.ntiles = nb;
+#define RTPS_DENS(S, P, T)                                                                                          \
+    if (win.tq && P)                                                                                                \
+        k_density<S, P, T, true><<<resident_grid((const void*)&k_density<S, P, T, true>, bs, lds, nb), bs, lds,      \
+                                   st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);                     \
+    else                                                                                                            \
+        k_density<S, P, T, false><<<nb, bs, lds, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed)
 #define RTPS_DENS_T(T)                                                                                              \
-    if (packed) { if (scaled) RTPS_DENS(true, true, T); else RTPS_DENS(false, true, T); }                             \
-    else { if (scaled) RTPS_DENS(true, false, T); else RTPS_DENS(false, false, T); }
+    if (packed) { if (scaled) { RTPS_DENS(true, true, T); } else { RTPS_DENS(false, true, T); } }                     \
+    else { if (scaled) { RTPS_DENS(true, false, T); } else { RTPS_DENS(false, false, T); } }
     if (stage) { RTPS_DENS_T(true) } else { RTPS_DENS_T(false) }
 #undef RTPS_DENS_T
 #undef RTPS_DENS
@@ -680,55 +782,73 @@ static inline size_t force_lds(int loop, int bs) {
 
 hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const float* dens, const float4* ve,
                         bool packed, const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp,
-                        const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st) {
+                        const Grid& g, const Win& win_in, float4* force, float4* xsph, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
-    const int bs = nbr_block(win);
+    const int bs = nbr_block(win_in);
     const int nb = blocks_for(num, bs);
-    const int loop = nbr_variant(win);
+    const int loop = nbr_variant(win_in);
     const size_t lds = force_lds(loop, bs);
     GravitySources none{};
     rtps_float4 z{0, 0, 0, 0};
+    Win win = win_in;
+    win.ntiles = nb;
+#define RTPS_FORCE_Q(S, P, L, Q, GRID)                                                                              \
+    k_force<S, P, false, L, Q><<<GRID, bs, lds, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force, xsph, \
+                                                     nullptr, nullptr, none, 0, z, z, 0.f, nullptr, nullptr, nullptr, \
+                                                     nullptr, nullptr)
 #define RTPS_FORCE_L(S, P, L)                                                                                       \
-    k_force<S, P, false, L><<<nb, bs, lds, st>>>(num, d_num, pos_s, dens, ve, start, end, sp, g, win, force, xsph,   \
-                                               nullptr, nullptr, none, 0, z, z, 0.f, nullptr, nullptr, nullptr,       \
-                                               nullptr, nullptr)
+    if (win.tq && P) {                                                                                              \
+        RTPS_FORCE_Q(S, P, L, true, resident_grid((const void*)&k_force<S, P, false, L, true>, bs, lds, nb));        \
+    } else {                                                                                                        \
+        RTPS_FORCE_Q(S, P, L, false, nb);                                                                           \
+    }
 #define RTPS_FORCE(S, P)                                                                                            \
-    if (loop == kLoopStaged) RTPS_FORCE_L(S, P, kLoopStaged);                                                       \
-    else if (loop == kLoopStagedBits) RTPS_FORCE_L(S, P, kLoopStagedBits);                                          \
-    else RTPS_FORCE_L(S, P, kLoopDirect)
+    if (loop == kLoopStaged) { RTPS_FORCE_L(S, P, kLoopStaged) }                                                    \
+    else if (loop == kLoopStagedBits) { RTPS_FORCE_L(S, P, kLoopStagedBits) }                                       \
+    else { RTPS_FORCE_L(S, P, kLoopDirect) }
     if (packed) { if (scaled) { RTPS_FORCE(true, true); } else { RTPS_FORCE(false, true); } }
     else { if (scaled) { RTPS_FORCE(true, false); } else { RTPS_FORCE(false, false); } }
 #undef RTPS_FORCE
 #undef RTPS_FORCE_L
+#undef RTPS_FORCE_Q
     return hipGetLastError();
 }
 
 hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, const float4* ve_packed,
                               const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
-                              const Win& win, float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
-                              const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,
-                              float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
-                              const float4* color_s, float4* color_u) {
+                              const Win& win_in, float4* force, float4* xsph, const float4* vel_s,
+                              const uint32_t* idx, const GravitySources& gsrc, int collision,
+                              const rtps_grid_params& gps, float dt, float4* pos_u, float4* vel_u, float4* veleval_u,
+                              hipStream_t st, const float4* color_s, float4* color_u) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (num <= 0) return hipSuccess;
     const bool scaled = sp.simulation_scale != 1.0f;
-    const int bs = nbr_block(win);
+    const int bs = nbr_block(win_in);
     const int nb = blocks_for(num, bs);
-    const int loop = nbr_variant(win);
+    const int loop = nbr_variant(win_in);
     const size_t lds = force_lds(loop, bs);
+    Win win = win_in;
+    win.ntiles = nb;
+#define RTPS_FFUSED_Q(S, L, Q, GRID)                                                                                \
+    k_force<S, true, true, L, Q><<<GRID, bs, lds, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, \
+                                                       force, xsph, vel_s, idx, gsrc, collision, gps.bnd_min,        \
+                                                       gps.bnd_max, dt, pos_u, vel_u, veleval_u, color_s, color_u)
 #define RTPS_FFUSED_L(S, L)                                                                                         \
-    k_force<S, true, true, L><<<nb, bs, lds, st>>>(num, d_num, pos_s, nullptr, ve_packed, start, end, sp, g, win, force, \
-                                                 xsph, vel_s, idx, gsrc, collision, gps.bnd_min, gps.bnd_max, dt,  \
-                                                 pos_u, vel_u, veleval_u, color_s, color_u)
+    if (win.tq) {                                                                                                   \
+        RTPS_FFUSED_Q(S, L, true, resident_grid((const void*)&k_force<S, true, true, L, true>, bs, lds, nb));        \
+    } else {                                                                                                        \
+        RTPS_FFUSED_Q(S, L, false, nb);                                                                             \
+    }
 #define RTPS_FFUSED(S)                                                                                              \
-    if (loop == kLoopStaged) RTPS_FFUSED_L(S, kLoopStaged);                                                         \
-    else if (loop == kLoopStagedBits) RTPS_FFUSED_L(S, kLoopStagedBits);                                            \
-    else RTPS_FFUSED_L(S, kLoopDirect)
-    if (scaled) { RTPS_FFUSED(true); } else { RTPS_FFUSED(false); }
+    if (loop == kLoopStaged) { RTPS_FFUSED_L(S, kLoopStaged) }                                                      \
+    else if (loop == kLoopStagedBits) { RTPS_FFUSED_L(S, kLoopStagedBits) }                                         \
+    else { RTPS_FFUSED_L(S, kLoopDirect) }
+    if (scaled) { RTPS_FFUSED(true) } else { RTPS_FFUSED(false) }
 #undef RTPS_FFUSED
 #undef RTPS_FFUSED_L
+#undef RTPS_FFUSED_Q
     return hipGetLastError();
 }
 

@@ -53,10 +53,12 @@ struct rtps_sph {
     uint32_t* sub_start = nullptr;
     bool sub_valid = false;   // the last cellindices wrote the table for the current sorted state
     int* d_num = nullptr;
+    int* tile_q = nullptr;   // per-XCD tile queue counters of the neighbour kernels (kTileQueueWords)
     void* sort_tmp = nullptr;
     // neighbour rows recorded by density for the force loops 2 (bit form) and 3 (list form)
     GravitySources gsrc{};
-    int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0;
+    int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0,
+        opt_queues = 0;
     int opt_autotune = 1;
     int opt_color = 0;   // permute colours every step (set by the first non-default colour pushed)
     int opt_frame = 0;   // surface vertices: 0 = world coordinates, 1 = the reference's [0, 10]^3 frame
@@ -103,7 +105,7 @@ struct rtps_sph {
         for (auto& g : gexec_t)
             if (g) (void)hipGraphExecDestroy(g);
         void* bufs[] = {pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s,  force_s,  xsph_s,   color_u,
-                        color_s, keys, idx, start, end, d_num, sort_tmp, sub_start};
+                        color_s, keys, idx, start, end, d_num, sort_tmp, sub_start, tile_q};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         void* sbufs[] = {cf_field, mc_cases, mc_counts, mc_offs, mc_tiles, mc_total, mc_words, mc_ntri, mesh_v, mesh_n,
@@ -141,6 +143,7 @@ Win win(const rtps_sph* s, bool force = false) {
     Win w{s->opt_window ? s->keys_sorted : nullptr, s->gps.grid_res.x, s->gps.grid_res.y, s->gps.grid_res.z,
           s->opt_remap, force ? s->block_force : s->block_density, force ? s->force_variant : -1};
     w.sub = sub_table(s);
+    w.tq = s->opt_queues ? s->tile_q : nullptr;
     return w;
 }
 
@@ -493,7 +496,7 @@ std::string tune_key(const rtps_sph* s) {
     unsigned long n2 = 1;
     while (n2 < (unsigned long)s->num_host) n2 <<= 1;
     return arch + "/cu" + std::to_string(cus) + "/n" + std::to_string(n2) + "/c" + std::to_string(s->nb_cells) +
-           "/w" + std::to_string(s->opt_window) + "/r" + std::to_string(s->opt_remap) + "/s" +
+           "/w" + std::to_string(s->opt_window) + "/r" + std::to_string(s->opt_remap) + "/q" + std::to_string(s->opt_queues) + "/s" +
            std::to_string(s->sp.simulation_scale != 1.0f ? 1 : 0) + "/i" + std::to_string(s->integrator);
 }
 
@@ -649,6 +652,8 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
         RTPS_TRY_HIP(dalloc(&s->start, C));
         RTPS_TRY_HIP(dalloc(&s->end, C));
         RTPS_TRY_HIP(dalloc(&s->d_num, 1));
+        RTPS_TRY_HIP(dalloc(&s->tile_q, kTileQueueWords));
+        RTPS_TRY_HIP(hipMemsetAsync(s->tile_q, 0, kTileQueueWords * sizeof(int), s->stream));
         if ((size_t)s->nb_cells * kSubCells * sizeof(uint32_t) <= ((size_t)2 << 30))   // at most 2 GiB
             RTPS_TRY_HIP(dalloc(&s->sub_start, (size_t)s->nb_cells * kSubCells));
         RTPS_TRY_HIP(hipMalloc(&s->sort_tmp, sort_temp_bytes((int)N, s->key_bits)));
@@ -1261,6 +1266,8 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_TILE_REMAP:
             if (value < 0 || value > 4096) return set_error(RTPS_ERR_ARG, "tile remap must be 0, 1 or a chunk of 2..4096");
             s->opt_remap = value;
+            s->graph_valid = false;
+            s->timed_graph_valid = false;
             break;
         case RTPS_OPT_SORT_TILE:
             if (value != 0 && !sort_config_valid(value / 100, value % 100))
@@ -1271,6 +1278,12 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_SURFACE_FRAME:
             if (value != 0 && value != 1) return set_error(RTPS_ERR_ARG, "surface frame must be 0 (world) or 1 (reference)");
             s->opt_frame = value;
+            break;
+        case RTPS_OPT_TILE_QUEUES:
+            if (value != 0 && value != 1) return set_error(RTPS_ERR_ARG, "tile queues option must be 0 or 1");
+            s->opt_queues = value;
+            s->graph_valid = false;
+            s->timed_graph_valid = false;
             break;
         case RTPS_OPT_BLOCK_DENSITY:
         case RTPS_OPT_BLOCK_FORCE:

@@ -307,7 +307,14 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *   RTPS_OPT_TILE_REMAP  tile order of the neighbour kernels over the 8 XCDs:
  *                        0 = hardware round-robin (default), 1 = one
  *                        contiguous range per XCD, C in 2..4096 = runs of C
- *                        consecutive tiles per XCD, interleaved */
+ *                        consecutive tiles per XCD, interleaved
+ *   RTPS_OPT_TILE_QUEUES 1 = the density and force kernels run as resident
+ *                        grids taking their tiles from eight queues, one per
+ *                        XCD over a contiguous eighth of the sorted particles,
+ *                        a drained queue's blocks moving on to the next (each
+ *                        XCD's L2 then serves one stretch of the neighbour
+ *                        rows); 0 = one tile per block in RTPS_OPT_TILE_REMAP
+ *                        order.  The same results either way. */
 #define RTPS_OPT_COLLISION 0
 #define RTPS_OPT_GRAPH 1
 #define RTPS_OPT_FORCE_OUT 2
@@ -322,6 +329,7 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
 #define RTPS_OPT_BLOCK_DENSITY 11
 #define RTPS_OPT_BLOCK_FORCE 12
 #define RTPS_OPT_SURFACE_FRAME 13
+#define RTPS_OPT_TILE_QUEUES 14
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))

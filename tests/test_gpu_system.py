@@ -327,6 +327,52 @@ def test_force_loop_variants_agree(cuda, scene):
         np.testing.assert_array_equal(out[v][1], out[0][1], err_msg=f"loop {v}")
 
 
+def test_tile_queues_give_the_same_step(cuda):
+    """RTPS_OPT_TILE_QUEUES: the density and force kernels run as resident
+    grids over eight per-XCD tile queues and give the one-tile-per-block
+    step's bits for each force loop and block size, over several steps (each
+    launch's last block clears the counters for the next), through a deletion
+    step (device-side count), with fewer tiles than resident blocks, on the
+    split (triangle-free euler) path, and when autotuned."""
+    from enjaparticles_amd._native import OPT_TILE_QUEUES
+    cases = [("dam", scenes.dam16k(), {}), ("small", scenes.dam_custom(10, 8, 6, seed=91), {}),
+             ("euler", scenes.dam16k(), {"integrator": "euler"})]
+    for name, sc, extra in cases:
+        settings = dict(sc.settings, **extra)
+        for bs, loop in ((64, 5), (128, 4), (256, 0)):
+            out = []
+            for q in (0, 1):
+                s = RTPS.generateSystemInstance(settings)
+                s.pushParticles(sc.pos, sc.vel)
+                s.setOption(4, loop)
+                s.setOption(11, bs)
+                s.setOption(12, bs)
+                s.setOption(OPT_TILE_QUEUES, q)
+                if name == "dam" and bs == 64:
+                    s.testDelete()   # the next update deletes 40 particles
+                s.step(4)
+                out.append((s.getNum(), s.read("pos"), s.read("vel"), s.read("density")))
+                s.close()
+            assert out[0][0] == out[1][0], (name, bs)
+            for a, b in zip(out[0][1:], out[1][1:]):
+                np.testing.assert_array_equal(b, a, err_msg=f"{name} block {bs} loop {loop}")
+    s = _system(scenes.dam16k())   # the variant option is process-wide: restore the default
+    s.setOption(4, DEFAULT_FORCE_LOOP)
+    s.close()
+    sc = scenes.dam16k()
+    a = _system(sc)
+    a.setOption(OPT_TILE_QUEUES, 1)
+    a.autotune(measure=True)
+    a.step(3)
+    b = _system(sc)
+    b.step(4)
+    np.testing.assert_array_equal(a.read("pos"), b.read("pos"))
+    with pytest.raises(RTPSError):
+        a.setOption(OPT_TILE_QUEUES, 2)
+    a.close()
+    b.close()
+
+
 def test_fixed_launch_options(cuda):
     """RTPS_OPT_BLOCK_DENSITY / RTPS_OPT_BLOCK_FORCE fix the neighbour launches
     (no autotuning): any block size gives the autotuned step's bits; other
