@@ -625,7 +625,7 @@ def main():
             "frac": achieved / PEAK_FP32_TFLOPS,
             "peak_nonpacked": PEAK_VALU_FP32_TFLOPS,
             "frac_nonpacked": achieved / PEAK_VALU_FP32_TFLOPS,
-            "traffic": measured_traffic(),
+            "traffic": measured_traffic("k_force", n, int(tune.get("block_force", 64))),
             "traffic_unit": "HBM bytes per launch on the timed state (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
             "algorithmic_bytes": 100.0 * n,
             "flop_per_launch": force_flop,

@@ -76,8 +76,7 @@ namespace {
 // One forward step of every scene from state slot `in` to `out`, recording into r.
 int forward_step(rtps_badj* a, int in, int out, BRecord& r, hipStream_t st) {
     const int T = a->T;
-    RTPS_TRY_HIP(launch_hash_b(T, a->sid, a->scenes, a->d_num, a->pos[in], a->keys, a->idx, r.start, a->C,
-                               a->queues, st));
+    RTPS_TRY_HIP(launch_hash_b(T, a->sid, a->scenes, a->d_num, a->pos[in], a->keys, a->idx, r.start, a->C, st));
     uint32_t *ks, *is;
     RTPS_TRY_HIP(sort_pairs_passes(T, a->L.shifts, a->L.widths, a->L.npasses, a->keys, a->idx, a->sort_tmp, st, &ks,
                                    &is));
@@ -89,7 +88,7 @@ int forward_step(rtps_badj* a, int in, int out, BRecord& r, hipStream_t st) {
                                 a->n_items, a->queues, a->scenes, a->d_num, r.pos_s, r.start, r.end, r.keys, r.ve_s,
                                 r.vel_s, r.idx, nullptr, nullptr, nullptr, st));
     RTPS_TRY_HIP(launch_pairs_b(kPairsForceStore, pairs_default_cfg(kPairsForceStore), a->force_grid, a->items,
-                                a->n_items, a->queues + kXcdQueues, a->scenes, a->d_num, r.pos_s, r.start, r.end,
+                                a->n_items, a->queues, a->scenes, a->d_num, r.pos_s, r.start, r.end,
                                 r.keys, r.ve_s, r.vel_s, r.idx, nullptr, nullptr, nullptr, st, r.force_s, a->xsph));
     RTPS_TRY_HIP(launch_badj_integrate(a->n_items, a->items, a->scenes, r.pos_s, r.vel_s, r.force_s, a->xsph, r.idx,
                                        r.start, a->pos[out], a->vel[out], a->ve[out], a->left_grid, st));
@@ -136,7 +135,8 @@ int rtps_badj_create(int num_scenes, const int* n_settings, const char* const* k
         const size_t N = (size_t)a->T, C = (size_t)a->C;
         RTPS_TRY_HIP(a->alloc(&a->scenes, (size_t)num_scenes));
         RTPS_TRY_HIP(a->alloc(&a->d_num, (size_t)num_scenes));
-        RTPS_TRY_HIP(a->alloc(&a->queues, (size_t)kQueueInts));
+        RTPS_TRY_HIP(a->alloc(&a->queues, (size_t)kTileQueueWords));
+        RTPS_TRY_HIP(hipMemset(a->queues, 0, kTileQueueWords * sizeof(int)));   // each launch's last block re-clears
         RTPS_TRY_HIP(a->alloc(&a->item_first, (size_t)num_scenes + 1));
         RTPS_TRY_HIP(a->alloc(&a->left_grid, 1));
         RTPS_TRY_HIP(a->alloc(&a->sid, N));

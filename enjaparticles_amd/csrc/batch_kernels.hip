@@ -43,10 +43,9 @@ __global__ __launch_bounds__(kBBlock) void k_hash_b(int T, const uint16_t* __res
                                                     const SceneDev* __restrict__ scenes, const int* __restrict__ d_num,
                                                     const float4* __restrict__ pos_u, uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ idx, uint32_t* __restrict__ start_clear,
-                                                    int n_clear, int* __restrict__ queues) {
+                                                    int n_clear) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_clear) start_clear[i] = kEmpty;
-    if (i < kQueueInts) queues[i] = 0;
     if (i >= T) return;
     const int s = sid[i];
     const SceneDev& S = scenes[s];
@@ -282,30 +281,12 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_pairs_b(const BatchItem* __res
                                                      float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
                                                      float4* __restrict__ force_out, float4* __restrict__ xsph_out,
                                                      const uint32_t* __restrict__ sub) {
-    __shared__ int sh_item;
     __shared__ float4 stage_b[BLOCK / 64][2 * kBStageCap];   // one 6 KB slice per wave
-    const int q0 = (int)(blockIdx.x % kXcdQueues);
-    int q = q0;   // thread 0's current queue
-    for (;;) {
-        if (threadIdx.x == 0) {
-            int item = -1;
-            for (;;) {
-                const int lo = (int)((long)n_items * q / kXcdQueues);
-                const int hi = (int)((long)n_items * (q + 1) / kXcdQueues);
-                const int t = atomicAdd(&queue[q], 1);
-                if (lo + t < hi) {
-                    item = lo + t;
-                    break;
-                }
-                q = (q + 1) % kXcdQueues;   // drained: the next queue (drained queues stay drained)
-                if (q == q0) break;         // all eight drained
-            }
-            sh_item = item;
-        }
-        __syncthreads();
-        const int it = __builtin_amdgcn_readfirstlane(sh_item);
-        __syncthreads();
-        if (it < 0) break;   // block-uniform: every block leaves once all queues are drained
+    int q = (int)(blockIdx.x % kTileQueues);   // thread 0's current queue
+    // the loop has one block-uniform exit (it < 0): every block leaves once all queues are drained
+    for (int it = first_tile(n_items);;) {
+        if (it < 0) it = claim_tile(queue, n_items, q);
+        if (it < 0) break;
         const BatchItem I = items[it];
         const int s = __builtin_amdgcn_readfirstlane(I.scene);
         const SceneDev& S = scenes[s];
@@ -315,7 +296,9 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_pairs_b(const BatchItem* __res
         if ((int)threadIdx.x < I.count && i - S.off < num)
             pairs_one<MODE>(i, s, S, deleted, pos_s, start, end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
                             force_out, xsph_out, sub, stage_b[threadIdx.x >> 6]);
+        it = -1;
     }
+    finish_tiles(queue);
 }
 
 __global__ void k_commit_b(int S, const SceneDev* __restrict__ scenes, int* __restrict__ d_num,
@@ -327,11 +310,11 @@ __global__ void k_commit_b(int S, const SceneDev* __restrict__ scenes, int* __re
 }
 
 hipError_t launch_hash_b(int T, const uint16_t* sid, const SceneDev* scenes, const int* d_num, const float4* pos_u,
-                         uint32_t* keys, uint32_t* idx, uint32_t* start, int n_cells, int* queues, hipStream_t st) {
+                         uint32_t* keys, uint32_t* idx, uint32_t* start, int n_cells, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     const int n = T > n_cells ? T : n_cells;
-    k_hash_b<<<bblocks(n > kQueueInts ? n : kQueueInts), kBBlock, 0, st>>>(T, sid, scenes, d_num, pos_u, keys, idx,
-                                                                         start, n_cells, queues);
+    if (n <= 0) return hipSuccess;
+    k_hash_b<<<bblocks(n), kBBlock, 0, st>>>(T, sid, scenes, d_num, pos_u, keys, idx, start, n_cells);
     return hipGetLastError();
 }
 
@@ -345,7 +328,7 @@ hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* sce
 int pairs_blocks_per_cu(const PairsCfg& c) { return c.waves * 256 / c.block; }
 
 bool pairs_cfg_valid(int mode, const PairsCfg& c) {
-    if (c.block != 128 && c.block != 256) return false;
+    if (c.block != 64 && c.block != 128 && c.block != 256) return false;
     return mode == kPairsDensity ? (c.waves >= 4 && c.waves <= 6) : (c.waves == 3 || c.waves == 4);
 }
 
@@ -378,7 +361,8 @@ hipError_t launch_pairs_b(int mode, const PairsCfg& cfg, int grid, const BatchIt
     };
     auto by_block = [&](auto m) {
         if (cfg.block == 256) by_waves(m, std::integral_constant<int, 256>{});
-        else by_waves(m, std::integral_constant<int, 128>{});
+        else if (cfg.block == 128) by_waves(m, std::integral_constant<int, 128>{});
+        else by_waves(m, std::integral_constant<int, 64>{});
     };
     switch (mode) {
         case kPairsDensity: by_block(std::integral_constant<int, kPairsDensity>{}); break;

@@ -14,6 +14,7 @@ struct BatchLayout {
     std::vector<SceneDev> scenes;   // offsets: scene capacities laid end to end
     std::vector<BatchItem> items;   // 256-slot tiles of every scene, in scene order
     std::vector<BatchItem> items128;   // the same in 128-slot tiles
+    std::vector<BatchItem> items64;    // and in 64-slot tiles
     std::vector<int> item_first;    // first item of scene s (num_scenes + 1 entries)
     long T = 0, total_cells = 0;
     int shifts[4] = {}, widths[4] = {}, npasses = 0;   // sort plan of the segmented keys

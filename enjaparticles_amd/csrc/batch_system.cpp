@@ -40,10 +40,11 @@ struct rtps_batch {
            *ve_s = nullptr;
     uint32_t *keys = nullptr, *idx = nullptr, *start = nullptr, *end = nullptr;
     uint32_t *keys_sorted = nullptr, *idx_sorted = nullptr;
-    // work items of 256 and of 128 slots (the two tile sizes of the pair kernels)
+    // work items of 256, 128 and 64 slots (the tile sizes of the pair kernels)
     BatchItem* items = nullptr;
     BatchItem* items128 = nullptr;
-    int n_items = 0, n_items128 = 0, cus = 256;
+    BatchItem* items64 = nullptr;
+    int n_items = 0, n_items128 = 0, n_items64 = 0, cus = 256;
     // launch configurations of the density and force pair kernels (the autotuner's choice)
     PairsCfg cfg_density = pairs_default_cfg(kPairsDensity), cfg_force = pairs_default_cfg(kPairsForceStep);
     int opt_autotune = 1;
@@ -63,7 +64,7 @@ struct rtps_batch {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         void* bufs[] = {scenes, d_num, sid, pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s, keys, idx,
-                        start, end, items, items128, queues, sort_tmp, sub};
+                        start, end, items, items128, items64, queues, sort_tmp, sub};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         for (auto& e : ev)
@@ -91,8 +92,12 @@ void mark(rtps_batch* b, int i) {
 }
 
 // Work items and persistent grid of a pair-kernel configuration.
-const BatchItem* items_of(const rtps_batch* b, const PairsCfg& c) { return c.block == 128 ? b->items128 : b->items; }
-int nitems_of(const rtps_batch* b, const PairsCfg& c) { return c.block == 128 ? b->n_items128 : b->n_items; }
+const BatchItem* items_of(const rtps_batch* b, const PairsCfg& c) {
+    return c.block == 64 ? b->items64 : (c.block == 128 ? b->items128 : b->items);
+}
+int nitems_of(const rtps_batch* b, const PairsCfg& c) {
+    return c.block == 64 ? b->n_items64 : (c.block == 128 ? b->n_items128 : b->n_items);
+}
 int grid_of(const rtps_batch* b, const PairsCfg& c) {
     return std::max(1, std::min(nitems_of(b, c), b->cus * pairs_blocks_per_cu(c)));
 }
@@ -106,7 +111,7 @@ int launch_density_b(rtps_batch* b, const PairsCfg& c) {
 
 int launch_force_b(rtps_batch* b, const PairsCfg& c) {
     RTPS_TRY_HIP(launch_pairs_b(kPairsForceStep, c, grid_of(b, c), items_of(b, c), nitems_of(b, c),
-                                b->queues + kXcdQueues, b->scenes, b->d_num, b->pos_s, b->start, b->end,
+                                b->queues, b->scenes, b->d_num, b->pos_s, b->start, b->end,
                                 b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted, b->pos_u, b->vel_u, b->veleval_u,
                                 b->stream, nullptr, nullptr, b->sub));
     return RTPS_OK;
@@ -115,7 +120,7 @@ int launch_force_b(rtps_batch* b, const PairsCfg& c) {
 int launch_front_b(rtps_batch* b) {
     mark(b, B_HASH);
     RTPS_TRY_HIP(launch_hash_b(b->T, b->sid, b->scenes, b->d_num, b->pos_u, b->keys, b->idx, b->start,
-                               b->total_cells, b->queues, b->stream));
+                               b->total_cells, b->stream));
     mark(b, B_SORT);
     RTPS_TRY_HIP(sort_pairs_passes(b->T, b->shifts, b->widths, b->npasses, b->keys, b->idx, b->sort_tmp, b->stream,
                                    &b->keys_sorted, &b->idx_sorted));
@@ -143,8 +148,8 @@ int launch_step(rtps_batch* b) {
 
 // ---------------------------------------------------------------------------
 // Autotuner of the pair kernels (no reference counterpart): threads per block
-// (= slots per work item, 128 or 256) and register budget (4, 5 or 6 waves per
-// SIMD) of the density and the force kernel, timed on the live sorted state
+// (= slots per work item, 64, 128 or 256) and register budget (4, 5 or 6 waves
+// per SIMD for density, 3 or 4 for force) of the density and the force kernel, timed on the live sorted state
 // of the first step; each kernel keeps its fastest configuration (a
 // configuration tried later must be 3% faster).  Both kernels are idempotent
 // on one sorted state (density rewrites veleval_s.w from the positions; the
@@ -232,19 +237,18 @@ int autotune_step_b(rtps_batch* b, bool measure) {
         (void)hipEventDestroy(e0);
         return hip_error(err, "hipEventCreate");
     }
-    const std::vector<PairsCfg> cands[2] = {{{256, 5}, {256, 4}, {256, 6}, {128, 5}, {128, 4}, {128, 6}},
-                                            {{256, 4}, {256, 3}, {128, 4}, {128, 3}}};
+    const std::vector<PairsCfg> cands[2] = {
+        {{256, 5}, {256, 4}, {256, 6}, {128, 5}, {128, 4}, {128, 6}, {64, 5}, {64, 4}, {64, 6}},
+        {{256, 4}, {256, 3}, {128, 4}, {128, 3}, {64, 4}, {64, 3}}};
     constexpr int kReps = 3;
     constexpr float kMargin = 0.97f;
     const char* verbose_env = std::getenv("RTPS_TUNE_VERBOSE");
     const bool verbose = verbose_env && verbose_env[0] == '1';
-    // ms per launch of kReps launches after one untimed one, each after its
-    // queue counters are cleared; < 0 on error
+    // ms per launch of kReps launches after one untimed one (each launch's
+    // last block clears the queue counters for the next); < 0 on error
     auto time_it = [&](int which, const PairsCfg& c) -> float {
         float total = 0.f;
         for (int r = 0; r <= kReps; ++r) {
-            if (launch_fill_u32((uint32_t*)(b->queues + which * kXcdQueues), 0u, kXcdQueues, b->stream) != hipSuccess)
-                return -1.f;
             if (hipEventRecord(e0, b->stream) != hipSuccess) return -1.f;
             if ((which == 0 ? launch_density_b(b, c) : launch_force_b(b, c)) != RTPS_OK) return -1.f;
             if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.f;
@@ -270,9 +274,7 @@ int autotune_step_b(rtps_batch* b, bool measure) {
             }
         }
         if (which == 0 && !failed) {   // the force kernel reads the densities of the chosen density launch
-            if (launch_fill_u32((uint32_t*)b->queues, 0u, kXcdQueues, b->stream) != hipSuccess ||
-                launch_density_b(b, best[0]) != RTPS_OK)
-                failed = true;
+            if (launch_density_b(b, best[0]) != RTPS_OK) failed = true;
         }
     }
     (void)hipEventDestroy(e0);
@@ -284,7 +286,6 @@ int autotune_step_b(rtps_batch* b, bool measure) {
     b->graph_valid = false;
     btune_put(key, best[0], best[1]);
     // finish the step: the force step with the chosen configuration, then the count commit
-    RTPS_TRY_HIP(launch_fill_u32((uint32_t*)(b->queues + kXcdQueues), 0u, kXcdQueues, b->stream));
     rc = launch_force_b(b, b->cfg_force);
     if (rc) return rc;
     RTPS_TRY_HIP(launch_commit_b(b->S, b->scenes, b->d_num, b->start, b->stream));
@@ -382,12 +383,14 @@ int build_batch_layout(int num_scenes, const int* n_settings, const char* const*
     // work items: 256-slot tiles of every scene
     L.items.clear();
     L.items128.clear();
+    L.items64.clear();
     L.item_first.assign(num_scenes + 1, 0);
     for (int s = 0; s < num_scenes; ++s) {
         const SceneDev& d = L.scenes[s];
         L.item_first[s] = (int)L.items.size();
         for (int f = 0; f < d.cap; f += 256) L.items.push_back(BatchItem{s, d.off + f, std::min(256, d.cap - f), 0});
         for (int f = 0; f < d.cap; f += 128) L.items128.push_back(BatchItem{s, d.off + f, std::min(128, d.cap - f), 0});
+        for (int f = 0; f < d.cap; f += 64) L.items64.push_back(BatchItem{s, d.off + f, std::min(64, d.cap - f), 0});
     }
     L.item_first[num_scenes] = (int)L.items.size();
     return RTPS_OK;
@@ -421,6 +424,7 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
         const std::vector<BatchItem>& items = L.items;
         b->n_items = (int)items.size();
         b->n_items128 = (int)L.items128.size();
+        b->n_items64 = (int)L.items64.size();
         int dev = 0;
         if (hipGetDevice(&dev) == hipSuccess) {
             hipDeviceProp_t prop;
@@ -451,7 +455,8 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
             RTPS_TRY_HIP(hipMalloc((void**)&b->sub, (size_t)cells * kSubCells * 4));
         RTPS_TRY_HIP(hipMalloc((void**)&b->items, sizeof(BatchItem) * items.size()));
         RTPS_TRY_HIP(hipMalloc((void**)&b->items128, sizeof(BatchItem) * L.items128.size()));
-        RTPS_TRY_HIP(hipMalloc((void**)&b->queues, kQueueInts * sizeof(int)));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->items64, sizeof(BatchItem) * L.items64.size()));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->queues, kTileQueueWords * sizeof(int)));
         RTPS_TRY_HIP(hipMalloc(&b->sort_tmp, sort_temp_bytes((int)T, 32)));
         RTPS_TRY_HIP(hipMemcpyAsync(b->scenes, b->scenes_h.data(), sizeof(SceneDev) * num_scenes,
                                     hipMemcpyHostToDevice, b->stream));
@@ -461,7 +466,9 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
                                     b->stream));
         RTPS_TRY_HIP(hipMemcpyAsync(b->items128, L.items128.data(), sizeof(BatchItem) * L.items128.size(),
                                     hipMemcpyHostToDevice, b->stream));
-        RTPS_TRY_HIP(hipMemsetAsync(b->queues, 0, kQueueInts * sizeof(int), b->stream));
+        RTPS_TRY_HIP(hipMemcpyAsync(b->items64, L.items64.data(), sizeof(BatchItem) * L.items64.size(),
+                                    hipMemcpyHostToDevice, b->stream));
+        RTPS_TRY_HIP(hipMemsetAsync(b->queues, 0, kTileQueueWords * sizeof(int), b->stream));
         RTPS_TRY_HIP(launch_fill_u32(b->start, 666u, cells, b->stream));
         RTPS_TRY_HIP(launch_fill_u32(b->end, 666u, cells, b->stream));
         RTPS_TRY_HIP(sort_clear_temp(b->sort_tmp, (int)T, b->stream));
@@ -612,7 +619,7 @@ int rtps_batch_set_option(rtps_batch* b, int option, int value) {
         case RTPS_OPT_BLOCK_FORCE: {   // value = block * 10 + waves, e.g. 2565 (256 threads, 5 waves per SIMD)
             const PairsCfg c{value / 10, value % 10};
             if (!pairs_cfg_valid(option == RTPS_OPT_BLOCK_DENSITY ? kPairsDensity : kPairsForceStep, c))
-                return set_error(RTPS_ERR_ARG, "pair kernel launch: block*10 + waves with block 128 or 256, waves 4..6 "
+                return set_error(RTPS_ERR_ARG, "pair kernel launch: block*10 + waves with block 64, 128 or 256, waves 4..6 "
                                                "(density) or 3..4 (force)");
             (option == RTPS_OPT_BLOCK_DENSITY ? b->cfg_density : b->cfg_force) = c;
             b->tuned = true;   // fixed by the caller

@@ -10,10 +10,9 @@ namespace rtps {
 
 constexpr uint32_t kSceneShift = 24;   // segmented key: (scene << 24) | local key
 constexpr int kMaxScenes = 256;
-// Work-item queues of the persistent neighbour kernels: one per XCD (items
-// split into 8 contiguous ranges), for each of the two kernels of a step.
-constexpr int kXcdQueues = 8;
-constexpr int kQueueInts = 2 * kXcdQueues;
+// The persistent pair kernels take their work items from the per-XCD tile
+// queues of sph_device.h (kTileQueueWords ints, zero between launches; each
+// launch's last block clears them), one set shared by the kernels of a step.
 
 struct SceneDev {
     rtps_sph_params sp;          // SPHParams of the scene
@@ -34,13 +33,13 @@ struct BatchItem {
 };
 
 hipError_t launch_hash_b(int T, const uint16_t* sid, const SceneDev* scenes, const int* d_num, const float4* pos_u,
-                         uint32_t* keys, uint32_t* idx, uint32_t* start, int n_cells, int* queues, hipStream_t st);
+                         uint32_t* keys, uint32_t* idx, uint32_t* start, int n_cells, hipStream_t st);
 // sub (optional): sub-cell start table, 16 entries per cell of every scene
 hipError_t launch_cellindices_b(int T, const uint32_t* keys, const SceneDev* scenes, uint32_t* start, uint32_t* end,
                                 hipStream_t st, uint32_t* sub = nullptr);
 constexpr int kPairsDensity = 0, kPairsForceStep = 1, kPairsForceStore = 2;
 // Launch configuration of a persistent pair kernel: `block` threads per block
-// (= slots per work item: 128 or 256) and the register budget of `waves`
+// (= slots per work item: 64, 128 or 256) and the register budget of `waves`
 // waves per SIMD (3, 4, 5 or 6: at most 168, 128, 96 or 80 VGPRs); the grid
 // is CUs x waves x 256 / block blocks.  The density kernel is built for 4, 5
 // and 6 waves; the force kernels for 3 and 4 only, since their live values

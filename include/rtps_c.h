@@ -485,15 +485,16 @@ int rtps_batch_read(rtps_batch* b, int scene, int field, void* host, int count);
  * RTPS_OPT_AUTOTUNE (1 = the first rtps_batch_step tunes the pair kernels,
  * default; see rtps_batch_autotune) and RTPS_OPT_BLOCK_DENSITY /
  * RTPS_OPT_BLOCK_FORCE = block * 10 + waves (e.g. 2565: 256 threads per block
- * = slots per work item, compiled for 5 waves per SIMD; block 128 or 256,
+ * = slots per work item, compiled for 5 waves per SIMD; block 64, 128 or 256,
  * waves 4..6 for density and 3..4 for force, whose registers would spill at
  * 5 waves), which fix that kernel's launch. */
 int rtps_batch_set_option(rtps_batch* b, int option, int value);
 /* Autotuner of the batched pair kernels (no reference counterpart): runs one
  * step (as rtps_batch_step(b, 1)) during which the density and the force
- * kernel are each timed on the live state over block sizes 128 and 256
+ * kernel are each timed on the live state over block sizes 64, 128 and 256
  * (slots per work item) and register budgets of 4, 5 and 6 waves per SIMD
- * (density) or 3 and 4 (force), and keep their fastest (a configuration tried later must be 3% faster).
+ * (density) or 3 and 4 (force), and keep their fastest (a configuration
+ * tried later must be 3% faster).
  * Cached per device and layout size class (and in RTPS_TUNE_CACHE when set);
  * measure = 0 takes a cached choice.  Every configuration computes the same
  * step.  Reports the configurations in use. */
