@@ -467,7 +467,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--chunk", type=int, default=100, help="timed steps between pair counts")
-    ap.add_argument("--no-stage-timers", action="store_true", help="time the graph without per-stage events")
+    ap.add_argument("--no-stage-timers", action="store_true", help="time the graph without any events")
+    ap.add_argument("--stage-steps", type=int, default=20,
+                    help="steps after the timed ones with events around every stage (per-stage breakdown)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-single-steps", type=int, default=2)
@@ -514,7 +516,9 @@ def main():
     s.pushParticles(sc.pos, sc.vel)
     s.setOption(N.OPT_GRAPH, 1)
     s.setOption(N.OPT_FORCE_VARIANT, args.force_variant)
-    s.setOption(N.OPT_TIMERS, 0 if args.no_stage_timers else 1)
+    # the timed steps record HIP events around the force stage only (RTPS_OPT_TIMERS 2, two events per
+    # step inside the graph); the per-stage breakdown comes from the steps that follow
+    s.setOption(N.OPT_TIMERS, 0 if args.no_stage_timers else 2)
     if args.launch:
         bd, bf, loop = (int(v) for v in args.launch.split(","))
         s.setOption(N.OPT_FORCE_VARIANT, loop)
@@ -528,9 +532,10 @@ def main():
     s.synchronize()
     print(f"# warmup {args.warmup} steps done", file=sys.stderr, flush=True)
 
-    # Timed region: `steps` graph replays in chunks; each step records its
-    # per-stage HIP events on the handle's stream inside the graph.  The pair
-    # counts of the FLOP model are taken between chunks, outside the timing.
+    # Timed region: `steps` graph replays in chunks; each step records HIP
+    # events around its force kernel on the handle's stream inside the graph.
+    # The pair counts of the FLOP model are taken between chunks, outside the
+    # timing.
     s.resetTimers()
     counts = [s.countPairs()]
     elapsed, done = 0.0, 0
@@ -559,7 +564,15 @@ def main():
         torch.cuda.synchronize()
         print("# reference copy: 268435456 bytes read + written", file=sys.stderr, flush=True)
         return
-    stages = s.stageTimes() if not args.no_stage_timers else {}
+    force_timed = s.stageTimes()["force"] if not args.no_stage_timers else None
+    stages = {}
+    if not args.no_stage_timers and args.stage_steps > 0:
+        # per-stage breakdown: the next stage_steps steps with events around every stage
+        s.setOption(N.OPT_TIMERS, 1)
+        s.resetTimers()
+        s.step(args.stage_steps)
+        s.synchronize()
+        stages = s.stageTimes()
     s.setOption(N.OPT_TIMERS, 0)
     # pair counts per particle, averaged over the chunk boundaries (trapezoid)
     sizes = [min(args.chunk, args.steps - c * args.chunk) for c in range(len(counts) - 1)]
@@ -588,34 +601,10 @@ def main():
         "pairs_per_particle": {"candidates_27cell": Kp, "accepted_incl_self": Mp, "candidates_tested": Ktp},
         "autotune": tune,
     }
-    if stages:
-        # HBM side: algorithmic (minimum) bytes of each stage over its measured time
-        import numpy as np
-        cells = s.nb_cells + 1
-        occupied = int(np.count_nonzero(s.read("cell_start") != np.uint32(0xFFFFFFFF)))
-        key_bits = (s.nb_cells + 1).bit_length() + 4      # cell key bits (incl. pad) + 4 sub-cell bits
-        passes = (key_bits + 7) // 8
-        alg = {
-            "hash": 24.0 * n + 4.0 * cells,                 # pos read, key + idx written, start[] cleared
-            "sort": passes * 20.0 * n,                      # per pass: keys read (histogram), pairs read + written
-            "cellindices": 4.0 * n + 8.0 * occupied,        # keys read, start/end of occupied cells written
-            "permute": 100.0 * n,                           # idx + 3 float4 gathered, 3 float4 written
-            "density": 20.0 * n,                            # own position read, density written (neighbours: cache)
-            "force": 100.0 * n,                             # pos, veleval|density, vel, idx read; 3 float4 written
-        }
-        hbm = {}
-        for k, b in alg.items():
-            ms = stages[k][0]
-            gbs = b / (ms * 1e-3) / 1e9
-            hbm[k] = {"bytes": b, "ms": round(ms, 5), "GBps": round(gbs, 1), "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4)}
-        step_bytes = sum(alg.values())
-        hbm["step"] = {"bytes": step_bytes, "ms": round(ms_per_step, 5),
-                       "GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
-                       "frac_of_peak": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
-        out["hbm"] = {"peak_GBps": PEAK_HBM_GBS, "occupied_cells": occupied, "stages": hbm}
-        force_ms, dens_ms = stages["force"][0], stages["density"][0]
+    if force_timed is not None and force_timed[1] > 0:
+        # the dominant kernel: HIP events around the force stage of every timed step
+        force_ms = force_timed[0]
         achieved = force_flop / (force_ms * 1e-3) / 1e12
-        out["stages_ms"] = {k: round(v[0], 5) for k, v in stages.items()}
         out["roofline"] = {
             "bound": "valu",
             "kernel": "k_force (fused force + walls + leapfrog)",
@@ -630,8 +619,43 @@ def main():
             "algorithmic_bytes": 100.0 * n,
             "flop_per_launch": force_flop,
             "avg_launch_ms": force_ms,
-            "launches": stages["force"][1],
+            "launches": force_timed[1],
         }
+    if stages:
+        # per-stage breakdown (the steps right after the timed ones); HBM side:
+        # algorithmic (minimum) bytes of each stage over its measured time
+        import numpy as np
+        cells = s.nb_cells + 1
+        occupied = int(np.count_nonzero(s.read("cell_start") != np.uint32(0xFFFFFFFF)))
+        key_bits = (s.nb_cells + 1).bit_length() + 4      # cell key bits (incl. pad) + 4 sub-cell bits
+        passes = (key_bits + 7) // 8
+        # the step runs the hash inside the sort's first kernel and the permute
+        # inside the cellindices kernel: those stage pairs are reported together
+        alg = {
+            # pos read, key + idx written, start[] cleared; per pass pairs read + written
+            "hash+sort": (("hash", "sort"), 24.0 * n + 4.0 * cells + passes * 16.0 * n),
+            # keys read, start/end of occupied cells written; idx + 3 float4 gathered, 3 float4 written
+            "cellindices+permute": (("cellindices", "permute"), 4.0 * n + 8.0 * occupied + 100.0 * n),
+            # own position read, density written (neighbours: cache)
+            "density": (("density",), 20.0 * n),
+            # pos, veleval|density, vel, idx read; 3 float4 written
+            "force": (("force",), 100.0 * n),
+        }
+        hbm = {}
+        for k, (names, b) in alg.items():
+            ms = sum(stages[q][0] for q in names)
+            gbs = b / (ms * 1e-3) / 1e9
+            hbm[k] = {"bytes": b, "ms": round(ms, 5), "GBps": round(gbs, 1), "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4)}
+        alg = {k: b for k, (_, b) in alg.items()}
+        step_bytes = sum(alg.values())
+        hbm["step"] = {"bytes": step_bytes, "ms": round(ms_per_step, 5),
+                       "GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                       "frac_of_peak": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+        out["hbm"] = {"peak_GBps": PEAK_HBM_GBS, "occupied_cells": occupied, "stages": hbm}
+        first = args.warmup + args.steps + 1
+        out["stages_ms"] = {k: round(v[0], 5) for k, v in stages.items()}
+        out["stages_steps"] = [first, first + args.stage_steps - 1]
+        dens_ms = stages["density"][0]
         out["density_roofline"] = {
             "achieved_tflops": dens_flop / (dens_ms * 1e-3) / 1e12,
             "frac": dens_flop / (dens_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
@@ -640,7 +664,8 @@ def main():
         }
     if args.long_steps > 0 and not args.profile:
         print("# long-run leg ...", file=sys.stderr, flush=True)
-        out["long_run"] = long_run_leg(s, args.long_steps, args.warmup + args.steps + 1)
+        out["long_run"] = long_run_leg(s, args.long_steps,
+                                       args.warmup + args.steps + (args.stage_steps if stages else 0) + 1)
         print(f"# long run: {out['long_run']['value']:.3e} p-u/s over steps {out['long_run']['first_step']}-"
               f"{out['long_run']['last_step']}", file=sys.stderr, flush=True)
     if args.surface_res > 0 and not args.profile:

@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include "rtps_internal.h"
+#include "sph_device.h"
 
 namespace rtps {
 
@@ -152,6 +153,63 @@ __global__ __launch_bounds__(kHistThreads) void k_radix_hist_all(const uint32_t*
     uint32_t kk[IT];
 #pragma unroll
     for (int k = 0; k < IT; ++k) kk[k] = kin[min(base + k * kHistThreads + threadIdx.x, (long)n - 1)];
+    for (int p = 0; p < plan.npasses; ++p) {
+        const int shift = plan.shift[p], bits = plan.bits[p];
+        const uint32_t mask = (1u << bits) - 1u;
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const bool valid = base + k * kHistThreads + threadIdx.x < n;
+            const uint32_t d = valid ? (kk[k] >> shift) & mask : 0u;
+            const uint64_t peers = match_digit(d, bits, valid);
+            if (valid && (peers & lt) == 0ull) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
+        }
+    }
+    __syncthreads();
+    const int copy = (int)(blockIdx.x % kCountCopies);
+    for (int e = threadIdx.x; e < plan.npasses * kMaxRadix; e += kHistThreads) {
+        const uint32_t c = (&h[0][0])[e];
+        const int p = e / kMaxRadix, d = e % kMaxRadix;
+        if (c) atomicAdd(&gcount[((size_t)p * kCountCopies + copy) * kMaxRadix + d], c);
+    }
+}
+
+// The first kernel of the system's step: the keys of hash.cl computed from
+// the positions (hash_key; keys past num get pad_key) and written with
+// idx = i as the sort's input, start[0..n_clear) cleared (the per-step host
+// clear of CellIndices.cpp:59-62), and the digit histograms of every pass
+// taken from the keys still in registers (k_radix_hist_all without reading
+// the keys back).
+template <int IT>
+__global__ __launch_bounds__(kHistThreads) void k_hash_hist_all(const float4* __restrict__ pos_u, int num_host,
+                                                                const int* __restrict__ d_num, int n,
+                                                                uint32_t pad_key, Grid g, int sub_bits,
+                                                                uint32_t* __restrict__ keys,
+                                                                uint32_t* __restrict__ idx,
+                                                                uint32_t* __restrict__ start_clear, long n_clear,
+                                                                PassPlan plan, uint32_t* __restrict__ gcount,
+                                                                uint32_t* __restrict__ ctr,
+                                                                uint32_t* __restrict__ status0, long status_words) {
+    __shared__ uint32_t h[kMaxPasses][kMaxRadix];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int e = threadIdx.x; e < kMaxPasses * kMaxRadix; e += kHistThreads) (&h[0][0])[e] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kMaxPasses) ctr[threadIdx.x] = 0;
+    const long stride = (long)gridDim.x * kHistThreads;
+    for (long e = (long)blockIdx.x * kHistThreads + threadIdx.x; e < status_words; e += stride) status0[e] = 0;
+    for (long e = (long)blockIdx.x * kHistThreads + threadIdx.x; e < n_clear; e += stride) start_clear[e] = kEmpty;
+    const int num = d_num ? *d_num : num_host;
+    __syncthreads();
+    const long base = (long)blockIdx.x * (kHistThreads * IT);
+    uint32_t kk[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const long e = base + k * kHistThreads + threadIdx.x;
+        kk[k] = e < num ? hash_key(pos_u[e], g, sub_bits) : pad_key;
+        if (e < n) {
+            keys[e] = kk[k];
+            idx[e] = (uint32_t)e;
+        }
+    }
     for (int p = 0; p < plan.npasses; ++p) {
         const int shift = plan.shift[p], bits = plan.bits[p];
         const uint32_t mask = (1u << bits) - 1u;
@@ -351,12 +409,18 @@ hipError_t sort_clear_temp(void* temp, int n, hipStream_t st) {
 // (keys, vals) or in the temp buffer's alternate arrays, returned through
 // out_keys / out_vals.  The temp buffer's digit counts must be zero on entry
 // (sort_clear_temp once; every sort leaves them zero again).
-hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int npasses, uint32_t* keys,
-                             uint32_t* vals, void* temp, hipStream_t st, uint32_t** out_keys, uint32_t** out_vals) {
+namespace {
+
+// The passes of sort_pairs_passes after a first kernel (`first`, given the
+// plan, the temp layout and the histogram grid) that has written the digit
+// counts of every pass, cleared the tile counters and status array 0, and
+// left the input keys in `keys`.
+template <typename First>
+hipError_t sort_passes_after(int n, const int* shifts, const int* widths, int npasses, uint32_t* keys, uint32_t* vals,
+                             void* temp, hipStream_t st, uint32_t** out_keys, uint32_t** out_vals, First first) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     *out_keys = keys;
     *out_vals = vals;
-    if (n <= 1 || npasses == 0) return hipSuccess;
     if (npasses > kMaxPasses || n > kMaxSortKeys) return hipErrorInvalidValue;
     SortTemp t = carve(temp, n);
     PassPlan plan{};
@@ -377,7 +441,7 @@ hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int np
     const int hist_blocks = ntiles_for(n, kHistThreads * 4);
     uint32_t* st0 = t.status;
     uint32_t* st1 = t.status + t.status_words;
-    k_radix_hist_all<4><<<hist_blocks, kHistThreads, 0, st>>>(keys, n, plan, t.gcount, t.ctr, st0, t.status_words);
+    first(plan, t, hist_blocks);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     uint32_t *ki = keys, *vi = vals, *ko = t.kalt, *vo = t.valt;
@@ -406,6 +470,44 @@ hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int np
     *out_keys = ki;
     *out_vals = vi;
     return hipSuccess;
+}
+
+}  // namespace
+
+// Sorts (keys, vals) by the bits [shift[p], shift[p] + width[p]) of each pass
+// p (least significant range first, every width <= 8).  The result ends in
+// (keys, vals) or in the temp buffer's alternate arrays, returned through
+// out_keys / out_vals.  The temp buffer's digit counts must be zero on entry
+// (sort_clear_temp once; every sort leaves them zero again).
+hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int npasses, uint32_t* keys,
+                             uint32_t* vals, void* temp, hipStream_t st, uint32_t** out_keys, uint32_t** out_vals) {
+    *out_keys = keys;
+    *out_vals = vals;
+    if (n <= 1 || npasses == 0) return hipSuccess;
+    return sort_passes_after(n, shifts, widths, npasses, keys, vals, temp, st, out_keys, out_vals,
+                             [&](const PassPlan& plan, const SortTemp& t, int hist_blocks) {
+                                 k_radix_hist_all<4><<<hist_blocks, kHistThreads, 0, st>>>(
+                                     keys, n, plan, t.gcount, t.ctr, t.status, t.status_words);
+                             });
+}
+
+hipError_t hash_sort_pairs(int n, int key_bits, const float4* pos_u, int num_host, const int* d_num, uint32_t pad_key,
+                           const Grid& g, int sub_bits, uint32_t* start_clear, int n_clear, uint32_t* keys,
+                           uint32_t* vals, void* temp, hipStream_t st, uint32_t** out_keys, uint32_t** out_vals) {
+    *out_keys = keys;
+    *out_vals = vals;
+    int shifts[kMaxPasses], widths[kMaxPasses];
+    const int np = sort_plan(key_bits, shifts, widths);
+    if (np < 0) return hipErrorInvalidValue;
+    if (n <= 1 || np == 0) {   // nothing to sort: the hash and the clear alone
+        return launch_hash(num_host, d_num, n, pad_key, pos_u, keys, vals, g, start_clear, n_clear, sub_bits, st);
+    }
+    return sort_passes_after(n, shifts, widths, np, keys, vals, temp, st, out_keys, out_vals,
+                             [&](const PassPlan& plan, const SortTemp& t, int hist_blocks) {
+                                 k_hash_hist_all<4><<<hist_blocks, kHistThreads, 0, st>>>(
+                                     pos_u, num_host, d_num, n, pad_key, g, sub_bits, keys, vals, start_clear,
+                                     (long)n_clear, plan, t.gcount, t.ctr, t.status, t.status_words);
+                             });
 }
 
 // Passes over the low key_bits bits, split evenly into digits of <= 8 bits;

@@ -45,6 +45,11 @@ hipError_t launch_read_nc(const uint32_t* start, int nb_cells, int* nc, hipStrea
 hipError_t launch_permute(int num, const int* d_num, const uint32_t* idx, const float4* pu, const float4* vu,
                           const float4* eu, float4* ps, float4* vs, float4* es, hipStream_t st,
                           const float4* cu = nullptr, float4* cs = nullptr);
+// launch_cellindices and launch_permute in one kernel (the system's step)
+hipError_t launch_cells_permute(int num, const int* d_num, const uint32_t* keys, int nb_cells, uint32_t* start,
+                                uint32_t* end, int sub_bits, uint32_t* sub, const uint32_t* idx, const float4* pu,
+                                const float4* vu, const float4* eu, float4* ps, float4* vs, float4* es,
+                                hipStream_t st, const float4* cu = nullptr, float4* cs = nullptr);
 hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
                           const rtps_sph_params& sp, const Grid& g, const Win& win, float* dens, float4* packed,
                           hipStream_t st);
@@ -149,6 +154,12 @@ hipError_t sort_pairs_inplace(int n, int key_bits, uint32_t* keys, uint32_t* val
 hipError_t sort_pairs_passes(int n, const int* shifts, const int* widths, int npasses, uint32_t* keys,
                              uint32_t* vals, void* temp, hipStream_t st, uint32_t** out_keys, uint32_t** out_vals);
 int sort_plan(int key_bits, int* shifts, int* widths);
+// launch_hash (keys of pos_u, pad_key past num, start[0..n_clear) cleared)
+// and sort_pairs_pingpong of its (keys, idx) with the hash fused into the
+// sort's histogram kernel: one kernel fewer per step.
+hipError_t hash_sort_pairs(int n, int key_bits, const float4* pos_u, int num_host, const int* d_num, uint32_t pad_key,
+                           const Grid& g, int sub_bits, uint32_t* start_clear, int n_clear, uint32_t* keys,
+                           uint32_t* vals, void* temp, hipStream_t st, uint32_t** out_keys, uint32_t** out_vals);
 hipError_t sort_segments(int batch, int length, int dir, uint32_t* dkeys, uint32_t* dvals, const uint32_t* skeys,
                          const uint32_t* svals, void* temp, hipStream_t st);
 

@@ -62,6 +62,27 @@ __device__ __forceinline__ int grid_hash(int gx, int gy, int gz, const Grid& g) 
     return sat_f2i(v);
 }
 
+// hash.cl:50-157: the linear cell of p, clamped to nb_cells (hash.cl:91-92).
+// sub_bits > 0 (system path): the key is (cell << sub_bits) | xq with xq the
+// x position inside the cell in 1/2^sub_bits (0 for the clamped cell).
+__device__ __forceinline__ uint32_t hash_key(const float4& p, const Grid& g, int sub_bits) {
+    const float ux = (p.x - g.min.x) * g.delta.x;
+    const int cx = sat_f2i(ux);
+    const int cy = sat_f2i((p.y - g.min.y) * g.delta.y);
+    const int cz = sat_f2i((p.z - g.min.z) * g.delta.z);
+    int h = grid_hash(cx, cy, cz, g);
+    const bool out = h > g.nb_cells || h < 0;
+    h = h > g.nb_cells ? g.nb_cells : h;   // hash.cl:91
+    h = h < 0 ? g.nb_cells : h;            // hash.cl:92
+    uint32_t key = (uint32_t)h;
+    if (sub_bits) {
+        int xq = sat_f2i((ux - (float)cx) * (float)(1 << sub_bits));
+        xq = out || h == g.nb_cells ? 0 : min(max(xq, 0), (1 << sub_bits) - 1);
+        key = (key << sub_bits) | (uint32_t)xq;
+    }
+    return key;
+}
+
 // Particle count for the kernels after cellindices: the out-of-grid deletion
 // path (SPH.cpp:220-250) shrinks num to nc = start[nb_cells] when
 // 0 <= nc <= num.  Returns true when the deletion path is active.

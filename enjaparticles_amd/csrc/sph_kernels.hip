@@ -37,22 +37,7 @@ __global__ __launch_bounds__(kBlock) void k_hash(int num_host, const int* __rest
     if (i < n_clear) start_clear[i] = kEmpty;
     const int num = d_num ? *d_num : num_host;
     if (i < num) {
-        float4 p = pos_u[i];
-        const float ux = (p.x - g.min.x) * g.delta.x;
-        const int cx = sat_f2i(ux);
-        const int cy = sat_f2i((p.y - g.min.y) * g.delta.y);
-        const int cz = sat_f2i((p.z - g.min.z) * g.delta.z);
-        int h = grid_hash(cx, cy, cz, g);
-        const bool out = h > g.nb_cells || h < 0;
-        h = h > g.nb_cells ? g.nb_cells : h;   // hash.cl:91
-        h = h < 0 ? g.nb_cells : h;            // hash.cl:92
-        uint32_t key = (uint32_t)h;
-        if (sub_bits) {
-            int xq = sat_f2i((ux - (float)cx) * (float)(1 << sub_bits));
-            xq = out || h == g.nb_cells ? 0 : min(max(xq, 0), (1 << sub_bits) - 1);
-            key = (key << sub_bits) | (uint32_t)xq;
-        }
-        keys[i] = key;
+        keys[i] = hash_key(pos_u[i], g, sub_bits);
         idx[i] = (uint32_t)i;
     } else if (i < n_pad) {
         keys[i] = pad_key;
@@ -63,13 +48,9 @@ __global__ __launch_bounds__(kBlock) void k_hash(int num_host, const int* __rest
 // a4: cell start/end by boundary detection on the sorted keys (semantics of
 // cellindices.cl:12-107: start[k] = first i with key k, end[k] = one past the
 // last; keys > nb_cells are ignored).
-__global__ __launch_bounds__(kBlock) void k_cellindices(int num_host, const int* __restrict__ d_num,
-                                                        const uint32_t* __restrict__ keys, uint32_t nb_cells,
-                                                        uint32_t* __restrict__ start, uint32_t* __restrict__ end,
-                                                        int sub_bits, uint32_t* __restrict__ sub) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int num = d_num ? *d_num : num_host;
-    if (i >= num) return;
+__device__ __forceinline__ void cellindices_one(int i, int num, const uint32_t* __restrict__ keys, uint32_t nb_cells,
+                                                uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                int sub_bits, uint32_t* __restrict__ sub) {
     uint32_t h = keys[i] >> sub_bits;
     if (h > nb_cells) return;
     if (sub && h < nb_cells) {
@@ -96,11 +77,32 @@ __global__ __launch_bounds__(kBlock) void k_cellindices(int num_host, const int*
     if (i == num - 1) end[h] = (uint32_t)num;
 }
 
+__global__ __launch_bounds__(kBlock) void k_cellindices(int num_host, const int* __restrict__ d_num,
+                                                        const uint32_t* __restrict__ keys, uint32_t nb_cells,
+                                                        uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                        int sub_bits, uint32_t* __restrict__ sub) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int num = d_num ? *d_num : num_host;
+    if (i >= num) return;
+    cellindices_one(i, num, keys, nb_cells, start, end, sub_bits, sub);
+}
+
 __global__ void k_read_nc(const uint32_t* __restrict__ start, int nb_cells, int* __restrict__ nc) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *nc = (int)start[nb_cells];
 }
 
 // a5: permute.cl:10-46 (pos, vel, veleval).
+__device__ __forceinline__ void permute_one(int i, const uint32_t* __restrict__ idx, const float4* __restrict__ pu,
+                                            const float4* __restrict__ vu, const float4* __restrict__ eu,
+                                            float4* __restrict__ ps, float4* __restrict__ vs, float4* __restrict__ es,
+                                            const float4* __restrict__ cu, float4* __restrict__ cs) {
+    const uint32_t s = idx[i];
+    ps[i] = pu[s];
+    vs[i] = vu[s];
+    es[i] = eu[s];
+    if (cu) cs[i] = cu[s];   // permute.cl:41 (colours, when the system keeps them)
+}
+
 __global__ __launch_bounds__(kBlock) void k_permute(int num_host, const int* __restrict__ d_num,
                                                     const uint32_t* __restrict__ idx, const float4* __restrict__ pu,
                                                     const float4* __restrict__ vu, const float4* __restrict__ eu,
@@ -110,11 +112,25 @@ __global__ __launch_bounds__(kBlock) void k_permute(int num_host, const int* __r
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int num = d_num ? *d_num : num_host;
     if (i >= num) return;
-    const uint32_t s = idx[i];
-    ps[i] = pu[s];
-    vs[i] = vu[s];
-    es[i] = eu[s];
-    if (cu) cs[i] = cu[s];   // permute.cl:41 (colours, when the system keeps them)
+    permute_one(i, idx, pu, vu, eu, ps, vs, es, cu, cs);
+}
+
+// a4 + a5 in one pass over the sorted order (the system's step): both read
+// the sorted arrays at i and neither reads what the other writes.
+__global__ __launch_bounds__(kBlock) void k_cells_permute(int num_host, const int* __restrict__ d_num,
+                                                          const uint32_t* __restrict__ keys, uint32_t nb_cells,
+                                                          uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                          int sub_bits, uint32_t* __restrict__ sub,
+                                                          const uint32_t* __restrict__ idx,
+                                                          const float4* __restrict__ pu, const float4* __restrict__ vu,
+                                                          const float4* __restrict__ eu, float4* __restrict__ ps,
+                                                          float4* __restrict__ vs, float4* __restrict__ es,
+                                                          const float4* __restrict__ cu, float4* __restrict__ cs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int num = d_num ? *d_num : num_host;
+    if (i >= num) return;
+    permute_one(i, idx, pu, vu, eu, ps, vs, es, cu, cs);
+    cellindices_one(i, num, keys, nb_cells, start, end, sub_bits, sub);
 }
 
 // ---------------------------------------------------------------------------
@@ -283,6 +299,11 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, const int* __
     int num = d_num ? *d_num : num_host;
     bool deleted = false;
     if (d_num) deleted = effective_num(num, start, g.nb_cells, num);
+    // FUSED: the step's last kernel commits the count of a deletion step for
+    // the next step (SPH.cpp:220-250).  Blocks that read d_num before or
+    // after this store get the same count: effective_num of the committed
+    // count is that count.
+    if (FUSED && deleted && tile == 0 && threadIdx.x == 0) *const_cast<int*>(d_num) = num;
     if (i >= num) return;
     const float scale = sp.simulation_scale;
     const float h = sp.smoothing_distance, h2 = h * h, eps = sp.EPSILON;
@@ -700,6 +721,18 @@ hipError_t launch_cellindices(int num, const int* d_num, const uint32_t* keys, i
     if (num <= 0) return hipSuccess;
     k_cellindices<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, keys, (uint32_t)nb_cells, start, end, sub_bits,
                                                        sub_bits == kSubBits ? sub : nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_permute(int num, const int* d_num, const uint32_t* keys, int nb_cells, uint32_t* start,
+                                uint32_t* end, int sub_bits, uint32_t* sub, const uint32_t* idx, const float4* pu,
+                                const float4* vu, const float4* eu, float4* ps, float4* vs, float4* es,
+                                hipStream_t st, const float4* cu, float4* cs) {
+    (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
+    if (num <= 0) return hipSuccess;
+    k_cells_permute<<<blocks_for(num), kBlock, 0, st>>>(num, d_num, keys, (uint32_t)nb_cells, start, end, sub_bits,
+                                                        sub_bits == kSubBits ? sub : nullptr, idx, pu, vu, eu, ps,
+                                                        vs, es, cu, cs);
     return hipGetLastError();
 }
 

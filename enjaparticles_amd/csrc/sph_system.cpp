@@ -215,6 +215,7 @@ struct StageTimer {
     explicit StageTimer(rtps_sph* s_, int set_ = 0) : s(s_), on(s_->opt_timers != 0), set(set_) {}
     void mark(int i) {
         if (!on) return;
+        if (s->opt_timers == 2 && i != ST_FORCE && i != ST_N) return;   // force stage only
         hipError_t e = external ? hipEventRecordWithFlags(s->ev[set][i], s->stream, hipEventRecordExternal)
                                 : hipEventRecord(s->ev[set][i], s->stream);
         if (e != hipSuccess && err == hipSuccess) err = e;
@@ -225,19 +226,21 @@ struct StageTimer {
 int launch_front(rtps_sph* s, StageTimer& t) {
     const int n = s->num_host;
     s->sorted_ready = true;
+    // the hash runs inside the sort's first (histogram) kernel: the "sort"
+    // stage holds both, the "hash" stage is empty
     t.mark(ST_HASH);
-    RTPS_TRY_HIP(launch_hash(n, s->d_num, n, s->pad_key, s->pos_u, s->keys, s->idx, s->g, s->start,
-                             s->nb_cells + 1, kSubBits, s->stream));
     t.mark(ST_SORT);
-    RTPS_TRY_HIP(sort_pairs_pingpong(n, s->key_bits, s->keys, s->idx, s->sort_tmp, s->stream, &s->keys_sorted,
-                                     &s->idx_sorted));
+    RTPS_TRY_HIP(hash_sort_pairs(n, s->key_bits, s->pos_u, n, s->d_num, s->pad_key, s->g, kSubBits, s->start,
+                                 s->nb_cells + 1, s->keys, s->idx, s->sort_tmp, s->stream, &s->keys_sorted,
+                                 &s->idx_sorted));
+    // cellindices and permute in one kernel: the "cellindices" stage holds
+    // both, the "permute" stage is empty
     t.mark(ST_CELL);
-    RTPS_TRY_HIP(launch_cellindices(n, s->d_num, s->keys_sorted, s->nb_cells, s->start, s->end, kSubBits, s->stream,
-                                    sub_table_to_write(s)));
+    RTPS_TRY_HIP(launch_cells_permute(n, s->d_num, s->keys_sorted, s->nb_cells, s->start, s->end, kSubBits,
+                                      sub_table_to_write(s), s->idx_sorted, s->pos_u, s->vel_u, s->veleval_u,
+                                      s->pos_s, s->vel_s, s->ve_s, s->stream, col_u(s), col_s(s)));
     s->sub_valid = sub_table_to_write(s) != nullptr;
     t.mark(ST_PERMUTE);
-    RTPS_TRY_HIP(launch_permute(n, s->d_num, s->idx_sorted, s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s,
-                                s->ve_s, s->stream, col_u(s), col_s(s)));
     t.mark(ST_DENSITY);
     RTPS_TRY_HIP(launch_density(n, s->d_num, s->pos_s, s->start, s->end, s->sp, s->gs, win(s), nullptr, s->ve_s,
                                 s->stream));
@@ -294,6 +297,7 @@ int launch_back(rtps_sph* s) {
                                         win(s, true), fo ? s->force_s : nullptr, fo ? s->xsph_s : nullptr, s->vel_s,
                                         s->idx_sorted, s->gsrc, s->opt_collision, s->gps, s->dt, s->pos_u, s->vel_u,
                                         s->veleval_u, s->stream, col_s(s), col_u(s)));
+        return RTPS_OK;   // the fused kernel commits the count of a deletion step itself
     }
     RTPS_TRY_HIP(launch_commit_num(s->d_num, s->start, s->nb_cells, s->stream));
     return RTPS_OK;
@@ -315,7 +319,7 @@ int collect_timers(rtps_sph* s, int set) {
     if (!s->ev_pending[set]) return RTPS_OK;
     s->ev_pending[set] = false;
     RTPS_TRY_HIP(hipEventSynchronize(s->ev[set][ST_N]));
-    for (int i = 0; i < ST_N; ++i) {
+    for (int i = s->opt_timers == 2 ? (int)ST_FORCE : 0; i < ST_N; ++i) {
         float ms = 0.f;
         RTPS_TRY_HIP(hipEventElapsedTime(&ms, s->ev[set][i], s->ev[set][i + 1]));
         s->stage_ms[i] += ms;
@@ -1252,7 +1256,11 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_COLLISION: s->opt_collision = value ? 1 : 0; break;
         case RTPS_OPT_GRAPH: s->opt_graph = value ? 1 : 0; break;
         case RTPS_OPT_FORCE_OUT: s->opt_force_out = value ? 1 : 0; break;
-        case RTPS_OPT_TIMERS: s->opt_timers = value ? 1 : 0; break;
+        case RTPS_OPT_TIMERS:
+            if (value < 0 || value > 2) return set_error(RTPS_ERR_ARG, "timers option must be 0, 1 or 2");
+            if (value != s->opt_timers) s->timed_graph_valid = false;   // the timed graphs hold the events
+            s->opt_timers = value;
+            break;
         case RTPS_OPT_FORCE_VARIANT:
             if (value != 0 && value != 4 && value != 5)
                 return set_error(RTPS_ERR_ARG, "force loop must be 0 (direct), 4 (staged rows) or 5 (staged rows "

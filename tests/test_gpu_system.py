@@ -253,24 +253,35 @@ def test_addbox_matches_addrect(cuda, nn, lo, hi):
 
 def test_stage_timers_do_not_change_the_step(cuda):
     """Per-stage timers (events inside the step graph, two alternating sets)
-    time every step and leave the results bit-identical."""
+    time every step and leave the results bit-identical; the force-only mode
+    (RTPS_OPT_TIMERS 2) times just the force stage.  The hash runs inside the
+    sort's first kernel and the permute inside the cellindices kernel, so
+    those two stages are empty."""
     sc = scenes.dam_custom(24, 24, 12, seed=78)
     a = _system(sc, graph=True)
     b = _system(sc, graph=True)
     b.setOption(3, 1)
     c = _system(sc, graph=False)
     c.setOption(3, 1)
-    for s in (a, b, c):
+    d = _system(sc, graph=True)
+    d.setOption(3, 2)
+    for s in (a, b, c, d):
         s.step(1)          # the first step tunes the launches (not timed)
         s.resetTimers()
         s.step(7)
-    np.testing.assert_array_equal(b.read("pos"), a.read("pos"))
-    np.testing.assert_array_equal(c.read("pos"), a.read("pos"))
+    for s in (b, c, d):
+        np.testing.assert_array_equal(s.read("pos"), a.read("pos"))
     for s in (b, c):
         st = s.stageTimes()
         assert all(cnt == 7 for _, cnt in st.values()), st
-        assert all(ms > 0.0 for ms, _ in st.values()), st
-    for s in (a, b, c):
+        assert all(ms > 0.0 for k, (ms, _) in st.items() if k not in ("hash", "permute")), st
+        assert st["hash"][0] >= 0.0 and st["permute"][0] >= 0.0, st
+    st = d.stageTimes()
+    assert st["force"][1] == 7 and st["force"][0] > 0.0, st
+    assert all(cnt == 0 for k, (_, cnt) in st.items() if k != "force"), st
+    with pytest.raises(RTPSError):
+        d.setOption(3, 3)
+    for s in (a, b, c, d):
         s.close()
 
 
