@@ -437,8 +437,10 @@ def compare4m_leg(steps: int = 100, launch: str = ""):
     s.setOption(N.OPT_TIMERS, 1)
     s.resetTimers()
     s.step(5)
+    K, M, _ = s.countPairs()
     res["plain"] = {"particles": sc.n, "value": sc.n * steps / tp, "ms_per_step": 1e3 * tp / steps,
-                    "stages_ms": {k: round(v[0], 4) for k, v in s.stageTimes().items()}}
+                    "stages_ms": {k: round(v[0], 4) for k, v in s.stageTimes().items()},
+                    "pairs_per_particle_at_end": {"candidates_27cell": K / sc.n, "accepted_incl_self": M / sc.n}}
     s.close()
     scs = scenes.batched_equal(64, 64, 64, 16, seed=1007)
     total = sum(x.n for x in scs)
@@ -457,6 +459,18 @@ def compare4m_leg(steps: int = 100, launch: str = ""):
     res["batched"] = {"scenes": 64, "particles": total, "value": total * steps / tb, "ms_per_step": 1e3 * tb / steps,
                       "stages_ms": {k: round(v[0], 4) for k, v in b.stageTimes().items()}}
     b.close()
+    # The two workloads evolve differently (the slabs are 16 particles deep, the
+    # dam 128): neighbour counts of one scene, stepped alone as a plain system
+    # the same number of steps, put the two on a per-pair footing.
+    p0 = RTPS.generateSystemInstance(scs[0].settings)
+    p0.pushParticles(scs[0].pos, scs[0].vel)
+    p0.step(10 + steps + 5)
+    K0, M0, _ = p0.countPairs()
+    n0 = p0.getNum()
+    p0.close()
+    res["batched"]["pairs_per_particle_at_end_scene0"] = {"candidates_27cell": K0 / n0, "accepted_incl_self": M0 / n0}
+    mp = res["plain"]["pairs_per_particle_at_end"]["accepted_incl_self"]
+    res["pair_updates_ratio_batched_over_plain"] = (res["batched"]["value"] * M0 / n0) / (res["plain"]["value"] * mp)
     res["steps"] = steps
     return res
 

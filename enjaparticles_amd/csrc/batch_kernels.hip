@@ -225,6 +225,39 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                 }
             }
         };
+        // the staged copies, a lane's window in chunks of up to 32 candidates:
+        // the tests of a chunk set one bit per accepted candidate, then the
+        // pair terms run once per set bit, lowest first (the same ascending
+        // order and arithmetic as walk, so the same sums)
+        auto walk_bits = [&](uint32_t a, uint32_t b, uint32_t off) {
+            for (uint32_t j0 = a; j0 < b; j0 += 32u) {
+                const uint32_t n = min(b - j0, 32u);
+                const float4* __restrict__ pc = stage + (j0 - off);
+                uint32_t m = 0u, k = 0u;
+                auto test = [&](float4 pj, uint32_t t) {
+                    const float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
+                    const float r2 = rx * rx + ry * ry + rz * rz;
+                    m |= (r2 <= h2 && j0 + t != (uint32_t)i) ? (1u << t) : 0u;
+                };
+                for (; k + kGroup <= n; k += kGroup) {
+                    float4 q[kGroup];
+#pragma unroll
+                    for (int t = 0; t < kGroup; ++t) q[t] = pc[k + t];
+#pragma unroll
+                    for (int t = 0; t < kGroup; ++t) test(q[t], k + (uint32_t)t);
+                }
+                for (; k < n; ++k) test(pc[k], k);
+                while (m != 0u) {
+                    const uint32_t t = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1u;
+                    const float4 pj = pc[t];
+                    const float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
+                    const float r2 = rx * rx + ry * ry + rz * rz;
+                    const float4 vj = stage_v[j0 + t - off];
+                    force_pair_r(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, vj.w, pj.w);
+                }
+            }
+        };
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b, A, B;
@@ -239,7 +272,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                         stage_v[t] = e;
                     }
                     __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
-                    walk(stage, stage_v, a, b, A, true);
+                    walk_bits(a, b, A);
                     __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
                 } else {
                     walk(pos_s, ve, a, b, 0u, false);
