@@ -478,8 +478,8 @@ def compare4m_leg(steps: int = 100, launch: str = ""):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--chunk", type=int, default=100, help="timed steps between pair counts")
     ap.add_argument("--no-stage-timers", action="store_true", help="time the graph without any events")
     ap.add_argument("--stage-steps", type=int, default=20,
