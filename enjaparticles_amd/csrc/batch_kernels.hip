@@ -226,30 +226,35 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
             }
         };
         // the staged copies, a lane's window in chunks of up to 32 candidates:
-        // the tests of a chunk set one bit per accepted candidate, then the
-        // pair terms run once per set bit, lowest first (the same ascending
-        // order and arithmetic as walk, so the same sums)
+        // the tests of a chunk shift one bit per candidate into a mask (the
+        // chunk's first candidate ends in its highest bit), the particle's own
+        // bit is cleared, then the pair terms run once per set bit, highest
+        // first (the same ascending order and arithmetic as walk, so the same
+        // sums)
         auto walk_bits = [&](uint32_t a, uint32_t b, uint32_t off) {
             for (uint32_t j0 = a; j0 < b; j0 += 32u) {
                 const uint32_t n = min(b - j0, 32u);
                 const float4* __restrict__ pc = stage + (j0 - off);
                 uint32_t m = 0u, k = 0u;
-                auto test = [&](float4 pj, uint32_t t) {
+                auto test = [&](float4 pj) {
                     const float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
                     const float r2 = rx * rx + ry * ry + rz * rz;
-                    m |= (r2 <= h2 && j0 + t != (uint32_t)i) ? (1u << t) : 0u;
+                    m = m + m + (r2 <= h2 ? 1u : 0u);
                 };
                 for (; k + kGroup <= n; k += kGroup) {
                     float4 q[kGroup];
 #pragma unroll
                     for (int t = 0; t < kGroup; ++t) q[t] = pc[k + t];
 #pragma unroll
-                    for (int t = 0; t < kGroup; ++t) test(q[t], k + (uint32_t)t);
+                    for (int t = 0; t < kGroup; ++t) test(q[t]);
                 }
-                for (; k < n; ++k) test(pc[k], k);
+                for (; k < n; ++k) test(pc[k]);
+                const uint32_t self = (uint32_t)i - j0;   // the particle itself passes (r = 0): drop it
+                if (self < n) m &= ~(1u << (n - 1u - self));
                 while (m != 0u) {
-                    const uint32_t t = (uint32_t)__builtin_ctz(m);
-                    m &= m - 1u;
+                    const uint32_t hb = 31u - (uint32_t)__builtin_clz(m);
+                    m ^= 1u << hb;
+                    const uint32_t t = n - 1u - hb;
                     const float4 pj = pc[t];
                     const float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
                     const float r2 = rx * rx + ry * ry + rz * rz;

@@ -430,34 +430,39 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, const int* __
             }
         };
         // Loop 5 on the staged copies: the lane's window is taken in chunks of
-        // up to 32 candidates; the tests of a chunk set one bit per accepted
-        // candidate, then the pair terms run once per set bit, lowest first.
-        // The wave's pair iterations are its lanes' largest accepted count
-        // instead of their longest window, and the pairs are summed in the
-        // same ascending order as walk (the same bits).
+        // up to 32 candidates; the tests of a chunk shift one bit per
+        // candidate into a mask (the chunk's first candidate ends in its
+        // highest bit), the particle's own bit is cleared once per chunk, then
+        // the pair terms run once per set bit, highest first.  The wave's pair
+        // iterations are its lanes' largest accepted count instead of their
+        // longest window, and the pairs are summed in the same ascending order
+        // as walk (the same bits).
         auto walk_bits = [&](const float4* __restrict__ P, const float4* __restrict__ V, uint32_t a, uint32_t b,
                              uint32_t off) {
             for (uint32_t j0 = a; j0 < b; j0 += 32u) {
                 const uint32_t n = min(b - j0, 32u);
                 const float4* __restrict__ pc = P + (j0 - off);
                 uint32_t m = 0u, k = 0u;
-                auto test = [&](float4 pj, uint32_t t) {
+                auto test = [&](float4 pj) {
                     if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
                     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                     const float r2 = rx * rx + ry * ry + rz * rz;
-                    m |= (r2 <= h2 && j0 + t != (uint32_t)i) ? (1u << t) : 0u;
+                    m = m + m + (r2 <= h2 ? 1u : 0u);
                 };
                 for (; k + kGroup <= n; k += kGroup) {
                     float4 q[kGroup];
 #pragma unroll
                     for (int t = 0; t < kGroup; ++t) q[t] = pc[k + t];
 #pragma unroll
-                    for (int t = 0; t < kGroup; ++t) test(q[t], k + (uint32_t)t);
+                    for (int t = 0; t < kGroup; ++t) test(q[t]);
                 }
-                for (; k < n; ++k) test(pc[k], k);
+                for (; k < n; ++k) test(pc[k]);
+                const uint32_t self = (uint32_t)i - j0;   // the particle itself passes (r = 0): drop it
+                if (self < n) m &= ~(1u << (n - 1u - self));
                 while (m != 0u) {
-                    const uint32_t t = (uint32_t)__builtin_ctz(m);
-                    m &= m - 1u;
+                    const uint32_t hb = 31u - (uint32_t)__builtin_clz(m);
+                    m ^= 1u << hb;
+                    const uint32_t t = n - 1u - hb;
                     const float4 pq = pc[t];
                     float4 pj = pq;
                     if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
