@@ -491,6 +491,8 @@ def main():
     ap.add_argument("--launch", default="", metavar="BD,BF,LOOP",
                     help="fixed neighbour launches instead of autotuning: density and force threads per block and "
                          "the force loop (profiling runs: counters perturb the autotuner's timings)")
+    ap.add_argument("--tile-queues", type=int, default=0, choices=(0, 1),
+                    help="1 = density and force as resident grids over per-XCD tile queues (RTPS_OPT_TILE_QUEUES)")
     ap.add_argument("--force-variant", type=int, default=5,
                     help="process-default force loop before autotuning: 5 = LDS-staged rows with accepted-candidate "
                          "bits, 4 = LDS-staged rows, 0 = direct")
@@ -530,6 +532,7 @@ def main():
     s.pushParticles(sc.pos, sc.vel)
     s.setOption(N.OPT_GRAPH, 1)
     s.setOption(N.OPT_FORCE_VARIANT, args.force_variant)
+    s.setOption(N.OPT_TILE_QUEUES, args.tile_queues)
     # the timed steps record HIP events around the force stage only (RTPS_OPT_TIMERS 2, two events per
     # step inside the graph); the per-stage breakdown comes from the steps that follow
     s.setOption(N.OPT_TIMERS, 0 if args.no_stage_timers else 2)
