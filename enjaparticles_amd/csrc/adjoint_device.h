@@ -252,6 +252,43 @@ __device__ __forceinline__ void adj_b2_body(int i, const float4* __restrict__ po
     float4* __restrict__ sv = stage + kAdjStageCap;
     float4* __restrict__ sga = stage + 2 * kAdjStageCap;
     float4* __restrict__ sgx = stage + 3 * kAdjStageCap;
+    // the staged copies, a lane's window in chunks of up to 32 candidates:
+    // the tests of a chunk shift one bit per candidate into a mask (the
+    // chunk's first candidate ends in its highest bit), the particle's own
+    // bit is cleared, then the pair terms run once per set bit, highest
+    // first: the same pairs in the same ascending order as walk (the same
+    // sums), with the wave's pair iterations its lanes' largest accepted
+    // count instead of their longest window
+    auto walk_bits = [&](uint32_t a, uint32_t b, uint32_t off) {
+        for (uint32_t j0 = a; j0 < b; j0 += 32u) {
+            const uint32_t n = min(b - j0, 32u);
+            const float4* __restrict__ pc = sp_ + (j0 - off);
+            uint32_t mk = 0u, k = 0u;
+            auto test = [&](const float4 p) {
+                const float3 r = make_float3(pi.x - p.x, pi.y - p.y, pi.z - p.z);
+                mk = mk + mk + (dot3(r, r) <= h2 ? 1u : 0u);
+            };
+            for (; k + kGroup <= n; k += kGroup) {
+                float4 q[kGroup];
+#pragma unroll
+                for (int t = 0; t < kGroup; ++t) q[t] = pc[k + t];
+#pragma unroll
+                for (int t = 0; t < kGroup; ++t) test(q[t]);
+            }
+            for (; k < n; ++k) test(pc[k]);
+            const uint32_t self = (uint32_t)i - j0;   // the particle itself passes (r = 0): drop it
+            if (self < n) mk &= ~(1u << (n - 1u - self));
+            while (mk != 0u) {
+                const uint32_t hb = 31u - (uint32_t)__builtin_clz(mk);
+                mk ^= 1u << hb;
+                const uint32_t t = n - 1u - hb;
+                const float4 p = pc[t];
+                const float3 r = make_float3(pi.x - p.x, pi.y - p.y, pi.z - p.z);
+                const uint32_t o = j0 + t - off;
+                pair(r, dot3(r, r), sv[o], sga[o], sgx[o]);
+            }
+        }
+    };
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t a, b, A, B;
@@ -266,7 +303,7 @@ __device__ __forceinline__ void adj_b2_body(int i, const float4* __restrict__ po
                     sgx[t] = gX_s[A + t];
                 }
                 __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
-                walk(sp_, sv, sga, sgx, a, b, A);
+                walk_bits(a, b, A);
                 __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
             } else {
                 walk(pos_s, ve_s, ga_s, gX_s, a, b, 0u);
