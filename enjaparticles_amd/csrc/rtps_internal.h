@@ -53,10 +53,10 @@ hipError_t launch_cells_permute(int num, const int* d_num, const uint32_t* keys,
 hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const uint32_t* start, const uint32_t* end,
                           const rtps_sph_params& sp, const Grid& g, const Win& win, float* dens, float4* packed,
                           hipStream_t st);
-hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const float* dens, const float4* ve,
+hipError_t launch_force(int num, int* d_num, const float4* pos_s, const float* dens, const float4* ve,
                         bool packed, const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp,
                         const Grid& g, const Win& win, float4* force, float4* xsph, hipStream_t st);
-hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, const float4* ve_packed,
+hipError_t launch_force_fused(int num, int* d_num, const float4* pos_s, const float4* ve_packed,
                               const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
                               const Win& win, float4* force, float4* xsph, const float4* vel_s, const uint32_t* idx,
                               const GravitySources& gsrc, int collision, const rtps_grid_params& gps, float dt,

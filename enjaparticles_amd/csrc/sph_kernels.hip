@@ -283,7 +283,7 @@ constexpr int kStagedWavesPerSimd = 5;
 constexpr int kQueuedWavesPerSimd = 4;
 
 template <bool SCALED, bool PACKED, bool FUSED, int LOOP>
-__device__ __forceinline__ void force_tile(int tile, int num_host, const int* __restrict__ d_num,
+__device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
                                            const float4* __restrict__ pos_s, const float* __restrict__ dens_in,
                                            const float4* __restrict__ ve_in, const uint32_t* __restrict__ start,
                                            const uint32_t* __restrict__ end, const rtps_sph_params& sp, const Grid& g,
@@ -302,8 +302,9 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, const int* __
     // FUSED: the step's last kernel commits the count of a deletion step for
     // the next step (SPH.cpp:220-250).  Blocks that read d_num before or
     // after this store get the same count: effective_num of the committed
-    // count is that count.
-    if (FUSED && deleted && tile == 0 && threadIdx.x == 0) *const_cast<int*>(d_num) = num;
+    // count is that count.  (d_num is not __restrict__: it is read and
+    // written here.)
+    if (FUSED && deleted && tile == 0 && threadIdx.x == 0) *d_num = num;
     if (i >= num) return;
     const float scale = sp.simulation_scale;
     const float h = sp.smoothing_distance, h2 = h * h, eps = sp.EPSILON;
@@ -522,7 +523,7 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, const int* __
 // grid working through the per-XCD tile queues of win.tq (separate instances,
 // at four waves per SIMD: the loop around the body needs the registers).
 template <bool SCALED, bool PACKED, bool FUSED, int LOOP, bool QUEUED>
-__global__ __launch_bounds__(kBlock, QUEUED ? kQueuedWavesPerSimd : (LOOP >= kLoopStaged ? kStagedWavesPerSimd : kForceWavesPerSimd)) void k_force(int num_host, const int* __restrict__ d_num,
+__global__ __launch_bounds__(kBlock, QUEUED ? kQueuedWavesPerSimd : (LOOP >= kLoopStaged ? kStagedWavesPerSimd : kForceWavesPerSimd)) void k_force(int num_host, int* d_num,
                                                   const float4* __restrict__ pos_s,
                                                   const float* __restrict__ dens_in,
                                                   const float4* __restrict__ ve_in,
@@ -818,7 +819,7 @@ static inline size_t force_lds(int loop, int bs) {
     return loop == kLoopDirect ? 0 : (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4);
 }
 
-hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const float* dens, const float4* ve,
+hipError_t launch_force(int num, int* d_num, const float4* pos_s, const float* dens, const float4* ve,
                         bool packed, const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp,
                         const Grid& g, const Win& win_in, float4* force, float4* xsph, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
@@ -854,7 +855,7 @@ hipError_t launch_force(int num, const int* d_num, const float4* pos_s, const fl
     return hipGetLastError();
 }
 
-hipError_t launch_force_fused(int num, const int* d_num, const float4* pos_s, const float4* ve_packed,
+hipError_t launch_force_fused(int num, int* d_num, const float4* pos_s, const float4* ve_packed,
                               const uint32_t* start, const uint32_t* end, const rtps_sph_params& sp, const Grid& g,
                               const Win& win_in, float4* force, float4* xsph, const float4* vel_s,
                               const uint32_t* idx, const GravitySources& gsrc, int collision,
