@@ -34,13 +34,12 @@
 #include <hip/hip_runtime.h>
 
 #include "rtps_internal.h"
+#include "sort_layout.h"
 #include "sph_device.h"
 
 namespace rtps {
 
 constexpr int kHistThreads = 1024;  // threads of the histogram kernel
-constexpr int kMaxRadix = 256;
-constexpr int kMaxPasses = 4;
 
 // One-sweep tile, process-wide: threads * 100 + keys per thread (0 = by
 // size: 1024 x 4 from 2^19 keys up, else 1024 x 2)
@@ -57,75 +56,11 @@ bool sort_config_valid(int block, int it) {
 
 int sort_passes(int key_bits) { return key_bits <= 0 ? 0 : (key_bits + 7) / 8; }
 
-// tiles of `tile` keys
-static inline int ntiles_for(int n, int tile) { return (n + tile - 1) / tile; }
-constexpr int kMinTile = 1024;   // the smallest tile: sizes the status arrays
-
-// Status word of the look-back: flag in bits 31-30, count in bits 29-0.
-constexpr uint32_t kFlagAgg = 1u << 30, kFlagPrefix = 2u << 30, kCountMask = (1u << 30) - 1u;
 constexpr int kLookback = 8;   // status words of earlier tiles loaded together in the look-back
-// Copies of each pass's digit totals: histogram block b adds into copy
-// b % kCountCopies, so a total receives ntiles / kCountCopies atomics instead
-// of ntiles (same-address atomics from every CU serialise); readers add the
-// copies up.
-constexpr int kCountCopies = 16;
-constexpr int kMaxSortKeys = (1 << 30) - 1;
-
-struct SortTemp {
-    uint32_t* gcount;   // [kMaxPasses][kCountCopies][kMaxRadix], zero between sorts (offset 0: independent of n)
-    uint32_t* ctr;      // [kMaxPasses] tile counters
-    uint32_t* status;   // [2][status_words]
-    long status_words;  // tiles x kMaxRadix for the smallest tile
-    uint32_t* kalt;
-    uint32_t* valt;
-};
-
-static SortTemp carve(void* temp, int n) {
-    char* p = (char*)temp;
-    SortTemp t;
-    t.gcount = (uint32_t*)p;
-    size_t off = (size_t)kMaxPasses * kCountCopies * kMaxRadix * sizeof(uint32_t);
-    t.ctr = (uint32_t*)(p + off);
-    off += 256;
-    t.status_words = (long)ntiles_for(n > 0 ? n : 1, kMinTile) * kMaxRadix;
-    t.status = (uint32_t*)(p + off);
-    off += 2 * (size_t)t.status_words * sizeof(uint32_t);
-    off = (off + 255) & ~(size_t)255;
-    t.kalt = (uint32_t*)(p + off);
-    t.valt = t.kalt + n;
-    return t;
-}
 
 size_t sort_temp_bytes(int n, int key_bits) {
     (void)key_bits;
-    const long nn = n > 0 ? n : 1;
-    size_t b = (size_t)kMaxPasses * kCountCopies * kMaxRadix * sizeof(uint32_t) + 256;
-    b += 2 * (size_t)ntiles_for((int)nn, kMinTile) * kMaxRadix * sizeof(uint32_t);
-    b = (b + 255) & ~(size_t)255;
-    b += 2 * (size_t)nn * sizeof(uint32_t);   // alternate keys / values
-    return b;
-}
-
-// Lanes of the calling wave whose digit equals this lane's (restricted to `valid` lanes).
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, bool valid) {
-    uint64_t peers = __ballot(valid);
-    for (int b = 0; b < bits; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bb = __ballot(bit);
-        peers &= bit ? bb : ~bb;
-    }
-    return peers;
-}
-
-// Inclusive scan of v across the 64 lanes of a wave.
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
+    return sort_layout(n, nullptr, nullptr);
 }
 
 struct PassPlan {
@@ -400,8 +335,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t st) {
 // Zeroes the digit counts (once per temp buffer; every sort leaves them zero).
 hipError_t sort_clear_temp(void* temp, int n, hipStream_t st) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
-    SortTemp t = carve(temp, n);
-    return launch_fill_u32(t.gcount, 0u, (long)kMaxPasses * kCountCopies * kMaxRadix, st);
+    (void)n;
+    return launch_fill_u32((uint32_t*)temp, 0u, (long)(kSortZeroBytes / sizeof(uint32_t)), st);
 }
 
 // Sorts (keys, vals) by the bits [shift[p], shift[p] + width[p]) of each pass

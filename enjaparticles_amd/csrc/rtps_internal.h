@@ -163,4 +163,16 @@ hipError_t hash_sort_pairs(int n, int key_bits, const float4* pos_u, int num_hos
 hipError_t sort_segments(int batch, int length, int dir, uint32_t* dkeys, uint32_t* dvals, const uint32_t* skeys,
                          const uint32_t* svals, void* temp, hipStream_t st);
 
+// ---- bucket sort of the step's cell keys (bucket_sort.hip) ----
+// True when hash_bucket_sort takes n keys of key_bits bits.
+bool bucket_sort_fits(int n, int key_bits);
+// Temp bytes of hash_bucket_sort for every count up to n (at least
+// sort_temp_bytes(n, key_bits)).
+size_t bucket_temp_bytes(int n, int key_bits);
+// hash_sort_pairs's result (keys of pos_u, pad_key past num, start[0..n_clear)
+// cleared, (keys, idx) sorted stably) always left in (keys, vals).
+hipError_t hash_bucket_sort(int n, int key_bits, const float4* pos_u, int num_host, const int* d_num, uint32_t pad_key,
+                            const Grid& g, int sub_bits, uint32_t* start_clear, int n_clear, uint32_t* keys,
+                            uint32_t* vals, void* temp, hipStream_t st);
+
 }  // namespace rtps

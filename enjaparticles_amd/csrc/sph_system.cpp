@@ -59,6 +59,7 @@ struct rtps_sph {
     GravitySources gsrc{};
     int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0,
         opt_queues = 0;
+    int opt_bucket = 1;   // the step's keys: bucket sort where the key space allows it (1) or LSD radix sort (0)
     int opt_autotune = 1;
     int opt_color = 0;   // permute colours every step (set by the first non-default colour pushed)
     int opt_frame = 0;   // surface vertices: 0 = world coordinates, 1 = the reference's [0, 10]^3 frame
@@ -230,9 +231,16 @@ int launch_front(rtps_sph* s, StageTimer& t) {
     // stage holds both, the "hash" stage is empty
     t.mark(ST_HASH);
     t.mark(ST_SORT);
-    RTPS_TRY_HIP(hash_sort_pairs(n, s->key_bits, s->pos_u, n, s->d_num, s->pad_key, s->g, kSubBits, s->start,
-                                 s->nb_cells + 1, s->keys, s->idx, s->sort_tmp, s->stream, &s->keys_sorted,
-                                 &s->idx_sorted));
+    if (s->opt_bucket && bucket_sort_fits(n, s->key_bits)) {
+        RTPS_TRY_HIP(hash_bucket_sort(n, s->key_bits, s->pos_u, n, s->d_num, s->pad_key, s->g, kSubBits, s->start,
+                                      s->nb_cells + 1, s->keys, s->idx, s->sort_tmp, s->stream));
+        s->keys_sorted = s->keys;
+        s->idx_sorted = s->idx;
+    } else {
+        RTPS_TRY_HIP(hash_sort_pairs(n, s->key_bits, s->pos_u, n, s->d_num, s->pad_key, s->g, kSubBits, s->start,
+                                     s->nb_cells + 1, s->keys, s->idx, s->sort_tmp, s->stream, &s->keys_sorted,
+                                     &s->idx_sorted));
+    }
     // cellindices and permute in one kernel: the "cellindices" stage holds
     // both, the "permute" stage is empty
     t.mark(ST_CELL);
@@ -660,7 +668,8 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
         RTPS_TRY_HIP(hipMemsetAsync(s->tile_q, 0, kTileQueueWords * sizeof(int), s->stream));
         if ((size_t)s->nb_cells * kSubCells * sizeof(uint32_t) <= ((size_t)2 << 30))   // at most 2 GiB
             RTPS_TRY_HIP(dalloc(&s->sub_start, (size_t)s->nb_cells * kSubCells));
-        RTPS_TRY_HIP(hipMalloc(&s->sort_tmp, sort_temp_bytes((int)N, s->key_bits)));
+        // the bucket sort's temp, a superset of the radix sort's, for every count up to N
+        RTPS_TRY_HIP(hipMalloc(&s->sort_tmp, bucket_temp_bytes((int)N, s->key_bits)));
         float4* zf[] = {s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s, s->ve_s, s->force_s, s->xsph_s};
         for (float4* p : zf) RTPS_TRY_HIP(hipMemsetAsync(p, 0, N * sizeof(float4), s->stream));
         {   // every slot starts with the default colour of pushParticles (System.h:160)
@@ -1286,6 +1295,12 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_SURFACE_FRAME:
             if (value != 0 && value != 1) return set_error(RTPS_ERR_ARG, "surface frame must be 0 (world) or 1 (reference)");
             s->opt_frame = value;
+            break;
+        case RTPS_OPT_BUCKET_SORT:
+            if (value != 0 && value != 1) return set_error(RTPS_ERR_ARG, "bucket sort option must be 0 or 1");
+            s->opt_bucket = value;
+            s->graph_valid = false;
+            s->timed_graph_valid = false;
             break;
         case RTPS_OPT_TILE_QUEUES:
             if (value != 0 && value != 1) return set_error(RTPS_ERR_ARG, "tile queues option must be 0 or 1");

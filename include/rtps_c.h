@@ -315,7 +315,14 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        a drained queue's blocks moving on to the next (each
  *                        XCD's L2 then serves one stretch of the neighbour
  *                        rows); 0 = one tile per block in RTPS_OPT_TILE_REMAP
- *                        order.  The same results either way. */
+ *                        order.  The same results either way.
+ *   RTPS_OPT_BUCKET_SORT 1 = the step sorts its cell keys by bucket (a
+ *                        histogram of key buckets, the keys placed by bucket,
+ *                        each bucket sorted by one wave) where the key space
+ *                        has at most 2^16 buckets averaging at most 1024 keys
+ *                        (default), 0 = the LSD radix sort of rtps_sort_pairs.
+ *                        The same order either way (keys ascending, equal keys
+ *                        in particle order). */
 #define RTPS_OPT_COLLISION 0
 #define RTPS_OPT_GRAPH 1
 #define RTPS_OPT_FORCE_OUT 2
@@ -331,6 +338,7 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
 #define RTPS_OPT_BLOCK_FORCE 12
 #define RTPS_OPT_SURFACE_FRAME 13
 #define RTPS_OPT_TILE_QUEUES 14
+#define RTPS_OPT_BUCKET_SORT 15
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))
