@@ -22,8 +22,8 @@
 //                   writes the run's keys of that bucket there in index order
 //                   with the particle indices.  No global atomics; the runs of
 //                   a bucket land in any order.
-//   k_bucket_sort   one wave per bucket of up to kSmallMax keys (one block
-//                   per larger bucket, k_bucket_sort_large): a counting sort
+//   k_bucket_sort   one wave per bucket of up to kSmallMax keys, then one
+//                   block per larger bucket: a counting sort
 //                   on the low key bits, then each run of equal keys ordered
 //                   by particle index.  A bucket above kBlockMax keys, or with
 //                   a run of more than kTieMax equal keys, takes stable digit
@@ -48,9 +48,8 @@ constexpr int kBucketBits = 16;            // log2 kMaxBuckets
 constexpr int kScanThreads = 256;          // threads of k_bucket_scan
 constexpr int kScanTile = 4 * kScanThreads;  // bucket sizes per block of k_bucket_scan
 constexpr int kSmallMax = 512;             // buckets of k_bucket_sort: up to 512 keys (8 per lane)
-constexpr int kBlockMax = 8192;            // buckets of k_bucket_sort_large in registers and LDS: up to 8192 keys
-constexpr int kSmallBlocks = 1024;         // blocks of k_bucket_sort (4 waves per SIMD)
-constexpr int kLargeBlocks = 256;          // blocks of k_bucket_sort_large
+constexpr int kBlockMax = 4096;            // buckets sorted by a block in registers and LDS: up to 4096 keys
+constexpr int kSortBlocks = 1024;          // blocks of k_bucket_sort (4 waves per SIMD)
 constexpr int kSortWaves = 4;              // buckets (waves) per block of k_bucket_sort
 constexpr int kMaxBucketPasses = 7;
 constexpr int kTieMax = 256;               // longest run of equal keys the tie fix orders
@@ -129,9 +128,12 @@ __global__ __launch_bounds__(kBkThreads) void k_bucket_hist(const float4* __rest
                                                             uint32_t* __restrict__ btab,
                                                             unsigned long long* __restrict__ scan_status) {
     __shared__ uint32_t h[kMaxBuckets / 2];   // two 16-bit counts per word (a tile has kBkTile keys)
+    __shared__ uint32_t touched[kBkTile];     // the tile's buckets, in the order first seen
+    __shared__ uint32_t ntouched;
     const int tid = threadIdx.x;
     const int nw = (nb + 1) / 2;
     for (int e = tid; e < nw; e += kBkThreads) h[e] = 0;
+    if (tid == 0) ntouched = 0u;
     const long stride = (long)gridDim.x * kBkThreads;
     for (long e = (long)blockIdx.x * kBkThreads + tid; e < n_clear; e += stride) start_clear[e] = kEmpty;
     if (blockIdx.x == 0 && tid < kMaxBuckets / kScanTile) scan_status[tid] = 0ull;
@@ -146,16 +148,16 @@ __global__ __launch_bounds__(kBkThreads) void k_bucket_hist(const float4* __rest
         const uint32_t key = valid && e < num ? hash_key(p, g, sub_bits) : pad_key;
         if (valid) keys[e] = key;
         if (valid) {
-            const uint32_t b = key >> low;
-            atomicAdd(&h[b >> 1], 1u << (16 * (b & 1u)));
+            const uint32_t b = key >> low, sh = 16u * (b & 1u);
+            if (((atomicAdd(&h[b >> 1], 1u << sh) >> sh) & 0xFFFFu) == 0u) touched[atomicAdd(&ntouched, 1u)] = b;
         }
     }
     __syncthreads();
+    // one global atomic per bucket the tile touches: the tile's offset in it
     uint32_t* tab = btab + (size_t)blockIdx.x * nb;
-    for (int e = tid; e < nw; e += kBkThreads) {
-        const uint32_t c = h[e], c0 = c & 0xFFFFu, c1 = c >> 16;
-        if (c0) tab[2 * e] = atomicAdd(&bhist[2 * e], c0);
-        if (c1) tab[2 * e + 1] = atomicAdd(&bhist[2 * e + 1], c1);
+    for (uint32_t e = tid; e < ntouched; e += kBkThreads) {
+        const uint32_t b = touched[e];
+        tab[b] = atomicAdd(&bhist[b], (h[b >> 1] >> (16u * (b & 1u))) & 0xFFFFu);
     }
 }
 
@@ -428,29 +430,6 @@ __device__ void sort_bucket_global(uint32_t* __restrict__ ka, uint32_t* __restri
     }
 }
 
-// The small buckets of the list (at most kSmallMax keys), one wave each
-// (grid-stride over the list).
-__global__ __launch_bounds__(64 * kSortWaves) void k_bucket_sort(uint32_t* __restrict__ ka, uint32_t* __restrict__ va,
-                                                                 uint32_t* __restrict__ kb, uint32_t* __restrict__ vb,
-                                                                 BucketPlan plan, uint32_t* __restrict__ bhist,
-                                                                 const uint32_t* __restrict__ bstart,
-                                                                 const uint32_t* __restrict__ blist,
-                                                                 const uint32_t* __restrict__ nlist) {
-    __shared__ BucketLds<kSmallMax / 64> lds[kSortWaves];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    BucketLds<kSmallMax / 64>& L = lds[w];
-    const uint32_t count = nlist[0];
-    for (uint32_t q = blockIdx.x * kSortWaves + w; q < count; q += gridDim.x * kSortWaves) {
-        const uint32_t b = blist[q];
-        const uint32_t S = bhist[b], s0 = bstart[b];
-        bool again;
-        if (S <= 256u) again = sort_bucket_regs<4>(ka, va, kb, vb, S, s0, plan.low, reinterpret_cast<BucketLds<4>&>(L));
-        else again = sort_bucket_regs<kSmallMax / 64>(ka, va, kb, vb, S, s0, plan.low, L);
-        if (again) sort_bucket_global(ka, va, kb, vb, S, s0, plan, L.cnt);
-        if (lane == 0) bhist[b] = 0u;
-    }
-}
-
 // The counting sort of sort_bucket_regs by the whole block (kSortWaves
 // waves) for S <= 64 * kSortWaves * SLOTS keys: element i in slot i / 256 of
 // wave (i / 64) % 4, lane i % 64.  Returns true (nothing written) when a run
@@ -514,30 +493,52 @@ __device__ __forceinline__ bool sort_bucket_block(const uint32_t* __restrict__ k
     return false;
 }
 
-// The large buckets of the list (from its back), one block each: up to
-// kBlockMax keys in registers and LDS, above that (or with a run of equal keys
-// longer than kTieMax) through global memory by the block's first wave.
-__global__ __launch_bounds__(64 * kSortWaves) void k_bucket_sort_large(uint32_t* __restrict__ ka,
-                                                                       uint32_t* __restrict__ va,
-                                                                       uint32_t* __restrict__ kb,
-                                                                       uint32_t* __restrict__ vb, BucketPlan plan,
-                                                                       uint32_t* __restrict__ bhist,
-                                                                       const uint32_t* __restrict__ bstart,
-                                                                       const uint32_t* __restrict__ blist,
-                                                                       const uint32_t* __restrict__ nlist) {
-    __shared__ uint32_t stv[kBlockMax];
-    __shared__ uint32_t cnt[1 << kBucketMaxLow];
-    __shared__ int flag;
-    const uint32_t count = nlist[1];
-    for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {   // block-uniform
+// The listed buckets: first the small ones (at most kSmallMax keys), one
+// wave each, then the large ones (from the back of the list), one block each:
+// up to kBlockMax keys in registers and LDS, above that (or with a run of
+// equal keys longer than kTieMax) through global memory by the block's first
+// wave.  Grid-stride over the lists; the two phases share the LDS.
+__global__ __launch_bounds__(64 * kSortWaves) void k_bucket_sort(uint32_t* __restrict__ ka, uint32_t* __restrict__ va,
+                                                                 uint32_t* __restrict__ kb, uint32_t* __restrict__ vb,
+                                                                 BucketPlan plan, uint32_t* __restrict__ bhist,
+                                                                 const uint32_t* __restrict__ bstart,
+                                                                 const uint32_t* __restrict__ blist,
+                                                                 const uint32_t* __restrict__ nlist) {
+    struct BlockLds {
+        uint32_t stv[kBlockMax];
+        uint32_t cnt[1 << kBucketMaxLow];
+        int flag;
+    };
+    union Lds {
+        BucketLds<kSmallMax / 64> wave[kSortWaves];
+        BlockLds block;
+    };
+    __shared__ Lds lds;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    BucketLds<kSmallMax / 64>& L = lds.wave[w];
+    const uint32_t nsmall = nlist[0], nlarge = nlist[1];
+    for (uint32_t q = blockIdx.x * kSortWaves + w; q < nsmall; q += gridDim.x * kSortWaves) {
+        const uint32_t b = blist[q];
+        const uint32_t S = bhist[b], s0 = bstart[b];
+        bool again;
+        if (S <= 256u) again = sort_bucket_regs<4>(ka, va, kb, vb, S, s0, plan.low, reinterpret_cast<BucketLds<4>&>(L));
+        else again = sort_bucket_regs<kSmallMax / 64>(ka, va, kb, vb, S, s0, plan.low, L);
+        if (again) sort_bucket_global(ka, va, kb, vb, S, s0, plan, L.cnt);
+        if (lane == 0) bhist[b] = 0u;
+    }
+    if (nlarge == 0u) return;   // uniform: no barrier below is reached
+    __syncthreads();   // the waves' LDS before the block's
+    BlockLds& B = lds.block;
+    for (uint32_t q = blockIdx.x; q < nlarge; q += gridDim.x) {   // block-uniform
         const uint32_t b = blist[plan.nb - 1 - q];
         const uint32_t S = bhist[b], s0 = bstart[b];
         constexpr uint32_t kT = 64 * kSortWaves;
         bool again = true;
-        if (S <= 4u * kT) again = sort_bucket_block<4>(ka, va, kb, vb, S, s0, plan.low, stv, cnt, &flag);
-        else if (S <= 8u * kT) again = sort_bucket_block<8>(ka, va, kb, vb, S, s0, plan.low, stv, cnt, &flag);
-        else if (S <= (uint32_t)kBlockMax) again = sort_bucket_block<kBlockMax / kT>(ka, va, kb, vb, S, s0, plan.low, stv, cnt, &flag);
-        if (again && threadIdx.x < 64) sort_bucket_global(ka, va, kb, vb, S, s0, plan, cnt);
+        if (S <= 4u * kT) again = sort_bucket_block<4>(ka, va, kb, vb, S, s0, plan.low, B.stv, B.cnt, &B.flag);
+        else if (S <= 8u * kT) again = sort_bucket_block<8>(ka, va, kb, vb, S, s0, plan.low, B.stv, B.cnt, &B.flag);
+        else if (S <= (uint32_t)kBlockMax)
+            again = sort_bucket_block<kBlockMax / kT>(ka, va, kb, vb, S, s0, plan.low, B.stv, B.cnt, &B.flag);
+        if (again && threadIdx.x < 64) sort_bucket_global(ka, va, kb, vb, S, s0, plan, B.cnt);
         if (threadIdx.x == 0) bhist[b] = 0u;
         __syncthreads();   // the LDS of this bucket before the next one's
     }
@@ -560,10 +561,8 @@ hipError_t hash_bucket_sort(int n, int key_bits, const float4* pos_u, int num_ho
     k_bucket_scan<<<ntiles_for(plan.nb, kScanTile), kScanThreads, 0, st>>>(plan.nb, t.bhist, t.bstart, t.blist, nlist,
                                                                            scan_status);
     k_bucket_place<<<blocks, kBkThreads, 0, st>>>(keys, n, plan.low, plan.nb, btab, t.bstart, t.kalt, t.valt);
-    k_bucket_sort<<<kSmallBlocks, 64 * kSortWaves, 0, st>>>(t.kalt, t.valt, keys, vals, plan, t.bhist, t.bstart,
-                                                            t.blist, nlist);
-    k_bucket_sort_large<<<kLargeBlocks, 64 * kSortWaves, 0, st>>>(t.kalt, t.valt, keys, vals, plan, t.bhist, t.bstart,
-                                                                  t.blist, nlist);
+    k_bucket_sort<<<kSortBlocks, 64 * kSortWaves, 0, st>>>(t.kalt, t.valt, keys, vals, plan, t.bhist, t.bstart,
+                                                           t.blist, nlist);
     return hipGetLastError();
 }
 

@@ -5,10 +5,10 @@ state after the steps, must be bit-identical.
 
 The scenes cover the bucket sort's paths: buckets of up to 256 and 512 keys
 sorted by one wave (the dam lattice; uniform positions), buckets above 512
-keys sorted by a block (clumps in one cell: 4, 8 and 32 slots per thread),
+keys sorted by a block (clumps in one cell: 4, 8 and 16 slots per thread),
 runs of equal keys ordered by particle index (every scene; up to ~190 keys
 long in the 3000-particle clump), the global-memory passes (a bucket above
-8192 keys; a run of more than 256 equal keys in a small bucket: particles
+4096 keys; a run of more than 256 equal keys in a small bucket: particles
 stacked on one point), out-of-grid particles (all in the clamped cell), a
 deletion step (pad keys past the device count), and systems of 1, 2 and 65
 particles.
@@ -69,8 +69,8 @@ def _cases():
         ("random-dense", box, _uniform(20000, 0.4, 0.6, 8), 2, False),      # ~400-800 keys per bucket
         ("clump-700", box, _clump(700, 0.5, 11), 2, False),                 # block, 4 slots
         ("clump-1500", box, _clump(1500, 0.5, 12), 2, False),               # block, 8 slots
-        ("clump-3000", box, _clump(3000, 0.5, 13), 1, False),               # block, 32 slots, long runs
-        ("clump-9000", box, _clump(9000, 0.5, 14), 1, False),               # global memory
+        ("clump-3000", box, _clump(3000, 0.5, 13), 1, False),               # block, 16 slots, long runs
+        ("clump-9000", box, _clump(9000, 0.5, 14), 1, False),               # above 4096: global memory
         ("stack-400", box, _clump(400, 0.0, 15), 1, False),                 # 400 equal keys: global memory
         ("out-of-grid", dam.settings, out_of_grid, 2, False),
         ("n1", box, dam.pos[:1], 2, False),
