@@ -646,11 +646,15 @@ def main():
         occupied = int(np.count_nonzero(s.read("cell_start") != np.uint32(0xFFFFFFFF)))
         key_bits = (s.nb_cells + 1).bit_length() + 4      # cell key bits (incl. pad) + 4 sub-cell bits
         passes = (key_bits + 7) // 8
+        low = max(key_bits - 16, min(key_bits, 8))         # the bucket sort's bits inside a bucket
+        bucket = low <= 8 and (1 << (key_bits - low)) * 1024 >= n
         # the step runs the hash inside the sort's first kernel and the permute
         # inside the cellindices kernel: those stage pairs are reported together
         alg = {
-            # pos read, key + idx written, start[] cleared; per pass pairs read + written
-            "hash+sort": (("hash", "sort"), 24.0 * n + 4.0 * cells + passes * 16.0 * n),
+            # bucket sort: pos read, keys written, start[] cleared; keys read, pairs
+            # placed by bucket; pairs read and written sorted.  LSD radix sort: pos
+            # read, key + idx written, start[] cleared; per pass pairs read + written
+            "hash+sort": (("hash", "sort"), (48.0 * n if bucket else 24.0 * n + passes * 16.0 * n) + 4.0 * cells),
             # keys read, start/end of occupied cells written; idx + 3 float4 gathered, 3 float4 written
             "cellindices+permute": (("cellindices", "permute"), 4.0 * n + 8.0 * occupied + 100.0 * n),
             # own position read, density written (neighbours: cache)
