@@ -26,7 +26,7 @@ FIELD = {
 }
 (OPT_COLLISION, OPT_GRAPH, OPT_FORCE_OUT, OPT_TIMERS, OPT_FORCE_VARIANT, OPT_WINDOW, OPT_TILE_REMAP, OPT_AUTOTUNE,
  OPT_SORT_TILE, OPT_COLOR, OPT_DENSITY_STAGE, OPT_BLOCK_DENSITY, OPT_BLOCK_FORCE, OPT_SURFACE_FRAME,
- OPT_TILE_QUEUES, OPT_BUCKET_SORT) = range(16)
+ OPT_TILE_QUEUES, OPT_BUCKET_SORT, OPT_SHARED_ROWS) = range(17)
 # The library's process-wide default of OPT_FORCE_VARIANT (the force loop of
 # handles that are not autotuned).
 DEFAULT_FORCE_LOOP = 5

@@ -152,7 +152,7 @@ __device__ __forceinline__ void adj_b2_body(int i, const float4* __restrict__ po
     const float inv_ri = __builtin_amdgcn_rcpf(ri);
     float3 gx = make_float3(0, 0, 0), ge = make_float3(0, 0, 0);
     float grho = 0.f;
-    // one accepted pair (q2 <= h2, j != i)
+    // one accepted pair (within_h, j != i)
     auto pair = [&](const float3 r, const float q2, const float4 vej, const float4 gaj4, const float4 gXj4) {
         const float3 ej = f3(vej), gaj = f3(gaj4), gXj = f3(gXj4);
         const float rj = vej.w;
@@ -224,7 +224,7 @@ __device__ __forceinline__ void adj_b2_body(int i, const float4* __restrict__ po
             for (int k = 0; k < kGroup; ++k) {
                 r[k] = make_float3(pi.x - p[k].x, pi.y - p[k].y, pi.z - p[k].z);
                 q2[k] = dot3(r[k], r[k]);
-                ok[k] = q2[k] <= h2 && j + k != (uint32_t)i;
+                ok[k] = within_h(h2_minus_r2(r[k].x, r[k].y, r[k].z, h2)) && j + k != (uint32_t)i;
             }
             float4 v[kGroup], ga[kGroup], gX[kGroup];
 #pragma unroll
@@ -242,7 +242,8 @@ __device__ __forceinline__ void adj_b2_body(int i, const float4* __restrict__ po
             const float4 pj = P[j - off];
             const float3 r = make_float3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
             const float q2 = dot3(r, r);
-            if (q2 <= h2 && j != (uint32_t)i) pair(r, q2, V[j - off], GA[j - off], GX[j - off]);
+            if (within_h(h2_minus_r2(r.x, r.y, r.z, h2)) && j != (uint32_t)i)
+                pair(r, q2, V[j - off], GA[j - off], GX[j - off]);
         }
     };
     // rows of at most kAdjStageCap wave-wide candidates are copied once per
@@ -266,7 +267,7 @@ __device__ __forceinline__ void adj_b2_body(int i, const float4* __restrict__ po
             uint32_t mk = 0u, k = 0u;
             auto test = [&](const float4 p) {
                 const float3 r = make_float3(pi.x - p.x, pi.y - p.y, pi.z - p.z);
-                mk = mk + mk + (dot3(r, r) <= h2 ? 1u : 0u);
+                mk = mk + mk + (within_h(h2_minus_r2(r.x, r.y, r.z, h2)) ? 1u : 0u);
             };
             for (; k + kGroup <= n; k += kGroup) {
                 float4 q[kGroup];

@@ -203,7 +203,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                     ry[k] = pi.y - p[k].y * scale;
                     rz[k] = pi.z - p[k].z * scale;
                     r2[k] = rx[k] * rx[k] + ry[k] * ry[k] + rz[k] * rz[k];
-                    ok[k] = r2[k] <= h2 && j + k != (uint32_t)i;
+                    ok[k] = within_h(h2_minus_r2(rx[k], ry[k], rz[k], h2)) && j + k != (uint32_t)i;
                 }
 #pragma unroll
                 for (int k = 0; k < kGroup; ++k)
@@ -218,7 +218,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                 float4 pj = P[j - off];
                 float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
                 float r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 <= h2 && j != (uint32_t)i) {
+                if (within_h(h2_minus_r2(rx, ry, rz, h2)) && j != (uint32_t)i) {
                     const float4 vj = V[j - off];
                     force_pair_r(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, vj.w,
                                  inv_w ? pj.w : fast_rcp(vj.w));
@@ -226,20 +226,19 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
             }
         };
         // the staged copies, a lane's window in chunks of up to 32 candidates:
-        // the tests of a chunk shift one bit per candidate into a mask (the
-        // chunk's first candidate ends in its highest bit), the particle's own
-        // bit is cleared, then the pair terms run once per set bit, highest
-        // first (the same ascending order and arithmetic as walk, so the same
-        // sums)
+        // the tests of a chunk shift one rejection bit per candidate into a
+        // word (rej_bit: the chunk's first candidate ends in its highest bit),
+        // the particle's own bit is cleared, then the pair terms run once per
+        // accepted bit, highest first (the same ascending order and arithmetic
+        // as walk, so the same sums)
         auto walk_bits = [&](uint32_t a, uint32_t b, uint32_t off) {
             for (uint32_t j0 = a; j0 < b; j0 += 32u) {
                 const uint32_t n = min(b - j0, 32u);
                 const float4* __restrict__ pc = stage + (j0 - off);
-                uint32_t m = 0u, k = 0u;
+                uint32_t rej = 0u, k = 0u;
                 auto test = [&](float4 pj) {
                     const float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
-                    const float r2 = rx * rx + ry * ry + rz * rz;
-                    m = m + m + (r2 <= h2 ? 1u : 0u);
+                    rej = rej_bit(rej, h2_minus_r2(rx, ry, rz, h2));
                 };
                 for (; k + kGroup <= n; k += kGroup) {
                     float4 q[kGroup];
@@ -249,6 +248,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                     for (int t = 0; t < kGroup; ++t) test(q[t]);
                 }
                 for (; k < n; ++k) test(pc[k]);
+                uint32_t m = ~rej & low_bits(n);
                 const uint32_t self = (uint32_t)i - j0;   // the particle itself passes (r = 0): drop it
                 if (self < n) m &= ~(1u << (n - 1u - self));
                 while (m != 0u) {

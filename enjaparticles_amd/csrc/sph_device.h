@@ -145,6 +145,14 @@ struct Win {
     // finished blocks, all zero between launches; nullptr = one tile per block
     int* tq = nullptr;
     int ntiles = 0;   // tiles of the launch (set by the launcher)
+    // Recorded rows (nullptr = none): for particle i and row r = 3 (dz + 1) +
+    // (dy + 1), the three words at rows + 3 (r * rows_stride + i) hold its
+    // window [a, b) and the rejection bits (rej_bit) of the window's last
+    // min(b - a, 32) candidates.  The density kernel writes them; force loop
+    // kLoopStagedBits then reads them instead of looking the windows up and
+    // testing those candidates again.
+    uint32_t* rows = nullptr;
+    uint32_t rows_stride = 0;   // particles per row plane (at least the particle count)
 };
 
 // Candidates per group in the neighbour loops (loads in flight per lane).
@@ -417,10 +425,29 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 // fused multiply-adds (one instruction less than |r|^2 first; the rounding stays
 // within the FP32 tolerance of the reference's (h^2 - |r|^2)^3).  The density
 // kernels (plain and batched) use this form.
+__device__ __forceinline__ float h2_minus_r2(float rx, float ry, float rz, float h2) {
+    return fmaf(-rz, rz, fmaf(-ry, ry, fmaf(-rx, rx, h2)));
+}
 __device__ __forceinline__ void poly6_add_r(float& sum, float rx, float ry, float rz, float h2) {
-    const float t = fmaxf(fmaf(-rz, rz, fmaf(-ry, ry, fmaf(-rx, rx, h2))), 0.0f);
+    const float t = fmaxf(h2_minus_r2(rx, ry, rz, h2), 0.0f);
     sum = fmaf(t * t, t, sum);
 }
+
+// Neighbour test of every force loop (plain, batched, adjoint): the pair is
+// within h when u = h2_minus_r2 has its sign bit clear (u >= 0: u is never
+// -0 for h > 0).  force.cl:64 tests rlen <= h; the two differ only for pairs
+// within rounding of r = h, where every pair term of force.cl:73-133 is
+// (h - r)^k ~ 0.  The density kernel forms the same u for each candidate,
+// so it can record the test for the force kernel (rej_bit).
+__device__ __forceinline__ bool within_h(float u) { return (int)__float_as_uint(u) >= 0; }
+// Shifts the test of one more candidate into a word of rejection bits:
+// (rej << 1) | sign(u), one instruction; after a window of L candidates
+// its candidate k (k >= L - 32) sits at bit L - 1 - k.
+__device__ __forceinline__ uint32_t rej_bit(uint32_t rej, float u) {
+    return __builtin_amdgcn_alignbit(rej, __float_as_uint(u), 31u);
+}
+// Ones in the low n bits, 1 <= n <= 32.
+__device__ __forceinline__ uint32_t low_bits(uint32_t n) { return 0xFFFFFFFFu >> (32u - n); }
 
 // Per-particle pair-sum accumulator of force.cl:43-140 (pt.force, pt.viscosity, pt.xsph).
 struct ForceAcc {

@@ -149,7 +149,10 @@ int g_density_stage = 1;
 // At most 56 VGPRs asked for (nine waves per SIMD): the row-by-row walk fits
 // (55; 57 and eight waves without the cap); the staged walk takes 63 anyway
 // (eight waves, whose LDS slices take 128 of the 160 KB).
-template <bool SCALED, bool PACKED, bool STAGE>
+// EMIT: each row's window and the rejection bits of its last 32 candidates
+// go to win.rows (one rej_bit per candidate, the same u as the poly6 term),
+// for every live particle and all nine rows (empty rows as [a, a)).
+template <bool SCALED, bool PACKED, bool STAGE, bool EMIT>
 __device__ __forceinline__ void density_tile(int tile, int num_host, const int* __restrict__ d_num,
                                              const float4* __restrict__ pos_s, const uint32_t* __restrict__ start,
                                              const uint32_t* __restrict__ end, const rtps_sph_params& sp,
@@ -166,9 +169,13 @@ __device__ __forceinline__ void density_tile(int tile, int num_host, const int* 
     const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
     const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
     float sum = 0.0f;
+    uint32_t rej = 0u;   // EMIT: the current row's rejection bits
     auto accum = [&](float4 pj) {   // the poly6 term of one candidate
         if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
-        poly6_add_r(sum, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, h2);
+        const float u = h2_minus_r2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, h2);
+        const float t = fmaxf(u, 0.0f);
+        sum = fmaf(t * t, t, sum);
+        if (EMIT) rej = rej_bit(rej, u);
     };
     // candidates [a, b) of one row at P[j - off], kGroupDensity loads at a time
     auto walk = [&](const float4* __restrict__ P, uint32_t a, uint32_t b, uint32_t off) {
@@ -183,6 +190,16 @@ __device__ __forceinline__ void density_tile(int tile, int num_host, const int* 
         }
         for (; j < b; ++j) accum(P[j - off]);
     };
+    uint32_t* __restrict__ rec = EMIT ? win.rows + 3 * (size_t)i : nullptr;
+    auto record = [&](int dz, int dy, uint32_t a, uint32_t b) {
+        if (EMIT) {
+            uint32_t* __restrict__ o = rec + 3 * (size_t)((3 * (dz + 1) + dy + 1) * win.rows_stride);
+            o[0] = a;
+            o[1] = b;
+            o[2] = rej;
+            rej = 0u;
+        }
+    };
     if constexpr (STAGE) {
         extern __shared__ float4 stage_rows[];   // kStageCap per wave
         float4* __restrict__ my = stage_rows + (threadIdx.x >> 6) * kStageCap;
@@ -196,15 +213,17 @@ __device__ __forceinline__ void density_tile(int tile, int num_host, const int* 
                 row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
                 const uint32_t A = __ockl_wfred_min_u32(a < b ? a : 0xFFFFFFFFu);
                 const uint32_t B = __ockl_wfred_max_u32(a < b ? b : 0u);
-                if (B <= A) continue;
-                if (B - A <= (uint32_t)kStageCap) {
-                    for (uint32_t t = rank; t < B - A; t += nlive) my[t] = pos_s[A + t];
-                    __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
-                    walk(my, a, b, A);
-                    __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
-                } else {
-                    walk(pos_s, a, b, 0u);
+                if (B > A) {
+                    if (B - A <= (uint32_t)kStageCap) {
+                        for (uint32_t t = rank; t < B - A; t += nlive) my[t] = pos_s[A + t];
+                        __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
+                        walk(my, a, b, A);
+                        __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
+                    } else {
+                        walk(pos_s, a, b, 0u);
+                    }
                 }
+                record(dz, dy, a, b);
             }
         }
     } else {
@@ -213,6 +232,7 @@ __device__ __forceinline__ void density_tile(int tile, int num_host, const int* 
                 uint32_t a, b;
                 row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
                 walk(pos_s, a, b, 0u);
+                record(dz, dy, a, b);
             }
         }
     }
@@ -224,7 +244,7 @@ __device__ __forceinline__ void density_tile(int tile, int num_host, const int* 
 // grid working through the per-XCD tile queues of win.tq.  The queued
 // kernels are separate instances: the loop around the body raises its
 // register demand, which must not cost the one-tile kernels.
-template <bool SCALED, bool PACKED, bool STAGE, bool QUEUED>
+template <bool SCALED, bool PACKED, bool STAGE, bool QUEUED, bool EMIT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k_density(int num_host, const int* __restrict__ d_num,
                                                     const float4* __restrict__ pos_s,
                                                     const uint32_t* __restrict__ start,
@@ -235,12 +255,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
         for (int t = first_tile(win.ntiles);;) {
             if (t < 0) t = claim_tile(win.tq, win.ntiles, q);
             if (t < 0) break;
-            density_tile<SCALED, PACKED, STAGE>(t, num_host, d_num, pos_s, start, end, sp, g, win, dens, packed);
+            density_tile<SCALED, PACKED, STAGE, EMIT>(t, num_host, d_num, pos_s, start, end, sp, g, win, dens, packed);
             t = -1;
         }
         finish_tiles(win.tq);
     } else {
-        density_tile<SCALED, PACKED, STAGE>(remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x), num_host, d_num,
+        density_tile<SCALED, PACKED, STAGE, EMIT>(remap_tile(win.remap, (int)blockIdx.x, (int)gridDim.x), num_host, d_num,
                                             pos_s, start, end, sp, g, win, dens, packed);
     }
 }
@@ -256,7 +276,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
 // kLoopDirect: each lane walks its own window of every row in global memory,
 // kGroup candidates per iteration (position loads, then the veleval loads of
 // the accepted ones, issued together) with the pair terms under a branch on
-// r <= h.
+// r <= h (within_h).
 // kLoopStaged: the direct loop, with each row's candidates (positions, their
 // w replaced by 1 / density, and veleval|density) copied once per wave into
 // LDS as for the density kernel's STAGE (at most kForceStageCap candidates;
@@ -269,7 +289,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(56))) void k
 // in the pair terms for nearly every candidate; with the bits the wave's
 // pair iterations are its lanes' largest accepted count.  Ascending j again:
 // the direct loop's bits.
-constexpr int kLoopDirect = 0, kLoopStaged = 4, kLoopStagedBits = 5;
+// kLoopShared (kLoopStagedBits when the density kernel recorded the rows,
+// Win::rows): the windows are read instead of looked up, and the last chunk
+// of each window (the whole window up to 32 candidates) takes its bits from
+// the record instead of testing: the density kernel formed the same u for
+// the same candidates.  The direct loop's bits once more.
+constexpr int kLoopDirect = 0, kLoopStaged = 4, kLoopStagedBits = 5, kLoopShared = 6;
 constexpr int kForceStageCap = 192;   // 6 KB per wave: the 20 waves per CU of kStagedWavesPerSimd take 120 KB
 
 // At least six waves per SIMD: the compiler keeps the force kernel within 80
@@ -355,7 +380,7 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
                         if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
                         rx[k] = pi.x - pj.x; ry[k] = pi.y - pj.y; rz[k] = pi.z - pj.z;
                         r2[k] = rx[k] * rx[k] + ry[k] * ry[k] + rz[k] * rz[k];
-                        ok[k] = r2[k] <= h2 && j + k != (uint32_t)i;
+                        ok[k] = within_h(h2_minus_r2(rx[k], ry[k], rz[k], h2)) && j + k != (uint32_t)i;
                     }
 #pragma unroll
                     for (int k = 0; k < kGroup; ++k)
@@ -370,12 +395,12 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
                 for (; j < b; ++j) {
                     float rx, ry, rz;
                     float r2 = sep(j, rx, ry, rz);
-                    if (r2 <= h2 && j != (uint32_t)i) pair(j, rx, ry, rz, r2);
+                    if (within_h(h2_minus_r2(rx, ry, rz, h2)) && j != (uint32_t)i) pair(j, rx, ry, rz, r2);
                 }
             }
         }
     }
-    if constexpr (LOOP == kLoopStaged || LOOP == kLoopStagedBits) {
+    if constexpr (LOOP == kLoopStaged || LOOP == kLoopStagedBits || LOOP == kLoopShared) {
         const int3 c = grid_cell(pi.x, pi.y, pi.z, g);
         const WindowCtx wc = window_ctx(pi.x, pi.y, pi.z, g, c, h, win.keys, win.rx, win.ry, win.rz);
         extern __shared__ float4 force_rows[];   // 2 x kForceStageCap per wave: positions, then veleval|density
@@ -403,7 +428,7 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
                     if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
                     rx[k] = pi.x - pj.x; ry[k] = pi.y - pj.y; rz[k] = pi.z - pj.z;
                     r2[k] = rx[k] * rx[k] + ry[k] * ry[k] + rz[k] * rz[k];
-                    ok[k] = r2[k] <= h2 && j + k != (uint32_t)i;
+                    ok[k] = within_h(h2_minus_r2(rx[k], ry[k], rz[k], h2)) && j + k != (uint32_t)i;
                 }
 #pragma unroll
                 for (int k = 0; k < kGroup; ++k)
@@ -422,7 +447,7 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
                 if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
                 const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                 const float r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 <= h2 && j != (uint32_t)i) {
+                if (within_h(h2_minus_r2(rx, ry, rz, h2)) && j != (uint32_t)i) {
                     const float4 vj = V[j - off];
                     const float dj = PACKED ? vj.w : dens_in[j];
                     force_pair_r(acc, rx, ry, rz, r2, h, h2, eps, K, rho0, Pi, di, vi, vj, dj,
@@ -430,34 +455,40 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
                 }
             }
         };
-        // Loop 5 on the staged copies: the lane's window is taken in chunks of
-        // up to 32 candidates; the tests of a chunk shift one bit per
-        // candidate into a mask (the chunk's first candidate ends in its
-        // highest bit), the particle's own bit is cleared once per chunk, then
-        // the pair terms run once per set bit, highest first.  The wave's pair
-        // iterations are its lanes' largest accepted count instead of their
-        // longest window, and the pairs are summed in the same ascending order
-        // as walk (the same bits).
+        // Loops 5 and 6 on the staged copies: the lane's window is taken in
+        // chunks, the first one ragged (1-32 candidates) and the rest 32
+        // long, so that the last chunk is the window's last min(b - a, 32)
+        // candidates.  A chunk's tests shift one rejection bit per candidate
+        // into a word (rej_bit: the chunk's first candidate ends in its
+        // highest bit); loop 6 takes the last chunk's word from the rows the
+        // density kernel recorded (rej_last) instead.  The particle's own bit
+        // is cleared, then the pair terms run once per accepted bit, highest
+        // first.  The wave's pair iterations are its lanes' largest accepted
+        // count instead of their longest window, and the pairs are summed in
+        // the same ascending order as walk (the same bits).
         auto walk_bits = [&](const float4* __restrict__ P, const float4* __restrict__ V, uint32_t a, uint32_t b,
-                             uint32_t off) {
-            for (uint32_t j0 = a; j0 < b; j0 += 32u) {
-                const uint32_t n = min(b - j0, 32u);
+                             uint32_t off, uint32_t rej_last) {
+            uint32_t n = ((b - a - 1u) & 31u) + 1u;
+            for (uint32_t j0 = a; j0 < b; j0 += n, n = 32u) {
                 const float4* __restrict__ pc = P + (j0 - off);
-                uint32_t m = 0u, k = 0u;
-                auto test = [&](float4 pj) {
-                    if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
-                    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                    const float r2 = rx * rx + ry * ry + rz * rz;
-                    m = m + m + (r2 <= h2 ? 1u : 0u);
-                };
-                for (; k + kGroup <= n; k += kGroup) {
-                    float4 q[kGroup];
+                uint32_t rej = rej_last;
+                if (LOOP != kLoopShared || j0 + n != b) {
+                    rej = 0u;
+                    uint32_t k = 0u;
+                    auto test = [&](float4 pj) {
+                        if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
+                        rej = rej_bit(rej, h2_minus_r2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, h2));
+                    };
+                    for (; k + kGroup <= n; k += kGroup) {
+                        float4 q[kGroup];
 #pragma unroll
-                    for (int t = 0; t < kGroup; ++t) q[t] = pc[k + t];
+                        for (int t = 0; t < kGroup; ++t) q[t] = pc[k + t];
 #pragma unroll
-                    for (int t = 0; t < kGroup; ++t) test(q[t]);
+                        for (int t = 0; t < kGroup; ++t) test(q[t]);
+                    }
+                    for (; k < n; ++k) test(pc[k]);
                 }
-                for (; k < n; ++k) test(pc[k]);
+                uint32_t m = ~rej & low_bits(n);
                 const uint32_t self = (uint32_t)i - j0;   // the particle itself passes (r = 0): drop it
                 if (self < n) m &= ~(1u << (n - 1u - self));
                 while (m != 0u) {
@@ -475,10 +506,19 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
                 }
             }
         };
+        // the row's window and last-chunk rejection bits: recorded (loop 6) or looked up
+        const uint32_t* __restrict__ rec = LOOP == kLoopShared ? win.rows + 3 * (size_t)i : nullptr;
         for (int dz = -1; dz <= 1; ++dz) {
             for (int dy = -1; dy <= 1; ++dy) {
-                uint32_t a, b;
-                row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
+                uint32_t a, b, rej_last = 0u;
+                if constexpr (LOOP == kLoopShared) {
+                    const uint32_t* __restrict__ o = rec + 3 * (size_t)((3 * (dz + 1) + dy + 1) * win.rows_stride);
+                    a = o[0];
+                    b = o[1];
+                    rej_last = o[2];
+                } else {
+                    row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
+                }
                 const uint32_t A = __ockl_wfred_min_u32(a < b ? a : 0xFFFFFFFFu);
                 const uint32_t B = __ockl_wfred_max_u32(a < b ? b : 0u);
                 if (B <= A) continue;
@@ -491,8 +531,8 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
                         my_v[t] = e;
                     }
                     __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
-                    if constexpr (LOOP == kLoopStagedBits) walk_bits(my_p, my_v, a, b, A);
-                    else walk(my_p, my_v, a, b, A, true);
+                    if constexpr (LOOP == kLoopStaged) walk(my_p, my_v, a, b, A, true);
+                    else walk_bits(my_p, my_v, a, b, A, rej_last);
                     __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
                 } else {
                     walk(pos_s, ve_in, a, b, 0u, false);
@@ -798,18 +838,23 @@ hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const 
     const int nb = blocks_for(num, bs);
     const bool stage = g_density_stage != 0;
     const size_t lds = stage ? (size_t)(bs / 64) * kStageCap * sizeof(float4) : 0;
+    if (win_in.rows && win_in.rows_stride < (uint32_t)num) return hipErrorInvalidValue;
     Win win = win_in;
     win.ntiles = nb;
-#define RTPS_DENS(S, P, T)                                                                                          \
+#define RTPS_DENS(S, P, T, E)                                                                                       \
     if (win.tq && P)                                                                                                \
-        k_density<S, P, T, true><<<resident_grid((const void*)&k_density<S, P, T, true>, bs, lds, nb), bs, lds,      \
-                                   st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);                     \
+        k_density<S, P, T, true, E><<<resident_grid((const void*)&k_density<S, P, T, true, E>, bs, lds, nb), bs,     \
+                                      lds, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed);             \
     else                                                                                                            \
-        k_density<S, P, T, false><<<nb, bs, lds, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed)
-#define RTPS_DENS_T(T)                                                                                              \
-    if (packed) { if (scaled) { RTPS_DENS(true, true, T); } else { RTPS_DENS(false, true, T); } }                     \
-    else { if (scaled) { RTPS_DENS(true, false, T); } else { RTPS_DENS(false, false, T); } }
-    if (stage) { RTPS_DENS_T(true) } else { RTPS_DENS_T(false) }
+        k_density<S, P, T, false, E><<<nb, bs, lds, st>>>(num, d_num, pos_s, start, end, sp, g, win, dens, packed)
+#define RTPS_DENS_T(T, E)                                                                                           \
+    if (packed) { if (scaled) { RTPS_DENS(true, true, T, E); } else { RTPS_DENS(false, true, T, E); } }               \
+    else { if (scaled) { RTPS_DENS(true, false, T, E); } else { RTPS_DENS(false, false, T, E); } }
+    if (win.rows) {
+        if (stage) { RTPS_DENS_T(true, true) } else { RTPS_DENS_T(false, true) }
+    } else {
+        if (stage) { RTPS_DENS_T(true, false) } else { RTPS_DENS_T(false, false) }
+    }
 #undef RTPS_DENS_T
 #undef RTPS_DENS
     return hipGetLastError();
@@ -817,6 +862,12 @@ hipError_t launch_density(int num, const int* d_num, const float4* pos_s, const 
 
 static inline size_t force_lds(int loop, int bs) {
     return loop == kLoopDirect ? 0 : (size_t)(bs / 64) * 2 * kForceStageCap * sizeof(float4);
+}
+
+// The force loop of a launch: loop 5 becomes 6 when the rows are recorded.
+static inline int force_loop(const Win& w) {
+    const int v = nbr_variant(w);
+    return v == kLoopStagedBits && w.rows ? kLoopShared : v;
 }
 
 hipError_t launch_force(int num, int* d_num, const float4* pos_s, const float* dens, const float4* ve,
@@ -827,7 +878,8 @@ hipError_t launch_force(int num, int* d_num, const float4* pos_s, const float* d
     const bool scaled = sp.simulation_scale != 1.0f;
     const int bs = nbr_block(win_in);
     const int nb = blocks_for(num, bs);
-    const int loop = nbr_variant(win_in);
+    const int loop = force_loop(win_in);
+    if (win_in.rows && win_in.rows_stride < (uint32_t)num) return hipErrorInvalidValue;
     const size_t lds = force_lds(loop, bs);
     GravitySources none{};
     rtps_float4 z{0, 0, 0, 0};
@@ -846,6 +898,7 @@ hipError_t launch_force(int num, int* d_num, const float4* pos_s, const float* d
 #define RTPS_FORCE(S, P)                                                                                            \
     if (loop == kLoopStaged) { RTPS_FORCE_L(S, P, kLoopStaged) }                                                    \
     else if (loop == kLoopStagedBits) { RTPS_FORCE_L(S, P, kLoopStagedBits) }                                       \
+    else if (loop == kLoopShared) { RTPS_FORCE_L(S, P, kLoopShared) }                                               \
     else { RTPS_FORCE_L(S, P, kLoopDirect) }
     if (packed) { if (scaled) { RTPS_FORCE(true, true); } else { RTPS_FORCE(false, true); } }
     else { if (scaled) { RTPS_FORCE(true, false); } else { RTPS_FORCE(false, false); } }
@@ -866,7 +919,8 @@ hipError_t launch_force_fused(int num, int* d_num, const float4* pos_s, const fl
     const bool scaled = sp.simulation_scale != 1.0f;
     const int bs = nbr_block(win_in);
     const int nb = blocks_for(num, bs);
-    const int loop = nbr_variant(win_in);
+    const int loop = force_loop(win_in);
+    if (win_in.rows && win_in.rows_stride < (uint32_t)num) return hipErrorInvalidValue;
     const size_t lds = force_lds(loop, bs);
     Win win = win_in;
     win.ntiles = nb;
@@ -883,6 +937,7 @@ hipError_t launch_force_fused(int num, int* d_num, const float4* pos_s, const fl
 #define RTPS_FFUSED(S)                                                                                              \
     if (loop == kLoopStaged) { RTPS_FFUSED_L(S, kLoopStaged) }                                                      \
     else if (loop == kLoopStagedBits) { RTPS_FFUSED_L(S, kLoopStagedBits) }                                         \
+    else if (loop == kLoopShared) { RTPS_FFUSED_L(S, kLoopShared) }                                                 \
     else { RTPS_FFUSED_L(S, kLoopDirect) }
     if (scaled) { RTPS_FFUSED(true) } else { RTPS_FFUSED(false) }
 #undef RTPS_FFUSED

@@ -55,11 +55,14 @@ struct rtps_sph {
     int* d_num = nullptr;
     int* tile_q = nullptr;   // per-XCD tile queue counters of the neighbour kernels (kTileQueueWords)
     void* sort_tmp = nullptr;
-    // neighbour rows recorded by density for the force loops 2 (bit form) and 3 (list form)
+    // the neighbour rows the density kernel records for force loop 5 (Win::rows):
+    // 9 rows x 3 words per particle slot
+    uint32_t* rows = nullptr;
     GravitySources gsrc{};
     int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0,
         opt_queues = 0;
     int opt_bucket = 1;   // the step's keys: bucket sort where the key space allows it (1) or LSD radix sort (0)
+    int opt_rows = 1;     // density records the neighbour rows for force loop 5 (1) or force looks them up (0)
     int opt_autotune = 1;
     int opt_color = 0;   // permute colours every step (set by the first non-default colour pushed)
     int opt_frame = 0;   // surface vertices: 0 = world coordinates, 1 = the reference's [0, 10]^3 frame
@@ -106,7 +109,7 @@ struct rtps_sph {
         for (auto& g : gexec_t)
             if (g) (void)hipGraphExecDestroy(g);
         void* bufs[] = {pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s,  force_s,  xsph_s,   color_u,
-                        color_s, keys, idx, start, end, d_num, sort_tmp, sub_start, tile_q};
+                        color_s, keys, idx, start, end, d_num, sort_tmp, sub_start, tile_q, rows};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         void* sbufs[] = {cf_field, mc_cases, mc_counts, mc_offs, mc_tiles, mc_total, mc_words, mc_ntri, mesh_v, mesh_n,
@@ -145,6 +148,13 @@ Win win(const rtps_sph* s, bool force = false) {
           s->opt_remap, force ? s->block_force : s->block_density, force ? s->force_variant : -1};
     w.sub = sub_table(s);
     w.tq = s->opt_queues ? s->tile_q : nullptr;
+    // both launches of a step take the same choice: density records the rows
+    // exactly when the force launch will read them (loop 5)
+    const int var = s->force_variant >= 0 ? s->force_variant : g_force_variant;
+    if (s->opt_rows && s->rows && var == 5) {
+        w.rows = s->rows;
+        w.rows_stride = (uint32_t)s->max_num;
+    }
     return w;
 }
 
@@ -670,6 +680,7 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
             RTPS_TRY_HIP(dalloc(&s->sub_start, (size_t)s->nb_cells * kSubCells));
         // the bucket sort's temp, a superset of the radix sort's, for every count up to N
         RTPS_TRY_HIP(hipMalloc(&s->sort_tmp, bucket_temp_bytes((int)N, s->key_bits)));
+        RTPS_TRY_HIP(dalloc(&s->rows, 27 * N));
         float4* zf[] = {s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s, s->ve_s, s->force_s, s->xsph_s};
         for (float4* p : zf) RTPS_TRY_HIP(hipMemsetAsync(p, 0, N * sizeof(float4), s->stream));
         {   // every slot starts with the default colour of pushParticles (System.h:160)
@@ -1301,6 +1312,10 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
             s->opt_bucket = value;
             s->graph_valid = false;
             s->timed_graph_valid = false;
+            break;
+        case RTPS_OPT_SHARED_ROWS:
+            if (value != 0 && value != 1) return set_error(RTPS_ERR_ARG, "shared rows option must be 0 or 1");
+            s->opt_rows = value;
             break;
         case RTPS_OPT_TILE_QUEUES:
             if (value != 0 && value != 1) return set_error(RTPS_ERR_ARG, "tile queues option must be 0 or 1");

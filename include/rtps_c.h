@@ -322,7 +322,15 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        has at most 2^16 buckets averaging at most 1024 keys
  *                        (default), 0 = the LSD radix sort of rtps_sort_pairs.
  *                        The same order either way (keys ascending, equal keys
- *                        in particle order). */
+ *                        in particle order).
+ *   RTPS_OPT_SHARED_ROWS 1 = with force loop 5, the density kernel records each
+ *                        particle's nine neighbour windows and the within-h
+ *                        bits of their last 32 candidates, and the force
+ *                        kernel reads them instead of looking the windows up
+ *                        and testing those candidates again (default; 108
+ *                        bytes of device memory per particle slot, allocated
+ *                        at creation); 0 = the force kernel looks them up.
+ *                        The same results either way. */
 #define RTPS_OPT_COLLISION 0
 #define RTPS_OPT_GRAPH 1
 #define RTPS_OPT_FORCE_OUT 2
@@ -339,6 +347,7 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
 #define RTPS_OPT_SURFACE_FRAME 13
 #define RTPS_OPT_TILE_QUEUES 14
 #define RTPS_OPT_BUCKET_SORT 15
+#define RTPS_OPT_SHARED_ROWS 16
 int rtps_sph_set_option(rtps_sph* s, int option, int value);
 
 /* Autotuner (no reference counterpart): runs one step (as rtps_sph_step(s, 1))
