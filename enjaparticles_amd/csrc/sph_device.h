@@ -28,6 +28,7 @@ struct Grid {
     float3 delta;
     float rx, ry;        // grid_res.x, grid_res.y as floats (the hash is evaluated in float)
     int nb_cells;
+    int irx, iry;        // (int) rx, (int) ry: kernel arguments, so row offsets are scalar arithmetic
 };
 
 inline Grid make_grid(const rtps_grid_params& g) {
@@ -37,6 +38,8 @@ inline Grid make_grid(const rtps_grid_params& g) {
     d.rx = g.grid_res.x;
     d.ry = g.grid_res.y;
     d.nb_cells = g.nb_cells;
+    d.irx = (int)d.rx;
+    d.iry = (int)d.ry;
     return d;
 }
 
@@ -162,6 +165,8 @@ struct WindowCtx {
     float ux, uy, uz;        // continuous cell coordinates (p - min) * delta
     float wy, wz;            // cell widths in y and z (1 / delta)
     float hx;                // h in x cell units (h * delta.x)
+    int hc;                  // linear hash of cell (c.x - 1, c.y, c.z): row (dy, dz) starts at
+                             // hc + (dz * res_y + dy) * res_x
     bool on;                 // windowing allowed for this particle
 };
 
@@ -174,6 +179,10 @@ __device__ __forceinline__ WindowCtx window_ctx(float px, float py, float pz, co
     w.wy = 1.0f / g.delta.y;
     w.wz = 1.0f / g.delta.z;
     w.hx = h * g.delta.x;
+    w.hc = (c.z * g.iry + c.y) * g.irx + (c.x - 1);
+    // kept opaque: otherwise the compiler folds hc + (dz * res_y + dy) * res_x
+    // back into one multiply per row (a quarter-rate integer multiply)
+    asm volatile("" : "+v"(w.hc));
     // inside the grid and away from its x edges (no aliased cells in the row)
     w.on = keys != nullptr && w.ux >= 1.0f && w.uy >= 0.0f && w.uz >= 0.0f && c.x + 1 < (int)res_x &&
            w.uy < res_y && w.uz < res_z;
@@ -216,27 +225,29 @@ __device__ __forceinline__ void row_window_table(const WindowCtx& w, int3 c, int
         return;
     }
     const uint32_t nbc = (uint32_t)g.nb_cells;
-    const int hb = ((c.z + dz) * (int)g.ry + (c.y + dy)) * (int)g.rx + (c.x - 1);
-    // hash of cell k of the row, nb_cells when outside the grid (no arrays:
-    // dynamically indexed ones would live in scratch memory)
-    auto hash_k = [&](int k) {
-        const uint32_t hh = (uint32_t)(hb + k);
-        return hh < nbc ? hh : nbc;
+    // the row's first cell: the particle's hc plus a wave-uniform offset
+    // (scalar arithmetic on the grid size)
+    const int hb = w.hc + (dz * g.iry + dy) * g.irx;
+    // base[k] through a 32-bit byte offset (k * 4 < 2^29 here): one address
+    // register per offset, shared by the start and end loads
+    auto ld = [](const uint32_t* __restrict__ base, uint32_t k) {
+        return *(const uint32_t*)((const char*)base + k * 4u);
     };
-    const uint32_t h0 = hash_k(0), h1 = hash_k(1), h2c = hash_k(2);
+    // hash of cell k of the row, nb_cells when outside the grid
+    const uint32_t h0 = min((uint32_t)hb, nbc), h1 = min((uint32_t)(hb + 1), nbc), h2c = min((uint32_t)(hb + 2), nbc);
     // hardware square root (about 1 ulp): the window is widened by a whole
     // sub-cell on each side, so the rounding never drops a candidate
     const float half = __builtin_amdgcn_sqrtf(fmaxf(h2 - d2, 0.0f)) * (w.hx * __builtin_amdgcn_rsqf(h2));
     const float rel = w.ux - (float)(c.x - 1);
     const int r_lo = max((int)floorf((rel - half) * (float)kSubCells) - 1, 0);
     const int r_hi = min((int)floorf((rel + half) * (float)kSubCells) + 1, 3 * kSubCells - 1);
-    // table entry q of the cell holding sub-key r (cell 2 for r past the row)
-    auto entry = [&](int r) {
-        const uint32_t hc = hash_k(min(r >> kSubBits, 2));
-        return sub[(size_t)min(hc, nbc - 1u) * kSubCells + (uint32_t)(r & (kSubCells - 1))];
-    };
-    const uint32_t s0 = start[h0], s1 = start[h1], s2 = start[h2c];
-    const uint32_t e0 = end[h0], e1 = end[h1], e2 = end[h2c];
+    // table entry of sub-key r: the row's cells are consecutive, so entry q
+    // of cell r >> kSubBits is sub[hb * kSubCells + r]; indexes outside the
+    // table are clamped into it (their cells are outside the grid, or r is
+    // past the row: such entries are discarded below)
+    auto entry = [&](int r) { return ld(sub, min((uint32_t)(hb * kSubCells + r), nbc * kSubCells - 1u)); };
+    const uint32_t s0 = ld(start, h0), s1 = ld(start, h1), s2 = ld(start, h2c);
+    const uint32_t e0 = ld(end, h0), e1 = ld(end, h1), e2 = ld(end, h2c);
     const uint32_t t_lo = entry(r_lo), t_hi = entry(r_hi + 1);
     const bool o0 = h0 < nbc && s0 != kEmpty, o1 = h1 < nbc && s1 != kEmpty, o2 = h2c < nbc && s2 != kEmpty;
     const uint32_t hi = max(o0 ? e0 : 0u, max(o1 ? e1 : 0u, o2 ? e2 : 0u));
