@@ -229,8 +229,8 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
         // the tests of a chunk shift one rejection bit per candidate into a
         // word (rej_bit: the chunk's first candidate ends in its highest bit),
         // the particle's own bit is cleared, then the pair terms run once per
-        // accepted bit, highest first (the same ascending order and arithmetic
-        // as walk, so the same sums)
+        // accepted candidate, in ascending order (the same order and
+        // arithmetic as walk, so the same sums)
         auto walk_bits = [&](uint32_t a, uint32_t b, uint32_t off) {
             for (uint32_t j0 = a; j0 < b; j0 += 32u) {
                 const uint32_t n = min(b - j0, 32u);
@@ -248,13 +248,13 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                     for (int t = 0; t < kGroup; ++t) test(q[t]);
                 }
                 for (; k < n; ++k) test(pc[k]);
-                uint32_t m = ~rej & low_bits(n);
+                // bit t = candidate t (reversed once): lowest set bit first
+                uint32_t m = __builtin_bitreverse32(~rej) >> (32u - n);
                 const uint32_t self = (uint32_t)i - j0;   // the particle itself passes (r = 0): drop it
-                if (self < n) m &= ~(1u << (n - 1u - self));
+                if (self < n) m &= ~(1u << self);
                 while (m != 0u) {
-                    const uint32_t hb = 31u - (uint32_t)__builtin_clz(m);
-                    m ^= 1u << hb;
-                    const uint32_t t = n - 1u - hb;
+                    const uint32_t t = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1u;
                     const float4 pj = pc[t];
                     const float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
                     const float r2 = rx * rx + ry * ry + rz * rz;

@@ -462,8 +462,8 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
         // into a word (rej_bit: the chunk's first candidate ends in its
         // highest bit); loop 6 takes the last chunk's word from the rows the
         // density kernel recorded (rej_last) instead.  The particle's own bit
-        // is cleared, then the pair terms run once per accepted bit, highest
-        // first.  The wave's pair iterations are its lanes' largest accepted
+        // is cleared, then the pair terms run once per accepted candidate, in
+        // ascending order.  The wave's pair iterations are its lanes' largest accepted
         // count instead of their longest window, and the pairs are summed in
         // the same ascending order as walk (the same bits).
         auto walk_bits = [&](const float4* __restrict__ P, const float4* __restrict__ V, uint32_t a, uint32_t b,
@@ -488,13 +488,14 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
                     }
                     for (; k < n; ++k) test(pc[k]);
                 }
-                uint32_t m = ~rej & low_bits(n);
+                // accepted bits reversed once, so that bit t is candidate t: the
+                // pair loop takes the lowest set bit (ascending j as before)
+                uint32_t m = __builtin_bitreverse32(~rej) >> (32u - n);
                 const uint32_t self = (uint32_t)i - j0;   // the particle itself passes (r = 0): drop it
-                if (self < n) m &= ~(1u << (n - 1u - self));
+                if (self < n) m &= ~(1u << self);
                 while (m != 0u) {
-                    const uint32_t hb = 31u - (uint32_t)__builtin_clz(m);
-                    m ^= 1u << hb;
-                    const uint32_t t = n - 1u - hb;
+                    const uint32_t t = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1u;
                     const float4 pq = pc[t];
                     float4 pj = pq;
                     if (SCALED) { pj.x *= scale; pj.y *= scale; pj.z *= scale; }
