@@ -107,7 +107,11 @@ __global__ __launch_bounds__(kBBlock) void k_cellindices_b(int T, const uint32_t
 // persistent loop so that the loop has no divergent exits: every lane reaches
 // the loop's barriers together (a divergent continue there lets the compiler
 // retire lanes one by one, and a parked lane 0 would stop fetching items).
-template <int MODE>
+// ROWS: density records each row's window and the rejection bits of its last
+// 32 candidates at rows + 3 (r * rows_stride + i) (Win::rows of the plain
+// kernels); the force modes read them instead of looking the rows up, and
+// take the last chunk's bits from the record.
+template <int MODE, bool ROWS>
 __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool deleted,
                                           const float4* __restrict__ pos_s, const uint32_t* __restrict__ start,
                                           const uint32_t* __restrict__ end, const uint32_t* __restrict__ keys,
@@ -115,7 +119,8 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                                           const uint32_t* __restrict__ idx, float4* __restrict__ pos_u,
                                           float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
                                           float4* __restrict__ force_out, float4* __restrict__ xsph_out,
-                                          const uint32_t* __restrict__ sub_all, float4* __restrict__ stage) {
+                                          const uint32_t* __restrict__ sub_all, float4* __restrict__ stage,
+                                          uint32_t* __restrict__ rows, uint32_t rows_stride) {
     const rtps_sph_params& sp = S.sp;
     const uint32_t* sub = sub_all ? sub_all + (size_t)S.cell_base * kSubCells : nullptr;
     const float scale = sp.simulation_scale;
@@ -143,10 +148,16 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
         A = __ockl_wfred_min_u32(a < b ? a : 0xFFFFFFFFu);
         B = __ockl_wfred_max_u32(a < b ? b : 0u);
     };
+    // the row's record of particle i (ROWS)
+    auto rec = [&](int dz, int dy) { return rows + 3 * ((size_t)(3 * (dz + 1) + dy + 1) * rows_stride + (size_t)i); };
     if (MODE == kPairsDensity) {
         float sum = 0.0f;
-        auto term = [&](float4 pj) {
-            poly6_add_r(sum, pi.x - pj.x * scale, pi.y - pj.y * scale, pi.z - pj.z * scale, h2);
+        uint32_t rej = 0u;   // ROWS: the current row's rejection bits
+        auto term = [&](float4 pj) {   // poly6_add_r, keeping h^2 - r^2 for the record
+            const float u = h2_minus_r2(pi.x - pj.x * scale, pi.y - pj.y * scale, pi.z - pj.z * scale, h2);
+            const float t = fmaxf(u, 0.0f);
+            sum = fmaf(t * t, t, sum);
+            if (ROWS) rej = rej_bit(rej, u);
         };
         // candidate j at P[j - off]
         auto walk = [&](const float4* __restrict__ P, uint32_t a, uint32_t b, uint32_t off) {
@@ -166,14 +177,22 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                 uint32_t a, b, A, B;
                 row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix, sub);
                 wave_range(a, b, A, B);
-                if (B <= A) continue;
-                if (B - A <= (uint32_t)(2 * kBStageCap)) {   // density uses the whole slice
-                    for (uint32_t t = rank; t < B - A; t += nlive) stage[t] = pos_s[A + t];
-                    __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
-                    walk(stage, a, b, A);
-                    __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
-                } else {
-                    walk(pos_s, a, b, 0u);
+                if (B > A) {
+                    if (B - A <= (uint32_t)(2 * kBStageCap)) {   // density uses the whole slice
+                        for (uint32_t t = rank; t < B - A; t += nlive) stage[t] = pos_s[A + t];
+                        __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
+                        walk(stage, a, b, A);
+                        __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
+                    } else {
+                        walk(pos_s, a, b, 0u);
+                    }
+                }
+                if (ROWS) {
+                    uint32_t* __restrict__ o = rec(dz, dy);
+                    o[0] = a;
+                    o[1] = b;
+                    o[2] = rej;
+                    rej = 0u;
                 }
             }
         ve[i].w = sp.wpoly6_coef * sp.mass * sum;
@@ -225,29 +244,35 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                 }
             }
         };
-        // the staged copies, a lane's window in chunks of up to 32 candidates:
-        // the tests of a chunk shift one rejection bit per candidate into a
-        // word (rej_bit: the chunk's first candidate ends in its highest bit),
-        // the particle's own bit is cleared, then the pair terms run once per
-        // accepted candidate, in ascending order (the same order and
+        // the staged copies, a lane's window in chunks (a ragged first one of
+        // 1-32 candidates, then 32s, so the last chunk is the window's last
+        // min(b - a, 32)): the tests of a chunk shift one rejection bit per
+        // candidate into a word (rej_bit: the chunk's first candidate ends in
+        // its highest bit), or (ROWS) the last chunk's word comes from the
+        // record; the particle's own bit is cleared, then the pair terms run
+        // once per accepted candidate, in ascending order (the same order and
         // arithmetic as walk, so the same sums)
-        auto walk_bits = [&](uint32_t a, uint32_t b, uint32_t off) {
-            for (uint32_t j0 = a; j0 < b; j0 += 32u) {
-                const uint32_t n = min(b - j0, 32u);
+        auto walk_bits = [&](uint32_t a, uint32_t b, uint32_t off, uint32_t rej_last) {
+            uint32_t n = ((b - a - 1u) & 31u) + 1u;
+            for (uint32_t j0 = a; j0 < b; j0 += n, n = 32u) {
                 const float4* __restrict__ pc = stage + (j0 - off);
-                uint32_t rej = 0u, k = 0u;
-                auto test = [&](float4 pj) {
-                    const float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
-                    rej = rej_bit(rej, h2_minus_r2(rx, ry, rz, h2));
-                };
-                for (; k + kGroup <= n; k += kGroup) {
-                    float4 q[kGroup];
+                uint32_t rej = rej_last;
+                if (!ROWS || j0 + n != b) {
+                    rej = 0u;
+                    uint32_t k = 0u;
+                    auto test = [&](float4 pj) {
+                        const float rx = pi.x - pj.x * scale, ry = pi.y - pj.y * scale, rz = pi.z - pj.z * scale;
+                        rej = rej_bit(rej, h2_minus_r2(rx, ry, rz, h2));
+                    };
+                    for (; k + kGroup <= n; k += kGroup) {
+                        float4 q[kGroup];
 #pragma unroll
-                    for (int t = 0; t < kGroup; ++t) q[t] = pc[k + t];
+                        for (int t = 0; t < kGroup; ++t) q[t] = pc[k + t];
 #pragma unroll
-                    for (int t = 0; t < kGroup; ++t) test(q[t]);
+                        for (int t = 0; t < kGroup; ++t) test(q[t]);
+                    }
+                    for (; k < n; ++k) test(pc[k]);
                 }
-                for (; k < n; ++k) test(pc[k]);
                 // bit t = candidate t (reversed once): lowest set bit first
                 uint32_t m = __builtin_bitreverse32(~rej) >> (32u - n);
                 const uint32_t self = (uint32_t)i - j0;   // the particle itself passes (r = 0): drop it
@@ -265,8 +290,15 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
         };
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy) {
-                uint32_t a, b, A, B;
-                row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix, sub);
+                uint32_t a, b, A, B, rej_last = 0u;
+                if constexpr (ROWS) {
+                    const uint32_t* __restrict__ o = rec(dz, dy);
+                    a = o[0];
+                    b = o[1];
+                    rej_last = o[2];
+                } else {
+                    row_window(wc, c, dy, dz, h2, g, st, en, keys, a, b, prefix, sub);
+                }
                 wave_range(a, b, A, B);
                 if (B <= A) continue;
                 if (B - A <= (uint32_t)kBStageCap) {
@@ -277,7 +309,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
                         stage_v[t] = e;
                     }
                     __builtin_amdgcn_wave_barrier();   // the copies before any lane reads them
-                    walk_bits(a, b, A);
+                    walk_bits(a, b, A, rej_last);
                     __builtin_amdgcn_wave_barrier();   // every read before the next row's copies
                 } else {
                     walk(pos_s, ve, a, b, 0u, false);
@@ -307,7 +339,7 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
 // order for the adjoint's recorded step).  BLOCK threads per block = slots per
 // work item; compiled for WAVES waves per SIMD (the register budget that keeps
 // the grid's WAVES * 256 / BLOCK blocks per CU resident).
-template <int MODE, int BLOCK, int WAVES>
+template <int MODE, int BLOCK, int WAVES, bool ROWS>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_pairs_b(const BatchItem* __restrict__ items, int n_items,
                                                      int* __restrict__ queue, const SceneDev* __restrict__ scenes,
                                                      const int* __restrict__ d_num, const float4* __restrict__ pos_s,
@@ -318,7 +350,8 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_pairs_b(const BatchItem* __res
                                                      const uint32_t* __restrict__ idx, float4* __restrict__ pos_u,
                                                      float4* __restrict__ vel_u, float4* __restrict__ veleval_u,
                                                      float4* __restrict__ force_out, float4* __restrict__ xsph_out,
-                                                     const uint32_t* __restrict__ sub) {
+                                                     const uint32_t* __restrict__ sub, uint32_t* __restrict__ rows,
+                                                     uint32_t rows_stride) {
     __shared__ float4 stage_b[BLOCK / 64][2 * kBStageCap];   // one 6 KB slice per wave
     int q = (int)(blockIdx.x % kTileQueues);   // thread 0's current queue
     // the loop has one block-uniform exit (it < 0): every block leaves once all queues are drained
@@ -332,8 +365,8 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_pairs_b(const BatchItem* __res
         const bool deleted = scene_num(S, d_num[s], start, num);
         const int i = I.first + (int)threadIdx.x;
         if ((int)threadIdx.x < I.count && i - S.off < num)
-            pairs_one<MODE>(i, s, S, deleted, pos_s, start, end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
-                            force_out, xsph_out, sub, stage_b[threadIdx.x >> 6]);
+            pairs_one<MODE, ROWS>(i, s, S, deleted, pos_s, start, end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
+                                  force_out, xsph_out, sub, stage_b[threadIdx.x >> 6], rows, rows_stride);
         it = -1;
     }
     finish_tiles(queue);
@@ -376,16 +409,24 @@ hipError_t launch_pairs_b(int mode, const PairsCfg& cfg, int grid, const BatchIt
                           const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
                           const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,
                           const uint32_t* idx, float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
-                          float4* force_out, float4* xsph_out, const uint32_t* sub) {
+                          float4* force_out, float4* xsph_out, const uint32_t* sub, uint32_t* rows,
+                          uint32_t rows_stride) {
     (void)hipGetLastError();   // drop a stale error left by unrelated earlier HIP calls
     if (mode == kPairsForceStore && (!force_out || !xsph_out)) return hipErrorInvalidValue;
+    if (rows && rows_stride == 0) return hipErrorInvalidValue;
     if (!pairs_cfg_valid(mode, cfg)) return hipErrorInvalidValue;
     // Instantiated shapes: density at 4, 5 and 6 waves, the force modes at 3
     // and 4 (pairs_cfg_valid has admitted only these).
     auto go = [&](auto m, auto b, auto w) {
-        k_pairs_b<decltype(m)::value, decltype(b)::value, decltype(w)::value><<<grid, decltype(b)::value, 0, st>>>(
-            items, n_items, queue, scenes, d_num, pos_s, start, end, keys, ve, vel_s, idx, pos_u, vel_u, veleval_u,
-            force_out, xsph_out, sub);
+        constexpr int M = decltype(m)::value, BL = decltype(b)::value, W = decltype(w)::value;
+        if (rows)
+            k_pairs_b<M, BL, W, true><<<grid, BL, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start, end,
+                                                           keys, ve, vel_s, idx, pos_u, vel_u, veleval_u, force_out,
+                                                           xsph_out, sub, rows, rows_stride);
+        else
+            k_pairs_b<M, BL, W, false><<<grid, BL, 0, st>>>(items, n_items, queue, scenes, d_num, pos_s, start, end,
+                                                            keys, ve, vel_s, idx, pos_u, vel_u, veleval_u, force_out,
+                                                            xsph_out, sub, nullptr, 0u);
     };
     auto by_waves = [&](auto m, auto b) {
         if constexpr (decltype(m)::value == kPairsDensity) {

@@ -52,6 +52,10 @@ struct rtps_batch {
     int* queues = nullptr;
     void* sort_tmp = nullptr;
     uint32_t* sub = nullptr;   // sub-cell starts of the windowed rows, 16 per cell
+    // neighbour rows the density kernel records for the force kernel: 9 rows
+    // x 3 words per slot (used while opt_rows is set)
+    uint32_t* rows = nullptr;
+    int opt_rows = 1;
     int shifts[4] = {}, widths[4] = {}, npasses = 0;
     int opt_graph = 1, opt_timers = 0;
     hipGraphExec_t gexec = nullptr;
@@ -64,7 +68,7 @@ struct rtps_batch {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         void* bufs[] = {scenes, d_num, sid, pos_u, vel_u, veleval_u, pos_s, vel_s, ve_s, keys, idx,
-                        start, end, items, items128, items64, queues, sort_tmp, sub};
+                        start, end, items, items128, items64, queues, sort_tmp, sub, rows};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         for (auto& e : ev)
@@ -102,10 +106,15 @@ int grid_of(const rtps_batch* b, const PairsCfg& c) {
     return std::max(1, std::min(nitems_of(b, c), b->cus * pairs_blocks_per_cu(c)));
 }
 
+// The recorded rows of a step (both launches of a step take the same choice:
+// density records them exactly when the force launch reads them).
+uint32_t* rows_of(const rtps_batch* b) { return b->opt_rows ? b->rows : nullptr; }
+
 int launch_density_b(rtps_batch* b, const PairsCfg& c) {
     RTPS_TRY_HIP(launch_pairs_b(kPairsDensity, c, grid_of(b, c), items_of(b, c), nitems_of(b, c), b->queues,
                                 b->scenes, b->d_num, b->pos_s, b->start, b->end, b->keys_sorted, b->ve_s, b->vel_s,
-                                b->idx_sorted, b->pos_u, b->vel_u, b->veleval_u, b->stream, nullptr, nullptr, b->sub));
+                                b->idx_sorted, b->pos_u, b->vel_u, b->veleval_u, b->stream, nullptr, nullptr, b->sub,
+                                rows_of(b), (uint32_t)b->T));
     return RTPS_OK;
 }
 
@@ -113,7 +122,7 @@ int launch_force_b(rtps_batch* b, const PairsCfg& c) {
     RTPS_TRY_HIP(launch_pairs_b(kPairsForceStep, c, grid_of(b, c), items_of(b, c), nitems_of(b, c),
                                 b->queues, b->scenes, b->d_num, b->pos_s, b->start, b->end,
                                 b->keys_sorted, b->ve_s, b->vel_s, b->idx_sorted, b->pos_u, b->vel_u, b->veleval_u,
-                                b->stream, nullptr, nullptr, b->sub));
+                                b->stream, nullptr, nullptr, b->sub, rows_of(b), (uint32_t)b->T));
     return RTPS_OK;
 }
 
@@ -458,6 +467,7 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
         RTPS_TRY_HIP(hipMalloc((void**)&b->items64, sizeof(BatchItem) * L.items64.size()));
         RTPS_TRY_HIP(hipMalloc((void**)&b->queues, kTileQueueWords * sizeof(int)));
         RTPS_TRY_HIP(hipMalloc(&b->sort_tmp, sort_temp_bytes((int)T, 32)));
+        RTPS_TRY_HIP(hipMalloc((void**)&b->rows, NT * 27 * 4));
         RTPS_TRY_HIP(hipMemcpyAsync(b->scenes, b->scenes_h.data(), sizeof(SceneDev) * num_scenes,
                                     hipMemcpyHostToDevice, b->stream));
         RTPS_TRY_HIP(hipMemsetAsync(b->d_num, 0, sizeof(int) * num_scenes, b->stream));
@@ -615,6 +625,10 @@ int rtps_batch_set_option(rtps_batch* b, int option, int value) {
         case RTPS_OPT_GRAPH: b->opt_graph = value ? 1 : 0; break;
         case RTPS_OPT_TIMERS: b->opt_timers = value ? 1 : 0; break;
         case RTPS_OPT_AUTOTUNE: b->opt_autotune = value ? 1 : 0; break;
+        case RTPS_OPT_SHARED_ROWS:
+            if (value != 0 && value != 1) return set_error(RTPS_ERR_ARG, "shared rows option must be 0 or 1");
+            b->opt_rows = value;
+            break;
         case RTPS_OPT_BLOCK_DENSITY:
         case RTPS_OPT_BLOCK_FORCE: {   // value = block * 10 + waves, e.g. 2565 (256 threads, 5 waves per SIMD)
             const PairsCfg c{value / 10, value % 10};

@@ -57,7 +57,8 @@ hipError_t launch_pairs_b(int mode, const PairsCfg& cfg, int grid, const BatchIt
                           const SceneDev* scenes, const int* d_num, const float4* pos_s, const uint32_t* start,
                           const uint32_t* end, const uint32_t* keys, float4* ve, const float4* vel_s,
                           const uint32_t* idx, float4* pos_u, float4* vel_u, float4* veleval_u, hipStream_t st,
-                          float4* force_out = nullptr, float4* xsph_out = nullptr, const uint32_t* sub = nullptr);
+                          float4* force_out = nullptr, float4* xsph_out = nullptr, const uint32_t* sub = nullptr,
+                          uint32_t* rows = nullptr, uint32_t rows_stride = 0);
 hipError_t launch_commit_b(int S, const SceneDev* scenes, int* d_num, const uint32_t* start, hipStream_t st);
 
 }  // namespace rtps

@@ -182,7 +182,7 @@ __device__ __forceinline__ WindowCtx window_ctx(float px, float py, float pz, co
     w.hc = (c.z * g.iry + c.y) * g.irx + (c.x - 1);
     // kept opaque: otherwise the compiler folds hc + (dz * res_y + dy) * res_x
     // back into one multiply per row (a quarter-rate integer multiply)
-    asm volatile("" : "+v"(w.hc));
+    asm("" : "+v"(w.hc));
     // inside the grid and away from its x edges (no aliased cells in the row)
     w.on = keys != nullptr && w.ux >= 1.0f && w.uy >= 0.0f && w.uz >= 0.0f && c.x + 1 < (int)res_x &&
            w.uy < res_y && w.uz < res_z;

@@ -505,7 +505,10 @@ int rtps_batch_read(rtps_batch* b, int scene, int field, void* host, int count);
  * RTPS_OPT_BLOCK_FORCE = block * 10 + waves (e.g. 2565: 256 threads per block
  * = slots per work item, compiled for 5 waves per SIMD; block 64, 128 or 256,
  * waves 4..6 for density and 3..4 for force, whose registers would spill at
- * 5 waves), which fix that kernel's launch. */
+ * 5 waves), which fix that kernel's launch; RTPS_OPT_SHARED_ROWS (1 = the
+ * density kernel records each particle's neighbour windows and within-h bits
+ * for the force kernel, default; 0 = the force kernel looks them up; the same
+ * results either way; 108 bytes of device memory per slot). */
 int rtps_batch_set_option(rtps_batch* b, int option, int value);
 /* Autotuner of the batched pair kernels (no reference counterpart): runs one
  * step (as rtps_batch_step(b, 1)) during which the density and the force

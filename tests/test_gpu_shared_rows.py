@@ -20,7 +20,7 @@ import oracle as O
 from enjaparticles_amd import scenes
 from enjaparticles_amd._native import (DEFAULT_FORCE_LOOP, OPT_BLOCK_DENSITY, OPT_BLOCK_FORCE, OPT_FORCE_VARIANT,
                                        OPT_SHARED_ROWS, OPT_TILE_QUEUES, RTPSError)
-from enjaparticles_amd.rtps import RTPS
+from enjaparticles_amd.rtps import RTPS, BatchedSPH
 
 pytestmark = pytest.mark.gpu
 
@@ -124,8 +124,51 @@ def test_full_size_dam_recorded_rows(cuda):
         np.testing.assert_array_equal(x, y)
 
 
+def _batched_run(scs, rows, steps):
+    b = BatchedSPH([sc.settings for sc in scs])
+    for s, sc in enumerate(scs):
+        b.pushParticles(s, sc.pos, sc.vel)
+    b.setOption(OPT_SHARED_ROWS, rows)
+    b.step(steps)
+    out = [(b.getNum(s), b.read(s, "pos"), b.read(s, "vel"), b.read(s, "density")) for s in range(len(scs))]
+    b.close()
+    return out
+
+
+def test_batched_recorded_rows_give_the_looked_up_step(cuda):
+    """The batched pair kernels with recorded rows (the default) against the
+    same kernels looking the rows up: every scene bit for bit, including a
+    scene at a simulation scale of 0.31, a scene losing particles out of its
+    grid (deletion), scenes of 0, 1, 2 and 65 particles, and a dense scene
+    whose windows exceed 32 candidates."""
+    scs = scenes.batched(6, 600, 6000, seed=88)
+    for s, n in [(0, 1), (1, 65), (2, 0), (3, 2)]:
+        sc = scs[s]
+        scs[s] = scenes.Scene(sc.name, sc.settings, sc.pos[:n].copy(), sc.vel[:n].copy(), sc.steps, sc.dt)
+    st, pos = _scaled()
+    scs.append(scenes.Scene("scaled", st, pos, np.zeros_like(pos), 3, 0.003))
+    big = scenes.batched(1, 6000, 6000, seed=89)[0]
+    pos = big.pos.copy()
+    pos[::50, 2] = 60.0                       # out of the scene's grid: deleted
+    scs.append(scenes.Scene("delete", big.settings, pos, big.vel, 3, big.dt))
+    box = dict(scenes.dam16k().settings, max_num_particles="20000")
+    dense = _uniform(20000, 0.3, 0.7, 10)
+    scs.append(scenes.Scene("dense", box, dense, np.zeros_like(dense), 2, 0.003))
+    a = _batched_run(scs, 1, 3)
+    b = _batched_run(scs, 0, 3)
+    for s, (x, y) in enumerate(zip(a, b)):
+        assert x[0] == y[0], s
+        for u, v, what in zip(x[1:], y[1:], ("pos", "vel", "density")):
+            np.testing.assert_array_equal(u, v, err_msg=f"scene {s}: {what}")
+    assert a[-2][0] == len(pos) - len(pos[::50])
+
+
 def test_shared_rows_option_is_checked(cuda):
     s = RTPS.generateSystemInstance(scenes.dam16k().settings)
     with pytest.raises(RTPSError):
         s.setOption(OPT_SHARED_ROWS, 2)
     s.close()
+    b = BatchedSPH([scenes.dam16k().settings])
+    with pytest.raises(RTPSError):
+        b.setOption(OPT_SHARED_ROWS, 2)
+    b.close()
