@@ -151,8 +151,11 @@ struct Win {
     // Recorded rows (nullptr = none): for particle i and row r = 3 (dz + 1) +
     // (dy + 1), the three words at rows + 3 (r * rows_stride + i) hold its
     // window [a, b) and the rejection bits (rej_bit) of the window's last
-    // min(b - a, 32) candidates.  The density kernel writes them; force loop
-    // kLoopStagedBits then reads them instead of looking the windows up and
+    // min(b - a, 32) candidates; after the 27 rows_stride words, two words per
+    // row for each wave of 64 particles (18 per wave, from rows + 27 rows_stride
+    // + 18 (i >> 6)) hold the wave's union [A, B) of its live lanes' windows.
+    // The density kernel writes them; force loop kLoopStagedBits then reads
+    // them instead of looking the windows up, reducing them over the wave and
     // testing those candidates again.
     uint32_t* rows = nullptr;
     uint32_t rows_stride = 0;   // particles per row plane (at least the particle count)

@@ -191,22 +191,30 @@ __device__ __forceinline__ void density_tile(int tile, int num_host, const int* 
         for (; j < b; ++j) accum(P[j - off]);
     };
     uint32_t* __restrict__ rec = EMIT ? win.rows + 3 * (size_t)i : nullptr;
-    auto record = [&](int dz, int dy, uint32_t a, uint32_t b) {
+    // the wave's row unions [A, B) follow the particles' records, two words
+    // per row for each wave of 64 particles (written by its first live lane)
+    uint32_t* __restrict__ wrec = EMIT ? win.rows + 27 * (size_t)win.rows_stride + 18 * (size_t)(i >> 6) : nullptr;
+    const uint64_t live = __ballot(true);   // lanes past num have returned
+    const uint32_t nlive = (uint32_t)__popcll(live);
+    const uint32_t rank = (uint32_t)__popcll(live & ((1ull << (threadIdx.x & 63)) - 1ull));
+    auto record = [&](int dz, int dy, uint32_t a, uint32_t b, uint32_t A, uint32_t B) {
         if (EMIT) {
-            uint32_t* __restrict__ o = rec + 3 * (size_t)((3 * (dz + 1) + dy + 1) * win.rows_stride);
+            const int r = 3 * (dz + 1) + dy + 1;
+            uint32_t* __restrict__ o = rec + 3 * (size_t)(r * win.rows_stride);
             o[0] = a;
             o[1] = b;
             o[2] = rej;
             rej = 0u;
+            if (rank == 0) {
+                wrec[2 * r] = A;
+                wrec[2 * r + 1] = B;
+            }
         }
     };
     if constexpr (STAGE) {
         extern __shared__ float4 stage_rows[];   // kStageCap per wave
         float4* __restrict__ my = stage_rows + (threadIdx.x >> 6) * kStageCap;
-        // the wave's live lanes share the copies (lanes past num have returned)
-        const uint64_t live = __ballot(true);
-        const uint32_t nlive = (uint32_t)__popcll(live);
-        const uint32_t rank = (uint32_t)__popcll(live & ((1ull << (threadIdx.x & 63)) - 1ull));
+        // the wave's live lanes (rank of nlive) share the copies
         for (int dz = -1; dz <= 1; ++dz) {
             for (int dy = -1; dy <= 1; ++dy) {
                 uint32_t a, b;
@@ -223,7 +231,7 @@ __device__ __forceinline__ void density_tile(int tile, int num_host, const int* 
                         walk(pos_s, a, b, 0u);
                     }
                 }
-                record(dz, dy, a, b);
+                record(dz, dy, a, b, A, B);
             }
         }
     } else {
@@ -232,7 +240,9 @@ __device__ __forceinline__ void density_tile(int tile, int num_host, const int* 
                 uint32_t a, b;
                 row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
                 walk(pos_s, a, b, 0u);
-                record(dz, dy, a, b);
+                if (EMIT)
+                    record(dz, dy, a, b, __ockl_wfred_min_u32(a < b ? a : 0xFFFFFFFFu),
+                           __ockl_wfred_max_u32(a < b ? b : 0u));
             }
         }
     }
@@ -511,17 +521,22 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
         const uint32_t* __restrict__ rec = LOOP == kLoopShared ? win.rows + 3 * (size_t)i : nullptr;
         for (int dz = -1; dz <= 1; ++dz) {
             for (int dy = -1; dy <= 1; ++dy) {
-                uint32_t a, b, rej_last = 0u;
+                uint32_t a, b, A, B, rej_last = 0u;
                 if constexpr (LOOP == kLoopShared) {
-                    const uint32_t* __restrict__ o = rec + 3 * (size_t)((3 * (dz + 1) + dy + 1) * win.rows_stride);
+                    const int r = 3 * (dz + 1) + dy + 1;
+                    const uint32_t* __restrict__ o = rec + 3 * (size_t)(r * win.rows_stride);
                     a = o[0];
                     b = o[1];
                     rej_last = o[2];
+                    // the wave's union, recorded by the density kernel for the same 64 particles
+                    const uint32_t* __restrict__ u = win.rows + 27 * (size_t)win.rows_stride + 18 * (size_t)(i >> 6);
+                    A = __builtin_amdgcn_readfirstlane(u[2 * r]);
+                    B = __builtin_amdgcn_readfirstlane(u[2 * r + 1]);
                 } else {
                     row_window(wc, c, dy, dz, h2, g, start, end, win.keys, a, b, 0u, win.sub);
+                    A = __ockl_wfred_min_u32(a < b ? a : 0xFFFFFFFFu);
+                    B = __ockl_wfred_max_u32(a < b ? b : 0u);
                 }
-                const uint32_t A = __ockl_wfred_min_u32(a < b ? a : 0xFFFFFFFFu);
-                const uint32_t B = __ockl_wfred_max_u32(a < b ? b : 0u);
                 if (B <= A) continue;
                 if (B - A <= (uint32_t)kForceStageCap) {
                     // the copy keeps 1 / d_j in the position's unused w: one

@@ -56,7 +56,7 @@ struct rtps_sph {
     int* tile_q = nullptr;   // per-XCD tile queue counters of the neighbour kernels (kTileQueueWords)
     void* sort_tmp = nullptr;
     // the neighbour rows the density kernel records for force loop 5 (Win::rows):
-    // 9 rows x 3 words per particle slot
+    // 9 rows x 3 words per particle slot, then 9 x 2 words per wave of 64 slots
     uint32_t* rows = nullptr;
     GravitySources gsrc{};
     int opt_collision = 1, opt_graph = 1, opt_force_out = 0, opt_timers = 0, opt_window = 1, opt_remap = 0,
@@ -680,7 +680,7 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
             RTPS_TRY_HIP(dalloc(&s->sub_start, (size_t)s->nb_cells * kSubCells));
         // the bucket sort's temp, a superset of the radix sort's, for every count up to N
         RTPS_TRY_HIP(hipMalloc(&s->sort_tmp, bucket_temp_bytes((int)N, s->key_bits)));
-        RTPS_TRY_HIP(dalloc(&s->rows, 27 * N));
+        RTPS_TRY_HIP(dalloc(&s->rows, 27 * N + 18 * ((N + 63) / 64)));
         float4* zf[] = {s->pos_u, s->vel_u, s->veleval_u, s->pos_s, s->vel_s, s->ve_s, s->force_s, s->xsph_s};
         for (float4* p : zf) RTPS_TRY_HIP(hipMemsetAsync(p, 0, N * sizeof(float4), s->stream));
         {   // every slot starts with the default colour of pushParticles (System.h:160)
