@@ -161,16 +161,16 @@ __device__ __forceinline__ void pairs_one(int i, int s, const SceneDev& S, bool 
         };
         // candidate j at P[j - off]
         auto walk = [&](const float4* __restrict__ P, uint32_t a, uint32_t b, uint32_t off) {
-            uint32_t j = a;
-            for (; j + kGroup <= b; j += kGroup) {   // loads of a group issued together
-                const float4* __restrict__ pg = P + (j - off);
+            const float4* __restrict__ pg = P + (a - off);
+            const uint32_t n = a < b ? b - a : 0u;
+            for (uint32_t gr = n / kGroup; gr != 0u; --gr, pg += kGroup) {   // loads of a group issued together
                 float4 p[kGroup];
 #pragma unroll
                 for (int k = 0; k < kGroup; ++k) p[k] = pg[k];
 #pragma unroll
                 for (int k = 0; k < kGroup; ++k) term(p[k]);
             }
-            for (; j < b; ++j) term(P[j - off]);
+            for (uint32_t t = n % kGroup; t != 0u; --t, ++pg) term(*pg);
         };
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy) {

@@ -177,18 +177,20 @@ __device__ __forceinline__ void density_tile(int tile, int num_host, const int* 
         sum = fmaf(t * t, t, sum);
         if (EMIT) rej = rej_bit(rej, u);
     };
-    // candidates [a, b) of one row at P[j - off], kGroupDensity loads at a time
+    // candidates [a, b) of one row at P[j - off], kGroupDensity loads at a
+    // time; one pointer steps through the window (immediate offsets for the
+    // group's loads) under a count of the groups left
     auto walk = [&](const float4* __restrict__ P, uint32_t a, uint32_t b, uint32_t off) {
-        uint32_t j = a;
-        for (; j + kGroupDensity <= b; j += kGroupDensity) {
-            const float4* __restrict__ pg = P + (j - off);   // immediate offsets for the group's loads
+        const float4* __restrict__ pg = P + (a - off);
+        const uint32_t n = a < b ? b - a : 0u;
+        for (uint32_t g = n / kGroupDensity; g != 0u; --g, pg += kGroupDensity) {
             float4 p[kGroupDensity];
 #pragma unroll
             for (int k = 0; k < kGroupDensity; ++k) p[k] = pg[k];
 #pragma unroll
             for (int k = 0; k < kGroupDensity; ++k) accum(p[k]);
         }
-        for (; j < b; ++j) accum(P[j - off]);
+        for (uint32_t t = n % kGroupDensity; t != 0u; --t, ++pg) accum(*pg);
     };
     uint32_t* __restrict__ rec = EMIT ? win.rows + 3 * (size_t)i : nullptr;
     // the wave's row unions [A, B) follow the particles' records, two words
