@@ -240,8 +240,8 @@ int autotune_step_b(rtps_batch* b, bool measure) {
     b->opt_timers = timers;
     if (rc) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    RTPS_TRY_HIP(hipEventCreate(&e0));
-    hipError_t err = hipEventCreate(&e1);
+    RTPS_TRY_HIP(hipEventCreateWithFlags(&e0, kTimingEvent));
+    hipError_t err = hipEventCreateWithFlags(&e1, kTimingEvent);
     if (err != hipSuccess) {
         (void)hipEventDestroy(e0);
         return hip_error(err, "hipEventCreate");
@@ -484,7 +484,7 @@ int rtps_batch_create(int num_scenes, const int* n_settings, const char* const* 
         RTPS_TRY_HIP(sort_clear_temp(b->sort_tmp, (int)T, b->stream));
         b->keys_sorted = b->keys;
         b->idx_sorted = b->idx;
-        for (auto& e : b->ev) RTPS_TRY_HIP(hipEventCreate(&e));
+        for (auto& e : b->ev) RTPS_TRY_HIP(hipEventCreateWithFlags(&e, kTimingEvent));
         RTPS_TRY_HIP(hipStreamSynchronize(b->stream));
         *out = b.release();
         return RTPS_OK;

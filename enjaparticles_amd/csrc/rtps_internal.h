@@ -12,6 +12,13 @@
 
 namespace rtps {
 
+// Flags of the library's timing events (stage timers, autotuners): they only
+// time work on their stream and are never used to make device memory visible
+// to the host, so their records skip the system-scope fence (a cache
+// write-back and invalidate per record, which also leaves the next kernel's
+// caches cold).
+constexpr unsigned kTimingEvent = hipEventDisableSystemFence;
+
 struct Grid;
 struct GravitySources;
 struct Win;

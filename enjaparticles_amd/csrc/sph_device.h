@@ -524,8 +524,12 @@ __device__ __forceinline__ void force_epilogue(const ForceAcc& acc, float di, co
 }
 
 // gravity.cl:9-36 for one particle: accel += mag * fast_normalize(vect).
+// A source of magnitude alpha * mass = 0 adds exactly zero to every
+// component (0 times a finite unit vector), so it is skipped (a uniform
+// branch: the sources are kernel arguments).
 __device__ __forceinline__ void gravity_sources(float4& f, float4 pos, const GravitySources& gs, float scale) {
     for (int s = 0; s < gs.n; ++s) {
+        if (gs.alpha[s] * gs.mass[s] == 0.0f) continue;
         float vx = scale * (gs.pos[s].x - pos.x), vy = scale * (gs.pos[s].y - pos.y), vz = scale * (gs.pos[s].z - pos.z);
         float d2 = vx * vx + vy * vy + vz * vz;
         float inv = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;

@@ -226,7 +226,7 @@ struct StageTimer {
     explicit StageTimer(rtps_sph* s_, int set_ = 0) : s(s_), on(s_->opt_timers != 0), set(set_) {}
     void mark(int i) {
         if (!on) return;
-        if (s->opt_timers == 2 && i != ST_FORCE && i != ST_N) return;   // force stage only
+        if (s->opt_timers >= 2 && i != ST_FORCE && i != ST_N) return;   // force stage only
         hipError_t e = external ? hipEventRecordWithFlags(s->ev[set][i], s->stream, hipEventRecordExternal)
                                 : hipEventRecord(s->ev[set][i], s->stream);
         if (e != hipSuccess && err == hipSuccess) err = e;
@@ -337,7 +337,7 @@ int collect_timers(rtps_sph* s, int set) {
     if (!s->ev_pending[set]) return RTPS_OK;
     s->ev_pending[set] = false;
     RTPS_TRY_HIP(hipEventSynchronize(s->ev[set][ST_N]));
-    for (int i = s->opt_timers == 2 ? (int)ST_FORCE : 0; i < ST_N; ++i) {
+    for (int i = s->opt_timers >= 2 ? (int)ST_FORCE : 0; i < ST_N; ++i) {
         float ms = 0.f;
         RTPS_TRY_HIP(hipEventElapsedTime(&ms, s->ev[set][i], s->ev[set][i + 1]));
         s->stage_ms[i] += ms;
@@ -402,6 +402,9 @@ bool probe_graph_events(rtps_sph* s) {
 // Timed steps: step k records event set k & 1; the set is read back just
 // before it is recorded again (two steps later), so the host never waits for
 // the step the GPU is running and the timers add no gap between steps.
+// RTPS_OPT_TIMERS 3 times the force stage on one step in this many.
+constexpr int kTimerSample = 4;
+
 int step_timed(rtps_sph* s, int nsteps) {
     if (s->opt_graph && s->graph_events_ok < 0) s->graph_events_ok = probe_graph_events(s) ? 1 : 0;
     const bool graph = s->opt_graph && s->graph_events_ok == 1;
@@ -412,8 +415,26 @@ int step_timed(rtps_sph* s, int nsteps) {
         }
         s->timed_graph_valid = true;
     }
+    // RTPS_OPT_TIMERS 3: only every kTimerSample-th step carries the events;
+    // the steps between run the untimed graph (or launches)
+    const int every = s->opt_timers == 3 ? kTimerSample : 1;
+    if (graph && every > 1 && !s->graph_valid) {
+        int rc = capture_graph(s);
+        if (rc) return rc;
+    }
     for (int k = 0; k < nsteps; ++k) {
-        const int set = k & 1;
+        if (k % every != 0) {
+            if (graph) {
+                RTPS_TRY_HIP(hipGraphLaunch(s->gexec, s->stream));
+            } else {
+                StageTimer t(s);
+                t.on = false;
+                int rc = launch_step(s, t);
+                if (rc) return rc;
+            }
+            continue;
+        }
+        const int set = (k / every) & 1;
         int rc = collect_timers(s, set);
         if (rc) return rc;
         if (graph) {
@@ -545,8 +566,8 @@ int autotune_step(rtps_sph* s, bool measure) {
     int rc = launch_front(s, off);
     if (rc) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    RTPS_TRY_HIP(hipEventCreate(&e0));
-    hipError_t e = hipEventCreate(&e1);
+    RTPS_TRY_HIP(hipEventCreateWithFlags(&e0, kTimingEvent));
+    hipError_t e = hipEventCreateWithFlags(&e1, kTimingEvent);
     if (e != hipSuccess) {
         (void)hipEventDestroy(e0);
         return hip_error(e, "hipEventCreate");
@@ -699,7 +720,7 @@ int rtps_sph_create(const char* const* keys, const char* const* vals, int n, rtp
         s->keys_sorted = s->keys;
         s->idx_sorted = s->idx;
         for (auto& set : s->ev)
-            for (auto& e : set) RTPS_TRY_HIP(hipEventCreate(&e));
+            for (auto& e : set) RTPS_TRY_HIP(hipEventCreateWithFlags(&e, kTimingEvent));
         RTPS_TRY_HIP(hipStreamSynchronize(s->stream));
         *out = s.release();
         return RTPS_OK;
@@ -1277,7 +1298,7 @@ int rtps_sph_set_option(rtps_sph* s, int option, int value) {
         case RTPS_OPT_GRAPH: s->opt_graph = value ? 1 : 0; break;
         case RTPS_OPT_FORCE_OUT: s->opt_force_out = value ? 1 : 0; break;
         case RTPS_OPT_TIMERS:
-            if (value < 0 || value > 2) return set_error(RTPS_ERR_ARG, "timers option must be 0, 1 or 2");
+            if (value < 0 || value > 3) return set_error(RTPS_ERR_ARG, "timers option must be 0, 1, 2 or 3");
             if (value != s->opt_timers) s->timed_graph_valid = false;   // the timed graphs hold the events
             s->opt_timers = value;
             break;

@@ -267,7 +267,11 @@ int rtps_sph_get_params(rtps_sph* s, rtps_sph_params* sphp, rtps_grid_params* gp
  *                        alternate, so the host reads step k-2's times while
  *                        step k-1 runs and rtps_sph_step returns after the
  *                        last step finished; 2 = the same for the force
- *                        stage only (two events per step)
+ *                        stage only (two events per step); 3 = the force
+ *                        stage only on the first of every four steps of an
+ *                        rtps_sph_step call (the others run without events:
+ *                        each event record costs a few microseconds between
+ *                        kernels)
  *   RTPS_OPT_FORCE_VARIANT  force-loop kernel, process-wide default for handles
  *                        that are not autotuned: 0 = each lane walks its
  *                        windows in global memory, one branch per candidate;

@@ -254,7 +254,8 @@ def test_addbox_matches_addrect(cuda, nn, lo, hi):
 def test_stage_timers_do_not_change_the_step(cuda):
     """Per-stage timers (events inside the step graph, two alternating sets)
     time every step and leave the results bit-identical; the force-only mode
-    (RTPS_OPT_TIMERS 2) times just the force stage.  The hash runs inside the
+    (RTPS_OPT_TIMERS 2) times just the force stage, and mode 3 the force stage
+    on every fourth step (graph and eager launches).  The hash runs inside the
     sort's first kernel and the permute inside the cellindices kernel, so
     those two stages are empty."""
     sc = scenes.dam_custom(24, 24, 12, seed=78)
@@ -265,12 +266,20 @@ def test_stage_timers_do_not_change_the_step(cuda):
     c.setOption(3, 1)
     d = _system(sc, graph=True)
     d.setOption(3, 2)
-    for s in (a, b, c, d):
+    e = _system(sc, graph=True)
+    e.setOption(3, 3)
+    f = _system(sc, graph=False)
+    f.setOption(3, 3)
+    for s in (a, b, c, d, e, f):
         s.step(1)          # the first step tunes the launches (not timed)
         s.resetTimers()
         s.step(7)
-    for s in (b, c, d):
+    for s in (b, c, d, e, f):
         np.testing.assert_array_equal(s.read("pos"), a.read("pos"))
+    for s in (e, f):       # sampled force timers: steps 0 and 4 of the 7
+        st = s.stageTimes()
+        assert st["force"][1] == 2 and st["force"][0] > 0.0, st
+        assert all(cnt == 0 for k, (_, cnt) in st.items() if k != "force"), st
     for s in (b, c):
         st = s.stageTimes()
         assert all(cnt == 7 for _, cnt in st.values()), st
@@ -280,8 +289,8 @@ def test_stage_timers_do_not_change_the_step(cuda):
     assert st["force"][1] == 7 and st["force"][0] > 0.0, st
     assert all(cnt == 0 for k, (_, cnt) in st.items() if k != "force"), st
     with pytest.raises(RTPSError):
-        d.setOption(3, 3)
-    for s in (a, b, c, d):
+        d.setOption(3, 4)
+    for s in (a, b, c, d, e, f):
         s.close()
 
 
