@@ -585,11 +585,13 @@ __device__ __forceinline__ void collision_wall(float4& f, float4 p, float4 v, co
 }
 
 // leapfrog.cl:30-121 for one particle; returns the new unsorted records.
+// scaled = false: the caller knows simulation_scale is exactly 1, so the
+// multiplications and divisions by it (identities then) are left out.
 __device__ __forceinline__ void leapfrog(float4 p, float4 v, float4 f, float4 xs, const rtps_sph_params& sp, float dt,
-                                         float4& pos_out, float4& vel_out, float4& veleval_out) {
+                                         float4& pos_out, float4& vel_out, float4& veleval_out, bool scaled = true) {
 #pragma clang fp contract(off)
     const float s = sp.simulation_scale;
-    p.x *= s; p.y *= s; p.z *= s;
+    if (scaled) { p.x *= s; p.y *= s; p.z *= s; }
     f.x += sp.gravity.x; f.y += sp.gravity.y; f.z += sp.gravity.z;
     float speed = __builtin_sqrtf(f.x * f.x + f.y * f.y + f.z * f.z);
     if (speed > sp.velocity_limit) {
@@ -601,7 +603,7 @@ __device__ __forceinline__ void leapfrog(float4 p, float4 v, float4 f, float4 xs
     vn.x += xf * xs.x; vn.y += xf * xs.y; vn.z += xf * xs.z; vn.w += xf * xs.w;
     veleval_out = make_float4(0.5f * (v.x + vn.x), 0.5f * (v.y + vn.y), 0.5f * (v.z + vn.z), 0.5f * (v.w + vn.w));
     p.x += dt * vn.x; p.y += dt * vn.y; p.z += dt * vn.z;
-    pos_out = make_float4(p.x / s, p.y / s, p.z / s, 1.0f);
+    pos_out = scaled ? make_float4(p.x / s, p.y / s, p.z / s, 1.0f) : make_float4(p.x, p.y, p.z, 1.0f);
     vel_out = vn;
 }
 

@@ -566,7 +566,7 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
         const float4 v = vel_s[i];
         if (collision) collision_wall(F, p_unscaled, v, sp, bmin, bmax);
         float4 po, vo, eo;
-        leapfrog(p_unscaled, v, F, X, sp, dt, po, vo, eo);
+        leapfrog(p_unscaled, v, F, X, sp, dt, po, vo, eo, SCALED);
         const uint32_t o = deleted ? (uint32_t)i : idx[i];
         pos_u[o] = po;
         vel_u[o] = vo;
