@@ -491,6 +491,8 @@ def main():
     ap.add_argument("--launch", default="", metavar="BD,BF,LOOP",
                     help="fixed neighbour launches instead of autotuning: density and force threads per block and "
                          "the force loop (profiling runs: counters perturb the autotuner's timings)")
+    ap.add_argument("--shared-rows", type=int, default=-1, choices=(-1, 0, 1),
+                    help="RTPS_OPT_SHARED_ROWS of the plain system (-1 = the library default)")
     ap.add_argument("--tile-queues", type=int, default=0, choices=(0, 1),
                     help="1 = density and force as resident grids over per-XCD tile queues (RTPS_OPT_TILE_QUEUES)")
     ap.add_argument("--force-variant", type=int, default=5,
@@ -533,6 +535,8 @@ def main():
     s.setOption(N.OPT_GRAPH, 1)
     s.setOption(N.OPT_FORCE_VARIANT, args.force_variant)
     s.setOption(N.OPT_TILE_QUEUES, args.tile_queues)
+    if args.shared_rows >= 0:
+        s.setOption(N.OPT_SHARED_ROWS, args.shared_rows)
     # the timed steps record HIP events around the force stage on every fourth step (RTPS_OPT_TIMERS 3:
     # two events inside the graph of steps 0, 4, 8, ... of the timed region, the others replay the
     # graph without events, whose records cost a few microseconds each); the per-stage breakdown comes
