@@ -400,7 +400,7 @@ int pairs_blocks_per_cu(const PairsCfg& c) { return c.waves * 256 / c.block; }
 
 bool pairs_cfg_valid(int mode, const PairsCfg& c) {
     if (c.block != 64 && c.block != 128 && c.block != 256) return false;
-    return mode == kPairsDensity ? (c.waves >= 4 && c.waves <= 6) : (c.waves == 3 || c.waves == 4);
+    return mode == kPairsDensity ? (c.waves >= 4 && c.waves <= 6) : (c.waves >= 3 && c.waves <= 5);
 }
 
 PairsCfg pairs_default_cfg(int mode) { return mode == kPairsDensity ? PairsCfg{256, 5} : PairsCfg{256, 4}; }
@@ -415,8 +415,8 @@ hipError_t launch_pairs_b(int mode, const PairsCfg& cfg, int grid, const BatchIt
     if (mode == kPairsForceStore && (!force_out || !xsph_out)) return hipErrorInvalidValue;
     if (rows && rows_stride == 0) return hipErrorInvalidValue;
     if (!pairs_cfg_valid(mode, cfg)) return hipErrorInvalidValue;
-    // Instantiated shapes: density at 4, 5 and 6 waves, the force modes at 3
-    // and 4 (pairs_cfg_valid has admitted only these).
+    // Instantiated shapes: density at 4, 5 and 6 waves, the force modes at 3,
+    // 4 and 5 (pairs_cfg_valid has admitted only these).
     auto go = [&](auto m, auto b, auto w) {
         constexpr int M = decltype(m)::value, BL = decltype(b)::value, W = decltype(w)::value;
         if (rows)
@@ -435,7 +435,8 @@ hipError_t launch_pairs_b(int mode, const PairsCfg& cfg, int grid, const BatchIt
             else go(m, b, std::integral_constant<int, 6>{});
         } else {
             if (cfg.waves == 3) go(m, b, std::integral_constant<int, 3>{});
-            else go(m, b, std::integral_constant<int, 4>{});
+            else if (cfg.waves == 4) go(m, b, std::integral_constant<int, 4>{});
+            else go(m, b, std::integral_constant<int, 5>{});
         }
     };
     auto by_block = [&](auto m) {

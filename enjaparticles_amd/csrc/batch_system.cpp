@@ -248,7 +248,7 @@ int autotune_step_b(rtps_batch* b, bool measure) {
     }
     const std::vector<PairsCfg> cands[2] = {
         {{256, 5}, {256, 4}, {256, 6}, {128, 5}, {128, 4}, {128, 6}, {64, 5}, {64, 4}, {64, 6}},
-        {{256, 4}, {256, 3}, {128, 4}, {128, 3}, {64, 4}, {64, 3}}};
+        {{256, 4}, {256, 3}, {128, 4}, {128, 3}, {64, 4}, {64, 3}, {256, 5}, {128, 5}, {64, 5}}};
     constexpr int kReps = 3;
     constexpr float kMargin = 0.97f;
     const char* verbose_env = std::getenv("RTPS_TUNE_VERBOSE");

@@ -42,10 +42,11 @@ constexpr int kPairsDensity = 0, kPairsForceStep = 1, kPairsForceStore = 2;
 // (= slots per work item: 64, 128 or 256) and the register budget of `waves`
 // waves per SIMD (3, 4, 5 or 6: at most 168, 128, 96 or 80 VGPRs); the grid
 // is CUs x waves x 256 / block blocks.  The density kernel is built for 4, 5
-// and 6 waves; the force kernels for 3 and 4 only, since their live values
-// exceed the 96 registers of 5 waves and would spill to scratch memory.  The
-// batched system's autotuner picks one per kernel; the defaults are 256
-// threads at 5 waves (density) and at 4 waves (force).
+// and 6 waves; the force kernels for 3, 4 and 5 (5 fits their registers only
+// on recorded rows: looking the rows up, their live values exceed the 96
+// registers of 5 waves and spill to scratch memory).  The batched system's
+// autotuner picks one per kernel; the defaults are 256 threads at 5 waves
+// (density) and at 4 waves (force).
 struct PairsCfg {
     int block = 256;
     int waves = 5;

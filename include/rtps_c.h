@@ -508,8 +508,9 @@ int rtps_batch_read(rtps_batch* b, int scene, int field, void* host, int count);
  * default; see rtps_batch_autotune) and RTPS_OPT_BLOCK_DENSITY /
  * RTPS_OPT_BLOCK_FORCE = block * 10 + waves (e.g. 2565: 256 threads per block
  * = slots per work item, compiled for 5 waves per SIMD; block 64, 128 or 256,
- * waves 4..6 for density and 3..4 for force, whose registers would spill at
- * 5 waves), which fix that kernel's launch; RTPS_OPT_SHARED_ROWS (1 = the
+ * waves 4..6 for density and 3..5 for force, whose registers spill at 5 waves
+ * unless the rows are recorded), which fix that kernel's launch;
+ * RTPS_OPT_SHARED_ROWS (1 = the
  * density kernel records each particle's neighbour windows and within-h bits
  * for the force kernel, default; 0 = the force kernel looks them up; the same
  * results either way; 108 bytes of device memory per slot). */
