@@ -202,7 +202,7 @@ __device__ __forceinline__ void density_tile(int tile, int num_host, const int* 
     auto record = [&](int dz, int dy, uint32_t a, uint32_t b, uint32_t A, uint32_t B) {
         if (EMIT) {
             const int r = 3 * (dz + 1) + dy + 1;
-            uint32_t* __restrict__ o = rec + 3 * (size_t)(r * win.rows_stride);
+            uint32_t* __restrict__ o = rec + 3 * ((size_t)r * win.rows_stride);
             o[0] = a;
             o[1] = b;
             o[2] = rej;
@@ -526,7 +526,7 @@ __device__ __forceinline__ void force_tile(int tile, int num_host, int* d_num,
                 uint32_t a, b, A, B, rej_last = 0u;
                 if constexpr (LOOP == kLoopShared) {
                     const int r = 3 * (dz + 1) + dy + 1;
-                    const uint32_t* __restrict__ o = rec + 3 * (size_t)(r * win.rows_stride);
+                    const uint32_t* __restrict__ o = rec + 3 * ((size_t)r * win.rows_stride);
                     a = o[0];
                     b = o[1];
                     rej_last = o[2];
